@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark of the RPY grand-mobility hot path on one MI355X.
+
+A step is one full grand-mobility apply (all four reference blocks:
+seg<-seg 6x6, seg<-blob, blob<-seg, blob<-blob) over the 46,722-particle
+swimmer (1024 filaments x 30 segments + 16,002 blobs on a sphere of radius
+40a), forces and torques i.i.d. N(0,1), inputs resident in HBM.  The line
+reports pair-interactions/s of the FP32-Kahan kernel (`value`), the FP64
+kernel beside it, the FP32 error against FP64, the pair kernel's achieved
+FLOP/s against the vector roofline, and the CPU FP64 oracle timed on the
+host cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from generic_mobility_interface_class_amd import capi, geometry  # noqa: E402
+
+FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40}      # include/gmi_flops.h
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def grand_pairs_flops(ns, nb):
+    pairs = ns * ns + 2 * ns * nb + nb * nb
+    flops = FLOPS["ss"] * ns * ns + (FLOPS["sb"] + FLOPS["bs"]) * ns * nb + FLOPS["bb"] * nb * nb
+    return pairs, flops
+
+
+def time_apply(ctx, steps, warmup):
+    for _ in range(warmup):
+        ctx.call("gmi_evaluate_grand_mobility")
+    ctx.call("gmi_wait_for_device")
+    ctx.call("gmi_profile_begin")
+    ctx.call("gmi_timer_start")
+    for _ in range(steps):
+        ctx.call("gmi_evaluate_grand_mobility")
+    ms = capi.C.c_double(0.0)
+    ctx.call("gmi_timer_stop", capi.C.byref(ms))
+    pair_ms = capi.C.c_double(0.0)
+    n = capi.C.c_int(0)
+    ctx.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
+    ctx.call("gmi_wait_for_device")
+    return ms.value, pair_ms.value / max(1, n.value)
+
+
+def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
+    """Oracle (CPU FP64 restatement) on a bounded sample of target rows
+    against all sources; returns pairs/s."""
+    from oracle import binding as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ns, nb = len(xs), len(xb)
+
+    def run(rs, rb):
+        t0 = time.perf_counter()
+        v = orc.Mss(xs, fs, 1.0, rows=(0, rs), nthreads=threads)
+        orc.Msb(xs, xb, fb, 1.0, 1.0, V=v, rows=(0, rs), nthreads=threads)
+        orc.Mbs(xb, xs, fs, 1.0, 1.0, rows=(0, rb), nthreads=threads)
+        orc.Mbb(xb, fb, 1.0, rows=(0, rb), nthreads=threads)
+        return time.perf_counter() - t0, (rs + rb) * (ns + nb)
+
+    dt, pairs = run(64, 32)
+    scale = max(1.0, seconds / max(dt, 1e-6))
+    rs = min(ns, int(64 * scale))
+    rb = min(nb, int(32 * scale))
+    dt, pairs = run(rs, rb)
+    return {
+        "value": pairs / dt,
+        "unit": "pair-interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rs} segment + {rb} blob target rows x all {ns + nb} sources "
+                  f"(all four blocks, FP64, {dt:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="apply_46722")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one MI355X; "
+              "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
+        sys.exit(2)
+
+    g = geometry.workload_geometry(args.workload)
+    xs, xb = g.positions()
+    fs, fb = geometry.random_forces(g.ns, g.nblob, seed=3)
+    pairs, flops = grand_pairs_flops(g.ns, g.nblob)
+
+    res = {}
+    vel = {}
+    for name, prec in (("fp32_kahan", capi.GMI_FP32_KAHAN), ("fp64", capi.GMI_FP64)):
+        ctx = capi.Context([g.ns], [g.nblob], g.nseg, precision=prec)
+        ctx.upload(xs, fs, xb, fb)
+        steps = args.steps if name == "fp32_kahan" else max(3, args.steps // 2)
+        total_ms, kern_ms = time_apply(ctx, steps, args.warmup)
+        vel[name] = ctx.velocities()
+        res[name] = dict(steps=steps, ms_per_step=total_ms / steps, kernel_ms=kern_ms)
+        ctx.close()
+        print(f"# {name}: {total_ms / steps:.3f} ms/apply, pair kernel {kern_ms:.3f} ms",
+              file=sys.stderr, flush=True)
+
+    v32s, v32b = vel["fp32_kahan"]
+    v64s, v64b = vel["fp64"]
+    num = np.sqrt(np.sum((v32s - v64s) ** 2) + np.sum((v32b - v64b) ** 2))
+    den = np.sqrt(np.sum(v64s ** 2) + np.sum(v64b ** 2))
+    err_norm = float(num / den)
+    err_max = float(max(np.abs(v32s - v64s).max() / np.abs(v64s).max(),
+                        np.abs(v32b - v64b).max() / np.abs(v64b).max()))
+
+    r32, r64 = res["fp32_kahan"], res["fp64"]
+    achieved32 = flops / (r32["kernel_ms"] * 1e-3) / 1e12
+    achieved64 = flops / (r64["kernel_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": "RPY pair-interactions/s (% FP32 peak)",
+        "value": pairs / (r32["ms_per_step"] * 1e-3),
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": r32["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (geodesic sphere + Fibonacci filaments, N(0,1) forces, seed 3)",
+        "config": {
+            "workload": args.workload,
+            "segments": g.ns, "blobs": g.nblob,
+            "pairs_per_apply": pairs, "flop_per_apply": flops,
+            "blocks": "ss+sb+bs+bb", "precision": "FP32 pairs, Kahan cross-tile, FP64 reduce",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved32, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved32 / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "k_pair<float>",
+            "kernel_ms": r32["kernel_ms"],
+        },
+        "fp64": {
+            "value": pairs / (r64["ms_per_step"] * 1e-3),
+            "ms_per_step": r64["ms_per_step"],
+            "kernel_ms": r64["kernel_ms"],
+            "achieved_tflops": round(achieved64, 3),
+            "frac_fp64_peak": round(achieved64 / PEAK_FP64_TFLOPS, 4),
+        },
+        "fp32_error_vs_fp64": {"norm_rel": err_norm, "max_rel": err_max},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(xs, xb, fs, fb, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+"""ctypes binding of the C ABI in include/gmi.h.
+
+This is the binding a Python caller of the library uses; it carries no
+compute of its own.  Loading fails loudly when libgmi.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgmi.so")
+
+GMI_OK = 0
+GMI_ERR_ARG = -1
+GMI_ERR_HIP = -2
+GMI_ERR_STATE = -3
+GMI_ERR_NOT_FINITE = -4
+GMI_ERR_SINGULAR = -5
+
+GMI_FP64 = 0
+GMI_FP32_KAHAN = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class GmiParams(C.Structure):
+    _fields_ = [
+        ("n_swimmers", C.c_int),
+        ("nseg_per_swimmer", _ip),
+        ("nblob_per_swimmer", _ip),
+        ("nseg_per_filament", C.c_int),
+        ("swimmers_interact", C.c_int),
+        ("mu", C.c_double),
+        ("a_seg", C.c_double),
+        ("a_blob", C.c_double),
+        ("precision", C.c_int),
+        ("device", C.c_int),
+    ]
+
+
+# (name, restype, argtypes) of every exported function declared in gmi.h
+SIGNATURES = [
+    ("gmi_device_count", C.c_int, [_ip]),
+    ("gmi_create", C.c_int, [C.POINTER(C.c_void_p), C.POINTER(GmiParams)]),
+    ("gmi_destroy", None, [C.c_void_p]),
+    ("gmi_last_error", C.c_char_p, [C.c_void_p]),
+    ("gmi_set_precision", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_num_segments", C.c_int, [C.c_void_p]),
+    ("gmi_num_blobs", C.c_int, [C.c_void_p]),
+    ("gmi_copy_segment_positions_to_device", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_segment_forces_to_device", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_blob_positions_to_device", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_blob_forces_to_device", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_interparticle_blob_forces_to_host", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_blob_velocities_to_host", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_copy_segment_velocities_to_host", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_apply_interparticle_forces", C.c_int, [C.c_void_p, C.c_double]),
+    ("gmi_wait_for_device", C.c_int, [C.c_void_p]),
+    ("gmi_evaluate_segment_segment_mobility", C.c_int, [C.c_void_p]),
+    ("gmi_evaluate_segment_blob_mobility", C.c_int, [C.c_void_p]),
+    ("gmi_evaluate_blob_blob_mobility", C.c_int, [C.c_void_p]),
+    ("gmi_evaluate_blob_segment_mobility", C.c_int, [C.c_void_p]),
+    ("gmi_evaluate_grand_mobility", C.c_int, [C.c_void_p]),
+    ("gmi_timer_start", C.c_int, [C.c_void_p]),
+    ("gmi_timer_stop", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_profile_begin", C.c_int, [C.c_void_p]),
+    ("gmi_profile_end", C.c_int, [C.c_void_p, _dp, _ip]),
+    ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libgmi.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing; build it with "
+            "`python -m generic_mobility_interface_class_amd.build`")
+    lib = C.CDLL(path)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    load().gmi_device_count(C.byref(n))
+    return int(n.value)
+
+
+class GmiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gmi error {code}: {msg}")
+        self.code = code
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device context (gmi_create / gmi_destroy)."""
+
+    def __init__(self, nseg_per_swimmer, nblob_per_swimmer, nseg_per_filament,
+                 mu=1.0, a_seg=1.0, a_blob=1.0, precision=GMI_FP64,
+                 interact=True, device=0):
+        self.lib = load()
+        ns = np.ascontiguousarray(nseg_per_swimmer, dtype=np.int32)
+        nb = np.ascontiguousarray(nblob_per_swimmer, dtype=np.int32)
+        self._ns, self._nb = ns, nb
+        p = GmiParams(len(ns), ns.ctypes.data_as(_ip), nb.ctypes.data_as(_ip),
+                      int(nseg_per_filament), int(bool(interact)), float(mu),
+                      float(a_seg), float(a_blob), int(precision), int(device))
+        h = C.c_void_p()
+        rc = self.lib.gmi_create(C.byref(h), C.byref(p))
+        if rc != GMI_OK:
+            msg = self.lib.gmi_last_error(h) if h.value else b"bad parameters"
+            if h.value:
+                self.lib.gmi_destroy(h)
+            raise GmiError(rc, msg.decode())
+        self.h = h
+        self.Ns = int(ns.sum())
+        self.Nb = int(nb.sum())
+
+    def check(self, rc):
+        if rc != GMI_OK:
+            raise GmiError(rc, self.lib.gmi_last_error(self.h).decode())
+
+    def call(self, name, *args):
+        self.check(getattr(self.lib, name)(self.h, *args))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gmi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # convenience wrappers -------------------------------------------------
+    def upload(self, xs=None, fs=None, xb=None, fb=None):
+        if xs is not None:
+            self._xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1)
+            self.call("gmi_copy_segment_positions_to_device", ptr(self._xs))
+        if fs is not None:
+            self._fs = np.ascontiguousarray(fs, dtype=np.float64).reshape(-1)
+            self.call("gmi_copy_segment_forces_to_device", ptr(self._fs))
+        if xb is not None:
+            self._xb = np.ascontiguousarray(xb, dtype=np.float64).reshape(-1)
+            self.call("gmi_copy_blob_positions_to_device", ptr(self._xb))
+        if fb is not None:
+            self._fb = np.ascontiguousarray(fb, dtype=np.float64).reshape(-1)
+            self.call("gmi_copy_blob_forces_to_device", ptr(self._fb))
+
+    def velocities(self):
+        vs = np.zeros(6 * self.Ns)
+        vb = np.zeros(3 * self.Nb)
+        self.call("gmi_copy_segment_velocities_to_host", ptr(vs))
+        self.call("gmi_copy_blob_velocities_to_host", ptr(vb))
+        self.call("gmi_wait_for_device")
+        return vs.reshape(-1, 6), vb.reshape(-1, 3)
+
+    def set_precision(self, p):
+        self.call("gmi_set_precision", int(p))

@@ -1,0 +1,464 @@
+// gmi_capi.hip -- the C ABI (include/gmi.h): context, buffers, launches.
+//
+// Replaces rpy_mobility_solver (rpy_mobility_solver.cu:14-311).  Design:
+//  * one context = one device + one stream; all device vectors resident;
+//  * device layout is SoA (component-major, leading dimension padded to a
+//    multiple of 64) so that every kernel load is coalesced;
+//  * host copies go through per-buffer pinned staging: H2D is asynchronous
+//    (the pinned region is reused only after its previous copy finished),
+//    D2H completes at gmi_wait_for_device();
+//  * pair launches run from precomputed work lists (target tile x source
+//    split), so uneven swimmers and small problems still fill the GPU.
+#include "gmi_internal.hpp"
+#include "gmi_ctx.hpp"
+#include "../../include/gmi.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+
+using namespace gmi;
+
+namespace gmi {
+
+int fail(gmi_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_check(gmi_ctx *c, hipError_t e, const char *what) {
+  if (e == hipSuccess) return GMI_OK;
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  return fail(c, GMI_ERR_HIP, buf);
+}
+
+int pad64(int n) { return ((n + 63) / 64) * 64; }
+
+// Build the work list of one block mask.
+static int build_list(gmi_ctx *c, int mask, WorkList &wl) {
+  wl.h_tiles.clear();
+  std::vector<WorkItem> work;
+  const bool seg_targets = mask & (kSS | kSB);
+  const bool blob_targets = mask & (kBS | kBB);
+  struct G { int s0, ns, b0, nb; };
+  std::vector<G> groups;
+  if (c->interact || c->nsw == 1) {
+    groups.push_back({0, c->Ns, 0, c->Nb});
+  } else {
+    for (int g = 0; g < c->nsw; ++g)
+      groups.push_back({c->seg_off[g], c->nseg[g], c->blob_off[g], c->nblob[g]});
+  }
+  // total pair work, to spread splits in proportion
+  long long tot_tiles = 0;
+  for (const G &g : groups) {
+    if (seg_targets) tot_tiles += (g.ns + kBlock - 1) / kBlock;
+    if (blob_targets) tot_tiles += (g.nb + kBlock - 1) / kBlock;
+  }
+  const long long target = std::max<long long>(1, c->target_blocks);
+  const int base_split =
+      int(std::max<long long>(1, (target + tot_tiles - 1) / std::max<long long>(1, tot_tiles)));
+  int part0 = 0;
+  for (const G &g : groups) {
+    for (int sp = 0; sp < 2; ++sp) {
+      const bool segt = (sp == 0);
+      if (segt ? !seg_targets : !blob_targets) continue;
+      const int nt_all = segt ? g.ns : g.nb;
+      const int t_off = segt ? g.s0 : g.b0;
+      const bool use_s = segt ? (mask & kSS) : (mask & kBS);
+      const bool use_b = segt ? (mask & kSB) : (mask & kBB);
+      const int L = (use_s ? g.ns : 0) + (use_b ? g.nb : 0);
+      const int max_split = std::max(1, (L + c->min_chunk - 1) / c->min_chunk);
+      const int nsplit = std::max(1, std::min(base_split, max_split));
+      for (int t = 0; t < nt_all; t += kBlock) {
+        TileDesc T;
+        T.species = segt ? 0 : 1;
+        T.t0 = t_off + t;
+        T.nt = std::min(kBlock, nt_all - t);
+        T.sseg0 = g.s0; T.nsseg = g.ns;
+        T.sblob0 = g.b0; T.nsblob = g.nb;
+        T.nsplit = nsplit;
+        T.part0 = (nsplit > 1) ? part0 : 0;
+        if (nsplit > 1) part0 += nsplit;
+        const int id = int(wl.h_tiles.size());
+        wl.h_tiles.push_back(T);
+        for (int s = 0; s < nsplit; ++s) work.push_back({id, s});
+      }
+    }
+  }
+  wl.ntiles = int(wl.h_tiles.size());
+  wl.nwork = int(work.size());
+  wl.nslots = part0;
+  wl.nsplit_tiles = 0;
+  for (const TileDesc &T : wl.h_tiles) wl.nsplit_tiles += (T.nsplit > 1);
+  if (wl.d_tiles) (void)hipFree(wl.d_tiles);
+  if (wl.d_work) (void)hipFree(wl.d_work);
+  wl.d_tiles = nullptr; wl.d_work = nullptr;
+  if (wl.ntiles == 0) return GMI_OK;
+  int rc;
+  if ((rc = hip_check(c, hipMalloc(&wl.d_tiles, sizeof(TileDesc) * wl.ntiles), "hipMalloc tiles")))
+    return rc;
+  if ((rc = hip_check(c, hipMalloc(&wl.d_work, sizeof(WorkItem) * wl.nwork), "hipMalloc work")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpy(wl.d_tiles, wl.h_tiles.data(), sizeof(TileDesc) * wl.ntiles,
+                                   hipMemcpyHostToDevice), "upload tiles")))
+    return rc;
+  return hip_check(c, hipMemcpy(wl.d_work, work.data(), sizeof(WorkItem) * wl.nwork,
+                                hipMemcpyHostToDevice), "upload work");
+}
+
+int rebuild_lists(gmi_ctx *c) {
+  int rc;
+  size_t slots = 0;
+  for (int mask = 1; mask < 16; ++mask) {
+    if ((rc = build_list(c, mask, c->lists[mask]))) return rc;
+    slots = std::max(slots, size_t(c->lists[mask].nslots));
+  }
+  if ((rc = build_list(c, kSS | kBB, c->barrier_tiles))) return rc;
+  if (slots > c->part_slots) {
+    if (c->part) (void)hipFree(c->part);
+    c->part = nullptr;
+    if ((rc = hip_check(c, hipMalloc(&c->part, slots * 6 * kBlock * sizeof(double)),
+                        "hipMalloc partials")))
+      return rc;
+    c->part_slots = slots;
+  }
+  return GMI_OK;
+}
+
+int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
+              const double *Fs, const double *Fb, double *Vs, double *Vb,
+              const double *Xs, const double *Xb) {
+  const WorkList &wl = c->lists[mask];
+  PairArgs a;
+  a.Xs = Xs ? Xs : c->Xs; a.Fs = Fs ? Fs : c->Fs; a.lds = c->lds;
+  a.Xb = Xb ? Xb : c->Xb; a.Fb = Fb ? Fb : c->Fb; a.ldb = c->ldb;
+  a.Vs = Vs ? Vs : c->Vs; a.Vb = Vb ? Vb : c->Vb;
+  a.part = c->part;
+  a.tiles = wl.d_tiles;
+  a.work = wl.d_work;
+  a.mask = mask;
+  a.accumulate = accumulate;
+  a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
+  hipEvent_t ev_mid = nullptr;
+  if (c->profiling && wl.nwork > 0) {
+    int rc;
+    while (c->prof_events.size() < c->prof_used + 2) {
+      hipEvent_t e;
+      if ((rc = hip_check(c, hipEventCreate(&e), "event"))) return rc;
+      c->prof_events.push_back(e);
+    }
+    if ((rc = hip_check(c, hipEventRecord(c->prof_events[c->prof_used], c->stream), "event")))
+      return rc;
+    ev_mid = c->prof_events[c->prof_used + 1];
+    c->prof_used += 2;
+  }
+  return hip_check(c, launch_pair(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0,
+                                  fp32, c->stream, ev_mid),
+                   "pair launch");
+}
+
+}  // namespace gmi
+
+// ---- staging helpers -------------------------------------------------------
+
+static int h2d(gmi_ctx *c, int slot, const double *host, int n, int nc,
+               double *dst_soa, int ld) {
+  if (!host) return fail(c, GMI_ERR_ARG, "null host pointer");
+  if (n == 0) return GMI_OK;
+  gmi_ctx::Stage &st = c->stage[slot];
+  int rc;
+  // the pinned region may still feed an earlier asynchronous copy
+  if ((rc = hip_check(c, hipEventSynchronize(st.done), "stage wait"))) return rc;
+  std::memcpy(st.pinned, host, sizeof(double) * size_t(n) * nc);
+  if ((rc = hip_check(c, hipMemcpyAsync(st.dev, st.pinned, sizeof(double) * size_t(n) * nc,
+                                        hipMemcpyHostToDevice, c->stream), "H2D")))
+    return rc;
+  if ((rc = hip_check(c, hipEventRecord(st.done, c->stream), "stage record"))) return rc;
+  return hip_check(c, launch_aos_to_soa(dst_soa, ld, st.dev, n, nc, c->stream), "aos->soa");
+}
+
+static int flush_pending(gmi_ctx *c) {
+  if (c->pending.empty()) return GMI_OK;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  for (const gmi_ctx::Pending &p : c->pending) std::memcpy(p.dst, p.src, p.bytes);
+  c->pending.clear();
+  return GMI_OK;
+}
+
+static int d2h(gmi_ctx *c, int slot, double *host, int n, int nc,
+               const double *src_soa, int ld) {
+  if (!host) return fail(c, GMI_ERR_ARG, "null host pointer");
+  if (n == 0) return GMI_OK;
+  gmi_ctx::Stage &st = c->stage[slot];
+  int rc;
+  for (const gmi_ctx::Pending &p : c->pending)
+    if (p.src == st.pinned) {
+      if ((rc = flush_pending(c))) return rc;
+      break;
+    }
+  if ((rc = hip_check(c, hipEventSynchronize(st.done), "stage wait"))) return rc;
+  if ((rc = hip_check(c, launch_soa_to_aos(st.dev, src_soa, ld, n, nc, c->stream), "soa->aos")))
+    return rc;
+  const size_t bytes = sizeof(double) * size_t(n) * nc;
+  if ((rc = hip_check(c, hipMemcpyAsync(st.pinned, st.dev, bytes, hipMemcpyDeviceToHost,
+                                        c->stream), "D2H")))
+    return rc;
+  if ((rc = hip_check(c, hipEventRecord(st.done, c->stream), "stage record"))) return rc;
+  c->pending.push_back({host, st.pinned, bytes});
+  return GMI_OK;
+}
+
+// ---- C ABI -------------------------------------------------------------------
+
+extern "C" {
+
+int gmi_device_count(int *count) {
+  if (!count) return GMI_ERR_ARG;
+  *count = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return GMI_OK;  // no runtime / no device: zero devices
+  }
+  *count = n;
+  return GMI_OK;
+}
+
+int gmi_create(gmi_ctx **out, const gmi_params *p) {
+  if (!out || !p) return GMI_ERR_ARG;
+  *out = nullptr;
+  if (p->n_swimmers < 1 || !p->nseg_per_swimmer || !p->nblob_per_swimmer ||
+      p->nseg_per_filament < 1 || !(p->mu > 0) || !(p->a_seg > 0) || !(p->a_blob > 0) ||
+      (p->precision != GMI_FP64 && p->precision != GMI_FP32_KAHAN))
+    return GMI_ERR_ARG;
+  gmi_ctx *c = new gmi_ctx();
+  c->device = p->device;
+  c->nsw = p->n_swimmers;
+  c->nseg.assign(p->nseg_per_swimmer, p->nseg_per_swimmer + p->n_swimmers);
+  c->nblob.assign(p->nblob_per_swimmer, p->nblob_per_swimmer + p->n_swimmers);
+  c->Ns = 0; c->Nb = 0;
+  for (int g = 0; g < c->nsw; ++g) {
+    if (c->nseg[g] < 0 || c->nblob[g] < 0 || c->nseg[g] % p->nseg_per_filament) {
+      delete c;
+      return GMI_ERR_ARG;
+    }
+    c->seg_off.push_back(c->Ns);
+    c->blob_off.push_back(c->Nb);
+    c->Ns += c->nseg[g];
+    c->Nb += c->nblob[g];
+  }
+  c->nseg_fil = p->nseg_per_filament;
+  c->interact = p->swimmers_interact != 0;
+  c->mu = p->mu; c->as = p->a_seg; c->ab = p->a_blob;
+  c->precision = p->precision;
+  c->lds = pad64(std::max(c->Ns, 1));
+  c->ldb = pad64(std::max(c->Nb, 1));
+  *out = c;
+  int rc;
+  if ((rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice"))) return rc;
+  if ((rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking),
+                      "stream")))
+    return rc;
+  if ((rc = hip_check(c, hipEventCreate(&c->ev0), "event"))) return rc;
+  if ((rc = hip_check(c, hipEventCreate(&c->ev1), "event"))) return rc;
+  auto dalloc = [&](double **ptr, size_t n) {
+    return hip_check(c, hipMalloc(ptr, sizeof(double) * std::max<size_t>(n, 1)), "hipMalloc");
+  };
+  if ((rc = dalloc(&c->Xs, 3 * size_t(c->lds)))) return rc;
+  if ((rc = dalloc(&c->Fs, 6 * size_t(c->lds)))) return rc;
+  if ((rc = dalloc(&c->Vs, 6 * size_t(c->lds)))) return rc;
+  if ((rc = dalloc(&c->Xb, 3 * size_t(c->ldb)))) return rc;
+  if ((rc = dalloc(&c->Fb, 3 * size_t(c->ldb)))) return rc;
+  if ((rc = dalloc(&c->Vb, 3 * size_t(c->ldb)))) return rc;
+  if ((rc = dalloc(&c->Frep, 3 * size_t(c->ldb)))) return rc;
+  // zero every vector once so that partial uploads never expose garbage
+  double *bufs[] = {c->Xs, c->Fs, c->Vs, c->Xb, c->Fb, c->Vb, c->Frep};
+  size_t lens[] = {3u * c->lds, 6u * c->lds, 6u * c->lds, 3u * c->ldb,
+                   3u * c->ldb, 3u * c->ldb, 3u * c->ldb};
+  for (int k = 0; k < 7; ++k)
+    if ((rc = hip_check(c, hipMemset(bufs[k], 0, sizeof(double) * lens[k]), "memset")))
+      return rc;
+  const size_t stage_n[gmi_ctx::kNumStage] = {
+      3u * c->Ns, 6u * c->Ns, 3u * c->Nb, 3u * c->Nb, 3u * c->Nb, 3u * c->Nb, 6u * c->Ns};
+  for (int k = 0; k < gmi_ctx::kNumStage; ++k) {
+    gmi_ctx::Stage &st = c->stage[k];
+    const size_t bytes = sizeof(double) * std::max<size_t>(stage_n[k], 1);
+    if ((rc = hip_check(c, hipHostMalloc((void **)&st.pinned, bytes, hipHostMallocDefault),
+                        "hipHostMalloc")))
+      return rc;
+    if ((rc = hip_check(c, hipMalloc(&st.dev, bytes), "hipMalloc stage"))) return rc;
+    if ((rc = hip_check(c, hipEventCreateWithFlags(&st.done, hipEventDisableTiming), "event")))
+      return rc;
+    if ((rc = hip_check(c, hipEventRecord(st.done, c->stream), "event record"))) return rc;
+  }
+  if ((rc = rebuild_lists(c))) return rc;
+  return GMI_OK;
+}
+
+void gmi_destroy(gmi_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->release_extra();
+  double *bufs[] = {c->Xs, c->Fs, c->Vs, c->Xb, c->Fb, c->Vb, c->Frep, c->part};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &kv : c->lists) {
+    if (kv.second.d_tiles) (void)hipFree(kv.second.d_tiles);
+    if (kv.second.d_work) (void)hipFree(kv.second.d_work);
+  }
+  if (c->barrier_tiles.d_tiles) (void)hipFree(c->barrier_tiles.d_tiles);
+  if (c->barrier_tiles.d_work) (void)hipFree(c->barrier_tiles.d_work);
+  for (auto &st : c->stage) {
+    if (st.pinned) (void)hipHostFree(st.pinned);
+    if (st.dev) (void)hipFree(st.dev);
+    if (st.done) (void)hipEventDestroy(st.done);
+  }
+  for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *gmi_last_error(const gmi_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int gmi_set_precision(gmi_ctx *c, int precision) {
+  if (!c) return GMI_ERR_ARG;
+  if (precision != GMI_FP64 && precision != GMI_FP32_KAHAN)
+    return fail(c, GMI_ERR_ARG, "unknown precision");
+  c->precision = precision;
+  return GMI_OK;
+}
+
+int gmi_num_segments(const gmi_ctx *c) { return c ? c->Ns : GMI_ERR_ARG; }
+int gmi_num_blobs(const gmi_ctx *c) { return c ? c->Nb : GMI_ERR_ARG; }
+
+int gmi_copy_segment_positions_to_device(gmi_ctx *c, const double *x) {
+  if (!c) return GMI_ERR_ARG;
+  return h2d(c, gmi_ctx::kStXs, x, c->Ns, 3, c->Xs, c->lds);
+}
+int gmi_copy_segment_forces_to_device(gmi_ctx *c, const double *f) {
+  if (!c) return GMI_ERR_ARG;
+  return h2d(c, gmi_ctx::kStFs, f, c->Ns, 6, c->Fs, c->lds);
+}
+int gmi_copy_blob_positions_to_device(gmi_ctx *c, const double *x) {
+  if (!c) return GMI_ERR_ARG;
+  return h2d(c, gmi_ctx::kStXb, x, c->Nb, 3, c->Xb, c->ldb);
+}
+int gmi_copy_blob_forces_to_device(gmi_ctx *c, const double *f) {
+  if (!c) return GMI_ERR_ARG;
+  return h2d(c, gmi_ctx::kStFb, f, c->Nb, 3, c->Fb, c->ldb);
+}
+int gmi_copy_interparticle_blob_forces_to_host(gmi_ctx *c, double *f) {
+  if (!c) return GMI_ERR_ARG;
+  return d2h(c, gmi_ctx::kStRep, f, c->Nb, 3, c->Frep, c->ldb);
+}
+int gmi_copy_blob_velocities_to_host(gmi_ctx *c, double *v) {
+  if (!c) return GMI_ERR_ARG;
+  return d2h(c, gmi_ctx::kStVb, v, c->Nb, 3, c->Vb, c->ldb);
+}
+int gmi_copy_segment_velocities_to_host(gmi_ctx *c, double *v) {
+  if (!c) return GMI_ERR_ARG;
+  return d2h(c, gmi_ctx::kStVs, v, c->Ns, 6, c->Vs, c->lds);
+}
+
+int gmi_apply_interparticle_forces(gmi_ctx *c, double K) {
+  if (!c) return GMI_ERR_ARG;
+  if (!std::isfinite(K)) return fail(c, GMI_ERR_ARG, "barrier K not finite");
+  const WorkList &wl = c->barrier_tiles;
+  return hip_check(c, launch_barrier(c->Fs, c->lds, c->Frep, c->ldb, c->Xs, c->Xb,
+                                     wl.d_tiles, wl.ntiles, c->nseg_fil, c->as, c->ab, K,
+                                     c->stream),
+                   "barrier launch");
+}
+
+int gmi_wait_for_device(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  return flush_pending(c);
+}
+
+int gmi_evaluate_segment_segment_mobility(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return run_pairs(c, kSS, 0, c->precision == GMI_FP32_KAHAN, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr);
+}
+int gmi_evaluate_segment_blob_mobility(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return run_pairs(c, kSB, 1, c->precision == GMI_FP32_KAHAN, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr);
+}
+int gmi_evaluate_blob_blob_mobility(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return run_pairs(c, kBB, 0, c->precision == GMI_FP32_KAHAN, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr);
+}
+int gmi_evaluate_blob_segment_mobility(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return run_pairs(c, kBS, 0, c->precision == GMI_FP32_KAHAN, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, nullptr);
+}
+int gmi_evaluate_grand_mobility(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return run_pairs(c, kSS | kSB | kBS | kBB, 0, c->precision == GMI_FP32_KAHAN, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+int gmi_timer_start(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  return hip_check(c, hipEventRecord(c->ev0, c->stream), "event record");
+}
+
+int gmi_timer_stop(gmi_ctx *c, double *ms) {
+  if (!c || !ms) return GMI_ERR_ARG;
+  int rc;
+  if ((rc = hip_check(c, hipEventRecord(c->ev1, c->stream), "event record"))) return rc;
+  if ((rc = hip_check(c, hipEventSynchronize(c->ev1), "event sync"))) return rc;
+  float t = 0.f;
+  if ((rc = hip_check(c, hipEventElapsedTime(&t, c->ev0, c->ev1), "elapsed"))) return rc;
+  *ms = t;
+  return GMI_OK;
+}
+
+int gmi_profile_begin(gmi_ctx *c) {
+  if (!c) return GMI_ERR_ARG;
+  c->profiling = true;
+  c->prof_used = 0;
+  return GMI_OK;
+}
+
+int gmi_profile_end(gmi_ctx *c, double *pair_ms, int *launches) {
+  if (!c || !pair_ms || !launches) return GMI_ERR_ARG;
+  int rc;
+  c->profiling = false;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  double tot = 0.0;
+  for (size_t k = 0; k + 1 < c->prof_used; k += 2) {
+    float t = 0.f;
+    if ((rc = hip_check(c, hipEventElapsedTime(&t, c->prof_events[k], c->prof_events[k + 1]),
+                        "elapsed")))
+      return rc;
+    tot += t;
+  }
+  *pair_ms = tot;
+  *launches = int(c->prof_used / 2);
+  c->prof_used = 0;
+  return GMI_OK;
+}
+
+int gmi_set_launch_shape(gmi_ctx *c, int target_blocks, int min_chunk) {
+  if (!c) return GMI_ERR_ARG;
+  if (target_blocks < 1 || min_chunk < 1) return fail(c, GMI_ERR_ARG, "bad launch shape");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  c->target_blocks = target_blocks;
+  c->min_chunk = min_chunk;
+  return rebuild_lists(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// gmi_ctx.hpp -- the opaque context behind the C ABI (not public).
+#pragma once
+#include "gmi_internal.hpp"
+
+struct gmi_ctx {
+  // pinned/device staging slots for host copies
+  enum { kStXs, kStFs, kStXb, kStFb, kStRep, kStVb, kStVs, kNumStage };
+  struct Stage {
+    double *pinned = nullptr;
+    double *dev = nullptr;
+    hipEvent_t done = nullptr;
+  };
+  struct Pending {
+    void *dst;
+    const void *src;
+    size_t bytes;
+  };
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // sizes
+  int nsw = 0;
+  std::vector<int> nseg, nblob, seg_off, blob_off;
+  int Ns = 0, Nb = 0, lds = 64, ldb = 64;
+  int nseg_fil = 1;
+  bool interact = true;
+  double mu = 1, as = 1, ab = 1;
+  int precision = 0;
+
+  // device SoA vectors
+  double *Xs = nullptr, *Fs = nullptr, *Vs = nullptr;
+  double *Xb = nullptr, *Fb = nullptr, *Vb = nullptr, *Frep = nullptr;
+  double *part = nullptr;
+  size_t part_slots = 0;
+
+  // launch shaping (set by the tuner)
+  int target_blocks = 2048;
+  int min_chunk = 512;
+
+  std::map<int, gmi::WorkList> lists;
+  gmi::WorkList barrier_tiles;
+  Stage stage[kNumStage];
+  std::vector<Pending> pending;
+  std::string err;
+
+  // pair-kernel profiling: events around every pair-kernel launch
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_events;  // pairs (start, stop)
+  size_t prof_used = 0;
+
+  // hook for subsystems that own further device memory
+  void (*extra_release)(gmi_ctx *) = nullptr;
+  void *extra = nullptr;
+  void release_extra() {
+    if (extra_release) extra_release(this);
+    extra_release = nullptr;
+    extra = nullptr;
+  }
+};
+
+namespace gmi {
+int fail(gmi_ctx *c, int code, const std::string &msg);
+int hip_check(gmi_ctx *c, hipError_t e, const char *what);
+int rebuild_lists(gmi_ctx *c);
+int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32, const double *Fs,
+              const double *Fb, double *Vs, double *Vb, const double *Xs,
+              const double *Xb);
+}  // namespace gmi

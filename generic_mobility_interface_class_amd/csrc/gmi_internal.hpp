@@ -1,0 +1,71 @@
+// gmi_internal.hpp -- shared declarations of the gfx950 library (not public).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <map>
+
+namespace gmi {
+
+// Targets per pair-kernel workgroup (one target per lane, 4 waves).
+constexpr int kBlock = 256;
+// Sources staged in LDS per tile.
+constexpr int kTile = 256;
+
+enum PairMask : int { kSS = 1, kSB = 2, kBS = 4, kBB = 8 };
+
+// One target tile: up to kBlock consecutive targets of one species inside one
+// swimmer group, and the source ranges of that group.
+struct TileDesc {
+  int species;        // 0: segment targets, 1: blob targets
+  int t0, nt;         // target range in the species array
+  int sseg0, nsseg;   // segment sources of the group
+  int sblob0, nsblob; // blob sources of the group
+  int nsplit;         // source splits (>1: partials reduced afterwards)
+  int part0;          // first partial slot of this tile
+};
+
+struct WorkItem {
+  int tile;
+  int split;
+};
+
+// Device-side SoA views.  Component c of particle i lives at p[c*ld + i].
+struct PairArgs {
+  const double *Xs; const double *Fs; int lds;
+  const double *Xb; const double *Fb; int ldb;
+  double *Vs; double *Vb;
+  double *part;             // [slot][6][kBlock] partial sums
+  const TileDesc *tiles;
+  const WorkItem *work;
+  int mask;                 // PairMask bits
+  int accumulate;           // 0: V = M F, 1: V += M F
+  double a_s, a_b, mu;
+};
+
+// Work list of one pair launch (built once per mask).
+struct WorkList {
+  TileDesc *d_tiles = nullptr;
+  WorkItem *d_work = nullptr;
+  int ntiles = 0, nwork = 0, nslots = 0, nsplit_tiles = 0;
+  std::vector<TileDesc> h_tiles;
+};
+
+// Launchers (gmi_pair.hip).
+// ev_mid (optional) is recorded between the pair kernel and the reduction.
+hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce,
+                       bool fp32, hipStream_t s, hipEvent_t ev_mid = nullptr);
+hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb,
+                          const double *Xs, const double *Xb,
+                          const TileDesc *tiles, int ntiles, int nseg_per_fil,
+                          double a_s, double a_b, double K, hipStream_t s);
+
+// AoS <-> SoA transposes (gmi_pair.hip).
+hipError_t launch_aos_to_soa(double *dst, int ld, const double *src, int n,
+                             int ncomp, hipStream_t s);
+hipError_t launch_soa_to_aos(double *dst, const double *src, int ld, int n,
+                             int ncomp, hipStream_t s);
+
+}  // namespace gmi

@@ -1,0 +1,563 @@
+// gmi_pair.hip -- gfx950 RPY pair kernels (all four mobility blocks).
+//
+// One launch evaluates any subset of the four reference blocks
+//   ss: segment (v, w) <- segment (f, tau)   cuda_functions.cu:246-425
+//   sb: segment (v, w) <- blob f             cuda_functions.cu:541-694
+//   bs: blob v         <- segment (f, tau)   cuda_functions.cu:697-863
+//   bb: blob v         <- blob f             cuda_functions.cu:429-539
+// over a work list of (target tile, source split) items.  Each workgroup
+// owns kBlock targets (one per lane, 4 waves of 64) and walks one contiguous
+// slice of its group's sources in LDS tiles of kTile.  Every lane of a wave
+// reads the same LDS source at the same time (a broadcast read), so LDS
+// bandwidth per pair is one source record per wave.
+//
+// Far pairs (the common case) use the unnormalised separation d = x_i - x_j
+// and powers of 1/|d| from one reciprocal square root:
+//   v += A f + B d + g (tau x d),   w += A' tau + B' d + g (f x d).
+// Near pairs (|d| <= a_i + a_j, including the self pair) take a divergent
+// branch that reproduces the reference's overlap formulas with
+// r_hat = d / (|d| + 1e-20).
+//
+// FP64 variant: v_rsq_f64 refined by one third-order Newton step.
+// FP32 variant: positions split hi + lo so that near separations keep full
+// single precision, v_rsq_f32, plain FP32 sums inside a tile and Kahan
+// compensated sums across tiles; partial sums across source splits are
+// reduced in FP64.
+#include "gmi_internal.hpp"
+
+namespace gmi {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+template <typename R> __device__ __forceinline__ R rsq(R x);
+template <> __device__ __forceinline__ float rsq<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+template <> __device__ __forceinline__ double rsq<double>(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-(x * y), y, 1.0);             // 1 - x y^2
+  return fma(y * e, fma(0.375, e, 0.5), y);          // third-order step
+}
+
+// Constants of one radius pair (a_i target, a_j source), precomputed in FP64.
+template <typename R> struct PC {
+  R far2;       // (a_i + a_j)^2: pairs with r^2 above it use the far forms
+  R c8;         // 1/(8 pi mu)
+  R c8s3;       // c8 (a_i^2 + a_j^2)/3
+  R c8s;        // c8 (a_i^2 + a_j^2)
+  R c16;        // 1/(16 pi mu)
+  R c16x3;      // 3/(16 pi mu)
+  // near-field (overlap) constants
+  R a;          // equal radius (ss / bb)
+  R c6a;        // 1/(6 pi mu a)
+  R c8a3;       // 1/(8 pi mu a^3)
+  R c16a3;      // 1/(16 pi mu a^3)
+  R as, ab, dlt, c6ab, c128, c6max, c8as3;  // unequal radii (sb / bs)
+};
+
+template <typename R>
+__host__ __device__ PC<R> make_pc(double ai, double aj, double mu) {
+  PC<R> k;
+  const double c8 = 1.0 / (8.0 * kPi * mu);
+  const double sig = ai * ai + aj * aj;
+  k.far2 = R((ai + aj) * (ai + aj));
+  k.c8 = R(c8);
+  k.c8s3 = R(c8 * sig / 3.0);
+  k.c8s = R(c8 * sig);
+  k.c16 = R(1.0 / (16.0 * kPi * mu));
+  k.c16x3 = R(3.0 / (16.0 * kPi * mu));
+  k.a = R(ai);
+  k.c6a = R(1.0 / (6.0 * kPi * mu * ai));
+  k.c8a3 = R(1.0 / (8.0 * kPi * mu * ai * ai * ai));
+  k.c16a3 = R(1.0 / (16.0 * kPi * mu * ai * ai * ai));
+  return k;
+}
+
+// Unequal-radius constants: as = segment radius, ab = blob radius.
+template <typename R>
+__host__ __device__ PC<R> make_pc_unequal(double as, double ab, double mu) {
+  PC<R> k = make_pc<R>(as, ab, mu);
+  k.as = R(as);
+  k.ab = R(ab);
+  k.dlt = R(as > ab ? as - ab : ab - as);
+  k.c6ab = R(1.0 / (6.0 * kPi * mu * as * ab));
+  k.c128 = R(1.0 / (128.0 * kPi * mu * ab * as * as * as));
+  k.c6max = R(1.0 / (6.0 * kPi * mu * (as > ab ? as : ab)));
+  k.c8as3 = R(as > ab ? 1.0 / (8.0 * kPi * mu * as * as * as) : 0.0);
+  return k;
+}
+
+template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
+
+// Equal-radius overlap coefficients (cuda_functions.cu:336-350, 364-386,
+// 400-406) expressed as multipliers of f, d and (x d).
+template <typename R>
+__device__ __forceinline__ void near_equal(
+    R r2, R fd, R td, const PC<R> &k, R &At, R &Bt, R &Ar, R &Br, R &g) {
+  const R r = dsqrt(r2);
+  const R rm1 = R(1) / (r + R(1e-20));
+  const R b = R(3) * r / (R(32) * k.a);
+  At = (R(1) - R(3) * b) * k.c6a;
+  Bt = ((b * k.c6a * fd) * rm1) * rm1;
+  const R t = r / k.a;
+  const R t3 = t * t * t;
+  Ar = (R(1) - R(27) * t / R(32) + R(5) * t3 / R(64)) * k.c8a3;
+  Br = (((R(9) * t / R(32) - R(3) * t3 / R(64)) * k.c8a3 * td) * rm1) * rm1;
+  g = (r * (R(1) - R(3) * r / (R(8) * k.a)) * k.c16a3) * rm1;
+}
+
+// Unequal-radius overlap coefficients (cuda_functions.cu:622-646, 656-664).
+template <typename R>
+__device__ __forceinline__ void near_unequal(
+    R r2, R fd, const PC<R> &k, R &At, R &Bt, R &g) {
+  const R r = dsqrt(r2);
+  const R rm1 = R(1) / (r + R(1e-20));
+  if (r > k.dlt) {
+    const R r3x32 = R(32) * r * r * r;
+    const R p = k.dlt * k.dlt + R(3) * r * r;
+    const R q = k.dlt * k.dlt - r * r;
+    At = (R(0.5) * (k.as + k.ab) - p * p / r3x32) * k.c6ab;
+    Bt = (((R(3) * q * q / r3x32) * k.c6ab * fd) * rm1) * rm1;
+    const R s = k.as - k.ab + r;
+    const R h = k.as - r;
+    g = (s * s * rm1 * rm1 *
+         (k.ab * k.ab + R(2) * k.ab * (r + k.as) - R(3) * h * h) * k.c128) *
+        rm1;
+  } else {
+    At = k.c6max;
+    Bt = R(0);
+    g = (r * k.c8as3) * rm1;
+  }
+}
+
+template <typename R> struct Acc {
+  R vx, vy, vz, wx, wy, wz;
+  __device__ void zero() { vx = vy = vz = wx = wy = wz = R(0); }
+};
+
+// ---- per-pair kernels ------------------------------------------------------
+
+// ss: 6x6 equal radius.
+template <typename R>
+__device__ __forceinline__ void pair_ss(R dx, R dy, R dz, R fx, R fy, R fz,
+                                        R tx, R ty, R tz, const PC<R> &k,
+                                        Acc<R> &o) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R fd = fx * dx + fy * dy + fz * dz;
+  const R td = tx * dx + ty * dy + tz * dz;
+  R At, Bt, Ar, Br, g;
+  if (__builtin_expect(r2 > k.far2, 1)) {
+    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+    At = ri * (k.c8 + k.c8s3 * ri2);
+    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+    Ar = -k.c16 * ri3;
+    Br = k.c16x3 * ri3 * ri2 * td;
+    g = k.c8 * ri3;
+  } else {
+    near_equal(r2, fd, td, k, At, Bt, Ar, Br, g);
+  }
+  o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
+  o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
+  o.vz += At * fz + Bt * dz + g * (tx * dy - ty * dx);
+  o.wx += Ar * tx + Br * dx + g * (fy * dz - fz * dy);
+  o.wy += Ar * ty + Br * dy + g * (fz * dx - fx * dz);
+  o.wz += Ar * tz + Br * dz + g * (fx * dy - fy * dx);
+}
+
+// sb: segment (v, w) <- blob f, unequal radii.
+template <typename R>
+__device__ __forceinline__ void pair_sb(R dx, R dy, R dz, R fx, R fy, R fz,
+                                        const PC<R> &k, Acc<R> &o) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R fd = fx * dx + fy * dy + fz * dz;
+  R At, Bt, g;
+  if (__builtin_expect(r2 > k.far2, 1)) {
+    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+    At = ri * (k.c8 + k.c8s3 * ri2);
+    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+    g = k.c8 * ri3;
+  } else {
+    near_unequal(r2, fd, k, At, Bt, g);
+  }
+  o.vx += At * fx + Bt * dx;
+  o.vy += At * fy + Bt * dy;
+  o.vz += At * fz + Bt * dz;
+  o.wx += g * (fy * dz - fz * dy);
+  o.wy += g * (fz * dx - fx * dz);
+  o.wz += g * (fx * dy - fy * dx);
+}
+
+// bs: blob v <- segment (f, tau), unequal radii.
+template <typename R>
+__device__ __forceinline__ void pair_bs(R dx, R dy, R dz, R fx, R fy, R fz,
+                                        R tx, R ty, R tz, const PC<R> &k,
+                                        Acc<R> &o) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R fd = fx * dx + fy * dy + fz * dz;
+  R At, Bt, g;
+  if (__builtin_expect(r2 > k.far2, 1)) {
+    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+    At = ri * (k.c8 + k.c8s3 * ri2);
+    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+    g = k.c8 * ri3;
+  } else {
+    near_unequal(r2, fd, k, At, Bt, g);
+  }
+  o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
+  o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
+  o.vz += At * fz + Bt * dz + g * (tx * dy - ty * dx);
+}
+
+// bb: blob v <- blob f, equal radius.
+template <typename R>
+__device__ __forceinline__ void pair_bb(R dx, R dy, R dz, R fx, R fy, R fz,
+                                        const PC<R> &k, Acc<R> &o) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R fd = fx * dx + fy * dy + fz * dz;
+  R At, Bt;
+  if (__builtin_expect(r2 > k.far2, 1)) {
+    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+    At = ri * (k.c8 + k.c8s3 * ri2);
+    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  } else {
+    R Ar, Br, g;
+    near_equal(r2, fd, R(0), k, At, Bt, Ar, Br, g);
+  }
+  o.vx += At * fx + Bt * dx;
+  o.vy += At * fy + Bt * dy;
+  o.vz += At * fz + Bt * dz;
+}
+
+// ---- precision traits ----------------------------------------------------
+// Position of one particle in the kernel's arithmetic: FP64 keeps the
+// double; FP32 keeps hi = float(x) and lo = float(x - hi), and forms
+// separations as (hi_i - hi_j) + (lo_i - lo_j).
+template <typename R> struct Pos;
+template <> struct Pos<double> {
+  double x, y, z;
+  __device__ void load(double px, double py, double pz) { x = px; y = py; z = pz; }
+};
+template <> struct Pos<float> {
+  float x, y, z, lx, ly, lz;
+  __device__ void load(double px, double py, double pz) {
+    x = float(px); y = float(py); z = float(pz);
+    lx = float(px - double(x)); ly = float(py - double(y)); lz = float(pz - double(z));
+  }
+};
+
+template <typename R> struct Lds {
+  R x[kTile], y[kTile], z[kTile];
+  R fx[kTile], fy[kTile], fz[kTile];
+  R tx[kTile], ty[kTile], tz[kTile];
+  float lx[kTile], ly[kTile], lz[kTile];   // used by FP32 only
+};
+
+template <typename R>
+__device__ __forceinline__ void sep(const Pos<R> &p, const Lds<R> &s, int j,
+                                    R &dx, R &dy, R &dz);
+template <>
+__device__ __forceinline__ void sep<double>(const Pos<double> &p,
+                                            const Lds<double> &s, int j,
+                                            double &dx, double &dy, double &dz) {
+  dx = p.x - s.x[j]; dy = p.y - s.y[j]; dz = p.z - s.z[j];
+}
+template <>
+__device__ __forceinline__ void sep<float>(const Pos<float> &p,
+                                           const Lds<float> &s, int j,
+                                           float &dx, float &dy, float &dz) {
+  dx = (p.x - s.x[j]) + (p.lx - s.lx[j]);
+  dy = (p.y - s.y[j]) + (p.ly - s.ly[j]);
+  dz = (p.z - s.z[j]) + (p.lz - s.lz[j]);
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_pos(Lds<R> &s, int t, double px,
+                                          double py, double pz);
+template <>
+__device__ __forceinline__ void stage_pos<double>(Lds<double> &s, int t,
+                                                  double px, double py,
+                                                  double pz) {
+  s.x[t] = px; s.y[t] = py; s.z[t] = pz;
+}
+template <>
+__device__ __forceinline__ void stage_pos<float>(Lds<float> &s, int t,
+                                                 double px, double py,
+                                                 double pz) {
+  const float hx = float(px), hy = float(py), hz = float(pz);
+  s.x[t] = hx; s.y[t] = hy; s.z[t] = hz;
+  s.lx[t] = float(px - double(hx));
+  s.ly[t] = float(py - double(hy));
+  s.lz[t] = float(pz - double(hz));
+}
+
+// Running sum of one target: FP64 plain, FP32 Kahan-compensated per tile.
+template <typename R> struct Sum;
+template <> struct Sum<double> {
+  Acc<double> s;
+  __device__ void zero() { s.zero(); }
+  __device__ void add(const Acc<double> &t) {
+    s.vx += t.vx; s.vy += t.vy; s.vz += t.vz;
+    s.wx += t.wx; s.wy += t.wy; s.wz += t.wz;
+  }
+  __device__ void get(double *o) const {
+    o[0] = s.vx; o[1] = s.vy; o[2] = s.vz; o[3] = s.wx; o[4] = s.wy; o[5] = s.wz;
+  }
+};
+__device__ __forceinline__ void kahan(float &sum, float &c, float x) {
+  const float y = x - c;
+  const float t = sum + y;
+  c = (t - sum) - y;
+  sum = t;
+}
+template <> struct Sum<float> {
+  Acc<float> s, c;
+  __device__ void zero() { s.zero(); c.zero(); }
+  __device__ void add(const Acc<float> &t) {
+    kahan(s.vx, c.vx, t.vx); kahan(s.vy, c.vy, t.vy); kahan(s.vz, c.vz, t.vz);
+    kahan(s.wx, c.wx, t.wx); kahan(s.wy, c.wy, t.wy); kahan(s.wz, c.wz, t.wz);
+  }
+  __device__ void get(double *o) const {
+    o[0] = double(s.vx) - double(c.vx); o[1] = double(s.vy) - double(c.vy);
+    o[2] = double(s.vz) - double(c.vz); o[3] = double(s.wx) - double(c.wx);
+    o[4] = double(s.wy) - double(c.wy); o[5] = double(s.wz) - double(c.wz);
+  }
+};
+
+enum Kind { SS, SB, BS, BB };
+
+// Walk sources [j0, j1) of one species in LDS tiles and accumulate into sum.
+template <typename R, int K>
+__device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
+                                      const double *__restrict__ X,
+                                      const double *__restrict__ F, int ld,
+                                      int j0, int j1, const PC<R> &k,
+                                      Sum<R> &sum) {
+  constexpr bool six = (K == SS || K == BS);  // segment sources carry tau
+  const int tid = threadIdx.x;
+  for (int base = j0; base < j1; base += kTile) {
+    const int n = min(kTile, j1 - base);
+    __syncthreads();
+    if (tid < n) {
+      const int j = base + tid;
+      stage_pos<R>(lds, tid, X[j], X[ld + j], X[2 * ld + j]);
+      lds.fx[tid] = R(F[j]);
+      lds.fy[tid] = R(F[ld + j]);
+      lds.fz[tid] = R(F[2 * ld + j]);
+      if (six) {
+        lds.tx[tid] = R(F[3 * ld + j]);
+        lds.ty[tid] = R(F[4 * ld + j]);
+        lds.tz[tid] = R(F[5 * ld + j]);
+      }
+    }
+    __syncthreads();
+    Acc<R> acc;
+    acc.zero();
+#pragma unroll 2
+    for (int j = 0; j < n; ++j) {
+      R dx, dy, dz;
+      sep<R>(p, lds, j, dx, dy, dz);
+      if (K == SS)
+        pair_ss<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], lds.tx[j],
+                   lds.ty[j], lds.tz[j], k, acc);
+      else if (K == SB)
+        pair_sb<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
+      else if (K == BS)
+        pair_bs<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], lds.tx[j],
+                   lds.ty[j], lds.tz[j], k, acc);
+      else
+        pair_bb<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
+    }
+    sum.add(acc);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
+  __shared__ Lds<R> lds;
+  const WorkItem w = a.work[blockIdx.x];
+  const TileDesc T = a.tiles[w.tile];
+  const int tid = threadIdx.x;
+  const bool seg_t = (T.species == 0);
+  const bool use_s = seg_t ? (a.mask & kSS) : (a.mask & kBS);
+  const bool use_b = seg_t ? (a.mask & kSB) : (a.mask & kBB);
+  const int ns = use_s ? T.nsseg : 0;
+  const int nb = use_b ? T.nsblob : 0;
+  const long L = long(ns) + nb;
+  const int beg = int(L * w.split / T.nsplit);
+  const int end = int(L * (w.split + 1) / T.nsplit);
+
+  // Lanes past the end of the tile mirror the tile's first target.
+  const int ti = T.t0 + (tid < T.nt ? tid : 0);
+  Pos<R> p;
+  if (seg_t)
+    p.load(a.Xs[ti], a.Xs[a.lds + ti], a.Xs[2 * a.lds + ti]);
+  else
+    p.load(a.Xb[ti], a.Xb[a.ldb + ti], a.Xb[2 * a.ldb + ti]);
+
+  Sum<R> sum;
+  sum.zero();
+  const int s0 = beg, s1 = min(end, ns);
+  const int b0 = max(beg, ns) - ns, b1 = end - ns;
+  if (seg_t) {
+    if (s1 > s0) {
+      const PC<R> k = make_pc<R>(a.a_s, a.a_s, a.mu);
+      sweep<R, SS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+    }
+    if (b1 > b0) {
+      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
+      sweep<R, SB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+    }
+  } else {
+    if (s1 > s0) {
+      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
+      sweep<R, BS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+    }
+    if (b1 > b0) {
+      const PC<R> k = make_pc<R>(a.a_b, a.a_b, a.mu);
+      sweep<R, BB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+    }
+  }
+  if (tid >= T.nt) return;
+  double o[6];
+  sum.get(o);
+  if (T.nsplit == 1) {
+    if (seg_t) {
+      double *V = a.Vs + T.t0 + tid;
+      for (int c = 0; c < 6; ++c)
+        V[c * a.lds] = a.accumulate ? V[c * a.lds] + o[c] : o[c];
+    } else {
+      double *V = a.Vb + T.t0 + tid;
+      for (int c = 0; c < 3; ++c)
+        V[c * a.ldb] = a.accumulate ? V[c * a.ldb] + o[c] : o[c];
+    }
+  } else {
+    double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
+    const int nc = seg_t ? 6 : 3;
+    for (int c = 0; c < nc; ++c) P[c * kBlock] = o[c];
+  }
+}
+
+// Deterministic reduction of split partials: fixed split order.
+__global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
+  const TileDesc T = a.tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (T.nsplit == 1 || tid >= T.nt) return;
+  const bool seg_t = (T.species == 0);
+  const int nc = seg_t ? 6 : 3;
+  double *V = seg_t ? a.Vs : a.Vb;
+  const int ld = seg_t ? a.lds : a.ldb;
+  for (int c = 0; c < nc; ++c) {
+    double s = 0.0;
+    for (int q = 0; q < T.nsplit; ++q)
+      s += a.part[size_t(T.part0 + q) * 6 * kBlock + c * kBlock + tid];
+    double *v = V + c * ld + T.t0 + tid;
+    *v = a.accumulate ? *v + s : s;
+  }
+}
+
+// Steric barrier (cuda_functions.cu:961-1157).  One lane per target; the
+// sources of the target's group are read straight from global memory (all
+// lanes of a wave read the same address, served from cache).
+__global__ __launch_bounds__(kBlock) void k_barrier(
+    double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
+    const double *__restrict__ Xs, const double *__restrict__ Xb,
+    const TileDesc *__restrict__ tiles, int nseg_fil, double as, double ab,
+    double K) {
+  const TileDesc T = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (tid >= T.nt) return;
+  const int i = T.t0 + tid;
+  const bool seg_t = (T.species == 0);
+  const double *X = seg_t ? Xs : Xb;
+  const int ld = seg_t ? lds : ldb;
+  const double xi = X[i], yi = X[ld + i], zi = X[2 * ld + i];
+  double fx = 0, fy = 0, fz = 0;
+  auto contrib = [&](double dx, double dy, double dz, double asum) {
+    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < 1.1 * asum) {
+      double fac = fmin(1.0, 1.0 - (10.0 / asum) * (dist - asum));
+      fac = K * fac * fac * fac * fac;
+      const double dm1 = 1.0 / dist;
+      fx += fac * dx * dm1; fy += fac * dy * dm1; fz += fac * dz * dm1;
+    }
+  };
+  const int fili = i / nseg_fil;
+  for (int j = T.sseg0; j < T.sseg0 + T.nsseg; ++j) {
+    if (seg_t && fili == j / nseg_fil && abs(i - j) <= 1) continue;
+    contrib(xi - Xs[j], yi - Xs[lds + j], zi - Xs[2 * lds + j],
+            seg_t ? 2.0 * as : as + ab);
+  }
+  for (int j = T.sblob0; j < T.sblob0 + T.nsblob; ++j) {
+    if (!seg_t && j == i) continue;
+    contrib(xi - Xb[j], yi - Xb[ldb + j], zi - Xb[2 * ldb + j],
+            seg_t ? as + ab : 2.0 * ab);
+  }
+  if (seg_t) {
+    Fs[i] += fx; Fs[lds + i] += fy; Fs[2 * lds + i] += fz;
+  } else {
+    Frep[i] = fx; Frep[ldb + i] = fy; Frep[2 * ldb + i] = fz;
+  }
+}
+
+__global__ void k_aos_to_soa(double *__restrict__ dst, int ld,
+                             const double *__restrict__ src, int n, int nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < nc; ++c) dst[c * ld + i] = src[size_t(nc) * i + c];
+}
+
+__global__ void k_soa_to_aos(double *__restrict__ dst,
+                             const double *__restrict__ src, int ld, int n,
+                             int nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < nc; ++c) dst[size_t(nc) * i + c] = src[c * ld + i];
+}
+
+}  // namespace
+
+hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce,
+                       bool fp32, hipStream_t s, hipEvent_t ev_mid) {
+  if (nwork <= 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(k_pair<float>, dim3(nwork), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_pair<double>, dim3(nwork), dim3(kBlock), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+  if (ntiles_reduce > 0) {
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb,
+                          const double *Xs, const double *Xb,
+                          const TileDesc *tiles, int ntiles, int nseg_fil,
+                          double as, double ab, double K, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_barrier, dim3(ntiles), dim3(kBlock), 0, s, Fs, lds,
+                     Frep, ldb, Xs, Xb, tiles, nseg_fil, as, ab, K);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(double *dst, int ld, const double *src, int n,
+                             int nc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((n + 255) / 256), dim3(256), 0, s,
+                     dst, ld, src, n, nc);
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(double *dst, const double *src, int ld, int n,
+                             int nc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((n + 255) / 256), dim3(256), 0, s,
+                     dst, src, ld, n, nc);
+  return hipGetLastError();
+}
+
+}  // namespace gmi

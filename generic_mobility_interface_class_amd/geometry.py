@@ -1,0 +1,206 @@
+"""Deterministic synthetic swimmer geometry and inputs.
+
+The reference seeds its sphere surfaces by a spiral start plus repulsive
+relaxation (seeding.cu:13-389) and draws filament perturbations from
+std::random_device (filament.cpp:345-360).  Neither is reproducible, so the
+workloads here use closed-form geometry instead:
+
+* blob centres: vertices of a geodesic icosphere of frequency f, which has
+  10 f^2 + 2 vertices (642 -> f=8, 2562 -> 16, 4002 -> 20, 10242 -> 32,
+  16002 -> 40);
+* filament bases: Fibonacci-sphere points at radius
+  R + 1.05 (DL/2 + a_blob), the offset the reference uses between the blob
+  shell and the filament base sphere (swimmer.cpp:171 vs rigid_body.cpp:186);
+* filaments: straight, normal to the surface, segment k at base + k DL n
+  (filament.cpp:297-307);
+* random numbers: splitmix64-seeded xoshiro256**, normals by Box-Muller,
+  so every platform produces the same bits.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64(state: int):
+    state = (state + 0x9E3779B97F4A7C15) & _M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return state, z ^ (z >> 31)
+
+
+class Xoshiro256ss:
+    """xoshiro256** (Blackman & Vigna 2018), seeded through splitmix64."""
+
+    def __init__(self, seed: int):
+        st = seed & _M64
+        s = []
+        for _ in range(4):
+            st, z = _splitmix64(st)
+            s.append(z)
+        self.s = s
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & _M64
+
+    def next_u64(self) -> int:
+        s = self.s
+        result = (self._rotl((s[1] * 5) & _M64, 7) * 9) & _M64
+        t = (s[1] << 17) & _M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return result
+
+    def uniform(self, n: int) -> np.ndarray:
+        """n doubles in [0, 1) from the top 53 bits."""
+        out = np.empty(n, dtype=np.float64)
+        for i in range(n):
+            out[i] = (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+        return out
+
+    def normal(self, n: int) -> np.ndarray:
+        """n N(0,1) doubles by Box-Muller (pairs: cos branch, then sin)."""
+        m = (n + 1) // 2
+        u = self.uniform(2 * m)
+        u1 = 1.0 - u[0::2]          # (0, 1]
+        u2 = u[1::2]
+        rad = np.sqrt(-2.0 * np.log(u1))
+        z = np.empty(2 * m)
+        z[0::2] = rad * np.cos(2.0 * math.pi * u2)
+        z[1::2] = rad * np.sin(2.0 * math.pi * u2)
+        return z[:n]
+
+
+def icosahedron():
+    p = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0],
+                  [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
+                  [p, 0, -1], [p, 0, 1], [-p, 0, -1], [-p, 0, 1]], float)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+         (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+         (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+         (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    return v, f
+
+
+def geodesic_sphere(freq: int, radius: float = 1.0) -> np.ndarray:
+    """Vertices of the frequency-`freq` geodesic icosphere (10 f^2 + 2 points).
+
+    Each icosahedron face is split into f^2 triangles on a barycentric grid;
+    the grid points are projected to the sphere.  Shared edge and corner
+    points are emitted once, in first-visit order, so the order is fixed.
+    """
+    if freq < 1:
+        raise ValueError("freq must be >= 1")
+    v, faces = icosahedron()
+    pts = []
+    seen = {}
+    for (a, b, c) in faces:
+        A, B, C = v[a], v[b], v[c]
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                k = freq - i - j
+                # exact integer key of the barycentric point on the shared
+                # edge/corner: sort contributions by vertex id
+                key = tuple(sorted(((a, i), (b, j), (c, k)), key=lambda t: t[0]))
+                key = tuple(t for t in key if t[1] != 0)
+                if key in seen:
+                    continue
+                p = (i * A + j * B + k * C) / freq
+                p = p / np.linalg.norm(p)
+                seen[key] = len(pts)
+                pts.append(p)
+    out = np.array(pts) * radius
+    assert out.shape[0] == 10 * freq * freq + 2
+    return out
+
+
+def geodesic_freq_for(nblob: int) -> int:
+    f = int(round(math.sqrt((nblob - 2) / 10.0)))
+    if 10 * f * f + 2 != nblob:
+        raise ValueError(f"{nblob} is not a geodesic count 10 f^2 + 2")
+    return f
+
+
+def fibonacci_sphere(n: int) -> np.ndarray:
+    """n near-uniform unit vectors (golden-angle spiral)."""
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - (2.0 * i + 1.0) / n
+    rho = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([rho * np.cos(phi), rho * np.sin(phi), z], axis=1)
+
+
+@dataclass
+class SwimmerGeometry:
+    nfil: int
+    nseg: int
+    nblob: int
+    R: float                 # blob-shell radius
+    a_seg: float = 1.0
+    a_blob: float = 1.0
+    mu: float = 1.0
+    centre: np.ndarray = field(default_factory=lambda: np.zeros(3))
+
+    @property
+    def DL(self) -> float:   # config.hpp:204
+        return 2.2 * self.a_seg
+
+    @property
+    def ns(self) -> int:
+        return self.nfil * self.nseg
+
+    def blob_references(self) -> np.ndarray:
+        return geodesic_sphere(geodesic_freq_for(self.nblob), self.R)
+
+    def filament_directions(self) -> np.ndarray:
+        return fibonacci_sphere(self.nfil)
+
+    def base_radius(self) -> float:
+        return self.R + 1.05 * (0.5 * self.DL + self.a_blob)
+
+    def positions(self):
+        """(x_segs [Ns,3], x_blobs [Nb,3]) of the straight reference state."""
+        dirs = self.filament_directions()
+        base = self.centre + dirs * self.base_radius()
+        k = np.arange(self.nseg, dtype=np.float64)
+        xs = base[:, None, :] + (k[None, :, None] * self.DL) * dirs[:, None, :]
+        xb = self.centre + self.blob_references()
+        return xs.reshape(-1, 3).copy(), xb
+
+
+def random_forces(ns: int, nb: int, seed: int, blob_torques: bool = False):
+    """F_s [Ns,6] and F_b [Nb,3] (or [Nb,6]) i.i.d. N(0,1).
+
+    Draw order: all segment (f, tau) rows, then all blob rows.
+    """
+    rng = Xoshiro256ss(seed)
+    fs = rng.normal(6 * ns).reshape(ns, 6)
+    fb = rng.normal((6 if blob_torques else 3) * nb).reshape(nb, -1)
+    return fs, fb
+
+
+# Workload definitions (sizes of the synthetic configurations)
+WORKLOADS = {
+    # one swimmer, 962 particles, CPU-runnable apply
+    "apply_962": dict(nfil=16, nseg=20, nblob=642, R=10.0),
+    # implicit-timestep swimmer: 5120 segments, 4002 blobs
+    "step_9122": dict(nfil=256, nseg=20, nblob=4002, R=25.0),
+    # large single apply: 30720 segments + 16002 blobs = 46722 particles
+    "apply_46722": dict(nfil=1024, nseg=30, nblob=16002, R=40.0),
+}
+
+
+def workload_geometry(name: str) -> SwimmerGeometry:
+    return SwimmerGeometry(**WORKLOADS[name])

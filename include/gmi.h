@@ -1,0 +1,131 @@
+/* gmi.h -- C ABI of the MI355X (gfx950) RPY grand-mobility library.
+ *
+ * Drop-in boundary for the reference's mobility backend.  The reference
+ * exposes the abstract C++ class `mobility_solver` (mobility_solver.hpp:18-102)
+ * whose pure virtuals (mobility_solver.hpp:25-44) are implemented by
+ * `rpy_mobility_solver` (rpy_mobility_solver.cu:14-311).  Each entry point
+ * below replaces one of those members; the comment on each names it.
+ *
+ * Conventions
+ *  - Plain C types only; every call returns GMI_OK (0) or a negative
+ *    GMI_ERR_* code, and gmi_last_error() gives the message.  No exceptions
+ *    cross this boundary.
+ *  - Host arrays are caller-owned FP64 AoS in the reference layout
+ *    (mobility_solver.hpp:54-60): x_segs[3 Ns], f_segs[6 Ns] = (f, tau),
+ *    v_segs[6 Ns] = (v, omega), x_blobs/f_blobs/v_blobs[3 Nb]; index order
+ *    swimmer-major, then filament, then segment (blobs: swimmer, blob).
+ *  - Device buffers are owned by the context.  Work is queued on the
+ *    context's stream; copies to host complete at gmi_wait_for_device(),
+ *    which is the only synchronisation point (as wait_for_device(),
+ *    rpy_mobility_solver.cu:230-239).
+ *  - One context per device per host thread.
+ */
+#ifndef GMI_H
+#define GMI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMI_OK 0
+#define GMI_ERR_ARG (-1)        /* bad argument / size                        */
+#define GMI_ERR_HIP (-2)        /* HIP runtime error                          */
+#define GMI_ERR_STATE (-3)      /* call out of order                          */
+#define GMI_ERR_NOT_FINITE (-4) /* NaN/Inf in a result that must be finite    */
+#define GMI_ERR_SINGULAR (-5)   /* singular matrix (matrix.cpp:549-553)       */
+
+/* Arithmetic of the pair kernels. */
+#define GMI_FP64 0       /* FP64 throughout (parity path)                    */
+#define GMI_FP32_KAHAN 1 /* FP32 pairs, Kahan cross-tile sums, FP64 reduce    */
+
+typedef struct gmi_ctx gmi_ctx;
+
+/* Sizes and physics of one context (replaces the compile-time NSWIM, NFIL,
+ * NSEG, NBLOB, MU, RSEG, RBLOB of config.hpp:95-103). */
+typedef struct gmi_params {
+  int n_swimmers;               /* NSWIM                                      */
+  const int *nseg_per_swimmer;  /* NFIL*NSEG of each swimmer [n_swimmers]     */
+  const int *nblob_per_swimmer; /* NBLOB of each swimmer [n_swimmers]         */
+  int nseg_per_filament;        /* NSEG (barrier exclusion, filament layout)  */
+  int swimmers_interact;        /* 1: every particle couples to every other
+                                   (the reference's NSWIM semantics,
+                                   cuda_functions.cu:278,457); 0: swimmers
+                                   are hydrodynamically independent          */
+  double mu;                    /* MU                                         */
+  double a_seg;                 /* RSEG                                       */
+  double a_blob;                /* RBLOB                                      */
+  int precision;                /* GMI_FP64 or GMI_FP32_KAHAN                 */
+  int device;                   /* HIP device ordinal                         */
+} gmi_params;
+
+/* Number of visible HIP devices (cudaGetDeviceCount in
+ * allocate_host_memory, rpy_mobility_solver.cu:58). */
+int gmi_device_count(int *count);
+/* allocate_host_memory + allocate_device_memory (rpy_mobility_solver.cu:54-115) */
+int gmi_create(gmi_ctx **ctx, const gmi_params *params);
+/* free_host_memory + free_device_memory (rpy_mobility_solver.cu:14-52) */
+void gmi_destroy(gmi_ctx *ctx);
+const char *gmi_last_error(const gmi_ctx *ctx);
+int gmi_set_precision(gmi_ctx *ctx, int precision);
+int gmi_num_segments(const gmi_ctx *ctx);
+int gmi_num_blobs(const gmi_ctx *ctx);
+
+/* ---- host <-> device copies (rpy_mobility_solver.cu:119-205) ------------ */
+/* copy_segment_positions_to_device (rpy_mobility_solver.cu:119) */
+int gmi_copy_segment_positions_to_device(gmi_ctx *ctx, const double *x_segs);
+/* copy_segment_forces_to_device (rpy_mobility_solver.cu:130) */
+int gmi_copy_segment_forces_to_device(gmi_ctx *ctx, const double *f_segs);
+/* copy_blob_positions_to_device (rpy_mobility_solver.cu:141) */
+int gmi_copy_blob_positions_to_device(gmi_ctx *ctx, const double *x_blobs);
+/* copy_blob_forces_to_device (rpy_mobility_solver.cu:152) */
+int gmi_copy_blob_forces_to_device(gmi_ctx *ctx, const double *f_blobs);
+/* copy_interparticle_blob_forces_to_host (rpy_mobility_solver.cu:163) */
+int gmi_copy_interparticle_blob_forces_to_host(gmi_ctx *ctx, double *f_blobs_rep);
+/* copy_blob_velocities_to_host (rpy_mobility_solver.cu:177) */
+int gmi_copy_blob_velocities_to_host(gmi_ctx *ctx, double *v_blobs);
+/* copy_segment_velocities_to_host (rpy_mobility_solver.cu:191) */
+int gmi_copy_segment_velocities_to_host(gmi_ctx *ctx, double *v_segs);
+
+/* ---- device work (rpy_mobility_solver.cu:207-311) ----------------------- */
+/* apply_interparticle_forces (rpy_mobility_solver.cu:207-228): steric
+ * barrier; adds into the device segment forces and sets the device blob
+ * repulsion.  K = REPULSIVE_FORCE_FACTOR*END_FORCE_MAGNITUDE (config.hpp:239-247). */
+int gmi_apply_interparticle_forces(gmi_ctx *ctx, double K);
+/* wait_for_device (rpy_mobility_solver.cu:230-239) */
+int gmi_wait_for_device(gmi_ctx *ctx);
+/* evaluate_segment_segment_mobility -> Mss_mult: V_s = M_ss F_s (rpy_mobility_solver.cu:241-257) */
+int gmi_evaluate_segment_segment_mobility(gmi_ctx *ctx);
+/* evaluate_segment_blob_mobility -> Msb_mult: V_s += M_sb F_b (rpy_mobility_solver.cu:259-275) */
+int gmi_evaluate_segment_blob_mobility(gmi_ctx *ctx);
+/* evaluate_blob_blob_mobility -> Mbb_mult: V_b = M_bb F_b (rpy_mobility_solver.cu:277-293) */
+int gmi_evaluate_blob_blob_mobility(gmi_ctx *ctx);
+/* evaluate_blob_segment_mobility -> Mbs_mult: V_b = M_bs F_s (rpy_mobility_solver.cu:295-311) */
+int gmi_evaluate_blob_segment_mobility(gmi_ctx *ctx);
+/* All four blocks in one launch: V_s = M_ss F_s + M_sb F_b,
+ * V_b = M_bs F_s + M_bb F_b.  No single reference counterpart; equals the
+ * sum of the four evaluate_* calls above. */
+int gmi_evaluate_grand_mobility(gmi_ctx *ctx);
+
+/* ---- timing (no reference counterpart; the reference has no timers) ----- */
+/* Records a HIP event on the context's stream. */
+int gmi_timer_start(gmi_ctx *ctx);
+/* Records the stop event, waits for it, returns elapsed milliseconds. */
+int gmi_timer_stop(gmi_ctx *ctx, double *ms);
+
+/* Starts recording HIP events around every pair-kernel launch (the
+ * reduction of split partials excluded). */
+int gmi_profile_begin(gmi_ctx *ctx);
+/* Waits for the stream, returns the summed pair-kernel time in ms and the
+ * number of launches since gmi_profile_begin(), and stops recording. */
+int gmi_profile_end(gmi_ctx *ctx, double *pair_ms, int *launches);
+
+/* ---- launch shaping (no reference counterpart; the reference launches
+ * ceil(N/64) blocks of 64, rpy_mobility_solver.cu:249-305) --------------- */
+/* Pair launches aim at `target_blocks` workgroups by splitting each target
+ * tile's sources into chunks of at least `min_chunk` sources. */
+int gmi_set_launch_shape(gmi_ctx *ctx, int target_blocks, int min_chunk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMI_H */
