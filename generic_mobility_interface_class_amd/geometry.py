@@ -204,3 +204,80 @@ WORKLOADS = {
 
 def workload_geometry(name: str) -> SwimmerGeometry:
     return SwimmerGeometry(**WORKLOADS[name])
+
+
+@dataclass
+class SimConfig:
+    """Inputs of an implicit-timestep run (one or more independent swimmers).
+
+    Physics follows the reference defaults (config.hpp:100-154) where the
+    workload does not fix them: KB = KT = 1800, Broyden TOL 1e-4 with
+    JACOBIAN_CONFIDENCE_FACTOR 0.1 and at most 100 iterations, GMRES at most
+    150 iterations, one backward-Euler step before BDF2.  The driving is a
+    preferred curvature amp*sin(2 pi t/period + phase_f) with amp = 0.5/L,
+    period 1 and dt = period/300; the barrier constant is 0.1*KB (the
+    reference's value when no end force defines it, config.hpp:244-247).
+    """
+    swimmers: list            # list of SwimmerGeometry
+    phases: np.ndarray        # one per filament, all swimmers
+    dt: float = 1.0 / 300.0
+    kb: float = 1800.0
+    kt: float = 1800.0
+    tol: float = 1e-4
+    jacobian_confidence: float = 0.1
+    gmres_tol: float = 1e-8
+    max_broyden_iter: int = 100
+    gmres_max_iter: int = 150
+    num_euler_steps: int = 1
+    barrier_factor: float = 0.1       # K = barrier_factor * kb
+    curvature_period: float = 1.0
+    curvature_amp: float = None       # default 0.5 / L
+
+    def __post_init__(self):
+        g0 = self.swimmers[0]
+        if self.curvature_amp is None:
+            self.curvature_amp = 0.5 / (g0.nseg * g0.DL)
+        for g in self.swimmers:
+            if (g.nseg, g.a_seg, g.a_blob, g.mu) != (g0.nseg, g0.a_seg, g0.a_blob, g0.mu):
+                raise ValueError("swimmers must share NSEG, radii and viscosity")
+
+    @property
+    def nseg(self):
+        return self.swimmers[0].nseg
+
+    @property
+    def barrier_K(self):
+        return self.barrier_factor * self.kb
+
+    def arrays(self):
+        """Flat arrays shared by the library and the oracle."""
+        nfil = np.array([g.nfil for g in self.swimmers], np.int32)
+        nblob = np.array([g.nblob for g in self.swimmers], np.int32)
+        blob_refs = np.concatenate([g.blob_references().reshape(-1) for g in self.swimmers])
+        fil_refs = np.concatenate([(g.filament_directions() * g.base_radius()).reshape(-1)
+                                   for g in self.swimmers])
+        body_x = np.concatenate([np.asarray(g.centre, float) for g in self.swimmers])
+        radius = np.array([g.R for g in self.swimmers])
+        return dict(nfil=nfil, nblob=nblob, blob_refs=np.ascontiguousarray(blob_refs),
+                    fil_refs=np.ascontiguousarray(fil_refs), body_x=body_x,
+                    phases=np.ascontiguousarray(self.phases, dtype=np.float64), radius=radius)
+
+
+def sim_config(nfil, nseg, nblob, R, seed=0, **kw):
+    """Single swimmer with random driving phases U[0, 2 pi) from `seed`."""
+    g = SwimmerGeometry(nfil=nfil, nseg=nseg, nblob=nblob, R=R)
+    rng = Xoshiro256ss(seed)
+    phases = 2.0 * math.pi * rng.uniform(nfil)
+    return SimConfig(swimmers=[g], phases=phases, **kw)
+
+
+SIM_WORKLOADS = {
+    # 16 x 20 segments + 642 blobs (the 962-particle swimmer)
+    "step_962": dict(nfil=16, nseg=20, nblob=642, R=10.0),
+    # 256 x 20 segments + 4002 blobs
+    "step_9122": dict(nfil=256, nseg=20, nblob=4002, R=25.0),
+}
+
+
+def sim_workload(name, seed=0, **kw):
+    return sim_config(seed=seed, **SIM_WORKLOADS[name], **kw)

@@ -109,3 +109,117 @@ def ss_block(xi, xj, a, mu=1.0):
     load().orc_ss_block(_p(out), _p(np.ascontiguousarray(xi, float)),
                         _p(np.ascontiguousarray(xj, float)), a, mu)
     return out.reshape(6, 6)
+
+
+# ---------------------------------------------------------------- time step
+
+class OrcSimParams(C.Structure):
+    _fields_ = [
+        ("n_swimmers", C.c_int), ("nfil", C.POINTER(C.c_int)), ("nblob", C.POINTER(C.c_int)),
+        ("nseg", C.c_int), ("blob_refs", _dp), ("fil_refs", _dp), ("body_x", _dp),
+        ("fil_phase", _dp), ("mu", C.c_double), ("a_seg", C.c_double), ("a_blob", C.c_double),
+        ("kb", C.c_double), ("kt", C.c_double), ("dt", C.c_double), ("tol", C.c_double),
+        ("jacobian_confidence", C.c_double), ("gmres_tol", C.c_double),
+        ("max_broyden_iter", C.c_int), ("gmres_max_iter", C.c_int),
+        ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
+        ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
+        ("sphere_radius", C.c_double),
+    ]
+
+
+def _sim_lib():
+    lib = load()
+    if not getattr(lib, "_sim_ready", False):
+        vp = C.c_void_p
+        lib.orc_sim_create.restype = vp
+        lib.orc_sim_create.argtypes = [C.POINTER(OrcSimParams), C.c_int]
+        lib.orc_sim_destroy.argtypes = [vp]
+        lib.orc_sim_destroy.restype = None
+        lib.orc_sim_make_body_reference_matrices.argtypes = [vp]
+        lib.orc_sim_step.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        for n in ("orc_sim_num_segments", "orc_sim_num_blobs", "orc_sim_nbroy"):
+            getattr(lib, n).argtypes = [vp]
+        lib.orc_sim_get_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_sim_get_state.restype = None
+        lib.orc_sim_get_mobility.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_sim_get_mobility.restype = None
+        lib.orc_sim_get_broyden.argtypes = [vp, _dp]
+        lib.orc_sim_get_broyden.restype = None
+        lib.orc_sim_test_compute_velocities.argtypes = [vp]
+        lib.orc_sim_test_forces.argtypes = [vp]
+        lib.orc_sim_test_forces.restype = None
+        lib.orc_sim_test_prepare_jacobian.argtypes = [vp, C.c_int]
+        lib.orc_sim_test_filament_jacobian.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        lib.orc_sim_test_jacobian_inv_mult.argtypes = [vp, _dp, _dp]
+        lib.orc_sim_test_jacobian_inv_mult.restype = None
+        lib.orc_sim_test_set_driving.argtypes = [vp, C.c_int]
+        lib.orc_sim_test_set_driving.restype = None
+        lib._sim_ready = True
+    return lib
+
+
+class Sim:
+    """Oracle time stepper for a geometry.SimConfig."""
+
+    def __init__(self, cfg, nthreads=None):
+        self.lib = _sim_lib()
+        a = cfg.arrays()
+        self._keep = a
+        g0 = cfg.swimmers[0]
+        self.p = OrcSimParams(
+            len(a["nfil"]), a["nfil"].ctypes.data_as(C.POINTER(C.c_int)),
+            a["nblob"].ctypes.data_as(C.POINTER(C.c_int)), cfg.nseg,
+            _p(a["blob_refs"]), _p(a["fil_refs"]), _p(a["body_x"]), _p(a["phases"]),
+            g0.mu, g0.a_seg, g0.a_blob, cfg.kb, cfg.kt, cfg.dt, cfg.tol,
+            cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
+            cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
+            float(a["radius"][0]))
+        self.h = self.lib.orc_sim_create(C.byref(self.p), nthreads or nthreads_default())
+        self.Ns = self.lib.orc_sim_num_segments(self.h)
+        self.Nb = self.lib.orc_sim_num_blobs(self.h)
+        self.nbroy = self.lib.orc_sim_nbroy(self.h)
+        self.nsw = len(a["nfil"])
+        self.nfil_total = int(a["nfil"].sum())
+        self.nseg = cfg.nseg
+
+    def close(self):
+        if self.h:
+            self.lib.orc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def make_body_reference_matrices(self):
+        rc = self.lib.orc_sim_make_body_reference_matrices(self.h)
+        assert rc == 0, rc
+
+    def step(self, nt):
+        bi, gi = C.c_int(0), C.c_int(0)
+        rc = self.lib.orc_sim_step(self.h, nt, C.byref(bi), C.byref(gi))
+        return rc, bi.value, gi.value
+
+    def state(self):
+        S = self.nseg
+        x = np.zeros(3 * self.Ns); q = np.zeros(4 * self.Ns); u = np.zeros(3 * self.Ns)
+        lam = np.zeros(self.nfil_total * (3 * (S - 1) + 6)); body = np.zeros(10 * self.nsw)
+        self.lib.orc_sim_get_state(self.h, _p(x), _p(q), _p(u), _p(lam), _p(body))
+        return dict(x=x.reshape(-1, 3), q=q.reshape(-1, 4), u=u.reshape(-1, 3),
+                    lam=lam.reshape(self.nfil_total, -1), body=body.reshape(-1, 10))
+
+    def mobility(self):
+        vs = np.zeros(6 * self.Ns); vb = np.zeros(6 * self.nsw); fb = np.zeros(3 * self.Nb)
+        fs = np.zeros(6 * self.Ns); ref = np.zeros(72 * self.nsw)
+        self.lib.orc_sim_get_mobility(self.h, _p(vs), _p(vb), _p(fb), _p(fs), _p(ref))
+        ref = ref.reshape(self.nsw, 2, 6, 6)
+        return dict(v_segs=vs.reshape(-1, 6), v_bodies=vb.reshape(-1, 6),
+                    f_blobs=fb.reshape(-1, 3), f_segs=fs.reshape(-1, 6),
+                    Nref=ref[:, 0], KTKinv=ref[:, 1])
+
+    def broyden_error(self):
+        e = np.zeros(self.nbroy)
+        self.lib.orc_sim_get_broyden(self.h, _p(e))
+        return e
