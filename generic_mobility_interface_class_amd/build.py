@@ -18,6 +18,7 @@ LIB = os.path.join(PKG, "libgmi.so")
 HIP_SOURCES = [
     "gmi_pair.hip",
     "gmi_capi.hip",
+    "gmi_sim.hip",
 ]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would

@@ -70,7 +70,25 @@ SIGNATURES = [
     ("gmi_profile_begin", C.c_int, [C.c_void_p]),
     ("gmi_profile_end", C.c_int, [C.c_void_p, _dp, _ip]),
     ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    # gmi_sim.h
+    ("gmi_sim_create", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gmi_make_body_reference_matrices", C.c_int, [C.c_void_p]),
+    ("gmi_initial_guess", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_forces_and_torques", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_compute_velocities", C.c_int, [C.c_void_p, C.c_int, _ip]),
+    ("gmi_compute_errors", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),
+    ("gmi_prepare_jacobian_inv", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_broyden_find_update", C.c_int, [C.c_void_p]),
+    ("gmi_apply_update", C.c_int, [C.c_void_p]),
+    ("gmi_broyden_end_of_iter", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_step", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
+    ("gmi_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    ("gmi_get_mobility", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    ("gmi_get_broyden", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("gmi_sim_set_precision", C.c_int, [C.c_void_p, C.c_int]),
 ]
+
+GMI_ERR_NOT_CONVERGED = -6
 
 _lib = None
 
@@ -177,3 +195,70 @@ class Context:
 
     def set_precision(self, p):
         self.call("gmi_set_precision", int(p))
+
+
+class GmiSimParams(C.Structure):
+    _fields_ = [
+        ("nfil", _ip), ("blob_refs", _dp), ("fil_refs", _dp), ("body_x", _dp),
+        ("fil_phase", _dp), ("kb", C.c_double), ("kt", C.c_double), ("dt", C.c_double),
+        ("tol", C.c_double), ("jacobian_confidence", C.c_double), ("gmres_tol", C.c_double),
+        ("max_broyden_iter", C.c_int), ("gmres_max_iter", C.c_int),
+        ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
+        ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
+        ("sphere_radius", C.c_double),
+    ]
+
+
+class Sim(Context):
+    """Device-resident time stepper for a geometry.SimConfig (gmi_sim.h)."""
+
+    def __init__(self, cfg, precision=GMI_FP64, device=0):
+        a = cfg.arrays()
+        g0 = cfg.swimmers[0]
+        super().__init__(a["nfil"] * cfg.nseg, a["nblob"], cfg.nseg, mu=g0.mu,
+                         a_seg=g0.a_seg, a_blob=g0.a_blob, precision=precision,
+                         interact=False, device=device)
+        self._a = a
+        self.sp = GmiSimParams(
+            a["nfil"].ctypes.data_as(_ip), ptr(a["blob_refs"]), ptr(a["fil_refs"]),
+            ptr(a["body_x"]), ptr(a["phases"]), cfg.kb, cfg.kt, cfg.dt, cfg.tol,
+            cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
+            cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
+            float(a["radius"][0]))
+        self.call("gmi_sim_create", C.byref(self.sp))
+        self.nsw = len(a["nfil"])
+        self.nfil_total = int(a["nfil"].sum())
+        self.nseg = cfg.nseg
+        self.nbroy = 6 * (self.nfil_total * cfg.nseg + self.nsw)
+
+    def make_body_reference_matrices(self):
+        self.call("gmi_make_body_reference_matrices")
+
+    def step(self, nt):
+        bi, gi = C.c_int(0), C.c_int(0)
+        rc = self.lib.gmi_step(self.h, nt, C.byref(bi), C.byref(gi))
+        return rc, bi.value, gi.value
+
+    def state(self):
+        S = self.nseg
+        x = np.zeros(3 * self.Ns); q = np.zeros(4 * self.Ns); u = np.zeros(3 * self.Ns)
+        lam = np.zeros(self.nfil_total * (3 * S + 3)); body = np.zeros(10 * self.nsw)
+        self.call("gmi_get_state", ptr(x), ptr(q), ptr(u), ptr(lam), ptr(body))
+        return dict(x=x.reshape(-1, 3), q=q.reshape(-1, 4), u=u.reshape(-1, 3),
+                    lam=lam.reshape(self.nfil_total, -1), body=body.reshape(-1, 10))
+
+    def mobility(self):
+        vs = np.zeros(6 * self.Ns); vb = np.zeros(6 * self.nsw); fb = np.zeros(3 * self.Nb)
+        ref = np.zeros(72 * self.nsw)
+        self.call("gmi_get_mobility", ptr(vs), ptr(vb), ptr(fb), ptr(ref))
+        ref = ref.reshape(self.nsw, 2, 6, 6)
+        return dict(v_segs=vs.reshape(-1, 6), v_bodies=vb.reshape(-1, 6),
+                    f_blobs=fb.reshape(-1, 3), Nref=ref[:, 0], KTKinv=ref[:, 1])
+
+    def broyden(self, which=0):
+        e = np.zeros(self.nbroy)
+        self.call("gmi_get_broyden", int(which), ptr(e))
+        return e
+
+    def set_sim_precision(self, p):
+        self.call("gmi_sim_set_precision", int(p))

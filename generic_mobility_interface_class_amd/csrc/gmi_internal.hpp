@@ -43,6 +43,7 @@ struct PairArgs {
   int mask;                 // PairMask bits
   int accumulate;           // 0: V = M F, 1: V += M F
   double a_s, a_b, mu;
+  const int *skip = nullptr; // if non-null and *skip != 0 the launch is a no-op
 };
 
 // Work list of one pair launch (built once per mask).

@@ -375,6 +375,7 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
   __shared__ Lds<R> lds;
+  if (a.skip && *a.skip) return;
   const WorkItem w = a.work[blockIdx.x];
   const TileDesc T = a.tiles[w.tile];
   const int tid = threadIdx.x;
@@ -440,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
 
 // Deterministic reduction of split partials: fixed split order.
 __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
+  if (a.skip && *a.skip) return;
   const TileDesc T = a.tiles[blockIdx.x];
   const int tid = threadIdx.x;
   if (T.nsplit == 1 || tid >= T.nt) return;

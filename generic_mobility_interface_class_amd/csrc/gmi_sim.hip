@@ -1,0 +1,1908 @@
+// gmi_sim.hip -- device-resident implicit time step (include/gmi_sim.h).
+//
+// Everything the reference keeps in host objects (segment/filament/body
+// state, Broyden C and D, the GMRES Krylov basis) lives in HBM here, in
+// structure-of-arrays form; each reference method becomes one or two
+// kernels and the host only reads convergence flags.  Decomposition:
+//   per segment  : residual rows                      (thread per segment)
+//   per filament : initial guess, forces, robot arm,   (thread per filament)
+//                  update; Jacobian assembly, Gauss-   (workgroup per filament)
+//                  Jordan inverse, Schur blocks, J^-1 x
+//   per swimmer  : body state, Schur complement,       (thread / workgroup
+//                  preconditioner and K products        per swimmer)
+//   per vector   : GMRES CGS2 Arnoldi, Givens,          (grid-wide kernels)
+//                  Broyden rank-one updates
+// Numerics follow the reference (file:line on every kernel); GMRES uses
+// classical Gram-Schmidt applied twice instead of modified Gram-Schmidt so
+// that one Arnoldi step is two batched dot-product sweeps rather than k
+// dependent ones (same Krylov basis to rounding).
+#include "gmi_internal.hpp"
+#include "gmi_ctx.hpp"
+#include "gmi_lie.hpp"
+#include "../../include/gmi_sim.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace gmi {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// body record (per swimmer), doubles
+enum {
+  BX = 0, BXM1 = 3, BXM2 = 6, BU = 9, BUM1 = 12, BQ = 15, BQM1 = 19, BQI = 23,
+  BF = 32, BN = 41, BK = 77, BS = 113, BREC = 152
+};
+// device flags
+enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_N = 8 };
+
+struct SimDev {
+  // sizes
+  int S, Nf, Nsw, Ns, Nb, lds, ldb, nbroy, npad, maxit;
+  double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol, R;
+  int neuler;
+  double camp, cper;
+  // index maps
+  const int *sw_fil0, *sw_nfil, *sw_blob0, *sw_nblob, *fil_sw, *blob_sw;
+  // segment SoA
+  double *X, *Xm1, *Xm2, *Q, *Qm1, *U, *Um1, *Fint, *Fs, *Vs;
+  // filament
+  double *fil_ref, *phase, *lam, *lamm1, *lamm2;  // lam: [Nf][3S]
+  double *teth, *clamp;                           // [Nf][9]: value, m1, m2
+  double *fil_bf;                                 // [Nf][6]
+  // body
+  double *body;                                   // [Nsw][BREC]
+  double *blob_ref;                               // SoA [3][ldb]
+  double *Xb, *Vb, *Frep;
+  // Jacobian
+  double *Jw, *Jinv, *ecb, *Bblk, *fil_BA, *fil_b2;
+  int *flags;
+};
+
+__device__ __forceinline__ d3 rowvec(const double *p) { return {p[0], p[1], p[2]}; }
+
+// -------------------------------------------------------------------------
+// small dense helpers
+__device__ void rotate66(const double *N, const m3 &Q, double *out) {
+  const m3 QT = mtrans(Q);
+  for (int bi = 0; bi < 2; bi++)
+    for (int bj = 0; bj < 2; bj++) {
+      m3 B;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) B.a[3 * i + j] = N[6 * (3 * bi + i) + 3 * bj + j];
+      const m3 C = mm(mm(Q, B), QT);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[6 * (3 * bi + i) + 3 * bj + j] = C.a[3 * i + j];
+    }
+}
+
+// Gauss-Jordan with partial pivoting of a small row-major matrix (n <= 6).
+__device__ bool gj_small(double *A, int n) {
+  int piv[6];
+  for (int i = 0; i < n; i++) piv[i] = i;
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double best = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; i++)
+      if (fabs(A[i * n + k]) > best) { best = fabs(A[i * n + k]); p = i; }
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; j++) { const double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t; }
+      const int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+    }
+    const double inv = 1.0 / A[k * n + k];
+    A[k * n + k] = 1.0;
+    for (int j = 0; j < n; j++) A[k * n + j] *= inv;
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      const double f = A[i * n + k];
+      if (f == 0.0) continue;
+      A[i * n + k] = 0.0;
+      for (int j = 0; j < n; j++) A[i * n + j] -= f * A[k * n + j];
+    }
+  }
+  double B[36];
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) B[i * n + piv[j]] = A[i * n + j];
+  for (int k = 0; k < n * n; k++) A[k] = B[k];
+  return true;
+}
+
+template <int NT>
+__device__ double block_sum(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// -------------------------------------------------------------------------
+// filament helpers (one thread owns one filament)
+
+__device__ dq seg_q(const SimDev &a, int g) { return ld4(a.Q, a.lds, g); }
+__device__ d3 seg_t(const SimDev &a, int g) { return qtangent(seg_q(a, g)); }
+
+// filament.cpp:416-436
+__device__ void robot_arm(const SimDev &a, int f) {
+  const int g0 = f * a.S;
+  d3 x = ld3(a.X, a.lds, g0);
+  d3 t1 = seg_t(a, g0);
+  for (int j = 1; j < a.S; j++) {
+    const d3 t2 = seg_t(a, g0 + j);
+    x = x + (0.5 * a.DL) * (t1 + t2);
+    st3(a.X, a.lds, g0 + j, x);
+    t1 = t2;
+  }
+}
+
+// segment.cpp:81-90: u += du, q = exp(u) qm1
+__device__ void seg_update(const SimDev &a, int g, d3 du) {
+  const d3 u = ld3(a.U, a.lds, g) + du;
+  st3(a.U, a.lds, g, u);
+  st4(a.Q, a.lds, g, qmul(lie_exp(u), ld4(a.Qm1, a.lds, g)));
+}
+
+__device__ d3 base_position(const SimDev &a, int f, const double *b) {
+  const m3 R = rotm(ga4(b + BQ));
+  return ga3(b + BX) + mv(R, rowvec(a.fil_ref + 3 * f));
+}
+
+// filament.cpp:438-463
+__device__ void accept_state(const SimDev &a, int f, d3 xb, d3 ub) {
+  const int g0 = f * a.S;
+  st3(a.X, a.lds, g0, xb);
+  seg_update(a, g0, ub - ld3(a.U, a.lds, g0));
+  robot_arm(a, f);
+}
+
+// -------------------------------------------------------------------------
+// initial guess (rigid_body.cpp:247-332, segment.cpp:38-79, filament.cpp:825-1018)
+
+__global__ void k_body_initial_guess(SimDev a, int nt) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double *b = a.body + size_t(n) * BREC;
+  const d3 x = ga3(b + BX), xm1 = ga3(b + BXM1), xm2 = ga3(b + BXM2);
+  const d3 u = ga3(b + BU), um1 = ga3(b + BUM1);
+  d3 gx, gu;
+  if (nt <= a.neuler) {
+    gx = 2.0 * x - xm1;
+    gu = u;
+  } else {
+    gx = 3.0 * (x - xm1) + xm2;
+    gu = 2.0 * u - um1;
+  }
+  sa3(b + BXM2, xm1); sa3(b + BXM1, x); sa3(b + BX, gx);
+  sa3(b + BUM1, u); sa3(b + BU, gu);
+  const dq qm1 = ga4(b + BQ);
+  sa4(b + BQM1, qm1);
+  const dq q = qmul(lie_exp(gu), qm1);
+  sa4(b + BQ, q);
+  const m3 R = rotm(q);
+  for (int k = 0; k < 9; k++) b[BQI + k] = R.a[k];
+}
+
+__device__ void extrap(double *v, int n, int nt) {
+  // v[0..n) value, v[n..2n) m1, v[2n..3n) m2
+  for (int k = 0; k < n; k++) {
+    double &x = v[k], &m1 = v[n + k], &m2 = v[2 * n + k];
+    if (nt == 1) {
+      m1 = x;
+    } else if (nt == 2) {
+      m2 = m1; m1 = x; x = 2.0 * m1 - m2;
+    } else if (nt > 2) {
+      const double t = 3.0 * (x - m1) + m2;
+      m2 = m1; m1 = x; x = t;
+    }
+  }
+}
+
+__global__ void k_fil_initial_guess(SimDev a, int nt) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.Nf) return;
+  const int g0 = f * a.S;
+  for (int j = 0; j < a.S; j++) {
+    const int g = g0 + j;
+    const d3 x = ld3(a.X, a.lds, g);
+    st3(a.Xm2, a.lds, g, ld3(a.Xm1, a.lds, g));
+    st3(a.Xm1, a.lds, g, x);
+    d3 u = ld3(a.U, a.lds, g);
+    if (nt <= a.neuler) {
+      st3(a.Um1, a.lds, g, u);
+    } else {
+      const d3 t = 2.0 * u - ld3(a.Um1, a.lds, g);
+      st3(a.Um1, a.lds, g, u);
+      u = t;
+      st3(a.U, a.lds, g, u);
+    }
+    const dq q = seg_q(a, g);
+    st4(a.Qm1, a.lds, g, q);
+    st4(a.Q, a.lds, g, qmul(lie_exp(u), q));
+  }
+  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+  accept_state(a, f, base_position(a, f, b), ga3(b + BU));
+  // lambda: [value | m1 | m2] blocks of 3(S-1) inside a 9S record
+  double *L = a.lam + size_t(f) * 9 * a.S;
+  extrap(L, 3 * (a.S - 1), nt);
+  extrap(a.teth + 9 * f, 3, nt);
+  extrap(a.clamp + 9 * f, 3, nt);
+}
+
+// -------------------------------------------------------------------------
+// forces (filament.cpp:1096-1220, swimmer.cpp:276-325)
+
+// body-frame moment of link i (filament.cpp:1096-1110)
+__device__ d3 bfm(const SimDev &a, dq q1, dq q2, d3 st) {
+  const dq qm = qconj(qmid(q1, q2));
+  const double s = 1.0 / a.DL;
+  const dq dqs = {(q2.s - q1.s) * s, (q2.x - q1.x) * s, (q2.y - q1.y) * s, (q2.z - q1.z) * s};
+  const dq k = qmul(qm, dqs);
+  return {a.kt * (2.0 * k.x - st.x), a.kb * (2.0 * k.y - st.y), a.kb * (2.0 * k.z - st.z)};
+}
+
+__device__ d3 strain_twist(const SimDev &a, int f, double t) {
+  return {0.0, 0.0, a.camp * sin(2.0 * kPi * t / a.cper + a.phase[f])};
+}
+
+__global__ void k_fil_forces(SimDev a, double t) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.Nf) return;
+  const int S = a.S, g0 = f * S;
+  const d3 st = strain_twist(a, f, t);
+  const double *L = a.lam + size_t(f) * 9 * S;
+  double *F = a.Fint;
+  const int ld = a.lds;
+  for (int c = 0; c < 6; c++) F[c * ld + g0] = 0.0;
+  dq q1 = seg_q(a, g0);
+  d3 t1 = qtangent(q1);
+  for (int i = 0; i < S - 1; i++) {
+    const int gi = g0 + i, gn = gi + 1;
+    const d3 lam = rowvec(L + 3 * i);
+    const dq q2 = seg_q(a, gn);
+    const d3 t2 = qtangent(q2);
+    d3 Fi = {F[gi], F[ld + gi], F[2 * ld + gi]};
+    d3 Ti = {F[3 * ld + gi], F[4 * ld + gi], F[5 * ld + gi]};
+    Fi = Fi - lam;
+    const d3 Fn = lam;
+    Ti = Ti - (0.5 * a.DL) * cross3(t1, lam);
+    d3 Tn = (-0.5 * a.DL) * cross3(t2, lam);
+    const d3 M = mv(rotm(qmid(q1, q2)), bfm(a, q1, q2, st));
+    Ti = Ti + M;
+    Tn = Tn - M;
+    F[gi] = Fi.x; F[ld + gi] = Fi.y; F[2 * ld + gi] = Fi.z;
+    F[3 * ld + gi] = Ti.x; F[4 * ld + gi] = Ti.y; F[5 * ld + gi] = Ti.z;
+    F[gn] = Fn.x; F[ld + gn] = Fn.y; F[2 * ld + gn] = Fn.z;
+    F[3 * ld + gn] = Tn.x; F[4 * ld + gn] = Tn.y; F[5 * ld + gn] = Tn.z;
+    q1 = q2;
+    t1 = t2;
+  }
+  const d3 teth = rowvec(a.teth + 9 * f);
+  const d3 ct = dexpinv_T(ld3(a.U, ld, g0), rowvec(a.clamp + 9 * f));
+  F[g0] += teth.x; F[ld + g0] += teth.y; F[2 * ld + g0] += teth.z;
+  F[3 * ld + g0] += ct.x; F[4 * ld + g0] += ct.y; F[5 * ld + g0] += ct.z;
+  // this filament's share of the free-swimming balance
+  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+  const d3 bx = ga3(b + BX);
+  d3 sf = {0, 0, 0}, stq = {0, 0, 0};
+  for (int j = 0; j < S; j++) {
+    const int g = g0 + j;
+    const d3 fs = {F[g], F[ld + g], F[2 * ld + g]};
+    const d3 ts = {F[3 * ld + g], F[4 * ld + g], F[5 * ld + g]};
+    sf = sf + fs;
+    stq = stq + ts + cross3(ld3(a.X, ld, g) - bx, fs);
+  }
+  double *o = a.fil_bf + 6 * f;
+  o[0] = sf.x; o[1] = sf.y; o[2] = sf.z; o[3] = stq.x; o[4] = stq.y; o[5] = stq.z;
+}
+
+__global__ void k_body_forces(SimDev a) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double *b = a.body + size_t(n) * BREC;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
+  for (int f = f0; f < f1; f++)
+    for (int k = 0; k < 6; k++) s[k] -= a.fil_bf[6 * f + k];
+  for (int k = 0; k < 6; k++) b[BF + k] = s[k];
+}
+
+// -------------------------------------------------------------------------
+// compute_velocities pieces (mobility_solver.cpp:132-192, 361-421)
+
+__global__ void k_blob_positions(SimDev a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.Nb) return;
+  const double *b = a.body + size_t(a.blob_sw[m]) * BREC;
+  const m3 R = rotm(ga4(b + BQ));
+  st3(a.Xb, a.ldb, m, ga3(b + BX) + mv(R, ld3(a.blob_ref, a.ldb, m)));
+}
+
+// swimmer.f += sum over its blobs of (F_rep, r x F_rep)
+__global__ __launch_bounds__(256) void k_body_repulsion(SimDev a) {
+  __shared__ double sh[256];
+  const int n = blockIdx.x;
+  double *b = a.body + size_t(n) * BREC;
+  const m3 R = rotm(ga4(b + BQ));
+  const int m0 = a.sw_blob0[n], m1 = m0 + a.sw_nblob[n];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
+    const d3 fr = ld3(a.Frep, a.ldb, m);
+    const d3 c = cross3(mv(R, ld3(a.blob_ref, a.ldb, m)), fr);
+    acc[0] += fr.x; acc[1] += fr.y; acc[2] += fr.z;
+    acc[3] += c.x; acc[4] += c.y; acc[5] += c.z;
+  }
+  for (int k = 0; k < 6; k++) {
+    const double s = block_sum<256>(acc[k], sh);
+    if (threadIdx.x == 0) b[BF + k] += s;
+  }
+}
+
+// rhs = [-v_blobs ; -swimmer.f]  (GMRES vector: lambda SoA [3][ldb], U [6 Nsw])
+__global__ void k_assemble_rhs(SimDev a, double *rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * a.ldb) {
+    const int m = i % a.ldb;
+    rhs[i] = (m < a.Nb) ? -a.Vb[i] : 0.0;
+  } else if (i < a.npad) {
+    const int r = i - 3 * a.ldb;
+    rhs[i] = -a.body[size_t(r / 6) * BREC + BF + r % 6];
+  }
+}
+
+// right preconditioner (mobility_solver.cpp:651-727), workgroup per swimmer
+__global__ __launch_bounds__(256) void k_precond(SimDev a, const double *in, double *out,
+                                                 const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  __shared__ double ou[6];
+  const int n = blockIdx.x;
+  const double *b = a.body + size_t(n) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const int m0 = a.sw_blob0[n], m1 = m0 + a.sw_nblob[n];
+  const int ld = a.ldb;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
+    const d3 v = ld3(in, ld, m);
+    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, ld, m)), v);
+    acc[0] -= v.x; acc[1] -= v.y; acc[2] -= v.z;
+    acc[3] -= c.x; acc[4] -= c.y; acc[5] -= c.z;
+  }
+  double bb[6];
+  for (int k = 0; k < 6; k++) bb[k] = block_sum<256>(acc[k], sh);
+  const int vid = 3 * ld + 6 * n;
+  if (threadIdx.x == 0) {
+    double bm[36], kk[36];
+    rotate66(b + BN, Qm, bm);
+    rotate66(b + BK, Qm, kk);
+    for (int r = 0; r < 6; r++) {
+      double s1 = 0, s2 = 0;
+      for (int k = 0; k < 6; k++) { s1 += kk[6 * r + k] * bb[k]; s2 += bm[6 * r + k] * in[vid + k]; }
+      ou[r] = s1 - s2;
+    }
+    for (int r = 0; r < 6; r++) out[vid + r] = ou[r];
+  }
+  __syncthreads();
+  const double Minv = 6.0 * kPi * a.mu * a.ab;
+  const d3 V = {ou[0], ou[1], ou[2]}, W = {ou[3], ou[4], ou[5]};
+  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
+    const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
+    st3(out, ld, m, Minv * (ld3(in, ld, m) + V + cross3(W, d)));
+  }
+}
+
+// system_matrix_mult K products (mobility_solver.cpp:905-964):
+// out_lambda = M_bb z - (U + W x d), out_U = -K^T z_lambda
+__global__ __launch_bounds__(256) void k_kprod(SimDev a, const double *z, const double *vb,
+                                               double *out, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  const int n = blockIdx.x;
+  const double *b = a.body + size_t(n) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const int m0 = a.sw_blob0[n], m1 = m0 + a.sw_nblob[n];
+  const int ld = a.ldb;
+  const int vid = 3 * ld + 6 * n;
+  const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
+    const d3 l = ld3(z, ld, m);
+    const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
+    const d3 c = cross3(d, l);
+    acc[0] -= l.x; acc[1] -= l.y; acc[2] -= l.z;
+    acc[3] -= c.x; acc[4] -= c.y; acc[5] -= c.z;
+    st3(out, ld, m, ld3(vb, ld, m) - (V + cross3(W, d)));
+  }
+  for (int k = 0; k < 6; k++) {
+    const double s = block_sum<256>(acc[k], sh);
+    if (threadIdx.x == 0) out[vid + k] = s;
+  }
+}
+
+// -------------------------------------------------------------------------
+// vector kernels (GMRES / Broyden)
+
+// h[i] = <Q_i, w>, i < k (one workgroup per column)
+__global__ __launch_bounds__(256) void k_dots(const double *Qb, size_t ldq, int k, const double *w,
+                                              int n, double *h, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  const int i = blockIdx.x;
+  if (i >= k) return;
+  const double *q = Qb + ldq * i;
+  double s = 0;
+  for (int e = threadIdx.x; e < n; e += 256) s += q[e] * w[e];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) h[i] = s;
+}
+
+// w -= sum_i h[i] Q_i  (and optionally accumulate h into hacc)
+__global__ void k_sub(const double *Qb, size_t ldq, int k, const double *h, double *w, int n,
+                      const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = w[e];
+  for (int i = 0; i < k; i++) s -= h[i] * Qb[ldq * i + e];
+  w[e] = s;
+}
+
+// Normalise w into the new basis vector and do the Givens step
+// (mobility_solver.cpp:1183-1234).  One workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_norm_givens(double *w, int n, int it, const double *h1,
+                                                      const double *h2, double *H, int ldh,
+                                                      double *cs, double *sn, double *beta,
+                                                      int *flags, const double *scal, double tol,
+                                                      int maxit) {
+  if (flags[F_DONE]) return;
+  __shared__ double sh[1024];
+  double s = 0;
+  for (int e = threadIdx.x; e < n; e += 1024) s += w[e] * w[e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  const double qn = sqrt(sh[0]);
+  for (int e = threadIdx.x; e < n; e += 1024) w[e] /= qn;
+  if (threadIdx.x != 0) return;
+  const int c = it - 1;
+  auto h = [&](int i, int j) -> double & { return H[size_t(i) * ldh + j]; };
+  for (int i = 0; i < it; i++) h(i, c) = h1[i] + h2[i];
+  h(it, c) = qn;
+  for (int i = 1; i < it; i++) {
+    const double t = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
+    h(i, c) = cs[i - 1] * h(i, c) - sn[i - 1] * h(i - 1, c);
+    h(i - 1, c) = t;
+  }
+  if (h(it, c) == 0.0) {
+    cs[c] = 1.0; sn[c] = 0.0;
+  } else if (fabs(h(it, c)) > fabs(h(c, c))) {
+    const double t = h(c, c) / h(it, c);
+    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + t * t);
+    cs[c] = t * sn[c];
+  } else {
+    const double t = h(it, c) / h(c, c);
+    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + t * t);
+    sn[c] = t * cs[c];
+  }
+  h(c, c) = cs[c] * h(c, c) + sn[c] * h(it, c);
+  h(it, c) = 0.0;
+  beta[it] = -sn[c] * beta[c];
+  beta[c] = cs[c] * beta[c];
+  const double rel = fabs(beta[it]) / scal[0];
+  if (rel <= tol || it == maxit) {
+    flags[F_DONE] = 1;
+    flags[F_ITS] = it;
+  }
+}
+
+// back substitution for y (mobility_solver.cpp:1236-1257)
+__global__ void k_backsub(const double *H, int ldh, const double *beta, const int *flags,
+                          double *y) {
+  const int it = flags[F_ITS];
+  y[it - 1] = beta[it - 1] / H[size_t(it - 1) * ldh + it - 1];
+  for (int i = it - 2; i >= 0; i--) {
+    double s = beta[i];
+    for (int j = it - 1; j > i; j--) s -= H[size_t(i) * ldh + j] * y[j];
+    y[i] = s / H[size_t(i) * ldh + i];
+  }
+}
+
+__global__ void k_combine(const double *Qb, size_t ldq, const int *flags, const double *y,
+                          double *t, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int it = flags[F_ITS];
+  double s = 0;
+  for (int i = 0; i < it; i++) s += y[i] * Qb[ldq * i + e];
+  t[e] = s;
+}
+
+// out = a*x + b*y  (elementwise)
+__global__ void k_axpby(double *out, double a, const double *x, double b, const double *y, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = a * x[e] + (y ? b * y[e] : 0.0);
+}
+
+// scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup)
+__global__ __launch_bounds__(1024) void k_norms(const double *x, const double *y, int n,
+                                                double *scal, int o0, int o1) {
+  __shared__ double sh[1024];
+  for (int pass = 0; pass < 2; pass++) {
+    const double *v = pass ? y : x;
+    if (!v) break;
+    double s = 0;
+    for (int e = threadIdx.x; e < n; e += 1024) s += v[e] * v[e];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) scal[pass ? o1 : o0] = sqrt(sh[0]);
+    __syncthreads();
+  }
+}
+
+__global__ void k_scale_by(double *x, int n, const double *scal, int o) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] /= scal[o];
+}
+
+__global__ void k_set_ints(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// copy U of every swimmer into column `col` of N_ref
+__global__ void k_store_nref_col(SimDev a, const double *x, int col) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double *b = a.body + size_t(n) * BREC;
+  for (int r = 0; r < 6; r++) b[BN + 6 * r + col] = x[3 * a.ldb + 6 * n + r];
+}
+
+__global__ void k_unit_rhs(SimDev a, double *rhs, int slot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.npad) return;
+  double v = 0.0;
+  if (i >= 3 * a.ldb && (i - 3 * a.ldb) % 6 == slot) v = -1.0;
+  rhs[i] = v;
+}
+
+// -------------------------------------------------------------------------
+// residual (mobility_solver.cpp:1404-1504, 1641-1689)
+
+__global__ void k_errors(SimDev a, int nt, double *err, const double *vbody) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double c1 = -4.0 / 3.0, c2 = 1.0 / 3.0, c3 = -2.0 / 3.0;
+  const bool euler = nt < a.neuler;
+  const double DT = a.dt;
+  bool big = false, bad = false;
+  if (i < a.Ns) {
+    const int f = i / a.S, j = i % a.S;
+    const int sw = a.fil_sw[f];
+    const int id = 6 * a.S * f + 6 * sw + 3 * j;
+    const int ld = a.lds;
+    const d3 v = {a.Vs[i], a.Vs[ld + i], a.Vs[2 * ld + i]};
+    const d3 om = {a.Vs[3 * ld + i], a.Vs[4 * ld + i], a.Vs[5 * ld + i]};
+    const d3 x = ld3(a.X, ld, i), xm1 = ld3(a.Xm1, ld, i), xm2 = ld3(a.Xm2, ld, i);
+    const d3 u = ld3(a.U, ld, i), um1 = ld3(a.Um1, ld, i);
+    d3 e;
+    if (euler) e = x - xm1 - DT * v;
+    else e = x + c1 * xm1 + c2 * xm2 + c3 * DT * v;
+    sa3(err + id, e);
+    big = fabs(e.x) > a.DL * a.tol || fabs(e.y) > a.DL * a.tol || fabs(e.z) > a.DL * a.tol;
+    bad = !(isfinite(e.x) && isfinite(e.y) && isfinite(e.z));
+    const d3 W = dexpinv(u, om);
+    d3 l;
+    if (euler) l = u - DT * W;
+    else l = u - c2 * um1 + c3 * DT * W;
+    sa3(err + id + 3 * a.S, l);
+    big = big || fabs(l.x) > a.tol || fabs(l.y) > a.tol || fabs(l.z) > a.tol;
+    bad = bad || !(isfinite(l.x) && isfinite(l.y) && isfinite(l.z));
+  } else if (i < a.Ns + a.Nsw) {
+    const int n = i - a.Ns;
+    const double *b = a.body + size_t(n) * BREC;
+    const int id = 6 * a.S * (a.sw_fil0[n] + a.sw_nfil[n]) + 6 * n;
+    const double *U = vbody + 6 * n;
+    const d3 x = ga3(b + BX), xm1 = ga3(b + BXM1), xm2 = ga3(b + BXM2);
+    const d3 u = ga3(b + BU), um1 = ga3(b + BUM1);
+    const d3 w = dexpinv(u, rowvec(U + 3));
+    d3 e, l;
+    if (euler) {
+      e = x - xm1 - DT * rowvec(U);
+      l = u - DT * w;
+    } else {
+      e = x + c1 * xm1 + c2 * xm2 + c3 * DT * rowvec(U);
+      l = u - c2 * um1 + c3 * DT * w;
+    }
+    sa3(err + id, e);
+    sa3(err + id + 3, l);
+    big = fabs(e.x) > a.tol * a.DL || fabs(e.y) > a.tol * a.DL || fabs(e.z) > a.tol * a.DL ||
+          fabs(l.x) > a.tol || fabs(l.y) > a.tol || fabs(l.z) > a.tol;
+    bad = !(isfinite(e.x) && isfinite(e.y) && isfinite(e.z) && isfinite(l.x) &&
+            isfinite(l.y) && isfinite(l.z));
+  }
+  if (big) atomicOr(&a.flags[F_BIG], 1);
+  if (bad) atomicOr(&a.flags[F_NAN], 1);
+}
+
+// -------------------------------------------------------------------------
+// approximate Jacobian (filament.cpp:8-55, 1222-1539)
+
+__device__ d3 psi_col(dq q, int c, double &s0) {
+  // column c of psi_mat(q) (quaternion.cpp:315-329) as (s0, vec)
+  const d3 v = {q.x, q.y, q.z};
+  const d3 e = {double(c == 0), double(c == 1), double(c == 2)};
+  s0 = -0.5 * dot3(v, e);
+  const d3 r = mv(rcross(v), e);
+  return 0.5 * (r + q.s * e);
+}
+
+// filament.cpp:8-55
+__device__ m3 bfm_lie_derivative(const SimDev &a, dq q1, dq q2, bool upper) {
+  const dq qm = qconj(qmid(q1, q2));
+  const double s = 1.0 / a.DL;
+  const dq dqs = {(q2.s - q1.s) * s, (q2.x - q1.x) * s, (q2.y - q1.y) * s, (q2.z - q1.z) * s};
+  const dq qq = upper ? q2 : q1;
+  const double s1 = upper ? 1.0 / a.DL : -1.0 / a.DL;
+  const double s2 = upper ? 0.5 * a.DL : -0.5 * a.DL;
+  m3 out;
+  for (int c = 0; c < 3; c++) {
+    double p0;
+    const d3 pv = psi_col(qq, c, p0);
+    const dq A1 = {s1 * p0, s1 * pv.x, s1 * pv.y, s1 * pv.z};
+    const dq A2 = {s2 * A1.s, s2 * A1.x, s2 * A1.y, s2 * A1.z};
+    const dq r1 = qmul(qm, A1), r2 = qmul(dqs, A2);
+    out.a[0 * 3 + c] = 2.0 * (r1.x + r2.x) * a.kt;
+    out.a[1 * 3 + c] = 2.0 * (r1.y + r2.y) * a.kb;
+    out.a[2 * 3 + c] = 2.0 * (r1.z + r2.z) * a.kb;
+  }
+  return out;
+}
+
+__device__ m3 elastic_rc(d3 K, dq q) {
+  return madd(mscale(K.x, rcross(qtangent(q))),
+              madd(mscale(K.y, rcross(qnormal(q))), mscale(K.z, rcross(qbinormal(q)))));
+}
+
+__device__ void put3(double *J, int n, int r0, int c0, const m3 &B) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) J[size_t(r0 + i) * n + c0 + j] = B.a[3 * i + j];
+}
+__device__ m3 get3(const double *J, int n, int r0, int c0) {
+  m3 B;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) B.a[3 * i + j] = J[size_t(r0 + i) * n + c0 + j];
+  return B;
+}
+
+// Builds the 6S x 6S approximate Jacobian of filament blockIdx.x into Jw.
+__global__ __launch_bounds__(64) void k_fil_jacobian(SimDev a, int nt, double t) {
+  const int f = blockIdx.x;
+  const int S = a.S, n = 6 * S;
+  double *J = a.Jw + size_t(f) * n * n;
+  for (int e = threadIdx.x; e < n * n; e += 64) J[e] = 0.0;
+  __syncthreads();
+  const int k = threadIdx.x;
+  const bool euler = nt < a.neuler;
+  const double mu = a.mu, as = a.as, DL = a.DL, dt = a.dt;
+  const int g0 = f * S;
+  if (k < S) {
+    const int g = g0 + k;
+    // positions <- lambda (filament.cpp:1414-1421)
+    const double fac1 = euler ? -dt / (6.0 * kPi * mu * as) : -dt / (9.0 * kPi * mu * as);
+    for (int c = 0; c < 3; c++) {
+      const int i = 3 * k + c;
+      J[size_t(i) * n + i] = fac1;
+      if (i < 3 * S - 3) J[size_t(i) * n + i + 3] = -fac1;
+    }
+    // positions <- Lie algebra (robot arm, filament.cpp:1439-1455)
+    for (int j = 1; j <= k; j++) {
+      const m3 tm = mscale(DL * (j == k ? 0.5 : 1.0), rcross(seg_t(a, g0 + j)));
+      put3(J, n, 3 * k, 3 * (S + j), tm);
+    }
+    // Lie <- lambda (filament.cpp:1463-1486)
+    const double fac2 = euler ? DL * dt / (16.0 * kPi * mu * as * as * as)
+                              : DL * dt / (24.0 * kPi * mu * as * as * as);
+    {
+      const m3 um = rcross(ld3(a.U, a.lds, g));
+      const m3 tm = mscale(-1.0, rcross(seg_t(a, g)));
+      const m3 blk = mscale(fac2, madd(tm, madd(mscale(0.5, mm(um, tm)),
+                                                mscale(1.0 / 12.0, mm(mm(um, um), tm)))));
+      if (k <= S - 2) put3(J, n, 3 * (S + k), 3 * (k + 1), blk);
+      if (k >= 1) put3(J, n, 3 * (S + k), 3 * k, blk);
+    }
+    // Lie <- Lie (filament.cpp:1222-1404)
+    const double tsf = euler ? dt : 2.0 * dt / 3.0;
+    const double Tfac = 1.0 / (8.0 * kPi * mu * as * as * as);
+    const d3 st = strain_twist(a, f, t);
+    const d3 uk = ld3(a.U, a.lds, g);
+    const m3 Lie = mscale(-1.0, rcross(uk));
+    const d3 tau = {a.Fint[3 * a.lds + g], a.Fint[4 * a.lds + g], a.Fint[5 * a.lds + g]};
+    const m3 Om = rcross(Tfac * tau);
+    auto diag_block = [&](const m3 &dO) {
+      const m3 t1 = madd(Om, mm(Lie, dO));
+      const m3 t2 = madd(rcross(mv(Om, uk)), madd(mm(Lie, Om), mm(mm(Lie, Lie), dO)));
+      return mscale(-tsf, madd(msub(dO, mscale(0.5, t1)), mscale(1.0 / 12.0, t2)));
+    };
+    auto off_block = [&](const m3 &dO) {
+      return mscale(-tsf, madd(msub(dO, mscale(0.5, mm(Lie, dO))),
+                               mscale(1.0 / 12.0, mm(mm(Lie, Lie), dO))));
+    };
+    const dq qk = seg_q(a, g);
+    const d3 tk = qtangent(qk);
+    const double *L = a.lam + size_t(f) * 9 * S;
+    if (k == 0) {
+      const dq q1 = seg_q(a, g + 1);
+      const m3 Cp = mscale(-0.5 * DL, mm(rcross(rowvec(L)), rcross(tk)));
+      const d3 Kph = bfm(a, qk, q1, st);
+      const m3 R = rotm(qmid(qk, q1));
+      m3 El = madd(mscale(0.5, elastic_rc(Kph, qk)), mm(R, bfm_lie_derivative(a, qk, q1, false)));
+      put3(J, n, 3 * S, 3 * S, diag_block(mscale(Tfac, madd(El, Cp))));
+      El = madd(mscale(0.5, elastic_rc(Kph, q1)), mm(R, bfm_lie_derivative(a, qk, q1, true)));
+      put3(J, n, 3 * S, 3 * S + 3, off_block(mscale(Tfac, El)));
+    } else if (k == S - 1) {
+      const dq q0 = seg_q(a, g - 1);
+      const m3 Cp = mscale(-0.5 * DL, mm(rcross(rowvec(L + 3 * (S - 2))), rcross(tk)));
+      const d3 Kmh = bfm(a, q0, qk, st);
+      const m3 R = rotm(qmid(q0, qk));
+      m3 El = msub(mscale(-0.5, elastic_rc(Kmh, qk)), mm(R, bfm_lie_derivative(a, q0, qk, true)));
+      put3(J, n, 3 * S + 3 * k, 3 * S + 3 * k, diag_block(mscale(Tfac, madd(El, Cp))));
+      El = msub(mscale(-0.5, elastic_rc(Kmh, q0)), mm(R, bfm_lie_derivative(a, q0, qk, false)));
+      put3(J, n, 3 * S + 3 * k, 3 * S + 3 * k - 3, off_block(mscale(Tfac, El)));
+    } else {
+      const dq qm = seg_q(a, g - 1), qp = seg_q(a, g + 1);
+      const d3 Kph = bfm(a, qk, qp, st), Kmh = bfm(a, qm, qk, st);
+      const d3 lv = rowvec(L + 3 * (k - 1)) + rowvec(L + 3 * k);
+      const m3 Cp = mscale(-0.5 * DL, mm(rcross(lv), rcross(tk)));
+      const m3 Rp = rotm(qmid(qk, qp)), Rm = rotm(qmid(qm, qk));
+      m3 El = madd(mscale(0.5, elastic_rc(Kph, qk)), mm(Rp, bfm_lie_derivative(a, qk, qp, false)));
+      El = msub(El, mscale(0.5, elastic_rc(Kmh, qk)));
+      El = msub(El, mm(Rm, bfm_lie_derivative(a, qm, qk, true)));
+      put3(J, n, 3 * S + 3 * k, 3 * S + 3 * k, diag_block(mscale(Tfac, madd(Cp, El))));
+      El = msub(mscale(-0.5, elastic_rc(Kmh, qm)), mm(Rm, bfm_lie_derivative(a, qm, qk, false)));
+      put3(J, n, 3 * S + 3 * k, 3 * S + 3 * k - 3, off_block(mscale(Tfac, El)));
+      El = madd(mscale(0.5, elastic_rc(Kph, qp)), mm(Rp, bfm_lie_derivative(a, qk, qp, true)));
+      put3(J, n, 3 * S + 3 * k, 3 * S + 3 * k + 3, off_block(mscale(Tfac, El)));
+    }
+    for (int c = 0; c < 3; c++) J[size_t(3 * S + 3 * k + c) * n + 3 * S + 3 * k + c] += 1.0;
+  }
+  __syncthreads();
+  // clamped first frame (filament.cpp:1496-1532)
+  if (threadIdx.x == 0) {
+    const double fac3 = euler ? -dt / (8.0 * kPi * mu * as * as * as)
+                              : -dt / (12.0 * kPi * mu * as * as * as);
+    const m3 um = rcross(ld3(a.U, a.lds, g0));
+    const m3 blk = madd(meye(), madd(mscale(0.5, um), mscale(1.0 / 12.0, mm(um, um))));
+    const m3 blkT = madd(meye(), madd(mscale(-0.5, um), mscale(1.0 / 12.0, mm(um, um))));
+    const m3 e1 = get3(J, n, 3 * S, 3 * S), e2 = get3(J, n, 3 * S + 3, 3 * S);
+    double *E = a.ecb + 18 * f;
+    for (int q = 0; q < 9; q++) { E[q] = e1.a[q]; E[9 + q] = e2.a[q]; }
+    put3(J, n, 3 * S, 3 * S, mscale(fac3, mm(blk, blkT)));
+    put3(J, n, 3 * S + 3, 3 * S, mzero());
+  }
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting of each filament's
+// matrix (Jw), written column-unpermuted to Jinv.  Workgroup per filament.
+__global__ __launch_bounds__(256) void k_gj_invert(SimDev a) {
+  const int f = blockIdx.x;
+  const int n = 6 * a.S;
+  double *A = a.Jw + size_t(f) * n * n;
+  double *Out = a.Jinv + size_t(f) * n * n;
+  __shared__ int piv[256];
+  __shared__ double colk[256];
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += 256) piv[i] = i;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    // pivot: first row of maximal |A(i,k)|, i >= k
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + t; i < n; i += 256) {
+      const double v = fabs(A[size_t(i) * n + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    rv[t] = best; ri[t] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        const double v2 = rv[t + s];
+        const int i2 = ri[t + s];
+        if (v2 > rv[t] || (v2 == rv[t] && i2 < ri[t])) { rv[t] = v2; ri[t] = i2; }
+      }
+      __syncthreads();
+    }
+    const int p = ri[0];
+    const double pv = rv[0];
+    __syncthreads();
+    if (!(pv > 0.0)) {
+      if (t == 0) atomicOr(&a.flags[F_SING], 1);
+      return;
+    }
+    if (p != k) {
+      for (int j = t; j < n; j += 256) {
+        const double x = A[size_t(k) * n + j];
+        A[size_t(k) * n + j] = A[size_t(p) * n + j];
+        A[size_t(p) * n + j] = x;
+      }
+      if (t == 0) { const int x = piv[k]; piv[k] = piv[p]; piv[p] = x; }
+    }
+    __syncthreads();
+    const double inv = 1.0 / A[size_t(k) * n + k];
+    for (int i = t; i < n; i += 256) colk[i] = A[size_t(i) * n + k];
+    __syncthreads();
+    for (int j = t; j < n; j += 256)
+      A[size_t(k) * n + j] = (j == k ? 1.0 : A[size_t(k) * n + j]) * inv;
+    __syncthreads();
+    for (int e = t; e < n * n; e += 256) {
+      const int i = e / n, j = e % n;
+      if (i == k) continue;
+      const double fi = colk[i];
+      const double base = (j == k) ? 0.0 : A[e];
+      A[e] = base - fi * A[size_t(k) * n + j];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < n * n; e += 256) {
+    const int i = e / n, j = e % n;
+    Out[size_t(i) * n + piv[j]] = A[e];
+  }
+}
+
+// Schur-complement blocks of one filament (swimmer.cpp:526-680):
+// Bblk = B J^-1 (6 x 6S) and BA = Bblk A (6 x 6).
+__global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
+  const int f = blockIdx.x;
+  const int S = a.S, n = 6 * S;
+  __shared__ double B[6 * 240];
+  const int t = threadIdx.x;
+  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+  const bool euler = nt < a.neuler;
+  const double dtf = euler ? -a.dt : -2.0 * a.dt / 3.0;
+  m3 Qi;
+  for (int q = 0; q < 9; q++) Qi.a[q] = b[BQI + q];
+  const d3 ub = ga3(b + BU);
+  const m3 blc = mscale(-1.0, rcross(ub));
+  const m3 Dinv = madd(meye(), madd(mscale(1.0 / 12.0, mm(blc, blc)), mscale(-0.5, blc)));
+  const m3 DinvT = madd(meye(), madd(mscale(1.0 / 12.0, mm(blc, blc)), mscale(0.5, blc)));
+  const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
+  for (int e = t; e < 6 * n; e += 256) B[e] = 0.0;
+  __syncthreads();
+  if (t == 0) {
+    double bm[36];
+    rotate66(b + BN, Qi, bm);
+    // raw B: cols 0..2 = [-I ; rcross(d)], cols 3S..3S+2 = [0 ; -DinvT]
+    double raw[6][6];
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 6; c++) raw[r][c] = 0.0;
+    const m3 rd = rcross(d);
+    for (int i = 0; i < 3; i++) {
+      raw[i][i] = -1.0;
+      for (int j = 0; j < 3; j++) {
+        raw[3 + i][j] = rd.a[3 * i + j];
+        raw[3 + i][3 + j] = -DinvT.a[3 * i + j];
+      }
+    }
+    for (int r = 0; r < 6; r++)
+      for (int c = 0; c < 6; c++) {
+        double s = 0;
+        for (int k = 0; k < 6; k++) s += bm[6 * r + k] * raw[k][c];
+        B[r * n + (c < 3 ? c : 3 * S + c - 3)] = s;
+      }
+    // sphere-segment RPY corrections (swimmer.cpp:580-650)
+    const double R = a.R, mu = a.mu, as = a.as;
+    const d3 bx = ga3(b + BX);
+    auto add3 = [&](int r0, int c0, const m3 &M) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) B[(r0 + i) * n + c0 + j] += M.a[3 * i + j];
+    };
+    for (int s = 0; s < S; s++) {
+      const int g = f * S + s;
+      d3 rh = ld3(a.X, a.lds, g) - bx;
+      const double r = sqrt(dot3(rh, rh));
+      rh = (1.0 / r) * rh;
+      m3 rr;
+      for (int i = 0; i < 3; i++) {
+        const double ri = i == 0 ? rh.x : (i == 1 ? rh.y : rh.z);
+        rr.a[3 * i] = ri * rh.x; rr.a[3 * i + 1] = ri * rh.y; rr.a[3 * i + 2] = ri * rh.z;
+      }
+      const double sig = R * R + as * as;
+      const m3 Mtt = mscale(1.0 / (8.0 * kPi * mu * r),
+                            madd(mscale(1.0 + sig / (3.0 * r * r), meye()),
+                                 mscale(1.0 - sig / (r * r), rr)));
+      const m3 Mtr = mscale(1.0 / (8.0 * kPi * mu * r * r), rcross(rh));
+      const m3 Mrr = mscale(1.0 / (16.0 * kPi * mu * r * r * r), msub(mscale(3.0, rr), meye()));
+      const m3 tm = rcross(seg_t(a, g));
+      if (s == 0) {
+        add3(0, 0, Mtt);
+        add3(3, 0, Mtr);
+        add3(0, 3 * S, mm(Mtr, DinvT));
+        add3(3, 3 * S, mm(Mrr, DinvT));
+        add3(0, 3, msub(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
+        add3(3, 3, msub(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
+      } else if (s == S - 1) {
+        add3(0, 3 * S - 3, madd(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
+        add3(3, 3 * S - 3, madd(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
+      } else {
+        add3(0, 3 * s, madd(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
+        add3(3, 3 * s, madd(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
+        add3(0, 3 * (s + 1), msub(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
+        add3(3, 3 * (s + 1), msub(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
+      }
+    }
+  }
+  __syncthreads();
+  // rows 3..5 <- Dinv * rows 3..5 for every column block, then *dtf
+  for (int cb = t; cb < 2 * S; cb += 256) {
+    const int c0 = 3 * cb;
+    m3 blk;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) blk.a[3 * i + j] = B[(3 + i) * n + c0 + j];
+    const m3 r = mm(Dinv, blk);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) B[(3 + i) * n + c0 + j] = r.a[3 * i + j];
+  }
+  __syncthreads();
+  for (int e = t; e < 6 * n; e += 256) B[e] *= dtf;
+  __syncthreads();
+  // Bblk = B J^-1
+  const double *Ji = a.Jinv + size_t(f) * n * n;
+  double *Bb = a.Bblk + size_t(f) * 6 * n;
+  for (int e = t; e < 6 * n; e += 256) {
+    const int r = e / n, c = e % n;
+    double s = 0;
+    for (int k = 0; k < n; k++) s += B[r * n + k] * Ji[size_t(k) * n + c];
+    Bb[e] = s;
+  }
+  __syncthreads();
+  // BA = Bblk A (A as in swimmer.cpp:535-541,673-680)
+  if (t < 36) {
+    const int r = t / 6, c = t % 6;
+    const double *E = a.ecb + 18 * f;
+    const double dn = sqrt(dot3(d, d));
+    const d3 v = ((0.5 * a.DL + dn) / dn) * d;
+    const m3 Dm = rcross(v);
+    double s = 0;
+    if (c < 3) {
+      for (int k = 0; k < S; k++) s += Bb[r * n + 3 * k + c];
+    } else {
+      const int cc = c - 3;
+      for (int k = 0; k < S; k++)
+        for (int q = 0; q < 3; q++) s += Bb[r * n + 3 * k + q] * Dm.a[3 * q + cc];
+      for (int q = 0; q < 3; q++) {
+        s += Bb[r * n + 3 * S + q] * E[3 * q + cc];
+        s += Bb[r * n + 3 * S + 3 + q] * E[9 + 3 * q + cc];
+      }
+    }
+    a.fil_BA[36 * f + t] = s;
+  }
+}
+
+// Schur complement of each swimmer and its inverse (swimmer.cpp:469-684)
+__global__ void k_schur_body(SimDev a, int nt) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double *b = a.body + size_t(n) * BREC;
+  const bool euler = nt < a.neuler;
+  const double dtf = euler ? -a.dt : -2.0 * a.dt / 3.0;
+  const d3 ub = ga3(b + BU);
+  const m3 blc = mscale(-1.0, rcross(ub));
+  const m3 cbo = mscale(-1.0 / dtf, blc);
+  double Sm[36];
+  for (int k = 0; k < 36; k++) Sm[k] = (k % 7 == 0) ? 1.0 : 0.0;
+  auto add33 = [&](const m3 &M) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Sm[6 * (3 + i) + 3 + j] += M.a[3 * i + j];
+  };
+  add33(mscale(dtf, msub(mscale(1.0 / 12.0, mm(blc, cbo)), mscale(0.5, cbo))));
+  m3 Qi;
+  for (int q = 0; q < 9; q++) Qi.a[q] = b[BQI + q];
+  m3 old = mzero();
+  const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
+  for (int f = f0; f < f1; f++) {
+    const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
+    const d3 teth = rowvec(a.teth + 9 * f), cl = rowvec(a.clamp + 9 * f);
+    old = msub(old, mm(rcross(teth), rcross(d)));
+    const d3 ucl = cross3(ub, cl);
+    old = madd(old, msub(mscale(0.5, rcross(cl)),
+                         mscale(1.0 / 12.0, madd(rcross(ucl), mm(blc, rcross(cl))))));
+  }
+  double bm[36];
+  rotate66(b + BN, Qi, bm);
+  m3 bm33;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) bm33.a[3 * i + j] = bm[6 * (3 + i) + 3 + j];
+  old = mm(bm33, old);
+  add33(mscale(dtf, madd(msub(old, mscale(0.5, mm(blc, old))),
+                         mscale(1.0 / 12.0, mm(mm(blc, blc), old)))));
+  for (int f = f0; f < f1; f++)
+    for (int k = 0; k < 36; k++) Sm[k] -= a.fil_BA[36 * f + k];
+  if (!gj_small(Sm, 6)) atomicOr(&a.flags[F_SING], 1);
+  for (int k = 0; k < 36; k++) b[BS + k] = Sm[k];
+}
+
+// -------------------------------------------------------------------------
+// jacobian_inv_mult (swimmer.cpp:793-839)
+
+__device__ __forceinline__ int fil_off(const SimDev &a, int f) {
+  return 6 * a.S * f + 6 * a.fil_sw[f];
+}
+__device__ __forceinline__ int body_off(const SimDev &a, int n) {
+  return 6 * a.S * (a.sw_fil0[n] + a.sw_nfil[n]) + 6 * n;
+}
+
+__global__ __launch_bounds__(64) void k_jim1(SimDev a, const double *in) {
+  __shared__ double sh[64];
+  const int f = blockIdx.x;
+  const int n = 6 * a.S;
+  const double *x = in + fil_off(a, f);
+  const double *Bb = a.Bblk + size_t(f) * 6 * n;
+  for (int r = 0; r < 6; r++) {
+    double s = 0;
+    for (int c = threadIdx.x; c < n; c += 64) s += Bb[r * n + c] * x[c];
+    s = block_sum<64>(s, sh);
+    if (threadIdx.x == 0) a.fil_b2[6 * f + r] = s;
+  }
+}
+
+__global__ void k_jim_body(SimDev a, const double *in, double *out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  const int o = body_off(a, n);
+  double b2[6];
+  for (int k = 0; k < 6; k++) b2[k] = in[o + k];
+  const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
+  for (int f = f0; f < f1; f++)
+    for (int k = 0; k < 6; k++) b2[k] -= a.fil_b2[6 * f + k];
+  const double *Si = a.body + size_t(n) * BREC + BS;
+  for (int r = 0; r < 6; r++) {
+    double s = 0;
+    for (int c = 0; c < 6; c++) s += Si[6 * r + c] * b2[c];
+    out[o + r] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_jim2(SimDev a, const double *in, double *out) {
+  __shared__ double tmp[240];
+  const int f = blockIdx.x;
+  const int S = a.S, n = 6 * S;
+  const int sw = a.fil_sw[f];
+  const double *bo = out + body_off(a, sw);
+  const d3 dx = {bo[0], bo[1], bo[2]}, du = {bo[3], bo[4], bo[5]};
+  const double *b = a.body + size_t(sw) * BREC;
+  m3 Qi;
+  for (int q = 0; q < 9; q++) Qi.a[q] = b[BQI + q];
+  const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
+  const double dn = sqrt(dot3(d, d));
+  const m3 Dm = rcross(((0.5 * a.DL + dn) / dn) * d);
+  const d3 Ddu = mv(Dm, du);
+  const double *E = a.ecb + 18 * f;
+  m3 e1, e2;
+  for (int q = 0; q < 9; q++) { e1.a[q] = E[q]; e2.a[q] = E[9 + q]; }
+  const d3 a1 = mv(e1, du), a2 = mv(e2, du);
+  const double *x = in + fil_off(a, f);
+  for (int r = threadIdx.x; r < n; r += 256) {
+    double Av = 0.0;
+    if (r < 3 * S) {
+      const int c = r % 3;
+      Av = (c == 0 ? dx.x + Ddu.x : (c == 1 ? dx.y + Ddu.y : dx.z + Ddu.z));
+    } else if (r < 3 * S + 3) {
+      const int c = r - 3 * S;
+      Av = c == 0 ? a1.x : (c == 1 ? a1.y : a1.z);
+    } else if (r < 3 * S + 6) {
+      const int c = r - 3 * S - 3;
+      Av = c == 0 ? a2.x : (c == 1 ? a2.y : a2.z);
+    }
+    tmp[r] = x[r] - Av;
+  }
+  __syncthreads();
+  const double *Ji = a.Jinv + size_t(f) * n * n;
+  double *o = out + fil_off(a, f);
+  for (int r = threadIdx.x; r < n; r += 256) {
+    double s = 0;
+    for (int c = 0; c < n; c++) s += Ji[size_t(r) * n + c] * tmp[c];
+    o[r] = s;
+  }
+}
+
+// -------------------------------------------------------------------------
+// Broyden (broyden_solver.cpp:26-130)
+
+// upd = -gamma*upd - sum_i d[i] C_i   (find_update, and the new C column)
+__global__ void k_broyden_combine(double *out, const double *in, double gamma, const double *C,
+                                  size_t ldc, const double *d, int k, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = -gamma * in[e];
+  for (int i = 0; i < k; i++) s -= d[i] * C[ldc * i + e];
+  out[e] = s;
+}
+
+__global__ void k_scale2(double *x, double *y, int n, const double *scal, int o) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) { x[e] /= scal[o]; y[e] /= scal[o]; }
+}
+
+// -------------------------------------------------------------------------
+// state update (swimmer.cpp:847-871, rigid_body.cpp:354-381, filament.cpp:1541-1575)
+
+__global__ void k_body_update(SimDev a, const double *upd) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double *b = a.body + size_t(n) * BREC;
+  const double *u = upd + body_off(a, n);
+  for (int k = 0; k < 3; k++) { b[BX + k] += u[k]; b[BU + k] += u[3 + k]; }
+  sa4(b + BQ, qmul(lie_exp(ga3(b + BU)), ga4(b + BQM1)));
+}
+
+__global__ void k_fil_update(SimDev a, const double *upd) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.Nf) return;
+  const int S = a.S, g0 = f * S;
+  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+  accept_state(a, f, base_position(a, f, b), ga3(b + BU));
+  const double *u = upd + fil_off(a, f);
+  double *T = a.teth + 9 * f;
+  for (int k = 0; k < 3; k++) T[k] += u[k];
+  double *L = a.lam + size_t(f) * 9 * S;
+  for (int i = 1; i < S; i++) {
+    for (int k = 0; k < 3; k++) L[3 * (i - 1) + k] += u[3 * i + k];
+    seg_update(a, g0 + i, rowvec(u + 3 * (S + i)));
+  }
+  double *Cl = a.clamp + 9 * f;
+  for (int k = 0; k < 3; k++) Cl[k] += u[3 * S + k];
+  robot_arm(a, f);
+}
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// ==========================================================================
+// host side
+
+struct SimHost {
+  SimDev d;
+  gmi_sim_params p;
+  std::vector<int> nfil;
+  // device buffers owned
+  std::vector<void *> owned;
+  double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
+         *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
+         *tmp = nullptr, *scal = nullptr;
+  double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
+         *bd = nullptr;
+  int *h_flags = nullptr;
+  double *h_scal = nullptr;
+  int iter = 0;
+  int precision = GMI_FP64;
+  int last_gmres = 0;
+  int chunk = 6;
+};
+
+static SimHost *sim_of(gmi_ctx *c) { return static_cast<SimHost *>(c->extra); }
+
+static void sim_release(gmi_ctx *c) {
+  SimHost *s = sim_of(c);
+  if (!s) return;
+  for (void *p : s->owned) (void)hipFree(p);
+  if (s->h_flags) (void)hipHostFree(s->h_flags);
+  if (s->h_scal) (void)hipHostFree(s->h_scal);
+  delete s;
+}
+
+template <typename T>
+static int dalloc(gmi_ctx *c, SimHost *s, T **p, size_t n) {
+  void *q = nullptr;
+  int rc = hip_check(c, hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)), "hipMalloc sim");
+  if (rc) return rc;
+  s->owned.push_back(q);
+  *p = static_cast<T *>(q);
+  return hip_check(c, hipMemsetAsync(q, 0, sizeof(T) * std::max<size_t>(n, 1), c->stream),
+                   "memset sim");
+}
+
+template <typename T>
+static int upload(gmi_ctx *c, T *dst, const T *src, size_t n) {
+  if (n == 0) return GMI_OK;
+  int rc = hip_check(c, hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream),
+                     "upload");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+static int read_flags(gmi_ctx *c, SimHost *s) {
+  int rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->h_flags, s->d.flags, sizeof(int) * F_N,
+                                        hipMemcpyDeviceToHost, c->stream), "flags D2H")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
+
+// system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964)
+static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, bool fp32,
+                    const int *skip) {
+  SimDev &d = s->d;
+  PairArgs pa;
+  const WorkList &wl = c->lists[kBB];
+  pa.Xs = c->Xs; pa.Fs = c->Fs; pa.lds = c->lds;
+  pa.Xb = c->Xb; pa.Fb = z; pa.ldb = c->ldb;
+  pa.Vs = c->Vs; pa.Vb = c->Vb;
+  pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work;
+  pa.mask = kBB; pa.accumulate = 0;
+  pa.a_s = c->as; pa.a_b = c->ab; pa.mu = c->mu;
+  pa.skip = skip;
+  int rc = hip_check(c, launch_pair(pa, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, fp32, c->stream),
+                     "pair bb");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_kprod, dim3(d.Nsw), dim3(256), 0, c->stream, d, z, c->Vb, out, skip);
+  return launch_ok(c, "kprod");
+}
+
+static int precond(gmi_ctx *c, SimHost *s, const double *in, double *out, const int *skip) {
+  hipLaunchKernelGGL(k_precond, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d, in, out, skip);
+  return launch_ok(c, "precond");
+}
+
+// GMRES with right preconditioning (mobility_solver.cpp:970-1354) on s->rhs,
+// warm-started from s->x; CGS2 Arnoldi.  Returns iterations via *its.
+static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
+  SimDev &d = s->d;
+  const int n = d.npad;
+  const size_t ldq = size_t(n);
+  const int maxit = std::min(s->d.maxit, 3 * d.Nb + 6 * d.Nsw);
+  const int nb = nblk(n, 256);
+  int rc;
+  *its = 0;
+  // r = rhs - A x ; norms
+  double *r = s->Qk;
+  if ((rc = sys_mult(c, s, s->x, s->tmp, fp32, nullptr))) return rc;
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
+  if ((rc = launch_ok(c, "gmres init"))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal, s->scal, 2 * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream), "scal D2H")))
+    return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  const double nrhs = s->h_scal[0];
+  double b0 = s->h_scal[1];
+  if (nrhs == 0.0) {
+    return hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x");
+  }
+  if (std::fabs(b0) <= tol * nrhs) return GMI_OK;
+  if (std::fabs(b0) > nrhs) {
+    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
+      return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(r, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                          c->stream), "copy rhs")))
+      return rc;
+    b0 = nrhs;
+  }
+  s->h_scal[2] = b0;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->scal + 2, s->h_scal + 2, sizeof(double),
+                                        hipMemcpyHostToDevice, c->stream), "beta0")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->beta, s->h_scal + 2, sizeof(double),
+                                        hipMemcpyHostToDevice, c->stream), "beta0")))
+    return rc;
+  hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, r, n, s->scal, 2);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags, 2, 0);
+  if ((rc = launch_ok(c, "gmres start"))) return rc;
+  // scal[0] = ||rhs|| is read by k_norm_givens
+  const int ldh = s->d.maxit + 1;
+  int it = 1;
+  int done = 0;
+  while (!done) {
+    const int last = std::min(maxit, it + s->chunk - 1);
+    for (; it <= last; it++) {
+      double *qprev = s->Qk + ldq * (it - 1);
+      double *w = s->Qk + ldq * it;
+      if ((rc = precond(c, s, qprev, s->z, d.flags))) return rc;
+      if ((rc = sys_mult(c, s, s->z, w, fp32, d.flags))) return rc;
+      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h1,
+                         d.flags);
+      hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w, n,
+                         d.flags);
+      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h2,
+                         d.flags);
+      hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h2, w, n,
+                         d.flags);
+      hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(1024), 0, c->stream, w, n, it, s->h1, s->h2,
+                         s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit);
+      if ((rc = launch_ok(c, "arnoldi"))) return rc;
+    }
+    if ((rc = read_flags(c, s))) return rc;
+    done = s->h_flags[F_DONE];
+    if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
+  }
+  const int nits = s->h_flags[F_ITS];
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(1), 0, c->stream, s->H, ldh, s->beta, d.flags, s->y);
+  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, d.flags, s->y,
+                     s->tmp, n);
+  if ((rc = launch_ok(c, "gmres solution"))) return rc;
+  if ((rc = precond(c, s, s->tmp, s->z, nullptr))) return rc;
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
+  if ((rc = launch_ok(c, "gmres update"))) return rc;
+  *its = nits;
+  return GMI_OK;
+}
+
+// mixed precision: FP32 pair kernels inside FP64 iterative refinement
+static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
+  if (s->precision == GMI_FP64) return gmres(c, s, false, s->p.gmres_tol, its);
+  // refinement: x += GMRES32(r), r = rhs - A64 x, until ||r|| <= tol ||rhs||
+  SimDev &d = s->d;
+  const int n = d.npad, nb = nblk(n, 256);
+  int rc, total = 0;
+  double *rhs_save = s->D + size_t(s->p.max_broyden_iter) * d.nbroy;  // scratch vectors
+  if ((rc = hip_check(c, hipMemcpyAsync(rhs_save, s->rhs, sizeof(double) * n,
+                                        hipMemcpyDeviceToDevice, c->stream), "save rhs")))
+    return rc;
+  double *xsave = rhs_save + n;
+  for (int outer = 0; outer < 8; outer++) {
+    // residual with the FP64 operator
+    if ((rc = sys_mult(c, s, s->x, s->tmp, false, nullptr))) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->rhs, 1.0, rhs_save, -1.0,
+                       s->tmp, n);
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, rhs_save, s->rhs, n, s->scal, 4,
+                       5);
+    if ((rc = launch_ok(c, "refinement residual"))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal + 4, s->scal + 4, 2 * sizeof(double),
+                                          hipMemcpyDeviceToHost, c->stream), "scal")))
+      return rc;
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+    const double nb0 = s->h_scal[4], nr = s->h_scal[5];
+    if (nb0 == 0.0 || nr <= s->p.gmres_tol * nb0) break;
+    // correction solve in FP32 pairs, started from zero, to 1e-5 relative
+    if ((rc = hip_check(c, hipMemcpyAsync(xsave, s->x, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                          c->stream), "save x")))
+      return rc;
+    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
+      return rc;
+    int it = 0;
+    if ((rc = gmres(c, s, true, std::max(1e-5, s->p.gmres_tol), &it))) return rc;
+    total += it;
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, xsave, 1.0, s->x, n);
+    if ((rc = launch_ok(c, "refine"))) return rc;
+  }
+  if ((rc = hip_check(c, hipMemcpyAsync(s->rhs, rhs_save, sizeof(double) * n,
+                                        hipMemcpyDeviceToDevice, c->stream), "restore rhs")))
+    return rc;
+  *its = total;
+  return GMI_OK;
+}
+
+// read_positions_and_forces (mobility_solver.cpp:132-192)
+static int read_positions_and_forces(gmi_ctx *c, SimHost *s) {
+  SimDev &d = s->d;
+  hipLaunchKernelGGL(k_blob_positions, dim3(nblk(d.Nb, 256)), dim3(256), 0, c->stream, d);
+  int rc;
+  if ((rc = launch_ok(c, "blob positions"))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(c->Fs, d.Fint, sizeof(double) * 6 * c->lds,
+                                        hipMemcpyDeviceToDevice, c->stream), "forces to device")))
+    return rc;
+  if ((rc = gmi_apply_interparticle_forces(c, s->p.barrier_K))) return rc;
+  hipLaunchKernelGGL(k_body_repulsion, dim3(d.Nsw), dim3(256), 0, c->stream, d);
+  return launch_ok(c, "body repulsion");
+}
+
+static int compute_velocities(gmi_ctx *c, SimHost *s, int *its) {
+  int rc;
+  if ((rc = read_positions_and_forces(c, s))) return rc;
+  const bool fp32 = s->precision == GMI_FP32_KAHAN;
+  if ((rc = run_pairs(c, kSS | kBS, 0, fp32, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
+    return rc;
+  hipLaunchKernelGGL(k_assemble_rhs, dim3(nblk(s->d.npad, 256)), dim3(256), 0, c->stream, s->d,
+                     s->rhs);
+  if ((rc = launch_ok(c, "rhs"))) return rc;
+  int it = 0;
+  if ((rc = solve_linear_system(c, s, &it))) return rc;
+  s->last_gmres = it;
+  if (its) *its = it;
+  return run_pairs(c, kSB, 1, fp32, nullptr, s->x, nullptr, nullptr, nullptr, nullptr);
+}
+
+static int compute_errors(gmi_ctx *c, SimHost *s, int nt, double *dst, int *big, int *bad) {
+  SimDev &d = s->d;
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_BIG, 2, 0);
+  hipLaunchKernelGGL(k_errors, dim3(nblk(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d, nt, dst,
+                     s->x + 3 * d.ldb);
+  int rc;
+  if ((rc = launch_ok(c, "errors"))) return rc;
+  if ((rc = read_flags(c, s))) return rc;
+  if (big) *big = s->h_flags[F_BIG];
+  if (bad) *bad = s->h_flags[F_NAN];
+  return GMI_OK;
+}
+
+static int jacobian_inv_mult(gmi_ctx *c, SimHost *s, const double *in, double *out) {
+  SimDev &d = s->d;
+  hipLaunchKernelGGL(k_jim1, dim3(d.Nf), dim3(64), 0, c->stream, d, in);
+  hipLaunchKernelGGL(k_jim_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, in, out);
+  hipLaunchKernelGGL(k_jim2, dim3(d.Nf), dim3(256), 0, c->stream, d, in, out);
+  return launch_ok(c, "jacobian_inv_mult");
+}
+
+}  // namespace gmi
+
+using namespace gmi;
+
+extern "C" {
+
+int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
+  if (!c || !p || !p->nfil || !p->blob_refs || !p->fil_refs) return GMI_ERR_ARG;
+  if (c->extra) return fail(c, GMI_ERR_STATE, "context already has a sim");
+  if (c->interact && c->nsw > 1)
+    return fail(c, GMI_ERR_ARG, "a sim needs hydrodynamically independent swimmers");
+  const int S = c->nseg_fil;
+  if (S < 2 || 6 * S > 240) return fail(c, GMI_ERR_ARG, "NSEG must be in [2, 40]");
+  int Nf = 0;
+  for (int n = 0; n < c->nsw; n++) {
+    if (p->nfil[n] < 1 || p->nfil[n] * S != c->nseg[n])
+      return fail(c, GMI_ERR_ARG, "nfil*NSEG must match the context's segment counts");
+    Nf += p->nfil[n];
+  }
+  if (!(p->dt > 0) || !(p->tol > 0) || !(p->gmres_tol > 0) || p->max_broyden_iter < 1 ||
+      p->gmres_max_iter < 1 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
+    return fail(c, GMI_ERR_ARG, "bad sim parameters");
+  SimHost *s = new SimHost();
+  s->p = *p;
+  s->precision = c->precision;
+  c->extra = s;
+  c->extra_release = sim_release;
+  SimDev &d = s->d;
+  d.S = S; d.Nf = Nf; d.Nsw = c->nsw; d.Ns = c->Ns; d.Nb = c->Nb;
+  d.lds = c->lds; d.ldb = c->ldb;
+  d.nbroy = 6 * (Nf * S + c->nsw);
+  d.npad = 3 * c->ldb + 6 * c->nsw;
+  d.maxit = p->gmres_max_iter;
+  d.DL = 2.2 * c->as; d.mu = c->mu; d.as = c->as; d.ab = c->ab;
+  d.kb = p->kb; d.kt = p->kt; d.dt = p->dt; d.tol = p->tol; d.gamma = p->jacobian_confidence;
+  d.gtol = p->gmres_tol; d.R = p->sphere_radius; d.neuler = p->num_euler_steps;
+  d.camp = p->curvature_amp; d.cper = p->curvature_period;
+  d.X = c->Xs; d.Fs = c->Fs; d.Vs = c->Vs; d.Xb = c->Xb; d.Vb = c->Vb; d.Frep = c->Frep;
+  int rc;
+  const size_t ls = size_t(d.lds), nn = size_t(6 * S) * (6 * S);
+  int *ip;
+  if ((rc = dalloc(c, s, &ip, d.Nsw))) return rc; d.sw_fil0 = ip;
+  if ((rc = dalloc(c, s, &ip, d.Nsw))) return rc; d.sw_nfil = ip;
+  if ((rc = dalloc(c, s, &ip, d.Nsw))) return rc; d.sw_blob0 = ip;
+  if ((rc = dalloc(c, s, &ip, d.Nsw))) return rc; d.sw_nblob = ip;
+  if ((rc = dalloc(c, s, &ip, Nf))) return rc; d.fil_sw = ip;
+  if ((rc = dalloc(c, s, &ip, d.Nb))) return rc; d.blob_sw = ip;
+  if ((rc = dalloc(c, s, &d.flags, F_N))) return rc;
+  if ((rc = dalloc(c, s, &d.Xm1, 3 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.Xm2, 3 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.Q, 4 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.Qm1, 4 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.U, 3 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.Um1, 3 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.Fint, 6 * ls))) return rc;
+  if ((rc = dalloc(c, s, &d.fil_ref, 3 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.phase, Nf))) return rc;
+  if ((rc = dalloc(c, s, &d.lam, size_t(Nf) * 9 * S))) return rc;
+  if ((rc = dalloc(c, s, &d.teth, 9 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.clamp, 9 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.fil_bf, 6 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.body, size_t(d.Nsw) * BREC))) return rc;
+  if ((rc = dalloc(c, s, &d.blob_ref, 3 * size_t(d.ldb)))) return rc;
+  if ((rc = dalloc(c, s, &d.Jw, size_t(Nf) * nn))) return rc;
+  if ((rc = dalloc(c, s, &d.Jinv, size_t(Nf) * nn))) return rc;
+  if ((rc = dalloc(c, s, &d.ecb, 18 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.Bblk, size_t(Nf) * 36 * S))) return rc;
+  if ((rc = dalloc(c, s, &d.fil_BA, 36 * size_t(Nf)))) return rc;
+  if ((rc = dalloc(c, s, &d.fil_b2, 6 * size_t(Nf)))) return rc;
+  const size_t np = size_t(d.npad);
+  const int maxit = d.maxit;
+  if ((rc = dalloc(c, s, &s->x, np))) return rc;
+  if ((rc = dalloc(c, s, &s->rhs, np))) return rc;
+  if ((rc = dalloc(c, s, &s->Qk, np * (maxit + 1)))) return rc;
+  if ((rc = dalloc(c, s, &s->H, size_t(maxit + 1) * (maxit + 1)))) return rc;
+  if ((rc = dalloc(c, s, &s->cs, maxit + 1))) return rc;
+  if ((rc = dalloc(c, s, &s->sn, maxit + 1))) return rc;
+  if ((rc = dalloc(c, s, &s->beta, maxit + 2))) return rc;
+  if ((rc = dalloc(c, s, &s->h1, maxit + 1))) return rc;
+  if ((rc = dalloc(c, s, &s->h2, maxit + 1))) return rc;
+  if ((rc = dalloc(c, s, &s->y, maxit + 1))) return rc;
+  if ((rc = dalloc(c, s, &s->z, np))) return rc;
+  if ((rc = dalloc(c, s, &s->tmp, np))) return rc;
+  if ((rc = dalloc(c, s, &s->scal, 16))) return rc;
+  const size_t nbr = size_t(d.nbroy);
+  if ((rc = dalloc(c, s, &s->err, nbr))) return rc;
+  if ((rc = dalloc(c, s, &s->nerr, nbr))) return rc;
+  if ((rc = dalloc(c, s, &s->upd, nbr))) return rc;
+  if ((rc = dalloc(c, s, &s->C, nbr * p->max_broyden_iter))) return rc;
+  // D holds MAX_BROYDEN_ITER columns plus two GMRES-refinement scratch vectors
+  if ((rc = dalloc(c, s, &s->D, nbr * p->max_broyden_iter + 2 * std::max(nbr, np)))) return rc;
+  if ((rc = dalloc(c, s, &s->bd, p->max_broyden_iter + 1))) return rc;
+  if ((rc = hip_check(c, hipHostMalloc((void **)&s->h_flags, sizeof(int) * F_N,
+                                       hipHostMallocDefault), "pinned flags")))
+    return rc;
+  if ((rc = hip_check(c, hipHostMalloc((void **)&s->h_scal, sizeof(double) * 16,
+                                       hipHostMallocDefault), "pinned scal")))
+    return rc;
+  // ---- host-side initial state (swimmer.cpp:160-237, filament.cpp:197-340)
+  std::vector<int> fil0(d.Nsw), nfv(d.Nsw), blob0(d.Nsw), nblv(d.Nsw), fil_sw(Nf), blob_sw(d.Nb);
+  int fo = 0;
+  for (int n = 0; n < d.Nsw; n++) {
+    fil0[n] = fo; nfv[n] = p->nfil[n];
+    blob0[n] = c->blob_off[n]; nblv[n] = c->nblob[n];
+    for (int i = 0; i < p->nfil[n]; i++) fil_sw[fo + i] = n;
+    for (int m = 0; m < c->nblob[n]; m++) blob_sw[c->blob_off[n] + m] = n;
+    fo += p->nfil[n];
+  }
+  std::vector<double> body(size_t(d.Nsw) * BREC, 0.0);
+  for (int n = 0; n < d.Nsw; n++) {
+    double *b = &body[size_t(n) * BREC];
+    for (int k = 0; k < 3; k++) b[BX + k] = b[BXM1 + k] = b[BXM2 + k] = p->body_x ? p->body_x[3 * n + k] : 0.0;
+    b[BQ] = b[BQM1] = 1.0;
+    for (int k = 0; k < 6; k++) { b[BN + 7 * k] = 1.0; b[BK + 7 * k] = 1.0; }
+    b[BQI] = b[BQI + 4] = b[BQI + 8] = 1.0;
+  }
+  std::vector<double> bref(3 * size_t(d.ldb), 0.0);
+  for (int m = 0; m < d.Nb; m++)
+    for (int k = 0; k < 3; k++) bref[k * d.ldb + m] = p->blob_refs[3 * m + k];
+  std::vector<double> X(3 * ls, 0.0), Qv(4 * ls, 0.0), phase(Nf, 0.0);
+  for (int f = 0; f < Nf; f++) {
+    phase[f] = p->fil_phase ? p->fil_phase[f] : 0.0;
+    const double *r = p->fil_refs + 3 * f;
+    const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double dd[3] = {r[0] / rn, r[1] / rn, r[2] / rn};
+    // rotation taking the x axis to dd: sqrt of (dd.x, x cross dd)
+    double qs = dd[0], qx = 0.0, qy = -dd[2], qz = dd[1];
+    if (qs < -0.999999999) { qs = 0; qx = 0; qy = 0; qz = 1; }
+    else {
+      qs = std::sqrt(0.5 * (1.0 + qs));
+      const double t = 2.0 * qs;
+      qx /= t; qy /= t; qz /= t;
+    }
+    const double *bx = &body[size_t(fil_sw[f]) * BREC + BX];
+    for (int j = 0; j < S; j++) {
+      const int g = f * S + j;
+      for (int k = 0; k < 3; k++) X[k * ls + g] = bx[k] + r[k] + j * d.DL * dd[k];
+      Qv[g] = qs; Qv[ls + g] = qx; Qv[2 * ls + g] = qy; Qv[3 * ls + g] = qz;
+    }
+  }
+  if ((rc = upload(c, const_cast<int *>(d.sw_fil0), fil0.data(), d.Nsw))) return rc;
+  if ((rc = upload(c, const_cast<int *>(d.sw_nfil), nfv.data(), d.Nsw))) return rc;
+  if ((rc = upload(c, const_cast<int *>(d.sw_blob0), blob0.data(), d.Nsw))) return rc;
+  if ((rc = upload(c, const_cast<int *>(d.sw_nblob), nblv.data(), d.Nsw))) return rc;
+  if ((rc = upload(c, const_cast<int *>(d.fil_sw), fil_sw.data(), Nf))) return rc;
+  if ((rc = upload(c, const_cast<int *>(d.blob_sw), blob_sw.data(), d.Nb))) return rc;
+  if ((rc = upload(c, d.body, body.data(), body.size()))) return rc;
+  if ((rc = upload(c, d.blob_ref, bref.data(), bref.size()))) return rc;
+  if ((rc = upload(c, d.fil_ref, p->fil_refs, 3 * size_t(Nf)))) return rc;
+  if ((rc = upload(c, d.phase, phase.data(), phase.size()))) return rc;
+  if ((rc = upload(c, d.X, X.data(), X.size()))) return rc;
+  if ((rc = upload(c, d.Xm1, X.data(), X.size()))) return rc;
+  if ((rc = upload(c, d.Xm2, X.data(), X.size()))) return rc;
+  if ((rc = upload(c, d.Q, Qv.data(), Qv.size()))) return rc;
+  if ((rc = upload(c, d.Qm1, Qv.data(), Qv.size()))) return rc;
+  s->nfil.assign(p->nfil, p->nfil + d.Nsw);
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int gmi_sim_set_precision(gmi_ctx *c, int precision) {
+  if (!c || !sim_of(c)) return GMI_ERR_STATE;
+  if (precision != GMI_FP64 && precision != GMI_FP32_KAHAN)
+    return fail(c, GMI_ERR_ARG, "unknown precision");
+  sim_of(c)->precision = precision;
+  return GMI_OK;
+}
+
+int gmi_make_body_reference_matrices(gmi_ctx *c) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  int rc;
+  // (K^T K)^-1 per swimmer from its body-frame blob references (host, 6x6)
+  std::vector<double> body(size_t(d.Nsw) * BREC);
+  if ((rc = hip_check(c, hipMemcpy(body.data(), d.body, sizeof(double) * body.size(),
+                                   hipMemcpyDeviceToHost), "body D2H")))
+    return rc;
+  for (int n = 0; n < d.Nsw; n++) {
+    double K[36] = {0};
+    const int nb = c->nblob[n];
+    K[0] = K[7] = K[14] = nb;
+    for (int m = 0; m < nb; m++) {
+      const double *r = s->p.blob_refs + 3 * (c->blob_off[n] + m);
+      // rcross(r) = [[0, r2, -r1], [-r2, 0, r0], [r1, -r0, 0]]
+      const double rc3[9] = {0, r[2], -r[1], -r[2], 0, r[0], r[1], -r[0], 0};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          K[6 * i + 3 + j] += rc3[3 * i + j];
+          K[6 * (3 + i) + j] -= rc3[3 * i + j];
+          double sq = 0;
+          for (int k = 0; k < 3; k++) sq += rc3[3 * i + k] * rc3[3 * k + j];
+          K[6 * (3 + i) + 3 + j] -= sq;
+        }
+    }
+    // host Gauss-Jordan, partial pivoting
+    int piv[6] = {0, 1, 2, 3, 4, 5};
+    for (int k = 0; k < 6; k++) {
+      int pp = k;
+      double best = std::fabs(K[6 * k + k]);
+      for (int i = k + 1; i < 6; i++)
+        if (std::fabs(K[6 * i + k]) > best) { best = std::fabs(K[6 * i + k]); pp = i; }
+      if (best == 0.0) return fail(c, GMI_ERR_SINGULAR, "K^T K is singular");
+      if (pp != k) {
+        for (int j = 0; j < 6; j++) std::swap(K[6 * k + j], K[6 * pp + j]);
+        std::swap(piv[k], piv[pp]);
+      }
+      const double inv = 1.0 / K[6 * k + k];
+      K[6 * k + k] = 1.0;
+      for (int j = 0; j < 6; j++) K[6 * k + j] *= inv;
+      for (int i = 0; i < 6; i++) {
+        if (i == k) continue;
+        const double fct = K[6 * i + k];
+        if (fct == 0.0) continue;
+        K[6 * i + k] = 0.0;
+        for (int j = 0; j < 6; j++) K[6 * i + j] -= fct * K[6 * k + j];
+      }
+    }
+    double* b = &body[size_t(n) * BREC];
+    for (int j = 0; j < 6; j++)
+      for (int i = 0; i < 6; i++) b[BK + 6 * i + piv[j]] = K[6 * i + j];
+  }
+  if ((rc = upload(c, d.body, body.data(), body.size()))) return rc;
+  if ((rc = read_positions_and_forces(c, s))) return rc;
+  for (int n = 0; n < 6; n++) {
+    hipLaunchKernelGGL(k_unit_rhs, dim3(nblk(d.npad, 256)), dim3(256), 0, c->stream, d, s->rhs,
+                       5 - n);
+    if ((rc = launch_ok(c, "unit rhs"))) return rc;
+    int it = 0;
+    if ((rc = solve_linear_system(c, s, &it))) return rc;
+    hipLaunchKernelGGL(k_store_nref_col, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->x,
+                       5 - n);
+    if ((rc = launch_ok(c, "store N_ref"))) return rc;
+  }
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int gmi_initial_guess(gmi_ctx *c, int nt) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  hipLaunchKernelGGL(k_body_initial_guess, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d,
+                     nt);
+  hipLaunchKernelGGL(k_fil_initial_guess, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d,
+                     nt);
+  return launch_ok(c, "initial guess");
+}
+
+int gmi_forces_and_torques(gmi_ctx *c, int nt) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  const double t = (nt + 1) * s->p.dt;
+  hipLaunchKernelGGL(k_fil_forces, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, t);
+  hipLaunchKernelGGL(k_body_forces, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d);
+  return launch_ok(c, "forces");
+}
+
+int gmi_compute_velocities(gmi_ctx *c, int nt, int *gmres_iters) {
+  (void)nt;
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  int rc = compute_velocities(c, s, gmres_iters);
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int gmi_compute_errors(gmi_ctx *c, int nt, int which, int *too_large, int *not_finite) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (which != 0 && which != 1) return fail(c, GMI_ERR_ARG, "which must be 0 or 1");
+  return compute_errors(c, s, nt, which ? s->nerr : s->err, too_large, not_finite);
+}
+
+int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  const double t = (nt + 1) * s->p.dt;
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
+  hipLaunchKernelGGL(k_fil_jacobian, dim3(d.Nf), dim3(64), 0, c->stream, d, nt, t);
+  hipLaunchKernelGGL(k_gj_invert, dim3(d.Nf), dim3(256), 0, c->stream, d);
+  hipLaunchKernelGGL(k_schur_fil, dim3(d.Nf), dim3(256), 0, c->stream, d, nt);
+  hipLaunchKernelGGL(k_schur_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, nt);
+  int rc;
+  if ((rc = launch_ok(c, "prepare_jacobian_inv"))) return rc;
+  if ((rc = read_flags(c, s))) return rc;
+  if (s->h_flags[F_SING]) return fail(c, GMI_ERR_SINGULAR, "singular approximate Jacobian");
+  s->iter = 0;
+  return GMI_OK;
+}
+
+int gmi_broyden_find_update(gmi_ctx *c) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  int rc;
+  if ((rc = jacobian_inv_mult(c, s, s->err, s->upd))) return rc;
+  const int n = d.nbroy;
+  if (s->iter > 0)
+    hipLaunchKernelGGL(k_dots, dim3(s->iter), dim3(256), 0, c->stream, s->D, size_t(n), s->iter,
+                       s->err, n, s->bd, nullptr);
+  hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, s->upd,
+                     s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n);
+  return launch_ok(c, "find_update");
+}
+
+int gmi_apply_update(gmi_ctx *c) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, s->upd);
+  hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, s->upd);
+  return launch_ok(c, "update");
+}
+
+int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  const int n = d.nbroy;
+  if (too_large && s->iter < s->p.max_broyden_iter - 1) {
+    int rc;
+    if ((rc = jacobian_inv_mult(c, s, s->nerr, s->upd))) return rc;
+    const int it = s->iter;
+    double *Cit = s->C + size_t(n) * it, *Dit = s->D + size_t(n) * it;
+    if (it > 0)
+      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->D, size_t(n), it, s->nerr,
+                         n, s->bd, nullptr);
+    hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
+                       s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n);
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Dit, 1.0, s->nerr,
+                       -1.0, s->err, n);
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, Dit, nullptr, n, s->scal, 8, 9);
+    hipLaunchKernelGGL(k_scale2, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, Dit, n, s->scal,
+                       8);
+    if ((rc = launch_ok(c, "end_of_iter"))) return rc;
+    std::swap(s->err, s->nerr);
+  }
+  s->iter++;
+  return GMI_OK;
+}
+
+int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  int rc, big = 0, bad = 0, its = 0, total = 0;
+  if ((rc = gmi_initial_guess(c, nt))) return rc;
+  if ((rc = gmi_forces_and_torques(c, nt))) return rc;
+  if ((rc = compute_velocities(c, s, &its))) return rc;
+  total += its;
+  if ((rc = compute_errors(c, s, nt, s->err, &big, &bad))) return rc;
+  if (big && (rc = gmi_prepare_jacobian_inv(c, nt))) return rc;
+  s->iter = 0;
+  while (big && s->iter < s->p.max_broyden_iter) {
+    if ((rc = gmi_broyden_find_update(c))) return rc;
+    if ((rc = gmi_apply_update(c))) return rc;
+    if ((rc = gmi_forces_and_torques(c, nt))) return rc;
+    if ((rc = compute_velocities(c, s, &its))) return rc;
+    total += its;
+    if ((rc = compute_errors(c, s, nt, s->nerr, &big, &bad))) return rc;
+    if (bad) {
+      if (broyden_iters) *broyden_iters = s->iter + 1;
+      if (gmres_iters) *gmres_iters = total;
+      return fail(c, GMI_ERR_NOT_FINITE, "Broyden's method diverged");
+    }
+    if ((rc = gmi_broyden_end_of_iter(c, big))) return rc;
+  }
+  if (broyden_iters) *broyden_iters = s->iter;
+  if (gmres_iters) *gmres_iters = total;
+  if (big) return fail(c, GMI_ERR_NOT_CONVERGED, "Broyden's method did not converge");
+  return GMI_OK;
+}
+
+int gmi_get_state(gmi_ctx *c, double *seg_x, double *seg_q, double *seg_u, double *fil_lam,
+                  double *body_xqu) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  const size_t ls = size_t(d.lds);
+  std::vector<double> buf;
+  auto soa = [&](const double *src, int nc, double *dst) -> int {
+    buf.resize(nc * ls);
+    int r = hip_check(c, hipMemcpy(buf.data(), src, sizeof(double) * nc * ls, hipMemcpyDeviceToHost),
+                      "state D2H");
+    if (r) return r;
+    for (int g = 0; g < d.Ns; g++)
+      for (int k = 0; k < nc; k++) dst[nc * g + k] = buf[k * ls + g];
+    return GMI_OK;
+  };
+  if (seg_x && (rc = soa(d.X, 3, seg_x))) return rc;
+  if (seg_q && (rc = soa(d.Q, 4, seg_q))) return rc;
+  if (seg_u && (rc = soa(d.U, 3, seg_u))) return rc;
+  if (fil_lam) {
+    const int S = d.S;
+    std::vector<double> L(size_t(d.Nf) * 9 * S), T(9 * size_t(d.Nf)), Cl(9 * size_t(d.Nf));
+    if ((rc = hip_check(c, hipMemcpy(L.data(), d.lam, sizeof(double) * L.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    if ((rc = hip_check(c, hipMemcpy(T.data(), d.teth, sizeof(double) * T.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    if ((rc = hip_check(c, hipMemcpy(Cl.data(), d.clamp, sizeof(double) * Cl.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    for (int f = 0; f < d.Nf; f++) {
+      double *o = fil_lam + size_t(f) * (3 * S + 3);
+      for (int k = 0; k < 3 * (S - 1); k++) o[k] = L[size_t(f) * 9 * S + k];
+      for (int k = 0; k < 3; k++) { o[3 * (S - 1) + k] = T[9 * f + k]; o[3 * S + k] = Cl[9 * f + k]; }
+    }
+  }
+  if (body_xqu) {
+    std::vector<double> B(size_t(d.Nsw) * BREC);
+    if ((rc = hip_check(c, hipMemcpy(B.data(), d.body, sizeof(double) * B.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    for (int n = 0; n < d.Nsw; n++) {
+      const double *b = &B[size_t(n) * BREC];
+      double *o = body_xqu + 10 * n;
+      for (int k = 0; k < 3; k++) o[k] = b[BX + k];
+      for (int k = 0; k < 4; k++) o[3 + k] = b[BQ + k];
+      for (int k = 0; k < 3; k++) o[7 + k] = b[BU + k];
+    }
+  }
+  return GMI_OK;
+}
+
+int gmi_get_mobility(gmi_ctx *c, double *v_segs, double *v_bodies, double *f_blobs,
+                     double *body_ref66) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  SimDev &d = s->d;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  if (v_segs) {
+    std::vector<double> buf(6 * size_t(d.lds));
+    if ((rc = hip_check(c, hipMemcpy(buf.data(), c->Vs, sizeof(double) * buf.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    for (int g = 0; g < d.Ns; g++)
+      for (int k = 0; k < 6; k++) v_segs[6 * g + k] = buf[size_t(k) * d.lds + g];
+  }
+  std::vector<double> xv(d.npad);
+  if ((rc = hip_check(c, hipMemcpy(xv.data(), s->x, sizeof(double) * d.npad, hipMemcpyDeviceToHost), "D2H")))
+    return rc;
+  if (v_bodies)
+    for (int k = 0; k < 6 * d.Nsw; k++) v_bodies[k] = xv[3 * d.ldb + k];
+  if (f_blobs)
+    for (int m = 0; m < d.Nb; m++)
+      for (int k = 0; k < 3; k++) f_blobs[3 * m + k] = xv[size_t(k) * d.ldb + m];
+  if (body_ref66) {
+    std::vector<double> B(size_t(d.Nsw) * BREC);
+    if ((rc = hip_check(c, hipMemcpy(B.data(), d.body, sizeof(double) * B.size(), hipMemcpyDeviceToHost), "D2H")))
+      return rc;
+    for (int n = 0; n < d.Nsw; n++)
+      for (int k = 0; k < 36; k++) {
+        body_ref66[72 * n + k] = B[size_t(n) * BREC + BN + k];
+        body_ref66[72 * n + 36 + k] = B[size_t(n) * BREC + BK + k];
+      }
+  }
+  return GMI_OK;
+}
+
+int gmi_get_broyden(gmi_ctx *c, int which, double *out) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (!out || which < 0 || which > 2) return fail(c, GMI_ERR_ARG, "bad arguments");
+  const double *src = which == 0 ? s->err : (which == 1 ? s->nerr : s->upd);
+  return hip_check(c, hipMemcpy(out, src, sizeof(double) * s->d.nbroy, hipMemcpyDeviceToHost),
+                   "D2H");
+}
+
+}  // extern "C"

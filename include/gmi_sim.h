@@ -1,0 +1,99 @@
+/* gmi_sim.h -- C ABI of the device-resident implicit time step.
+ *
+ * Replaces the generic half of `mobility_solver` (mobility_solver.cpp) and
+ * the model/solver objects that drive it (swimmer.cpp, filament.cpp,
+ * segment.cpp, rigid_body.cpp, broyden_solver.cpp, cilia_sim_main.cpp:205-341).
+ * The whole state -- segment positions/orientations, Lagrange multipliers,
+ * body state, Broyden C/D matrices, GMRES Krylov basis -- lives on the
+ * device; the host only reads convergence flags.  Every entry point below
+ * names the reference member it replaces.
+ *
+ * Swimmers are hydrodynamically independent (a block-diagonal mobility) and
+ * may differ in filament and blob counts; they share NSEG and the radii.
+ * A sim is attached to a gmi_ctx created with the same n_swimmers and
+ * per-swimmer sizes (nseg_per_swimmer = nfil*NSEG) and swimmers_interact = 0.
+ */
+#ifndef GMI_SIM_H
+#define GMI_SIM_H
+
+#include "gmi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMI_ERR_NOT_CONVERGED (-6) /* Broyden hit MAX_BROYDEN_ITER (cilia_sim_main.cpp:328-335) */
+
+typedef struct gmi_sim_params {
+  const int *nfil;            /* NFIL of each swimmer                        */
+  const double *blob_refs;    /* body-frame blob positions [3 Nb], swimmer-major
+                                 (rigid_body::blob_references)               */
+  const double *fil_refs;     /* body-frame filament base displacements
+                                 [3 Nfil] (swimmer::filament_references)     */
+  const double *body_x;       /* initial body centres [3 n_swimmers] or NULL */
+  const double *fil_phase;    /* driving phase per filament or NULL          */
+  double kb, kt;              /* KB, KT (config.hpp:105-106)                 */
+  double dt;                  /* DT                                          */
+  double tol;                 /* Broyden TOL (config.hpp:143)                */
+  double jacobian_confidence; /* JACOBIAN_CONFIDENCE_FACTOR (config.hpp:140) */
+  double gmres_tol;           /* LINEAR_SYSTEM_TOL (config.hpp:154)          */
+  int max_broyden_iter;       /* MAX_BROYDEN_ITER (config.hpp:142)           */
+  int gmres_max_iter;         /* MAX_LINEAR_SYSTEM_ITER (config.hpp:153)     */
+  int num_euler_steps;        /* NUM_EULER_STEPS (config.hpp:167)            */
+  double barrier_K;           /* REPULSIVE_FORCE_FACTOR*END_FORCE_MAGNITUDE  */
+  double curvature_amp;       /* preferred curvature amplitude (strain_twist) */
+  double curvature_period;
+  double sphere_radius;       /* body radius of the Schur-complement RPY
+                                 correction (swimmer.cpp:580)                */
+} gmi_sim_params;
+
+/* swimmers[n].initial_setup for all n + mobility.initialise()
+ * (swimmer.cpp:13-237, filament.cpp:58-340, rigid_body.cpp:13-245,
+ * mobility_solver.cpp:22-130).  Straight filaments along the base normals. */
+int gmi_sim_create(gmi_ctx *ctx, const gmi_sim_params *params);
+/* make_body_reference_matrices (mobility_solver.cpp:1693-1771), per swimmer. */
+int gmi_make_body_reference_matrices(gmi_ctx *ctx);
+/* swimmer::initial_guess for every swimmer (swimmer.cpp:240-274). */
+int gmi_initial_guess(gmi_ctx *ctx, int nt);
+/* swimmer::forces_and_torques for every swimmer (swimmer.cpp:276-325). */
+int gmi_forces_and_torques(gmi_ctx *ctx, int nt);
+/* mobility_solver::compute_velocities (mobility_solver.cpp:1358-1402):
+ * barrier, M_ss/M_bs, rigid-body GMRES, M_sb.  gmres_iters may be NULL. */
+int gmi_compute_velocities(gmi_ctx *ctx, int nt, int *gmres_iters);
+/* mobility_solver::compute_errors into broyden.error (which = 0) or
+ * broyden.new_error (which = 1); *too_large as returned by the reference,
+ * *not_finite = 1 if any entry is NaN/Inf (mobility_solver.cpp:1404-1689). */
+int gmi_compute_errors(gmi_ctx *ctx, int nt, int which, int *too_large, int *not_finite);
+/* swimmer::prepare_jacobian_inv for every swimmer (swimmer.cpp:329-688). */
+int gmi_prepare_jacobian_inv(gmi_ctx *ctx, int nt);
+/* broyden_solver::find_update (broyden_solver.cpp:26-68). */
+int gmi_broyden_find_update(gmi_ctx *ctx);
+/* swimmer::update(&broyden.update[...]) for every swimmer (swimmer.cpp:847-871). */
+int gmi_apply_update(gmi_ctx *ctx);
+/* broyden_solver::end_of_iter (broyden_solver.cpp:70-130). */
+int gmi_broyden_end_of_iter(gmi_ctx *ctx, int too_large);
+/* One full time step: the body of the nt loop, cilia_sim_main.cpp:205-341.
+ * Returns GMI_OK, GMI_ERR_NOT_FINITE (divergence) or GMI_ERR_NOT_CONVERGED. */
+int gmi_step(gmi_ctx *ctx, int nt, int *broyden_iters, int *gmres_iters);
+
+/* State access (host arrays).  seg_x[3Ns], seg_q[4Ns] (s, v), seg_u[3Ns];
+ * fil_lam[Nfil * (3 NSEG + 3)] = lambda[3(NSEG-1)], tether[3], clamp[3];
+ * body_xqu[10 n_swimmers] = x[3], q[4], u[3].  NULL pointers are skipped. */
+int gmi_get_state(gmi_ctx *ctx, double *seg_x, double *seg_q, double *seg_u,
+                  double *fil_lam, double *body_xqu);
+/* v_segs[6Ns] (AoS), v_bodies[6 n_swimmers], f_blobs[3Nb] (GMRES solution,
+ * AoS), body_ref66[72 n_swimmers] = N_ref then (K^T K)^-1 per swimmer,
+ * row-major.  NULL pointers are skipped. */
+int gmi_get_mobility(gmi_ctx *ctx, double *v_segs, double *v_bodies, double *f_blobs,
+                     double *body_ref66);
+/* broyden.error (which = 0) / new_error (1) / update (2) [NBROY]. */
+int gmi_get_broyden(gmi_ctx *ctx, int which, double *out);
+/* Solution accuracy of the rigid-body solve: GMI_FP64, or GMI_FP32_KAHAN for
+ * FP32 pair kernels inside FP64 iterative refinement (the outer residual
+ * uses the FP64 kernel; GMRES tolerance is met in FP64). */
+int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMI_SIM_H */

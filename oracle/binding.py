@@ -154,6 +154,9 @@ def _sim_lib():
         lib.orc_sim_test_jacobian_inv_mult.restype = None
         lib.orc_sim_test_set_driving.argtypes = [vp, C.c_int]
         lib.orc_sim_test_set_driving.restype = None
+        lib.orc_sim_test_errors.argtypes = [vp, C.c_int, _dp]
+        lib.orc_sim_test_initial_guess.argtypes = [vp, C.c_int]
+        lib.orc_sim_test_initial_guess.restype = None
         lib._sim_ready = True
     return lib
 
@@ -223,3 +226,35 @@ class Sim:
         e = np.zeros(self.nbroy)
         self.lib.orc_sim_get_broyden(self.h, _p(e))
         return e
+
+    # pieces of one step, for stage-by-stage parity
+    def initial_guess(self, nt):
+        self.lib.orc_sim_test_initial_guess(self.h, nt)
+
+    def forces(self):
+        self.lib.orc_sim_test_forces(self.h)
+
+    def compute_velocities(self):
+        return self.lib.orc_sim_test_compute_velocities(self.h)
+
+    def errors(self, nt):
+        e = np.zeros(self.nbroy)
+        big = self.lib.orc_sim_test_errors(self.h, nt, _p(e))
+        return e, bool(big)
+
+    def prepare_jacobian(self, nt):
+        rc = self.lib.orc_sim_test_prepare_jacobian(self.h, nt)
+        assert rc == 0, rc
+
+    def jacobian_inv_mult(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(v)
+        self.lib.orc_sim_test_jacobian_inv_mult(self.h, _p(v), _p(out))
+        return out
+
+    def filament_jacobian(self, fil, nt):
+        n = 6 * self.nseg
+        J = np.zeros(n * n); Ji = np.zeros(n * n)
+        rc = self.lib.orc_sim_test_filament_jacobian(self.h, fil, nt, _p(J), _p(Ji))
+        assert rc == 0, rc
+        return J.reshape(n, n), Ji.reshape(n, n)

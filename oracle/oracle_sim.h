@@ -49,6 +49,8 @@ int orc_sim_test_prepare_jacobian(orc_sim *s, int nt);
 int orc_sim_test_filament_jacobian(orc_sim *s, int fil, int nt, double *J, double *Jinv);
 void orc_sim_test_jacobian_inv_mult(orc_sim *s, const double *in, double *out);
 void orc_sim_test_set_driving(orc_sim *s, int nt);
+int orc_sim_test_errors(orc_sim *s, int nt, double *out);
+void orc_sim_test_initial_guess(orc_sim *s, int nt);
 
 #ifdef __cplusplus
 }

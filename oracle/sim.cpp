@@ -1143,5 +1143,15 @@ void orc_sim_test_jacobian_inv_mult(orc_sim *s, const double *in, double *out) {
   for (const Swimmer &B : s->sw) sw_jacobian_inv_mult(B, &in[B.err0], &out[B.err0], s->DL());
 }
 void orc_sim_test_set_driving(orc_sim *s, int nt) { set_driving(s, nt); }
+int orc_sim_test_errors(orc_sim *s, int nt, double *out) {
+  std::vector<double> e(s->nbroy, 0.0);
+  const bool big = compute_errors(s, e, nt);
+  std::copy(e.begin(), e.end(), out);
+  return big ? 1 : 0;
+}
+void orc_sim_test_initial_guess(orc_sim *s, int nt) {
+  set_driving(s, nt);
+  for (Swimmer &B : s->sw) sw_initial_guess(B, nt, s->p.num_euler_steps, s->DL());
+}
 
 }  // extern "C"

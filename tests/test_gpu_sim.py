@@ -1,0 +1,130 @@
+"""GPU parity of the device-resident implicit step against the CPU oracle.
+
+Stage by stage first (forces + rigid-body solve, residual, Jacobian inverse
+product), then whole steps.  Bars:
+  * body mobility reference and rigid solve: the GMRES tolerance (the device
+    uses CGS2 Arnoldi, the reference MGS; both stop at 1e-8 relative);
+  * residual, J^-1 e: 1e-10 relative;
+  * trajectories: equal Broyden iteration counts and states within DL*TOL
+    (positions) / TOL (Lie elements) -- the Broyden tolerance.
+"""
+import numpy as np
+import pytest
+
+from generic_mobility_interface_class_amd import capi, geometry
+from oracle import binding as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    d = np.abs(b).max()
+    return float(np.abs(a - b).max() / d) if d > 0 else float(np.abs(a).max())
+
+
+def pair(name="step_962", seed=1, **kw):
+    cfg = geometry.sim_workload(name, seed=seed, **kw)
+    return cfg, orc.Sim(cfg), capi.Sim(cfg)
+
+
+def test_body_reference_matrices(gpu):
+    cfg, o, g = pair()
+    print("refmats step_962", flush=True)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    mo, mg = o.mobility(), g.mobility()
+    assert rel(mg["KTKinv"], mo["KTKinv"]) < 1e-12
+    assert rel(mg["Nref"], mo["Nref"]) < 1e-7
+
+
+def test_stages_at_first_step(gpu):
+    cfg, o, g = pair()
+    print("stages step_962", flush=True)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    o.initial_guess(0); o.forces(); o.compute_velocities()
+    g.call("gmi_initial_guess", 0)
+    g.call("gmi_forces_and_torques", 0)
+    its = capi.C.c_int(0)
+    g.call("gmi_compute_velocities", 0, capi.C.byref(its))
+    mo, mg = o.mobility(), g.mobility()
+    print("rigid solve rel err: v_segs %.2e U %.2e lambda %.2e" % (
+        rel(mg["v_segs"], mo["v_segs"]), rel(mg["v_bodies"], mo["v_bodies"]),
+        rel(mg["f_blobs"], mo["f_blobs"])), flush=True)
+    assert rel(mg["v_segs"], mo["v_segs"]) < 1e-7
+    assert rel(mg["f_blobs"], mo["f_blobs"]) < 1e-6
+    # residual
+    eo, big_o = o.errors(0)
+    big, bad = capi.C.c_int(0), capi.C.c_int(0)
+    g.call("gmi_compute_errors", 0, 0, capi.C.byref(big), capi.C.byref(bad))
+    eg = g.broyden(0)
+    assert bool(big.value) == big_o and not bad.value
+    assert rel(eg, eo) < 1e-6
+    # approximate Jacobian inverse (filaments + Schur complement) applied to e
+    o.prepare_jacobian(0)
+    g.call("gmi_prepare_jacobian_inv", 0)
+    g.call("gmi_broyden_find_update")           # iter 0: update = -gamma J^-1 e
+    ug = g.broyden(2)
+    uo = -cfg.jacobian_confidence * o.jacobian_inv_mult(eg)
+    assert rel(ug, uo) < 1e-10
+
+
+def test_trajectory_ten_steps(gpu):
+    cfg, o, g = pair()
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    DL = cfg.swimmers[0].DL
+    for nt in range(10):
+        ro = o.step(nt)
+        rg = g.step(nt)
+        so, sg = o.state(), g.state()
+        dx = np.abs(sg["x"] - so["x"]).max()
+        du = np.abs(sg["u"] - so["u"]).max()
+        print(f"step {nt}: oracle {ro} gpu {rg} |dx| {dx:.2e} |du| {du:.2e}", flush=True)
+        assert ro[0] == 0 and rg[0] == 0
+        assert abs(ro[1] - rg[1]) <= 1
+        assert dx <= DL * cfg.tol and du <= cfg.tol
+
+
+def test_mixed_precision_trajectory(gpu):
+    cfg = geometry.sim_workload("step_962", seed=1)
+    g64 = capi.Sim(cfg)
+    g32 = capi.Sim(cfg, precision=capi.GMI_FP32_KAHAN)
+    g32.set_sim_precision(capi.GMI_FP32_KAHAN)
+    g64.make_body_reference_matrices()
+    g32.make_body_reference_matrices()
+    DL = cfg.swimmers[0].DL
+    for nt in range(10):
+        r64 = g64.step(nt)
+        r32 = g32.step(nt)
+        s64, s32 = g64.state(), g32.state()
+        dx = np.abs(s32["x"] - s64["x"]).max()
+        du = np.abs(s32["u"] - s64["u"]).max()
+        print(f"mixed step {nt}: fp64 {r64} mixed {r32} |dx| {dx:.2e} |du| {du:.2e}", flush=True)
+        assert r64[0] == 0 and r32[0] == 0
+        assert dx <= DL * cfg.tol and du <= cfg.tol
+
+
+def test_uneven_independent_swimmers(gpu):
+    sw = [geometry.SwimmerGeometry(nfil=4, nseg=10, nblob=92, R=4.0,
+                                   centre=np.array([0.0, 0.0, 0.0])),
+          geometry.SwimmerGeometry(nfil=9, nseg=10, nblob=162, R=5.0,
+                                   centre=np.array([5.0, 0.0, 0.0])),
+          geometry.SwimmerGeometry(nfil=6, nseg=10, nblob=92, R=4.0,
+                                   centre=np.array([0.0, 60.0, 0.0]))]
+    rng = geometry.Xoshiro256ss(5)
+    cfg = geometry.SimConfig(swimmers=sw, phases=2 * np.pi * rng.uniform(19))
+    o, g = orc.Sim(cfg), capi.Sim(cfg)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    assert rel(g.mobility()["Nref"], o.mobility()["Nref"]) < 1e-7
+    DL = sw[0].DL
+    for nt in range(4):
+        ro, rg = o.step(nt), g.step(nt)
+        so, sg = o.state(), g.state()
+        dx = np.abs(sg["x"] - so["x"]).max()
+        print(f"uneven step {nt}: oracle {ro} gpu {rg} |dx| {dx:.2e}", flush=True)
+        assert ro[0] == 0 and rg[0] == 0
+        assert dx <= DL * cfg.tol
+        np.testing.assert_allclose(sg["body"], so["body"], rtol=0, atol=DL * cfg.tol)
