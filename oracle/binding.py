@@ -26,6 +26,14 @@ def load():
         lib.orc_Mbs.argtypes = [_dp, _dp, _dp, i, _dp, i, i, i, d, d, d, i]
         lib.orc_barrier.argtypes = [_dp, _dp, _dp, i, _dp, i, i, d, d, d, i]
         lib.orc_ss_block.argtypes = [_dp, _dp, _dp, d, d]
+        for fn in ("orc_lie_exp", "orc_lie_rotmat"):
+            getattr(lib, fn).argtypes = [_dp, _dp]
+            getattr(lib, fn).restype = None
+        for fn in ("orc_lie_midpoint", "orc_lie_dexp", "orc_lie_dexpinv",
+                   "orc_lie_dexpinv_transpose"):
+            getattr(lib, fn).argtypes = [_dp, _dp, _dp]
+            getattr(lib, fn).restype = None
+        lib.orc_invert.argtypes = [_dp, i]
         for f in (lib.orc_Mss, lib.orc_Mbb, lib.orc_Msb, lib.orc_Mbs,
                   lib.orc_barrier, lib.orc_ss_block):
             f.restype = None
@@ -258,3 +266,53 @@ class Sim:
         rc = self.lib.orc_sim_test_filament_jacobian(self.h, fil, nt, _p(J), _p(Ji))
         assert rc == 0, rc
         return J.reshape(n, n), Ji.reshape(n, n)
+
+
+# ---------------------------------------------------------------- Lie helpers
+
+def _v(a, n):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(n))
+
+
+def lie_exp(u):
+    q = np.zeros(4)
+    load().orc_lie_exp(_p(_v(u, 3)), _p(q))
+    return q
+
+
+def lie_midpoint(q1, q2):
+    q = np.zeros(4)
+    load().orc_lie_midpoint(_p(_v(q1, 4)), _p(_v(q2, 4)), _p(q))
+    return q
+
+
+def lie_rotmat(q):
+    R = np.zeros(9)
+    load().orc_lie_rotmat(_p(_v(q, 4)), _p(R))
+    return R.reshape(3, 3)
+
+
+def _uv(fn, u, v):
+    out = np.zeros(3)
+    getattr(load(), fn)(_p(_v(u, 3)), _p(_v(v, 3)), _p(out))
+    return out
+
+
+def dexp(u, v):
+    return _uv("orc_lie_dexp", u, v)
+
+
+def dexpinv(u, v):
+    return _uv("orc_lie_dexpinv", u, v)
+
+
+def dexpinv_transpose(u, v):
+    return _uv("orc_lie_dexpinv_transpose", u, v)
+
+
+def invert(A):
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    rc = load().orc_invert(_p(A), A.shape[0])
+    if rc != 0:
+        raise np.linalg.LinAlgError("singular")
+    return A

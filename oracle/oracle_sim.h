@@ -50,6 +50,13 @@ int orc_sim_test_filament_jacobian(orc_sim *s, int fil, int nt, double *J, doubl
 void orc_sim_test_jacobian_inv_mult(orc_sim *s, const double *in, double *out);
 void orc_sim_test_set_driving(orc_sim *s, int nt);
 int orc_sim_test_errors(orc_sim *s, int nt, double *out);
+void orc_lie_exp(const double *u, double *q);
+void orc_lie_midpoint(const double *q1, const double *q2, double *q);
+void orc_lie_rotmat(const double *q, double *R);
+void orc_lie_dexp(const double *u, const double *v, double *out);
+void orc_lie_dexpinv(const double *u, const double *v, double *out);
+void orc_lie_dexpinv_transpose(const double *u, const double *v, double *out);
+int orc_invert(double *A, int n);
 void orc_sim_test_initial_guess(orc_sim *s, int nt);
 
 #ifdef __cplusplus

@@ -1143,6 +1143,38 @@ void orc_sim_test_jacobian_inv_mult(orc_sim *s, const double *in, double *out) {
   for (const Swimmer &B : s->sw) sw_jacobian_inv_mult(B, &in[B.err0], &out[B.err0], s->DL());
 }
 void orc_sim_test_set_driving(orc_sim *s, int nt) { set_driving(s, nt); }
+// Lie-group helpers exposed for tests (q as s, v0, v1, v2)
+void orc_lie_exp(const double *u, double *q) {
+  const Quat r = lie_exp(u);
+  q[0] = r.s; q[1] = r.v[0]; q[2] = r.v[1]; q[3] = r.v[2];
+}
+void orc_lie_midpoint(const double *q1, const double *q2, double *q) {
+  Quat a, b;
+  a.s = q1[0]; b.s = q2[0];
+  for (int k = 0; k < 3; k++) { a.v[k] = q1[1 + k]; b.v[k] = q2[1 + k]; }
+  const Quat r = midpoint(a, b);
+  q[0] = r.s; q[1] = r.v[0]; q[2] = r.v[1]; q[3] = r.v[2];
+}
+void orc_lie_rotmat(const double *q, double *R) {
+  Quat a;
+  a.s = q[0];
+  for (int k = 0; k < 3; k++) a.v[k] = q[1 + k];
+  const M3 m = rot_mat(a);
+  for (int k = 0; k < 9; k++) R[k] = m.a[k];
+}
+void orc_lie_dexp(const double *u, const double *v, double *out) { dexp(out, u, v); }
+void orc_lie_dexpinv(const double *u, const double *v, double *out) { dexpinv(out, u, v); }
+void orc_lie_dexpinv_transpose(const double *u, const double *v, double *out) {
+  dexpinv_transpose(out, u, v);
+}
+int orc_invert(double *A, int n) {
+  Mat M(n, n);
+  std::copy(A, A + n * n, M.d.begin());
+  if (!invert(M)) return -5;
+  std::copy(M.d.begin(), M.d.end(), A);
+  return 0;
+}
+
 int orc_sim_test_errors(orc_sim *s, int nt, double *out) {
   std::vector<double> e(s->nbroy, 0.0);
   const bool big = compute_errors(s, e, nt);
