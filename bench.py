@@ -1,6 +1,10 @@
 #!/usr/bin/env python3
 """Benchmark of the RPY grand-mobility hot path on one MI355X.
 
+Second part (sim_steps): implicit time steps of the 256-filament x 20-segment
++ 4002-blob swimmer, plain FP64 against mixed precision, steps/s with the
+Broyden and GMRES iteration counts, and the CPU oracle's step rate.
+
 A step is one full grand-mobility apply (all four reference blocks:
 seg<-seg 6x6, seg<-blob, blob<-seg, blob<-blob) over the 46,722-particle
 swimmer (1024 filaments x 30 segments + 16,002 blobs on a sphere of radius
@@ -85,6 +89,66 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
     }
 
 
+def bench_step(args):
+    """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer:
+    plain FP64 vs mixed precision (FP32 pair kernels, FP64 refinement)."""
+    res = {}
+    states = {}
+    for name, prec in (("fp64", capi.GMI_FP64), ("mixed", capi.GMI_FP32_KAHAN)):
+        cfg = geometry.sim_workload(args.step_workload, seed=1)
+        sim = capi.Sim(cfg, precision=prec)
+        sim.set_sim_precision(prec)
+        sim.make_body_reference_matrices()
+        nt = 0
+        for _ in range(args.step_warmup):
+            rc, bi, gi = sim.step(nt)
+            assert rc == 0, rc
+            nt += 1
+        sim.call("gmi_wait_for_device")
+        t0 = time.perf_counter()
+        sim.call("gmi_timer_start")
+        tb = tg = 0
+        for _ in range(args.step_steps):
+            rc, bi, gi = sim.step(nt)
+            assert rc == 0, rc
+            tb += bi
+            tg += gi
+            nt += 1
+        ms = capi.C.c_double(0.0)
+        sim.call("gmi_timer_stop", capi.C.byref(ms))
+        wall = time.perf_counter() - t0
+        k = args.step_steps
+        res[name] = dict(steps_per_s=k / (ms.value * 1e-3), ms_per_step=ms.value / k,
+                         wall_ms_per_step=wall * 1e3 / k,
+                         broyden_iters_per_step=tb / k, gmres_iters_per_step=tg / k)
+        states[name] = sim.state()
+        sim.close()
+        print(f"# step {name}: {ms.value / k:.2f} ms/step, {tb / k:.2f} Broyden, "
+              f"{tg / k:.1f} GMRES it/step", file=sys.stderr, flush=True)
+    dx = float(np.abs(states["mixed"]["x"] - states["fp64"]["x"]).max())
+    out = {"workload": args.step_workload, "steps": args.step_steps, "warmup": args.step_warmup,
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"],
+           "mixed_vs_fp64_max_position_diff": dx}
+    return out
+
+
+def cpu_step_baseline(args):
+    """Oracle (CPU FP64) time step on the same swimmer: one step after the
+    body reference matrices (bounded sample)."""
+    from oracle import binding as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cfg = geometry.sim_workload(args.step_workload, seed=1)
+    sim = orc.Sim(cfg, nthreads=threads)
+    sim.make_body_reference_matrices()
+    t0 = time.perf_counter()
+    rc, bi, gi = sim.step(0)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"step 0 of {args.step_workload} ({bi} Broyden, {gi} GMRES iterations, "
+                      f"{dt:.1f} s; pair sums on {threads} threads, the rest single-threaded)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +157,10 @@ def main():
     ap.add_argument("--workload", default="apply_46722")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--step-workload", default="step_9122")
+    ap.add_argument("--step-steps", type=int, default=10)
+    ap.add_argument("--step-warmup", type=int, default=2)
+    ap.add_argument("--no-step", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one MI355X; "
@@ -168,6 +236,10 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xs, xb, fs, fb, args.cpu_seconds)
+    if not args.no_step:
+        line["sim_steps"] = bench_step(args)
+        if not args.no_cpu:
+            line["sim_steps"]["cpu_baseline"] = cpu_step_baseline(args)
     print(json.dumps(line), flush=True)
 
 

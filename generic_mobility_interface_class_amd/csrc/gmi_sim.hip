@@ -37,7 +37,8 @@ enum {
   BF = 32, BN = 41, BK = 77, BS = 113, BREC = 152
 };
 // device flags
-enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_N = 8 };
+enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
+       F_N = 8 };
 
 struct SimDev {
   // sizes
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(1024) void k_norm_givens(double *w, int n, int it, 
                                                       const double *h2, double *H, int ldh,
                                                       double *cs, double *sn, double *beta,
                                                       int *flags, const double *scal, double tol,
-                                                      int maxit) {
+                                                      int maxit, double switch_rel) {
   if (flags[F_DONE]) return;
   __shared__ double sh[1024];
   double s = 0;
@@ -503,7 +504,22 @@ __global__ __launch_bounds__(1024) void k_norm_givens(double *w, int n, int it, 
   if (rel <= tol || it == maxit) {
     flags[F_DONE] = 1;
     flags[F_ITS] = it;
+    flags[F_SKIP64] = 1;
+    flags[F_SKIP32] = 1;
+  } else if (rel <= switch_rel) {
+    // relaxed Krylov: once the residual has dropped below switch_rel the
+    // operator may carry a relative error up to tol / rel, so the FP32 pair
+    // kernel replaces the FP64 one for the remaining Arnoldi steps
+    flags[F_SKIP64] = 1;
+    flags[F_SKIP32] = 0;
   }
+}
+
+__global__ void k_gmres_reset(int *flags) {
+  flags[F_DONE] = 0;
+  flags[F_ITS] = 0;
+  flags[F_SKIP64] = 0;
+  flags[F_SKIP32] = 1;
 }
 
 // back substitution for y (mobility_solver.cpp:1236-1257)
@@ -1171,6 +1187,10 @@ __global__ void k_fil_update(SimDev a, const double *upd) {
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
+// Relative error of the FP32-Kahan pair operator against FP64 (measured
+// max-norm error ~6e-7 on 46.7k particles, bounded here by 1e-6).
+constexpr double kFp32OperatorError = 1e-6;
+
 }  // namespace
 
 // ==========================================================================
@@ -1236,9 +1256,11 @@ static int read_flags(gmi_ctx *c, SimHost *s) {
 
 static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
 
-// system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964)
-static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, bool fp32,
-                    const int *skip) {
+// system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964).
+// mode 0: FP64 pairs, 1: FP32 pairs, 2: both launched, each gated by its
+// device flag (F_SKIP64 / F_SKIP32) so the precision switch needs no sync.
+static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, int mode,
+                    const int *done) {
   SimDev &d = s->d;
   PairArgs pa;
   const WorkList &wl = c->lists[kBB];
@@ -1248,11 +1270,21 @@ static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, bool f
   pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work;
   pa.mask = kBB; pa.accumulate = 0;
   pa.a_s = c->as; pa.a_b = c->ab; pa.mu = c->mu;
-  pa.skip = skip;
-  int rc = hip_check(c, launch_pair(pa, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, fp32, c->stream),
-                     "pair bb");
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_kprod, dim3(d.Nsw), dim3(256), 0, c->stream, d, z, c->Vb, out, skip);
+  const int nred = wl.nsplit_tiles ? wl.ntiles : 0;
+  int rc;
+  if (mode == 2) {
+    pa.skip = d.flags + F_SKIP64;
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, false, c->stream), "pair bb64")))
+      return rc;
+    pa.skip = d.flags + F_SKIP32;
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, true, c->stream), "pair bb32")))
+      return rc;
+  } else {
+    pa.skip = done;
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, mode == 1, c->stream), "pair bb")))
+      return rc;
+  }
+  hipLaunchKernelGGL(k_kprod, dim3(d.Nsw), dim3(256), 0, c->stream, d, z, c->Vb, out, done);
   return launch_ok(c, "kprod");
 }
 
@@ -1262,8 +1294,10 @@ static int precond(gmi_ctx *c, SimHost *s, const double *in, double *out, const 
 }
 
 // GMRES with right preconditioning (mobility_solver.cpp:970-1354) on s->rhs,
-// warm-started from s->x; CGS2 Arnoldi.  Returns iterations via *its.
-static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
+// warm-started from s->x; CGS2 Arnoldi.  mixed: relaxed precision (FP64
+// operator until the residual drops below switch_rel, FP32 after).
+// Returns iterations via *its.
+static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   SimDev &d = s->d;
   const int n = d.npad;
   const size_t ldq = size_t(n);
@@ -1273,7 +1307,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
   *its = 0;
   // r = rhs - A x ; norms
   double *r = s->Qk;
-  if ((rc = sys_mult(c, s, s->x, s->tmp, fp32, nullptr))) return rc;
+  if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr))) return rc;
   hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
   if ((rc = launch_ok(c, "gmres init"))) return rc;
@@ -1303,8 +1337,9 @@ static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
                                         hipMemcpyHostToDevice, c->stream), "beta0")))
     return rc;
   hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, r, n, s->scal, 2);
-  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags, 2, 0);
+  hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, d.flags);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
+  const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
   // scal[0] = ||rhs|| is read by k_norm_givens
   const int ldh = s->d.maxit + 1;
   int it = 1;
@@ -1315,7 +1350,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
       double *qprev = s->Qk + ldq * (it - 1);
       double *w = s->Qk + ldq * it;
       if ((rc = precond(c, s, qprev, s->z, d.flags))) return rc;
-      if ((rc = sys_mult(c, s, s->z, w, fp32, d.flags))) return rc;
+      if ((rc = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags))) return rc;
       hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h1,
                          d.flags);
       hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w, n,
@@ -1325,7 +1360,8 @@ static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
       hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h2, w, n,
                          d.flags);
       hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(1024), 0, c->stream, w, n, it, s->h1, s->h2,
-                         s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit);
+                         s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
+                         switch_rel);
       if ((rc = launch_ok(c, "arnoldi"))) return rc;
     }
     if ((rc = read_flags(c, s))) return rc;
@@ -1344,25 +1380,23 @@ static int gmres(gmi_ctx *c, SimHost *s, bool fp32, double tol, int *its) {
   return GMI_OK;
 }
 
-// mixed precision: FP32 pair kernels inside FP64 iterative refinement
+// Rigid-body solve.  FP64: plain GMRES.  Mixed: relaxed-precision GMRES,
+// then the true residual with the FP64 operator; if it misses the tolerance
+// GMRES restarts from the current iterate (iterative refinement, FP64 outer).
 static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
   if (s->precision == GMI_FP64) return gmres(c, s, false, s->p.gmres_tol, its);
-  // refinement: x += GMRES32(r), r = rhs - A64 x, until ||r|| <= tol ||rhs||
   SimDev &d = s->d;
   const int n = d.npad, nb = nblk(n, 256);
   int rc, total = 0;
-  double *rhs_save = s->D + size_t(s->p.max_broyden_iter) * d.nbroy;  // scratch vectors
-  if ((rc = hip_check(c, hipMemcpyAsync(rhs_save, s->rhs, sizeof(double) * n,
-                                        hipMemcpyDeviceToDevice, c->stream), "save rhs")))
-    return rc;
-  double *xsave = rhs_save + n;
-  for (int outer = 0; outer < 8; outer++) {
-    // residual with the FP64 operator
-    if ((rc = sys_mult(c, s, s->x, s->tmp, false, nullptr))) return rc;
-    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->rhs, 1.0, rhs_save, -1.0,
+  for (int cycle = 0; cycle < 4; cycle++) {
+    int it = 0;
+    if ((rc = gmres(c, s, true, s->p.gmres_tol, &it))) return rc;
+    total += it;
+    // true residual with the FP64 operator
+    if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr))) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->z, 1.0, s->rhs, -1.0,
                        s->tmp, n);
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, rhs_save, s->rhs, n, s->scal, 4,
-                       5);
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5);
     if ((rc = launch_ok(c, "refinement residual"))) return rc;
     if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal + 4, s->scal + 4, 2 * sizeof(double),
                                           hipMemcpyDeviceToHost, c->stream), "scal")))
@@ -1370,21 +1404,7 @@ static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
     if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
     const double nb0 = s->h_scal[4], nr = s->h_scal[5];
     if (nb0 == 0.0 || nr <= s->p.gmres_tol * nb0) break;
-    // correction solve in FP32 pairs, started from zero, to 1e-5 relative
-    if ((rc = hip_check(c, hipMemcpyAsync(xsave, s->x, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                          c->stream), "save x")))
-      return rc;
-    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
-      return rc;
-    int it = 0;
-    if ((rc = gmres(c, s, true, std::max(1e-5, s->p.gmres_tol), &it))) return rc;
-    total += it;
-    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, xsave, 1.0, s->x, n);
-    if ((rc = launch_ok(c, "refine"))) return rc;
   }
-  if ((rc = hip_check(c, hipMemcpyAsync(s->rhs, rhs_save, sizeof(double) * n,
-                                        hipMemcpyDeviceToDevice, c->stream), "restore rhs")))
-    return rc;
   *its = total;
   return GMI_OK;
 }
