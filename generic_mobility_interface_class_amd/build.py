@@ -19,6 +19,7 @@ HIP_SOURCES = [
     "gmi_pair.hip",
     "gmi_capi.hip",
     "gmi_sim.hip",
+    "gmi_learn.hip",
 ]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would
@@ -52,6 +53,24 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+DRIVER = os.path.join(PKG, "gmi_drive")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """C++ driver of the host class (host/gmi_drive.cpp), linked to libgmi.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(PKG, "host", "gmi_drive.cpp"), "-o", DRIVER + ".tmp",
+           f"-L{PKG}", "-lgmi", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("driver build failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build_oracle(verbose: bool = False) -> str:
     od = os.path.join(ROOT, "oracle")
     r = subprocess.run(["make", "-C", od, "-B", "liboracle.so"], capture_output=True, text=True)
@@ -64,3 +83,4 @@ def build_oracle(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_driver(verbose=True))

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("gmi_profile_begin", C.c_int, [C.c_void_p]),
     ("gmi_profile_end", C.c_int, [C.c_void_p, _dp, _ip]),
     ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gmi_autotune", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     # gmi_sim.h
     ("gmi_sim_create", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gmi_make_body_reference_matrices", C.c_int, [C.c_void_p]),
@@ -86,6 +87,20 @@ SIGNATURES = [
     ("gmi_get_mobility", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
     ("gmi_get_broyden", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("gmi_sim_set_precision", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_sim_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gmi_warmstart_init", C.c_int, [C.c_void_p, C.c_int, C.c_ulonglong]),
+    ("gmi_warmstart_configure", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("gmi_warmstart_dataset_size", C.c_int, [C.c_void_p, _ip]),
+    ("gmi_warmstart_train", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                      C.c_ulonglong, _dp]),
+    ("gmi_warmstart_param_count", C.c_int, [C.c_void_p, _ip]),
+    ("gmi_warmstart_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]),
+    ("gmi_warmstart_forward", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int,
+                                        C.POINTER(C.c_float)]),
+    ("gmi_warmstart_loss_grad", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float), C.c_int, _dp,
+                                          C.POINTER(C.c_float)]),
 ]
 
 GMI_ERR_NOT_CONVERGED = -6
@@ -196,6 +211,11 @@ class Context:
     def set_precision(self, p):
         self.call("gmi_set_precision", int(p))
 
+    def autotune(self, reps=3):
+        t, ch = C.c_int(0), C.c_int(0)
+        self.call("gmi_autotune", int(reps), C.byref(t), C.byref(ch))
+        return t.value, ch.value
+
 
 class GmiSimParams(C.Structure):
     _fields_ = [
@@ -262,3 +282,53 @@ class Sim(Context):
 
     def set_sim_precision(self, p):
         self.call("gmi_sim_set_precision", int(p))
+
+    def set_graphs(self, use_graphs=True, chunk=6):
+        self.call("gmi_sim_set_graphs", int(bool(use_graphs)), int(chunk))
+
+    # ---- learned warm start ------------------------------------------
+    def warmstart_init(self, hidden=128, seed=0):
+        self.call("gmi_warmstart_init", int(hidden), int(seed))
+        n = C.c_int(0)
+        self.call("gmi_warmstart_param_count", C.byref(n))
+        self.nparams = n.value
+
+    def warmstart_configure(self, apply=False, collect=0, stride=1):
+        self.call("gmi_warmstart_configure", int(bool(apply)), int(collect), int(stride))
+
+    def warmstart_dataset_size(self):
+        n = C.c_int(0)
+        self.call("gmi_warmstart_dataset_size", C.byref(n))
+        return n.value
+
+    def warmstart_train(self, iters=2000, batch=1024, lr=1e-3, seed=0):
+        loss = C.c_double(0.0)
+        self.call("gmi_warmstart_train", int(iters), int(batch), float(lr), int(seed),
+                  C.byref(loss))
+        return loss.value
+
+    def warmstart_params(self, set_to=None):
+        out = np.zeros(self.nparams, np.float32)
+        src = None
+        if set_to is not None:
+            src = np.ascontiguousarray(set_to, np.float32)
+        self.call("gmi_warmstart_params", out.ctypes.data_as(C.POINTER(C.c_float)),
+                  None if src is None else src.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def warmstart_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 6), np.float32)
+        fp = C.POINTER(C.c_float)
+        self.call("gmi_warmstart_forward", X.ctypes.data_as(fp), X.shape[0], Y.ctypes.data_as(fp))
+        return Y
+
+    def warmstart_loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_double(0.0)
+        fp = C.POINTER(C.c_float)
+        self.call("gmi_warmstart_loss_grad", X.ctypes.data_as(fp), T.ctypes.data_as(fp),
+                  X.shape[0], C.byref(loss), g.ctypes.data_as(fp))
+        return loss.value, g

@@ -14,6 +14,7 @@
 #include "../../include/gmi.h"
 
 #include <algorithm>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <cmath>
@@ -110,6 +111,7 @@ static int build_list(gmi_ctx *c, int mask, WorkList &wl) {
 
 int rebuild_lists(gmi_ctx *c) {
   int rc;
+  c->list_generation++;
   size_t slots = 0;
   for (int mask = 1; mask < 16; ++mask) {
     if ((rc = build_list(c, mask, c->lists[mask]))) return rc;
@@ -448,6 +450,64 @@ int gmi_profile_end(gmi_ctx *c, double *pair_ms, int *launches) {
   *pair_ms = tot;
   *launches = int(c->prof_used / 2);
   c->prof_used = 0;
+  return GMI_OK;
+}
+
+int gmi_set_launch_shape(gmi_ctx *c, int target_blocks, int min_chunk);
+
+// Launch-shape autotuner: times the grand apply (all four blocks) on the
+// context's current device data for each candidate (target workgroups,
+// minimum source chunk) and keeps the fastest.  Results are cached per
+// problem shape for the life of the process.
+namespace {
+struct TuneKey {
+  int Ns, Nb, nsw, interact, precision;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(Ns, Nb, nsw, interact, precision) <
+           std::tie(o.Ns, o.Nb, o.nsw, o.interact, o.precision);
+  }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+}  // namespace
+
+int gmi_autotune(gmi_ctx *c, int reps, int *best_target, int *best_chunk) {
+  if (!c || reps < 1) return GMI_ERR_ARG;
+  const TuneKey key{c->Ns, c->Nb, c->nsw, int(c->interact), c->precision};
+  int rc;
+  auto it = g_tuned.find(key);
+  if (it == g_tuned.end()) {
+    const int targets[] = {512, 1024, 2048, 4096, 8192};
+    const int chunks[] = {256, 512, 1024, 2048};
+    double best = 1e300;
+    std::pair<int, int> arg{c->target_blocks, c->min_chunk};
+    hipEvent_t e0, e1;
+    if ((rc = hip_check(c, hipEventCreate(&e0), "event"))) return rc;
+    if ((rc = hip_check(c, hipEventCreate(&e1), "event"))) return rc;
+    const int mask = kSS | kSB | kBS | kBB;
+    const bool fp32 = c->precision == GMI_FP32_KAHAN;
+    for (int tb : targets)
+      for (int ch : chunks) {
+        if ((rc = gmi_set_launch_shape(c, tb, ch))) return rc;
+        if ((rc = run_pairs(c, mask, 0, fp32, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)))
+          return rc;
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; r++)
+          if ((rc = run_pairs(c, mask, 0, fp32, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr)))
+            return rc;
+        (void)hipEventRecord(e1, c->stream);
+        if ((rc = hip_check(c, hipEventSynchronize(e1), "tune sync"))) return rc;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; arg = {tb, ch}; }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    it = g_tuned.emplace(key, arg).first;
+  }
+  if ((rc = gmi_set_launch_shape(c, it->second.first, it->second.second))) return rc;
+  if (best_target) *best_target = it->second.first;
+  if (best_chunk) *best_chunk = it->second.second;
   return GMI_OK;
 }
 

@@ -1,0 +1,607 @@
+// gmi_learn.hip -- learned Broyden warm start (include/gmi_sim.h).
+//
+// A per-segment 3-layer MLP (nin -> H -> H -> 6, tanh hidden layers, BF16
+// weights, FP32 activations and accumulation) proposes, after the
+// reference's extrapolated initial guess (segment.cpp:38-79,
+// filament.cpp:908-1016), a correction to each segment's six Broyden
+// unknowns: the lambda-type triple (tether lambda for segment 0, else
+// lambda_{k-1}) and the Lie-type triple (clamp lambda for segment 0, else
+// u_k).  The plain Broyden/GMRES iteration still runs to the reference
+// tolerance afterwards, so the network can only change the iteration count.
+//
+// Training data are (features at the initial guess, converged - guess)
+// pairs recorded by gmi_step during plain runs.  Training runs on the GPU:
+// minibatch gather, forward, MSE, backward and Adam are kernels here; the
+// FP32 master weights are rounded to BF16 (round-to-nearest-even) after each
+// update and the forward pass reads only the BF16 copy.
+//
+// Features (nin = 27) per segment, lab frame, at the initial guess:
+//   x - x_base (3), tangent (3), x - x_{n-1} (3), x_{n-1} - x_{n-2} (3),
+//   u (3), u_{n-1} (3), dt v_prev (3), dt w_prev (3)  [velocities of the
+//   previous converged step], k/(NSEG-1), sin and cos of the driving phase.
+// Inputs and targets are standardised with statistics of the training set.
+#include "gmi_ctx.hpp"
+#include "gmi_lie.hpp"
+#include "gmi_sim_state.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace gmi {
+
+constexpr int kNin = 27;
+constexpr int kNout = 6;
+
+struct Learn {
+  int nin = kNin, nh = 128, nout = kNout;
+  int np = 0;                                  // parameter count
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // FP32 master, grad, Adam
+  uint16_t *pq = nullptr;                      // BF16 copy
+  float *xs = nullptr;                         // [mean(nin), std(nin), mean(nout), std(nout)]
+  // dataset
+  float *DX = nullptr, *DT = nullptr;
+  int cap = 0, count = 0, stride = 1;
+  bool collect = false, apply = false, trained = false;
+  // per-step buffers
+  float *FX = nullptr, *FY = nullptr;          // [Ns][nin], [Ns][nout]
+  double *G = nullptr;                         // guess snapshot [Ns][6]
+  double *corr = nullptr;                      // update vector [nbroy]
+  // training / test buffers (bmax rows)
+  int bmax = 0;
+  float *Xb = nullptr, *Tb = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr;
+  float *dY = nullptr, *dH1 = nullptr, *dH2 = nullptr, *loss = nullptr;
+  int *idx = nullptr;
+  int adam_t = 0;
+  std::vector<void *> owned;
+  // offsets of the parameter blocks
+  int oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+};
+
+namespace {
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return uint16_t((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) {
+  const uint32_t u = uint32_t(h) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// ---- dense layers --------------------------------------------------------
+// H[i, n] = act(sum_k A[i, k] W[k, n] + b[n]),  W row-major K x N in BF16.
+__global__ void k_dense_fwd(const float *A, int B, int K, const uint16_t *W, const float *b, int N,
+                            int act, float *H) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B * N) return;
+  const int i = e / N, n = e % N;
+  float s = b[n];
+  const float *a = A + size_t(i) * K;
+  for (int k = 0; k < K; k++) s = fmaf(a[k], bf2f(W[size_t(k) * N + n]), s);
+  H[e] = act ? tanhf(s) : s;
+}
+
+// dA[i, k] = (sum_n dZ[i, n] W[k, n]) * (1 - Hp[i, k]^2)
+__global__ void k_dense_bwd_data(const float *dZ, int B, int N, const uint16_t *W, int K,
+                                 const float *Hp, float *dA) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B * K) return;
+  const int i = e / K, k = e % K;
+  float s = 0.f;
+  const float *dz = dZ + size_t(i) * N;
+  const uint16_t *w = W + size_t(k) * N;
+  for (int n = 0; n < N; n++) s = fmaf(dz[n], bf2f(w[n]), s);
+  const float h = Hp[e];
+  dA[e] = s * (1.f - h * h);
+}
+
+// dW[k, n] = sum_i A[i, k] dZ[i, n];  db[n] = sum_i dZ[i, n]  (fixed order)
+__global__ void k_dense_bwd_weight(const float *A, const float *dZ, int B, int K, int N, float *dW,
+                                   float *db) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < K * N) {
+    const int k = e / N, n = e % N;
+    float s = 0.f;
+    for (int i = 0; i < B; i++) s = fmaf(A[size_t(i) * K + k], dZ[size_t(i) * N + n], s);
+    dW[e] = s;
+  } else if (e < K * N + N) {
+    const int n = e - K * N;
+    float s = 0.f;
+    for (int i = 0; i < B; i++) s += dZ[size_t(i) * N + n];
+    db[n] = s;
+  }
+}
+
+// dY = 2 (Y - T) / (B N); loss = mean (Y - T)^2 (one workgroup)
+__global__ __launch_bounds__(256) void k_mse(const float *Y, const float *T, int BN, float *dY,
+                                             float *loss) {
+  __shared__ float sh[256];
+  float s = 0.f;
+  for (int e = threadIdx.x; e < BN; e += 256) {
+    const float d = Y[e] - T[e];
+    dY[e] = 2.f * d / float(BN);
+    s += d * d;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = sh[0] / float(BN);
+}
+
+__global__ void k_adam(float *w, const float *g, float *m, float *v, uint16_t *wq, int n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  const float mm = b1 * m[e] + (1.f - b1) * gg;
+  const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  const float ww = w[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+  w[e] = ww;
+  wq[e] = f2bf(ww);
+}
+
+__global__ void k_quantize(const float *w, uint16_t *wq, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) wq[e] = f2bf(w[e]);
+}
+
+__global__ void k_gather(const float *DX, const float *DT, const int *idx, int B, int nin,
+                         int nout, float *X, float *T) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B * (nin + nout)) return;
+  const int i = e / (nin + nout), c = e % (nin + nout);
+  const int r = idx[i];
+  if (c < nin) X[size_t(i) * nin + c] = DX[size_t(r) * nin + c];
+  else T[size_t(i) * nout + c - nin] = DT[size_t(r) * nout + c - nin];
+}
+
+// ---- simulator side -------------------------------------------------------
+
+__device__ __forceinline__ d3 rowv(const double *p) { return {p[0], p[1], p[2]}; }
+
+// raw features of every segment at the initial guess
+__global__ void k_features(SimDev a, double t, float *FX) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= a.Ns) return;
+  const int f = gidx / a.S, k = gidx % a.S;
+  const int ld = a.lds;
+  const d3 x = ld3(a.X, ld, gidx), xb = ld3(a.X, ld, f * a.S);
+  const d3 tg = qtangent(ld4(a.Q, ld, gidx));
+  const d3 xm1 = ld3(a.Xm1, ld, gidx), xm2 = ld3(a.Xm2, ld, gidx);
+  const d3 u = ld3(a.U, ld, gidx), um1 = ld3(a.Um1, ld, gidx);
+  const d3 v = {a.Vs[gidx], a.Vs[ld + gidx], a.Vs[2 * ld + gidx]};
+  const d3 w = {a.Vs[3 * ld + gidx], a.Vs[4 * ld + gidx], a.Vs[5 * ld + gidx]};
+  const double ph = 2.0 * 3.14159265358979323846 * t / a.cper + a.phase[f];
+  const d3 rows[8] = {x - xb, tg, x - xm1, xm1 - xm2, u, um1, a.dt * v, a.dt * w};
+  float *o = FX + size_t(gidx) * kNin;
+  for (int r = 0; r < 8; r++) {
+    o[3 * r] = float(rows[r].x);
+    o[3 * r + 1] = float(rows[r].y);
+    o[3 * r + 2] = float(rows[r].z);
+  }
+  o[24] = float(a.S > 1 ? double(k) / (a.S - 1) : 0.0);
+  o[25] = float(sin(ph));
+  o[26] = float(cos(ph));
+}
+
+// the six Broyden unknowns of each segment, current values
+__device__ void unknowns(const SimDev &a, int gidx, double *o) {
+  const int f = gidx / a.S, k = gidx % a.S;
+  const double *lam = (k == 0) ? a.teth + 9 * f : a.lam + size_t(f) * 9 * a.S + 3 * (k - 1);
+  for (int c = 0; c < 3; c++) o[c] = lam[c];
+  if (k == 0) {
+    for (int c = 0; c < 3; c++) o[3 + c] = a.clamp[9 * f + c];
+  } else {
+    const d3 u = ld3(a.U, a.lds, gidx);
+    o[3] = u.x; o[4] = u.y; o[5] = u.z;
+  }
+}
+
+__global__ void k_snapshot(SimDev a, double *G) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= a.Ns) return;
+  unknowns(a, gidx, G + 6 * size_t(gidx));
+}
+
+// append (features, converged - guess) rows to the dataset
+__global__ void k_record(SimDev a, const double *G, const float *FX, float *DX, float *DT,
+                         int row0, int cap) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= a.Ns) return;
+  const int r = row0 + gidx;
+  if (r >= cap) return;
+  double c[6];
+  unknowns(a, gidx, c);
+  for (int q = 0; q < 6; q++) DT[size_t(r) * kNout + q] = float(c[q] - G[6 * size_t(gidx) + q]);
+  for (int q = 0; q < kNin; q++) DX[size_t(r) * kNin + q] = FX[size_t(gidx) * kNin + q];
+}
+
+__global__ void k_standardise(float *X, int rows, int n, const float *mean, const float *sd) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * n) return;
+  const int c = e % n;
+  X[e] = (X[e] - mean[c]) / sd[c];
+}
+
+// correction vector in the Broyden layout (body rows zero)
+__global__ void k_correction(SimDev a, const float *Y, const float *ym, const float *ysd,
+                             double *corr) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx < a.Ns) {
+    const int f = gidx / a.S, k = gidx % a.S;
+    const int off = 6 * a.S * f + 6 * a.fil_sw[f];
+    for (int c = 0; c < 3; c++) {
+      corr[off + 3 * k + c] = double(ym[c] + ysd[c] * Y[size_t(gidx) * kNout + c]);
+      corr[off + 3 * a.S + 3 * k + c] =
+          double(ym[3 + c] + ysd[3 + c] * Y[size_t(gidx) * kNout + 3 + c]);
+    }
+  } else if (gidx < a.Ns + a.Nsw) {
+    const int n = gidx - a.Ns;
+    const int off = 6 * a.S * (a.sw_fil0[n] + a.sw_nfil[n]) + 6 * n;
+    for (int c = 0; c < 6; c++) corr[off + c] = 0.0;
+  }
+}
+
+inline int nb(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// ---- host -----------------------------------------------------------------
+
+void learn_release(SimHost *s) {
+  Learn *L = s->learn;
+  if (!L) return;
+  for (void *q : L->owned) (void)hipFree(q);
+  delete L;
+  s->learn = nullptr;
+}
+
+template <typename T>
+static int lalloc(gmi_ctx *c, Learn *L, T **p, size_t n) {
+  void *q = nullptr;
+  int rc = hip_check(c, hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)), "hipMalloc learn");
+  if (rc) return rc;
+  L->owned.push_back(q);
+  *p = static_cast<T *>(q);
+  return hip_check(c, hipMemsetAsync(q, 0, sizeof(T) * std::max<size_t>(n, 1), c->stream),
+                   "memset learn");
+}
+
+static int ensure_batch(gmi_ctx *c, Learn *L, int B) {
+  if (B <= L->bmax) return GMI_OK;
+  int rc;
+  const size_t b = size_t(B);
+  if ((rc = lalloc(c, L, &L->Xb, b * L->nin))) return rc;
+  if ((rc = lalloc(c, L, &L->Tb, b * L->nout))) return rc;
+  if ((rc = lalloc(c, L, &L->H1, b * L->nh))) return rc;
+  if ((rc = lalloc(c, L, &L->H2, b * L->nh))) return rc;
+  if ((rc = lalloc(c, L, &L->Y, b * L->nout))) return rc;
+  if ((rc = lalloc(c, L, &L->dY, b * L->nout))) return rc;
+  if ((rc = lalloc(c, L, &L->dH1, b * L->nh))) return rc;
+  if ((rc = lalloc(c, L, &L->dH2, b * L->nh))) return rc;
+  if ((rc = lalloc(c, L, &L->idx, b))) return rc;
+  L->bmax = B;
+  return GMI_OK;
+}
+
+// forward of B rows of Xin (standardised) -> L->Y; keeps H1, H2
+static int mlp_forward(gmi_ctx *c, Learn *L, const float *Xin, int B, float *Yout) {
+  const int nh = L->nh;
+  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, Xin, B, L->nin,
+                     L->pq + L->oW1, L->p + L->ob1, nh, 1, L->H1);
+  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->H1, B, nh,
+                     L->pq + L->oW2, L->p + L->ob2, nh, 1, L->H2);
+  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * L->nout, 256)), dim3(256), 0, c->stream, L->H2, B,
+                     nh, L->pq + L->oW3, L->p + L->ob3, L->nout, 0, Yout);
+  return hip_check(c, hipGetLastError(), "mlp forward");
+}
+
+// loss + gradients into L->g for B rows (Xin, T standardised)
+static int mlp_backward(gmi_ctx *c, Learn *L, const float *Xin, const float *T, int B) {
+  const int nh = L->nh, no = L->nout, ni = L->nin;
+  int rc;
+  if ((rc = mlp_forward(c, L, Xin, B, L->Y))) return rc;
+  hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, c->stream, L->Y, T, B * no, L->dY, L->loss);
+  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(nh * no + no, 256)), dim3(256), 0, c->stream,
+                     L->H2, L->dY, B, nh, no, L->g + L->oW3, L->g + L->ob3);
+  hipLaunchKernelGGL(k_dense_bwd_data, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->dY, B,
+                     no, L->pq + L->oW3, nh, L->H2, L->dH2);
+  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(nh * nh + nh, 256)), dim3(256), 0, c->stream,
+                     L->H1, L->dH2, B, nh, nh, L->g + L->oW2, L->g + L->ob2);
+  hipLaunchKernelGGL(k_dense_bwd_data, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->dH2, B,
+                     nh, L->pq + L->oW2, nh, L->H1, L->dH1);
+  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(ni * nh + nh, 256)), dim3(256), 0, c->stream,
+                     Xin, L->dH1, B, ni, nh, L->g + L->oW1, L->g + L->ob1);
+  return hip_check(c, hipGetLastError(), "mlp backward");
+}
+
+// called by gmi_step after the initial guess: features, guess snapshot and,
+// when enabled, the network's correction
+int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied) {
+  Learn *L = s->learn;
+  *applied = false;
+  if (!L || (!L->collect && !(L->apply && L->trained)) || nt < 1) return GMI_OK;
+  SimDev &d = s->d;
+  const double t = (nt + 1) * s->p.dt;
+  hipLaunchKernelGGL(k_features, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, t, L->FX);
+  hipLaunchKernelGGL(k_snapshot, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G);
+  int rc;
+  if ((rc = hip_check(c, hipGetLastError(), "features"))) return rc;
+  if (L->apply && L->trained) {
+    if ((rc = ensure_batch(c, L, d.Ns))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(L->Xb, L->FX, sizeof(float) * d.Ns * L->nin,
+                                          hipMemcpyDeviceToDevice, c->stream), "copy features")))
+      return rc;
+    hipLaunchKernelGGL(k_standardise, dim3(nb(d.Ns * L->nin, 256)), dim3(256), 0, c->stream,
+                       L->Xb, d.Ns, L->nin, L->xs, L->xs + L->nin);
+    if ((rc = mlp_forward(c, L, L->Xb, d.Ns, L->FY))) return rc;
+    hipLaunchKernelGGL(k_correction, dim3(nb(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d,
+                       L->FY, L->xs + 2 * L->nin, L->xs + 2 * L->nin + L->nout, L->corr);
+    if ((rc = hip_check(c, hipGetLastError(), "correction"))) return rc;
+    *applied = true;
+  }
+  return GMI_OK;
+}
+
+double *learn_correction(SimHost *s) { return s->learn ? s->learn->corr : nullptr; }
+
+int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt) {
+  Learn *L = s->learn;
+  if (!L || !L->collect || nt < 1 || L->count >= L->cap || nt % L->stride) return GMI_OK;
+  SimDev &d = s->d;
+  hipLaunchKernelGGL(k_record, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G, L->FX, L->DX,
+                     L->DT, L->count, L->cap);
+  L->count = std::min(L->cap, L->count + d.Ns);
+  return hip_check(c, hipGetLastError(), "record");
+}
+
+}  // namespace gmi
+
+using namespace gmi;
+
+static Learn *learn_of(gmi_ctx *c) {
+  if (!c || !c->extra) return nullptr;
+  return static_cast<SimHost *>(c->extra)->learn;
+}
+
+extern "C" {
+
+int gmi_warmstart_init(gmi_ctx *c, int hidden, unsigned long long seed) {
+  if (!c || !c->extra) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (hidden < 1 || hidden > 1024) return fail(c, GMI_ERR_ARG, "hidden width out of range");
+  SimHost *s = static_cast<SimHost *>(c->extra);
+  if (s->learn) learn_release(s);
+  Learn *L = new Learn();
+  s->learn = L;
+  L->nh = hidden;
+  const int ni = L->nin, nh = L->nh, no = L->nout;
+  L->oW1 = 0; L->ob1 = ni * nh; L->oW2 = L->ob1 + nh; L->ob2 = L->oW2 + nh * nh;
+  L->oW3 = L->ob2 + nh; L->ob3 = L->oW3 + nh * no; L->np = L->ob3 + no;
+  int rc;
+  if ((rc = lalloc(c, L, &L->p, L->np))) return rc;
+  if ((rc = lalloc(c, L, &L->g, L->np))) return rc;
+  if ((rc = lalloc(c, L, &L->m, L->np))) return rc;
+  if ((rc = lalloc(c, L, &L->v, L->np))) return rc;
+  if ((rc = lalloc(c, L, &L->pq, L->np))) return rc;
+  if ((rc = lalloc(c, L, &L->xs, 2 * (ni + no)))) return rc;
+  if ((rc = lalloc(c, L, &L->FX, size_t(s->d.Ns) * ni))) return rc;
+  if ((rc = lalloc(c, L, &L->FY, size_t(s->d.Ns) * no))) return rc;
+  if ((rc = lalloc(c, L, &L->G, size_t(s->d.Ns) * 6))) return rc;
+  if ((rc = lalloc(c, L, &L->corr, size_t(s->d.nbroy)))) return rc;
+  if ((rc = lalloc(c, L, &L->loss, 1))) return rc;
+  // Glorot-uniform weights from splitmix64, zero biases; unit statistics
+  std::vector<float> P(L->np, 0.f);
+  uint64_t st = seed;
+  auto u01 = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto glorot = [&](int off, int fin, int fout) {
+    const double a = std::sqrt(6.0 / (fin + fout));
+    for (int k = 0; k < fin * fout; k++) P[off + k] = float((2.0 * u01() - 1.0) * a);
+  };
+  glorot(L->oW1, ni, nh);
+  glorot(L->oW2, nh, nh);
+  glorot(L->oW3, nh, no);
+  std::vector<float> stats(2 * (ni + no));
+  for (int k = 0; k < ni; k++) { stats[k] = 0.f; stats[ni + k] = 1.f; }
+  for (int k = 0; k < no; k++) { stats[2 * ni + k] = 0.f; stats[2 * ni + no + k] = 1.f; }
+  if ((rc = hip_check(c, hipMemcpy(L->p, P.data(), sizeof(float) * L->np, hipMemcpyHostToDevice), "params")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpy(L->xs, stats.data(), sizeof(float) * stats.size(),
+                                   hipMemcpyHostToDevice), "stats")))
+    return rc;
+  hipLaunchKernelGGL(k_quantize, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->pq, L->np);
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int gmi_warmstart_param_count(gmi_ctx *c, int *n) {
+  Learn *L = learn_of(c);
+  if (!L || !n) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  *n = L->np;
+  return GMI_OK;
+}
+
+int gmi_warmstart_params(gmi_ctx *c, float *get, const float *set) {
+  Learn *L = learn_of(c);
+  if (!L) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  int rc;
+  if (set) {
+    if ((rc = hip_check(c, hipMemcpy(L->p, set, sizeof(float) * L->np, hipMemcpyHostToDevice), "set")))
+      return rc;
+    hipLaunchKernelGGL(k_quantize, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->pq, L->np);
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  }
+  if (get)
+    return hip_check(c, hipMemcpy(get, L->p, sizeof(float) * L->np, hipMemcpyDeviceToHost), "get");
+  return GMI_OK;
+}
+
+int gmi_warmstart_configure(gmi_ctx *c, int apply, int collect_capacity, int collect_stride) {
+  Learn *L = learn_of(c);
+  if (!L) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  if (collect_capacity < 0 || collect_stride < 1)
+    return fail(c, GMI_ERR_ARG, "negative capacity or stride < 1");
+  L->stride = collect_stride;
+  int rc;
+  if (collect_capacity > 0 && collect_capacity != L->cap) {
+    if ((rc = lalloc(c, L, &L->DX, size_t(collect_capacity) * L->nin))) return rc;
+    if ((rc = lalloc(c, L, &L->DT, size_t(collect_capacity) * L->nout))) return rc;
+    L->cap = collect_capacity;
+    L->count = 0;
+  }
+  L->collect = collect_capacity > 0;
+  L->apply = apply != 0;
+  return GMI_OK;
+}
+
+int gmi_warmstart_dataset_size(gmi_ctx *c, int *rows) {
+  Learn *L = learn_of(c);
+  if (!L || !rows) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  *rows = L->count;
+  return GMI_OK;
+}
+
+int gmi_warmstart_forward(gmi_ctx *c, const float *X, int B, float *Y) {
+  Learn *L = learn_of(c);
+  if (!L || !X || !Y || B < 1) return c ? fail(c, GMI_ERR_ARG, "bad arguments") : GMI_ERR_ARG;
+  int rc;
+  if ((rc = ensure_batch(c, L, B))) return rc;
+  if ((rc = hip_check(c, hipMemcpy(L->Xb, X, sizeof(float) * B * L->nin, hipMemcpyHostToDevice), "X")))
+    return rc;
+  if ((rc = mlp_forward(c, L, L->Xb, B, L->Y))) return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  return hip_check(c, hipMemcpy(Y, L->Y, sizeof(float) * B * L->nout, hipMemcpyDeviceToHost), "Y");
+}
+
+int gmi_warmstart_loss_grad(gmi_ctx *c, const float *X, const float *T, int B, double *loss,
+                            float *grad) {
+  Learn *L = learn_of(c);
+  if (!L || !X || !T || B < 1) return c ? fail(c, GMI_ERR_ARG, "bad arguments") : GMI_ERR_ARG;
+  int rc;
+  if ((rc = ensure_batch(c, L, B))) return rc;
+  if ((rc = hip_check(c, hipMemcpy(L->Xb, X, sizeof(float) * B * L->nin, hipMemcpyHostToDevice), "X")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpy(L->Tb, T, sizeof(float) * B * L->nout, hipMemcpyHostToDevice), "T")))
+    return rc;
+  if ((rc = mlp_backward(c, L, L->Xb, L->Tb, B))) return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  float lf = 0.f;
+  if ((rc = hip_check(c, hipMemcpy(&lf, L->loss, sizeof(float), hipMemcpyDeviceToHost), "loss")))
+    return rc;
+  if (loss) *loss = lf;
+  if (grad)
+    return hip_check(c, hipMemcpy(grad, L->g, sizeof(float) * L->np, hipMemcpyDeviceToHost), "grad");
+  return GMI_OK;
+}
+
+int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned long long seed,
+                        double *final_loss) {
+  Learn *L = learn_of(c);
+  if (!L) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  if (L->count < 2 || iters < 1 || batch < 1 || !(lr > 0))
+    return fail(c, GMI_ERR_ARG, "need data, iters >= 1, batch >= 1, lr > 0");
+  int rc;
+  const int ni = L->nin, no = L->nout, N = L->count;
+  // dataset statistics (host, FP64), then standardise the dataset in place
+  std::vector<float> X(size_t(N) * ni), T(size_t(N) * no);
+  if ((rc = hip_check(c, hipMemcpy(X.data(), L->DX, sizeof(float) * X.size(), hipMemcpyDeviceToHost), "DX")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpy(T.data(), L->DT, sizeof(float) * T.size(), hipMemcpyDeviceToHost), "DT")))
+    return rc;
+  std::vector<float> stats(2 * (ni + no));
+  auto moments = [&](const std::vector<float> &A, int cols, float *mean, float *sd) {
+    for (int q = 0; q < cols; q++) {
+      double s = 0, s2 = 0;
+      for (int r = 0; r < N; r++) s += A[size_t(r) * cols + q];
+      const double mu = s / N;
+      for (int r = 0; r < N; r++) {
+        const double dd = A[size_t(r) * cols + q] - mu;
+        s2 += dd * dd;
+      }
+      const double sdv = std::sqrt(s2 / N);
+      mean[q] = float(mu);
+      sd[q] = float(sdv > 1e-12 ? sdv : 1.0);
+    }
+  };
+  moments(X, ni, &stats[0], &stats[ni]);
+  moments(T, no, &stats[2 * ni], &stats[2 * ni + no]);
+  if ((rc = hip_check(c, hipMemcpy(L->xs, stats.data(), sizeof(float) * stats.size(),
+                                   hipMemcpyHostToDevice), "stats")))
+    return rc;
+  float *SX = nullptr, *ST = nullptr;
+  if ((rc = lalloc(c, L, &SX, size_t(N) * ni))) return rc;
+  if ((rc = lalloc(c, L, &ST, size_t(N) * no))) return rc;
+  hipMemcpyAsync(SX, L->DX, sizeof(float) * N * ni, hipMemcpyDeviceToDevice, c->stream);
+  hipMemcpyAsync(ST, L->DT, sizeof(float) * N * no, hipMemcpyDeviceToDevice, c->stream);
+  hipLaunchKernelGGL(k_standardise, dim3(nb(N * ni, 256)), dim3(256), 0, c->stream, SX, N, ni,
+                     L->xs, L->xs + ni);
+  hipLaunchKernelGGL(k_standardise, dim3(nb(N * no, 256)), dim3(256), 0, c->stream, ST, N, no,
+                     L->xs + 2 * ni, L->xs + 2 * ni + no);
+  if ((rc = ensure_batch(c, L, batch))) return rc;
+  int *hidx = nullptr;
+  if ((rc = hip_check(c, hipHostMalloc((void **)&hidx, sizeof(int) * batch * 2, hipHostMallocDefault),
+                      "pinned idx")))
+    return rc;
+  uint64_t st = seed;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double lsum = 0.0;
+  int lcount = 0;
+  hipEvent_t ev;
+  if ((rc = hip_check(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event"))) return rc;
+  (void)hipEventRecord(ev, c->stream);
+  for (int it = 0; it < iters; it++) {
+    int *h = hidx + (it & 1) * batch;
+    (void)hipEventSynchronize(ev);  // the pinned half written two iterations ago is consumed
+    for (int i = 0; i < batch; i++) h[i] = int(next() % uint64_t(N));
+    hipMemcpyAsync(L->idx, h, sizeof(int) * batch, hipMemcpyHostToDevice, c->stream);
+    hipLaunchKernelGGL(k_gather, dim3(nb(batch * (ni + no), 256)), dim3(256), 0, c->stream, SX, ST,
+                       L->idx, batch, ni, no, L->Xb, L->Tb);
+    if ((rc = mlp_backward(c, L, L->Xb, L->Tb, batch))) break;
+    L->adam_t++;
+    const float c1 = 1.f - std::pow(b1, float(L->adam_t)), c2 = 1.f - std::pow(b2, float(L->adam_t));
+    hipLaunchKernelGGL(k_adam, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->g, L->m,
+                       L->v, L->pq, L->np, float(lr), b1, b2, eps, c1, c2);
+    (void)hipEventRecord(ev, c->stream);
+    if (it >= iters - 20) {
+      float lf = 0.f;
+      hipMemcpyAsync(&lf, L->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream);
+      (void)hipStreamSynchronize(c->stream);
+      lsum += lf;
+      lcount++;
+    }
+  }
+  (void)hipEventDestroy(ev);
+  (void)hipHostFree(hidx);
+  if (rc) return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  L->trained = true;
+  if (final_loss) *final_loss = lcount ? lsum / lcount : 0.0;
+  return GMI_OK;
+}
+
+}  // extern "C"

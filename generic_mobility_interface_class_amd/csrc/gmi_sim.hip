@@ -19,7 +19,7 @@
 #include "gmi_internal.hpp"
 #include "gmi_ctx.hpp"
 #include "gmi_lie.hpp"
-#include "../../include/gmi_sim.h"
+#include "gmi_sim_state.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -30,38 +30,6 @@ namespace gmi {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
-
-// body record (per swimmer), doubles
-enum {
-  BX = 0, BXM1 = 3, BXM2 = 6, BU = 9, BUM1 = 12, BQ = 15, BQM1 = 19, BQI = 23,
-  BF = 32, BN = 41, BK = 77, BS = 113, BREC = 152
-};
-// device flags
-enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
-       F_N = 8 };
-
-struct SimDev {
-  // sizes
-  int S, Nf, Nsw, Ns, Nb, lds, ldb, nbroy, npad, maxit;
-  double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol, R;
-  int neuler;
-  double camp, cper;
-  // index maps
-  const int *sw_fil0, *sw_nfil, *sw_blob0, *sw_nblob, *fil_sw, *blob_sw;
-  // segment SoA
-  double *X, *Xm1, *Xm2, *Q, *Qm1, *U, *Um1, *Fint, *Fs, *Vs;
-  // filament
-  double *fil_ref, *phase, *lam, *lamm1, *lamm2;  // lam: [Nf][3S]
-  double *teth, *clamp;                           // [Nf][9]: value, m1, m2
-  double *fil_bf;                                 // [Nf][6]
-  // body
-  double *body;                                   // [Nsw][BREC]
-  double *blob_ref;                               // SoA [3][ldb]
-  double *Xb, *Vb, *Frep;
-  // Jacobian
-  double *Jw, *Jinv, *ecb, *Bblk, *fil_BA, *fil_b2;
-  int *flags;
-};
 
 __device__ __forceinline__ d3 rowvec(const double *p) { return {p[0], p[1], p[2]}; }
 
@@ -1196,30 +1164,18 @@ constexpr double kFp32OperatorError = 1e-6;
 // ==========================================================================
 // host side
 
-struct SimHost {
-  SimDev d;
-  gmi_sim_params p;
-  std::vector<int> nfil;
-  // device buffers owned
-  std::vector<void *> owned;
-  double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
-         *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
-         *tmp = nullptr, *scal = nullptr;
-  double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
-         *bd = nullptr;
-  int *h_flags = nullptr;
-  double *h_scal = nullptr;
-  int iter = 0;
-  int precision = GMI_FP64;
-  int last_gmres = 0;
-  int chunk = 6;
-};
-
 static SimHost *sim_of(gmi_ctx *c) { return static_cast<SimHost *>(c->extra); }
+
+static void drop_graphs(SimHost *s) {
+  for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+  s->graphs.clear();
+}
 
 static void sim_release(gmi_ctx *c) {
   SimHost *s = sim_of(c);
   if (!s) return;
+  drop_graphs(s);
+  learn_release(s);
   for (void *p : s->owned) (void)hipFree(p);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
   if (s->h_scal) (void)hipHostFree(s->h_scal);
@@ -1342,15 +1298,15 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
   // scal[0] = ||rhs|| is read by k_norm_givens
   const int ldh = s->d.maxit + 1;
-  int it = 1;
-  int done = 0;
-  while (!done) {
-    const int last = std::min(maxit, it + s->chunk - 1);
-    for (; it <= last; it++) {
+  // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
+  // says the solve has converged, so a chunk needs no host decisions
+  auto enqueue = [&](int it0, int it1) -> int {
+    int r;
+    for (int it = it0; it <= it1; it++) {
       double *qprev = s->Qk + ldq * (it - 1);
       double *w = s->Qk + ldq * it;
-      if ((rc = precond(c, s, qprev, s->z, d.flags))) return rc;
-      if ((rc = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags))) return rc;
+      if ((r = precond(c, s, qprev, s->z, d.flags))) return r;
+      if ((r = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags))) return r;
       hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h1,
                          d.flags);
       hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w, n,
@@ -1362,8 +1318,43 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
       hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(1024), 0, c->stream, w, n, it, s->h1, s->h2,
                          s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
                          switch_rel);
-      if ((rc = launch_ok(c, "arnoldi"))) return rc;
+      if ((r = launch_ok(c, "arnoldi"))) return r;
     }
+    return GMI_OK;
+  };
+  if (s->graph_gen != c->list_generation) {
+    drop_graphs(s);
+    s->graph_gen = c->list_generation;
+  }
+  int it = 1;
+  int done = 0;
+  while (!done) {
+    const int last = std::min(maxit, it + s->chunk - 1);
+    if (s->use_graphs) {
+      // one captured graph per (precision mode, chunk start); replayed by
+      // every later solve of this context
+      const std::pair<int, int> key{mixed ? 1 : 0, it};
+      auto g = s->graphs.find(key);
+      if (g == s->graphs.end()) {
+        hipGraph_t graph;
+        if ((rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                            "begin capture")))
+          return rc;
+        const int erc = enqueue(it, last);
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (erc) return erc;
+        if ((rc = hip_check(c, ec, "end capture"))) return rc;
+        hipGraphExec_t exec;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if ((rc = hip_check(c, ei, "instantiate"))) return rc;
+        g = s->graphs.emplace(key, exec).first;
+      }
+      if ((rc = hip_check(c, hipGraphLaunch(g->second, c->stream), "graph launch"))) return rc;
+    } else {
+      if ((rc = enqueue(it, last))) return rc;
+    }
+    it = last + 1;
     if ((rc = read_flags(c, s))) return rc;
     done = s->h_flags[F_DONE];
     if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
@@ -1619,6 +1610,18 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
+int gmi_sim_set_graphs(gmi_ctx *c, int use_graphs, int chunk) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (chunk < 1 || chunk > 64) return fail(c, GMI_ERR_ARG, "chunk must be in [1, 64]");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  drop_graphs(s);
+  s->use_graphs = use_graphs != 0;
+  s->chunk = chunk;
+  return GMI_OK;
+}
+
 int gmi_sim_set_precision(gmi_ctx *c, int precision) {
   if (!c || !sim_of(c)) return GMI_ERR_STATE;
   if (precision != GMI_FP64 && precision != GMI_FP32_KAHAN)
@@ -1804,6 +1807,16 @@ int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   int rc, big = 0, bad = 0, its = 0, total = 0;
   if ((rc = gmi_initial_guess(c, nt))) return rc;
+  bool corrected = false;
+  if ((rc = learn_after_initial_guess(c, s, nt, &corrected))) return rc;
+  if (corrected) {
+    // network proposal: the extrapolated iterate plus the predicted
+    // correction, applied through the ordinary state update
+    double *corr = learn_correction(s);
+    hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, corr);
+    hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, corr);
+    if ((rc = launch_ok(c, "warm start"))) return rc;
+  }
   if ((rc = gmi_forces_and_torques(c, nt))) return rc;
   if ((rc = compute_velocities(c, s, &its))) return rc;
   total += its;
@@ -1827,7 +1840,7 @@ int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
   if (broyden_iters) *broyden_iters = s->iter;
   if (gmres_iters) *gmres_iters = total;
   if (big) return fail(c, GMI_ERR_NOT_CONVERGED, "Broyden's method did not converge");
-  return GMI_OK;
+  return learn_after_convergence(c, s, nt);
 }
 
 int gmi_get_state(gmi_ctx *c, double *seg_x, double *seg_q, double *seg_u, double *fil_lam,

@@ -1,0 +1,76 @@
+// gmi_sim_state.hpp -- device state of the implicit time step (not public).
+#pragma once
+#include "gmi_internal.hpp"
+#include <map>
+#include <utility>
+#include "../../include/gmi_sim.h"
+
+namespace gmi {
+
+// body record (per swimmer), doubles
+enum {
+  BX = 0, BXM1 = 3, BXM2 = 6, BU = 9, BUM1 = 12, BQ = 15, BQM1 = 19, BQI = 23,
+  BF = 32, BN = 41, BK = 77, BS = 113, BREC = 152
+};
+// device flags
+enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
+       F_N = 8 };
+
+struct SimDev {
+  // sizes
+  int S, Nf, Nsw, Ns, Nb, lds, ldb, nbroy, npad, maxit;
+  double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol, R;
+  int neuler;
+  double camp, cper;
+  // index maps
+  const int *sw_fil0, *sw_nfil, *sw_blob0, *sw_nblob, *fil_sw, *blob_sw;
+  // segment SoA
+  double *X, *Xm1, *Xm2, *Q, *Qm1, *U, *Um1, *Fint, *Fs, *Vs;
+  // filament
+  double *fil_ref, *phase, *lam, *lamm1, *lamm2;  // lam: [Nf][3S]
+  double *teth, *clamp;                           // [Nf][9]: value, m1, m2
+  double *fil_bf;                                 // [Nf][6]
+  // body
+  double *body;                                   // [Nsw][BREC]
+  double *blob_ref;                               // SoA [3][ldb]
+  double *Xb, *Vb, *Frep;
+  // Jacobian
+  double *Jw, *Jinv, *ecb, *Bblk, *fil_BA, *fil_b2;
+  int *flags;
+};
+
+
+struct Learn;  // learned warm start (gmi_learn.hip)
+
+struct SimHost {
+  SimDev d;
+  gmi_sim_params p;
+  std::vector<int> nfil;
+  // device buffers owned
+  std::vector<void *> owned;
+  double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
+         *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
+         *tmp = nullptr, *scal = nullptr;
+  double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
+         *bd = nullptr;
+  int *h_flags = nullptr;
+  double *h_scal = nullptr;
+  int iter = 0;
+  int precision = GMI_FP64;
+  int last_gmres = 0;
+  int chunk = 6;
+  Learn *learn = nullptr;
+  // captured Arnoldi chunks, keyed by (mixed, first iteration)
+  bool use_graphs = true;
+  int graph_gen = -1;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+};
+
+
+// learned warm start hooks (gmi_learn.hip)
+void learn_release(SimHost *s);
+int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied);
+int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt);
+double *learn_correction(SimHost *s);
+
+}  // namespace gmi

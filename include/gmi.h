@@ -124,6 +124,10 @@ int gmi_profile_end(gmi_ctx *ctx, double *pair_ms, int *launches);
 /* Pair launches aim at `target_blocks` workgroups by splitting each target
  * tile's sources into chunks of at least `min_chunk` sources. */
 int gmi_set_launch_shape(gmi_ctx *ctx, int target_blocks, int min_chunk);
+/* Times the grand apply on the context's current device data for a grid of
+ * launch shapes (reps launches each), applies the fastest and reports it.
+ * The choice is cached per (sizes, precision) for the process. */
+int gmi_autotune(gmi_ctx *ctx, int reps, int *best_target_blocks, int *best_min_chunk);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,39 @@ int gmi_get_broyden(gmi_ctx *ctx, int which, double *out);
  * FP32 pair kernels inside FP64 iterative refinement (the outer residual
  * uses the FP64 kernel; GMRES tolerance is met in FP64). */
 int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
+/* GMRES runs its Arnoldi steps in chunks of `chunk` iterations between host
+ * convergence checks; with use_graphs each chunk is captured once as a HIP
+ * graph and replayed (no reference counterpart).  Default: graphs on, 6. */
+int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
+
+/* ---- learned warm start (no reference counterpart: the reference's warm
+ * starts are the extrapolations of segment.cpp:53-73, filament.cpp:908-1016,
+ * rigid_body.cpp:251-271).  A per-segment MLP (27 -> hidden -> hidden -> 6,
+ * tanh, BF16 weights, FP32 accumulation) corrects each segment's six
+ * extrapolated Broyden unknowns before the first residual of a step.
+ * Broyden/GMRES still converge to the same tolerances afterwards. */
+/* Creates the network with Glorot-uniform weights from `seed`. */
+int gmi_warmstart_init(gmi_ctx *ctx, int hidden, unsigned long long seed);
+/* apply: use the trained network in gmi_step (steps nt >= 1);
+ * collect_capacity > 0: record (features, converged - guess) rows of the
+ * converged steps nt >= 1 with nt % collect_stride == 0 into a dataset of
+ * that many rows (0: stop recording). */
+int gmi_warmstart_configure(gmi_ctx *ctx, int apply, int collect_capacity, int collect_stride);
+int gmi_warmstart_dataset_size(gmi_ctx *ctx, int *rows);
+/* Adam on the MSE of standardised targets; returns the mean loss of the
+ * last 20 iterations. */
+int gmi_warmstart_train(gmi_ctx *ctx, int iters, int batch, double lr,
+                        unsigned long long seed, double *final_loss);
+int gmi_warmstart_param_count(gmi_ctx *ctx, int *n);
+/* FP32 master parameters [W1, b1, W2, b2, W3, b3] (row-major, W as
+ * inputs x outputs); get and/or set (set re-rounds the BF16 copy). */
+int gmi_warmstart_params(gmi_ctx *ctx, float *get, const float *set);
+/* Forward pass on B standardised rows X[B x 27] -> Y[B x 6]. */
+int gmi_warmstart_forward(gmi_ctx *ctx, const float *X, int B, float *Y);
+/* MSE loss of B rows and its gradient w.r.t. the parameters (through the
+ * BF16 weights), grad[param_count]. */
+int gmi_warmstart_loss_grad(gmi_ctx *ctx, const float *X, const float *T, int B,
+                            double *loss, float *grad);
 
 #ifdef __cplusplus
 }
