@@ -2,7 +2,8 @@
 """Benchmark of the RPY grand-mobility hot path on one MI355X.
 
 Second part (sim_steps): implicit time steps of the 256-filament x 20-segment
-+ 4002-blob swimmer, plain FP64 against mixed precision, steps/s with the
++ 4002-blob swimmer after one driving period: plain FP64 against the learned
+warm start with mixed precision (and the two ingredients alone), steps/s with
 Broyden and GMRES iteration counts, and the CPU oracle's step rate.
 
 A step is one full grand-mobility apply (all four reference blocks:
@@ -90,47 +91,72 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
 
 
 def bench_step(args):
-    """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer:
-    plain FP64 vs mixed precision (FP32 pair kernels, FP64 refinement)."""
-    res = {}
-    states = {}
-    for name, prec in (("fp64", capi.GMI_FP64), ("mixed", capi.GMI_FP32_KAHAN)):
-        cfg = geometry.sim_workload(args.step_workload, seed=1)
-        sim = capi.Sim(cfg, precision=prec)
+    """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer.
+
+    One driving period (300 steps) of plain FP64 steps first, recording
+    every 37th step's (features, converged - guess) rows for the learned
+    warm start; then timed windows of `step_steps` steps each:
+      fp64            plain FP64 (the reference method)
+      learned_mixed   network warm start + mixed precision
+      learned_fp64    network warm start, FP64
+      mixed           mixed precision, extrapolation warm start
+    """
+    cfg = geometry.sim_workload(args.step_workload, seed=1)
+    sim = capi.Sim(cfg, precision=capi.GMI_FP64)
+    sim.set_sim_precision(capi.GMI_FP64)
+    sim.warmstart_init(hidden=128, seed=7)
+    stride = max(1, args.step_warmup // 8)
+    sim.warmstart_configure(apply=False, collect=40960, stride=stride)
+    sim.make_body_reference_matrices()
+    nt = 0
+    t0 = time.perf_counter()
+    for _ in range(args.step_warmup):
+        rc, bi, gi = sim.step(nt)
+        assert rc == 0, rc
+        nt += 1
+    warm_s = time.perf_counter() - t0
+    print(f"# {args.step_warmup} warm-up steps in {warm_s:.1f} s, "
+          f"{sim.warmstart_dataset_size()} training rows", file=sys.stderr, flush=True)
+    rows = sim.warmstart_dataset_size()
+    sim.warmstart_configure(apply=False, collect=0)
+
+    def window(name, prec, learned):
+        nonlocal nt
         sim.set_sim_precision(prec)
-        sim.make_body_reference_matrices()
-        nt = 0
-        for _ in range(args.step_warmup):
-            rc, bi, gi = sim.step(nt)
-            assert rc == 0, rc
-            nt += 1
+        sim.warmstart_configure(apply=learned, collect=0)
         sim.call("gmi_wait_for_device")
-        t0 = time.perf_counter()
         sim.call("gmi_timer_start")
         tb = tg = 0
         for _ in range(args.step_steps):
             rc, bi, gi = sim.step(nt)
-            assert rc == 0, rc
+            assert rc == 0, (name, nt, rc)
             tb += bi
             tg += gi
             nt += 1
         ms = capi.C.c_double(0.0)
         sim.call("gmi_timer_stop", capi.C.byref(ms))
-        wall = time.perf_counter() - t0
         k = args.step_steps
-        res[name] = dict(steps_per_s=k / (ms.value * 1e-3), ms_per_step=ms.value / k,
-                         wall_ms_per_step=wall * 1e3 / k,
-                         broyden_iters_per_step=tb / k, gmres_iters_per_step=tg / k)
-        states[name] = sim.state()
-        sim.close()
-        print(f"# step {name}: {ms.value / k:.2f} ms/step, {tb / k:.2f} Broyden, "
+        r = dict(steps=f"{nt - k}-{nt - 1}", steps_per_s=k / (ms.value * 1e-3),
+                 ms_per_step=ms.value / k, broyden_iters_per_step=tb / k,
+                 gmres_iters_per_step=tg / k)
+        print(f"# step {name}: {r['ms_per_step']:.2f} ms/step, {tb / k:.2f} Broyden, "
               f"{tg / k:.1f} GMRES it/step", file=sys.stderr, flush=True)
-    dx = float(np.abs(states["mixed"]["x"] - states["fp64"]["x"]).max())
-    out = {"workload": args.step_workload, "steps": args.step_steps, "warmup": args.step_warmup,
-           "fp64": res["fp64"], "mixed": res["mixed"],
-           "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"],
-           "mixed_vs_fp64_max_position_diff": dx}
-    return out
+        return r
+
+    res = {"fp64": window("fp64", capi.GMI_FP64, False)}
+    t0 = time.perf_counter()
+    loss = sim.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    res["learned_mixed"] = window("learned_mixed", capi.GMI_FP32_KAHAN, True)
+    res["learned_fp64"] = window("learned_fp64", capi.GMI_FP64, True)
+    res["mixed"] = window("mixed", capi.GMI_FP32_KAHAN, False)
+    sim.close()
+    return {"workload": args.step_workload, "warmup_steps": args.step_warmup,
+            "steps_per_window": args.step_steps, **res,
+            "speedup_learned_mixed_vs_fp64":
+                res["learned_mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"],
+            "warm_start": {"training_rows": rows, "train_iters": 3000, "train_s": train_s,
+                           "final_loss_standardised": loss}}
 
 
 def cpu_step_baseline(args):
@@ -158,8 +184,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--step-workload", default="step_9122")
-    ap.add_argument("--step-steps", type=int, default=10)
-    ap.add_argument("--step-warmup", type=int, default=2)
+    ap.add_argument("--step-steps", type=int, default=20)
+    ap.add_argument("--step-warmup", type=int, default=300)
     ap.add_argument("--no-step", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -177,10 +203,12 @@ def main():
     for name, prec in (("fp32_kahan", capi.GMI_FP32_KAHAN), ("fp64", capi.GMI_FP64)):
         ctx = capi.Context([g.ns], [g.nblob], g.nseg, precision=prec)
         ctx.upload(xs, fs, xb, fb)
+        shape = ctx.autotune(reps=2)
         steps = args.steps if name == "fp32_kahan" else max(3, args.steps // 2)
         total_ms, kern_ms = time_apply(ctx, steps, args.warmup)
         vel[name] = ctx.velocities()
-        res[name] = dict(steps=steps, ms_per_step=total_ms / steps, kernel_ms=kern_ms)
+        res[name] = dict(steps=steps, ms_per_step=total_ms / steps, kernel_ms=kern_ms,
+                         launch_shape=shape)
         ctx.close()
         print(f"# {name}: {total_ms / steps:.3f} ms/apply, pair kernel {kern_ms:.3f} ms",
               file=sys.stderr, flush=True)
@@ -224,6 +252,7 @@ def main():
             "traffic": None,
             "kernel": "k_pair<float>",
             "kernel_ms": r32["kernel_ms"],
+            "launch_shape": r32["launch_shape"],
         },
         "fp64": {
             "value": pairs / (r64["ms_per_step"] * 1e-3),

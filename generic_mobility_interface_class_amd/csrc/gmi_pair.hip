@@ -459,42 +459,61 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
 }
 
 // Steric barrier (cuda_functions.cu:961-1157).  One lane per target; the
-// sources of the target's group are read straight from global memory (all
-// lanes of a wave read the same address, served from cache).
+// target's group sources are staged in LDS tiles shared by the workgroup.
+// The contact test compares squared distances, so the square root and the
+// division run only for the few pairs inside 1.1 (a_i + a_j).
+__device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, double asum,
+                                             double K, double &fx, double &fy, double &fz) {
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  const double lim = 1.1 * asum;
+  if (__builtin_expect(d2 < lim * lim, 0)) {
+    const double dist = sqrt(d2);
+    double fac = fmin(1.0, 1.0 - (10.0 / asum) * (dist - asum));
+    fac = K * fac * fac * fac * fac;
+    const double dm1 = 1.0 / dist;
+    fx += fac * dx * dm1; fy += fac * dy * dm1; fz += fac * dz * dm1;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_barrier(
     double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
     const double *__restrict__ Xs, const double *__restrict__ Xb,
     const TileDesc *__restrict__ tiles, int nseg_fil, double as, double ab,
     double K) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile];
   const TileDesc T = tiles[blockIdx.x];
   const int tid = threadIdx.x;
-  if (tid >= T.nt) return;
-  const int i = T.t0 + tid;
   const bool seg_t = (T.species == 0);
+  const int i = T.t0 + (tid < T.nt ? tid : 0);
   const double *X = seg_t ? Xs : Xb;
   const int ld = seg_t ? lds : ldb;
   const double xi = X[i], yi = X[ld + i], zi = X[2 * ld + i];
   double fx = 0, fy = 0, fz = 0;
-  auto contrib = [&](double dx, double dy, double dz, double asum) {
-    const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-    if (dist < 1.1 * asum) {
-      double fac = fmin(1.0, 1.0 - (10.0 / asum) * (dist - asum));
-      fac = K * fac * fac * fac * fac;
-      const double dm1 = 1.0 / dist;
-      fx += fac * dx * dm1; fy += fac * dy * dm1; fz += fac * dz * dm1;
-    }
-  };
   const int fili = i / nseg_fil;
-  for (int j = T.sseg0; j < T.sseg0 + T.nsseg; ++j) {
-    if (seg_t && fili == j / nseg_fil && abs(i - j) <= 1) continue;
-    contrib(xi - Xs[j], yi - Xs[lds + j], zi - Xs[2 * lds + j],
-            seg_t ? 2.0 * as : as + ab);
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool src_seg = (pass == 0);
+    const double *S = src_seg ? Xs : Xb;
+    const int sld = src_seg ? lds : ldb;
+    const int j0 = src_seg ? T.sseg0 : T.sblob0;
+    const int j1 = j0 + (src_seg ? T.nsseg : T.nsblob);
+    const double asum = src_seg ? (seg_t ? 2.0 * as : as + ab) : (seg_t ? as + ab : 2.0 * ab);
+    for (int base = j0; base < j1; base += kTile) {
+      const int n = min(kTile, j1 - base);
+      __syncthreads();
+      if (tid < n) {
+        sx[tid] = S[base + tid]; sy[tid] = S[sld + base + tid]; sz[tid] = S[2 * sld + base + tid];
+      }
+      __syncthreads();
+      for (int jj = 0; jj < n; ++jj) {
+        const int j = base + jj;
+        // exclusions: adjacent segments of one filament (:1040), a blob itself (:1120)
+        if (src_seg && seg_t && fili == j / nseg_fil && abs(i - j) <= 1) continue;
+        if (!src_seg && !seg_t && j == i) continue;
+        barrier_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], asum, K, fx, fy, fz);
+      }
+    }
   }
-  for (int j = T.sblob0; j < T.sblob0 + T.nsblob; ++j) {
-    if (!seg_t && j == i) continue;
-    contrib(xi - Xb[j], yi - Xb[ldb + j], zi - Xb[2 * ldb + j],
-            seg_t ? as + ab : 2.0 * ab);
-  }
+  if (tid >= T.nt) return;
   if (seg_t) {
     Fs[i] += fx; Fs[lds + i] += fy; Fs[2 * lds + i] += fz;
   } else {

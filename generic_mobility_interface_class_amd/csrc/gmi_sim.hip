@@ -327,73 +327,113 @@ __global__ void k_assemble_rhs(SimDev a, double *rhs) {
   }
 }
 
-// right preconditioner (mobility_solver.cpp:651-727), workgroup per swimmer
-__global__ __launch_bounds__(256) void k_precond(SimDev a, const double *in, double *out,
-                                                 const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  __shared__ double sh[256];
-  __shared__ double ou[6];
-  const int n = blockIdx.x;
-  const double *b = a.body + size_t(n) * BREC;
+// Blob tiles: up to 256 consecutive blobs of one swimmer (tile table built
+// at creation), so the per-blob parts of the preconditioner and of the K
+// products run on many workgroups; per-swimmer sums are reduced in tile
+// order (deterministic).
+// part[t][0..6) = -sum over the tile's blobs of (v_m, d_m x v_m)  (-K^T v)
+__device__ void tile_kt(const SimDev &a, const BTile &T, const double *v, double *part,
+                        double *sh) {
+  const double *b = a.body + size_t(T.sw) * BREC;
   const m3 Qm = rotm(ga4(b + BQ));
-  const int m0 = a.sw_blob0[n], m1 = m0 + a.sw_nblob[n];
   const int ld = a.ldb;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-    const d3 v = ld3(in, ld, m);
-    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, ld, m)), v);
-    acc[0] -= v.x; acc[1] -= v.y; acc[2] -= v.z;
-    acc[3] -= c.x; acc[4] -= c.y; acc[5] -= c.z;
+  if (threadIdx.x < T.n) {
+    const int m = T.m0 + threadIdx.x;
+    const d3 l = ld3(v, ld, m);
+    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, ld, m)), l);
+    acc[0] = -l.x; acc[1] = -l.y; acc[2] = -l.z;
+    acc[3] = -c.x; acc[4] = -c.y; acc[5] = -c.z;
   }
-  double bb[6];
-  for (int k = 0; k < 6; k++) bb[k] = block_sum<256>(acc[k], sh);
-  const int vid = 3 * ld + 6 * n;
-  if (threadIdx.x == 0) {
-    double bm[36], kk[36];
-    rotate66(b + BN, Qm, bm);
-    rotate66(b + BK, Qm, kk);
-    for (int r = 0; r < 6; r++) {
-      double s1 = 0, s2 = 0;
-      for (int k = 0; k < 6; k++) { s1 += kk[6 * r + k] * bb[k]; s2 += bm[6 * r + k] * in[vid + k]; }
-      ou[r] = s1 - s2;
-    }
-    for (int r = 0; r < 6; r++) out[vid + r] = ou[r];
-  }
-  __syncthreads();
-  const double Minv = 6.0 * kPi * a.mu * a.ab;
-  const d3 V = {ou[0], ou[1], ou[2]}, W = {ou[3], ou[4], ou[5]};
-  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-    const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
-    st3(out, ld, m, Minv * (ld3(in, ld, m) + V + cross3(W, d)));
+  for (int k = 0; k < 6; k++) {
+    const double r = block_sum<256>(acc[k], sh);
+    if (threadIdx.x == 0) part[k] = r;
   }
 }
 
-// system_matrix_mult K products (mobility_solver.cpp:905-964):
-// out_lambda = M_bb z - (U + W x d), out_U = -K^T z_lambda
-__global__ __launch_bounds__(256) void k_kprod(SimDev a, const double *z, const double *vb,
-                                               double *out, const int *flags) {
+__global__ __launch_bounds__(256) void k_tile_kt(SimDev a, const BTile *tiles, const double *v,
+                                                 double *part, const int *flags) {
   if (flags && flags[F_DONE]) return;
   __shared__ double sh[256];
-  const int n = blockIdx.x;
+  const BTile T = tiles[blockIdx.x];
+  tile_kt(a, T, v, part + 6 * blockIdx.x, sh);
+}
+
+// preconditioner, body rows (mobility_solver.cpp:651-717): thread per swimmer
+__global__ void k_precond_body(SimDev a, const int *sw_tile0, const int *sw_ntile,
+                               const double *part, const double *in, double *out, double *ou_all,
+                               const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
   const double *b = a.body + size_t(n) * BREC;
   const m3 Qm = rotm(ga4(b + BQ));
-  const int m0 = a.sw_blob0[n], m1 = m0 + a.sw_nblob[n];
+  double bb[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = sw_tile0[n]; t < sw_tile0[n] + sw_ntile[n]; t++)
+    for (int k = 0; k < 6; k++) bb[k] += part[6 * t + k];
+  double bm[36], kk[36];
+  rotate66(b + BN, Qm, bm);
+  rotate66(b + BK, Qm, kk);
+  const int vid = 3 * a.ldb + 6 * n;
+  for (int r = 0; r < 6; r++) {
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < 6; k++) { s1 += kk[6 * r + k] * bb[k]; s2 += bm[6 * r + k] * in[vid + k]; }
+    const double o = s1 - s2;
+    out[vid + r] = o;
+    ou_all[6 * n + r] = o;
+  }
+}
+
+// preconditioner, blob rows: out = M^-1 (in + U + W x d)
+__global__ __launch_bounds__(256) void k_precond_blobs(SimDev a, const BTile *tiles,
+                                                       const double *in, const double *ou_all,
+                                                       double *out, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const BTile T = tiles[blockIdx.x];
+  if (threadIdx.x >= T.n) return;
+  const int m = T.m0 + threadIdx.x;
+  const double *b = a.body + size_t(T.sw) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const double *ou = ou_all + 6 * T.sw;
+  const d3 V = {ou[0], ou[1], ou[2]}, W = {ou[3], ou[4], ou[5]};
   const int ld = a.ldb;
-  const int vid = 3 * ld + 6 * n;
-  const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int m = m0 + threadIdx.x; m < m1; m += 256) {
-    const d3 l = ld3(z, ld, m);
+  const double Minv = 6.0 * kPi * a.mu * a.ab;
+  const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
+  st3(out, ld, m, Minv * (ld3(in, ld, m) + V + cross3(W, d)));
+}
+
+// system_matrix_mult K products (mobility_solver.cpp:905-964), blob rows:
+// out_lambda = M_bb z - (U + W x d); with the tile's -K^T z partials
+__global__ __launch_bounds__(256) void k_kprod_blobs(SimDev a, const BTile *tiles, const double *z,
+                                                     const double *vb, double *out, double *part,
+                                                     const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  const BTile T = tiles[blockIdx.x];
+  const double *b = a.body + size_t(T.sw) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const int ld = a.ldb;
+  const int vid = 3 * ld + 6 * T.sw;
+  if (threadIdx.x < T.n) {
+    const int m = T.m0 + threadIdx.x;
+    const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
     const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
-    const d3 c = cross3(d, l);
-    acc[0] -= l.x; acc[1] -= l.y; acc[2] -= l.z;
-    acc[3] -= c.x; acc[4] -= c.y; acc[5] -= c.z;
     st3(out, ld, m, ld3(vb, ld, m) - (V + cross3(W, d)));
   }
-  for (int k = 0; k < 6; k++) {
-    const double s = block_sum<256>(acc[k], sh);
-    if (threadIdx.x == 0) out[vid + k] = s;
-  }
+  tile_kt(a, T, z, part + 6 * blockIdx.x, sh);
+}
+
+// body rows of the K products: out_U = -K^T z (sum of the swimmer's tiles)
+__global__ void k_kprod_body(SimDev a, const int *sw_tile0, const int *sw_ntile,
+                             const double *part, double *out, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  double bb[6] = {0, 0, 0, 0, 0, 0};
+  for (int t = sw_tile0[n]; t < sw_tile0[n] + sw_ntile[n]; t++)
+    for (int k = 0; k < 6; k++) bb[k] += part[6 * t + k];
+  const int vid = 3 * a.ldb + 6 * n;
+  for (int k = 0; k < 6; k++) out[vid + k] = bb[k];
 }
 
 // -------------------------------------------------------------------------
@@ -413,37 +453,43 @@ __global__ __launch_bounds__(256) void k_dots(const double *Qb, size_t ldq, int 
   if (threadIdx.x == 0) h[i] = s;
 }
 
-// w -= sum_i h[i] Q_i  (and optionally accumulate h into hacc)
-__global__ void k_sub(const double *Qb, size_t ldq, int k, const double *h, double *w, int n,
-                      const int *flags) {
+// w -= sum_i h[i] Q_i; with part_sq, each workgroup also stores the sum of
+// squares of its updated elements (the new basis vector's norm, in pieces)
+__global__ __launch_bounds__(256) void k_sub(const double *Qb, size_t ldq, int k, const double *h,
+                                             double *w, int n, const int *flags,
+                                             double *part_sq) {
   if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = w[e];
-  for (int i = 0; i < k; i++) s -= h[i] * Qb[ldq * i + e];
-  w[e] = s;
+  double s2 = 0.0;
+  if (e < n) {
+    double s = w[e];
+    for (int i = 0; i < k; i++) s -= h[i] * Qb[ldq * i + e];
+    w[e] = s;
+    s2 = s * s;
+  }
+  if (part_sq) {
+    s2 = block_sum<256>(s2, sh);
+    if (threadIdx.x == 0) part_sq[blockIdx.x] = s2;
+  }
 }
 
-// Normalise w into the new basis vector and do the Givens step
-// (mobility_solver.cpp:1183-1234).  One workgroup of 1024.
-__global__ __launch_bounds__(1024) void k_norm_givens(double *w, int n, int it, const double *h1,
-                                                      const double *h2, double *H, int ldh,
-                                                      double *cs, double *sn, double *beta,
-                                                      int *flags, const double *scal, double tol,
-                                                      int maxit, double switch_rel) {
+// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
+// new vector's norm comes from the partial sums of squares of the second
+// Gram-Schmidt sweep (fixed order); the division by it is k_scale_by.
+__global__ __launch_bounds__(256) void k_givens(const double *part_sq, int nparts, int it,
+                                                const double *h1, const double *h2, double *H,
+                                                int ldh, double *cs, double *sn, double *beta,
+                                                int *flags, double *scal, double tol, int maxit,
+                                                double switch_rel) {
   if (flags[F_DONE]) return;
-  __shared__ double sh[1024];
-  double s = 0;
-  for (int e = threadIdx.x; e < n; e += 1024) s += w[e] * w[e];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  const double qn = sqrt(sh[0]);
-  for (int e = threadIdx.x; e < n; e += 1024) w[e] /= qn;
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += part_sq[p];
+  s = block_sum<256>(s, sh);
   if (threadIdx.x != 0) return;
+  const double qn = sqrt(s);
+  scal[3] = qn;
   const int c = it - 1;
   auto h = [&](int i, int j) -> double & { return H[size_t(i) * ldh + j]; };
   for (int i = 0; i < it; i++) h(i, c) = h1[i] + h2[i];
@@ -490,15 +536,21 @@ __global__ void k_gmres_reset(int *flags) {
   flags[F_SKIP32] = 1;
 }
 
-// back substitution for y (mobility_solver.cpp:1236-1257)
-__global__ void k_backsub(const double *H, int ldh, const double *beta, const int *flags,
-                          double *y) {
+// back substitution for y (mobility_solver.cpp:1236-1257) by one wave:
+// column sweep from the last unknown, each lane updating its rows, which
+// subtracts the terms in the reference's order (j from it-1 down to i+1)
+__global__ __launch_bounds__(64) void k_backsub(const double *H, int ldh, const double *beta,
+                                                const int *flags, double *y) {
+  extern __shared__ double rb[];
   const int it = flags[F_ITS];
-  y[it - 1] = beta[it - 1] / H[size_t(it - 1) * ldh + it - 1];
-  for (int i = it - 2; i >= 0; i--) {
-    double s = beta[i];
-    for (int j = it - 1; j > i; j--) s -= H[size_t(i) * ldh + j] * y[j];
-    y[i] = s / H[size_t(i) * ldh + i];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < it; i += 64) rb[i] = beta[i];
+  __syncthreads();
+  for (int j = it - 1; j >= 0; j--) {
+    const double yj = rb[j] / H[size_t(j) * ldh + j];
+    for (int i = lane; i < j; i += 64) rb[i] -= H[size_t(i) * ldh + j] * yj;
+    if (lane == 0) y[j] = yj;
+    __syncthreads();
   }
 }
 
@@ -1240,12 +1292,21 @@ static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, int mo
     if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, mode == 1, c->stream), "pair bb")))
       return rc;
   }
-  hipLaunchKernelGGL(k_kprod, dim3(d.Nsw), dim3(256), 0, c->stream, d, z, c->Vb, out, done);
+  hipLaunchKernelGGL(k_kprod_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, z,
+                     c->Vb, out, s->tpart, done);
+  hipLaunchKernelGGL(k_kprod_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
+                     s->sw_tile0, s->sw_ntile, s->tpart, out, done);
   return launch_ok(c, "kprod");
 }
 
 static int precond(gmi_ctx *c, SimHost *s, const double *in, double *out, const int *skip) {
-  hipLaunchKernelGGL(k_precond, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d, in, out, skip);
+  const SimDev &d = s->d;
+  hipLaunchKernelGGL(k_tile_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, in,
+                     s->tpart, skip);
+  hipLaunchKernelGGL(k_precond_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
+                     s->sw_tile0, s->sw_ntile, s->tpart, in, out, s->ou, skip);
+  hipLaunchKernelGGL(k_precond_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, in,
+                     s->ou, out, skip);
   return launch_ok(c, "precond");
 }
 
@@ -1296,7 +1357,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, d.flags);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
-  // scal[0] = ||rhs|| is read by k_norm_givens
+  // scal[0] = ||rhs|| is read by k_givens
   const int ldh = s->d.maxit + 1;
   // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
   // says the solve has converged, so a chunk needs no host decisions
@@ -1310,14 +1371,15 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
       hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h1,
                          d.flags);
       hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w, n,
-                         d.flags);
+                         d.flags, nullptr);
       hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h2,
                          d.flags);
       hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h2, w, n,
-                         d.flags);
-      hipLaunchKernelGGL(k_norm_givens, dim3(1), dim3(1024), 0, c->stream, w, n, it, s->h1, s->h2,
+                         d.flags, s->part);
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, nb, it, s->h1, s->h2,
                          s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
                          switch_rel);
+      hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, w, n, s->scal, 3);
       if ((r = launch_ok(c, "arnoldi"))) return r;
     }
     return GMI_OK;
@@ -1360,7 +1422,8 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
     if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
   }
   const int nits = s->h_flags[F_ITS];
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(1), 0, c->stream, s->H, ldh, s->beta, d.flags, s->y);
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->d.maxit + 1), c->stream,
+                     s->H, ldh, s->beta, d.flags, s->y);
   hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, d.flags, s->y,
                      s->tmp, n);
   if ((rc = launch_ok(c, "gmres solution"))) return rc;
@@ -1471,7 +1534,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
     Nf += p->nfil[n];
   }
   if (!(p->dt > 0) || !(p->tol > 0) || !(p->gmres_tol > 0) || p->max_broyden_iter < 1 ||
-      p->gmres_max_iter < 1 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
+      p->gmres_max_iter < 1 || p->gmres_max_iter > 4000 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
     return fail(c, GMI_ERR_ARG, "bad sim parameters");
   SimHost *s = new SimHost();
   s->p = *p;
@@ -1535,6 +1598,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &s->z, np))) return rc;
   if ((rc = dalloc(c, s, &s->tmp, np))) return rc;
   if ((rc = dalloc(c, s, &s->scal, 16))) return rc;
+  if ((rc = dalloc(c, s, &s->part, nblk(d.npad, 256)))) return rc;
   const size_t nbr = size_t(d.nbroy);
   if ((rc = dalloc(c, s, &s->err, nbr))) return rc;
   if ((rc = dalloc(c, s, &s->nerr, nbr))) return rc;
@@ -1607,6 +1671,29 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, d.Q, Qv.data(), Qv.size()))) return rc;
   if ((rc = upload(c, d.Qm1, Qv.data(), Qv.size()))) return rc;
   s->nfil.assign(p->nfil, p->nfil + d.Nsw);
+  // blob tiles of <= 256 blobs inside one swimmer
+  std::vector<BTile> bt;
+  std::vector<int> t0(d.Nsw), nt(d.Nsw);
+  for (int n = 0; n < d.Nsw; n++) {
+    t0[n] = int(bt.size());
+    for (int m = 0; m < c->nblob[n]; m += 256)
+      bt.push_back({n, c->blob_off[n] + m, std::min(256, c->nblob[n] - m)});
+    nt[n] = int(bt.size()) - t0[n];
+  }
+  s->nbtiles = int(bt.size());
+  BTile *dbt;
+  if ((rc = dalloc(c, s, &dbt, bt.size()))) return rc;
+  s->btiles = dbt;
+  if ((rc = upload(c, dbt, bt.data(), bt.size()))) return rc;
+  int *i0, *i1;
+  if ((rc = dalloc(c, s, &i0, d.Nsw))) return rc;
+  if ((rc = dalloc(c, s, &i1, d.Nsw))) return rc;
+  if ((rc = upload(c, i0, t0.data(), d.Nsw))) return rc;
+  if ((rc = upload(c, i1, nt.data(), d.Nsw))) return rc;
+  s->sw_tile0 = i0;
+  s->sw_ntile = i1;
+  if ((rc = dalloc(c, s, &s->tpart, 6 * std::max<size_t>(bt.size(), 1)))) return rc;
+  if ((rc = dalloc(c, s, &s->ou, 6 * size_t(d.Nsw)))) return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 

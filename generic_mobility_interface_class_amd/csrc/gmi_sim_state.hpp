@@ -42,6 +42,11 @@ struct SimDev {
 
 struct Learn;  // learned warm start (gmi_learn.hip)
 
+// up to 256 consecutive blobs of one swimmer
+struct BTile {
+  int sw, m0, n;
+};
+
 struct SimHost {
   SimDev d;
   gmi_sim_params p;
@@ -50,7 +55,7 @@ struct SimHost {
   std::vector<void *> owned;
   double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
          *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
-         *tmp = nullptr, *scal = nullptr;
+         *tmp = nullptr, *scal = nullptr, *part = nullptr;
   double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
          *bd = nullptr;
   int *h_flags = nullptr;
@@ -60,6 +65,11 @@ struct SimHost {
   int last_gmres = 0;
   int chunk = 6;
   Learn *learn = nullptr;
+  // blob tiles for the preconditioner and K products
+  BTile *btiles = nullptr;
+  int nbtiles = 0;
+  int *sw_tile0 = nullptr, *sw_ntile = nullptr;
+  double *tpart = nullptr, *ou = nullptr;
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
   bool use_graphs = true;
   int graph_gen = -1;
