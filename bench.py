@@ -124,23 +124,12 @@ def bench_step(args):
         nonlocal nt
         sim.set_sim_precision(prec)
         sim.warmstart_configure(apply=learned, collect=0)
-        sim.call("gmi_wait_for_device")
-        sim.call("gmi_timer_start")
-        tb = tg = 0
-        for _ in range(args.step_steps):
-            rc, bi, gi = sim.step(nt)
-            assert rc == 0, (name, nt, rc)
-            tb += bi
-            tg += gi
-            nt += 1
-        ms = capi.C.c_double(0.0)
-        sim.call("gmi_timer_stop", capi.C.byref(ms))
-        k = args.step_steps
-        r = dict(steps=f"{nt - k}-{nt - 1}", steps_per_s=k / (ms.value * 1e-3),
-                 ms_per_step=ms.value / k, broyden_iters_per_step=tb / k,
-                 gmres_iters_per_step=tg / k)
-        print(f"# step {name}: {r['ms_per_step']:.2f} ms/step, {tb / k:.2f} Broyden, "
-              f"{tg / k:.1f} GMRES it/step", file=sys.stderr, flush=True)
+        a0, f0 = sim.warmstart_stats()
+        r = _timed_steps(sim, nt, args.step_steps, f"step {name}")
+        nt += args.step_steps
+        if learned:
+            a1, f1 = sim.warmstart_stats()
+            r["learned_steps"], r["fallbacks"] = a1 - a0, f1 - f0
         return r
 
     res = {"fp64": window("fp64", capi.GMI_FP64, False)}
@@ -155,6 +144,94 @@ def bench_step(args):
             "steps_per_window": args.step_steps, **res,
             "speedup_learned_mixed_vs_fp64":
                 res["learned_mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"],
+            "warm_start": {"training_rows": rows, "train_iters": 3000, "train_s": train_s,
+                           "final_loss_standardised": loss}}
+
+
+def _timed_steps(sim, nt0, k, name):
+    """k steps from nt0 timed with the context's HIP events.  A step that
+    fails (non-convergence, divergence) ends the window and is reported."""
+    sim.call("gmi_wait_for_device")
+    sim.call("gmi_timer_start")
+    tb = tg = 0
+    failed = None
+    for nt in range(nt0, nt0 + k):
+        rc, bi, gi = sim.step(nt)
+        tb += bi
+        tg += gi
+        if rc != 0:
+            failed = {"step": nt, "status": rc, "message": sim.last_error()}
+            print(f"# {name}: step {nt} failed: {failed['message']}", file=sys.stderr, flush=True)
+            k = nt - nt0 + 1
+            break
+    ms = capi.C.c_double(0.0)
+    sim.call("gmi_timer_stop", capi.C.byref(ms))
+    if failed:
+        return {"failed": failed, "steps": f"{nt0}-{nt0 + k - 1}", "steps_per_s": 0.0}
+    r = dict(steps=f"{nt0}-{nt0 + k - 1}", steps_per_s=k / (ms.value * 1e-3),
+             ms_per_step=ms.value / k, broyden_iters_per_step=tb / k,
+             gmres_iters_per_step=tg / k)
+    print(f"# {name}: {r['ms_per_step']:.2f} ms/step, {tb / k:.2f} Broyden, "
+          f"{tg / k:.1f} GMRES it/step", file=sys.stderr, flush=True)
+    return r
+
+
+def bench_ensemble(args):
+    """64 independent swimmers of uneven size (17..474 filaments x 20
+    segments, 642..7292 blobs; 306,518 particles) stepped as one batch:
+    every pair launch covers all swimmers, Arnoldi chunks replay as
+    captured graphs.  Two FP64 steps untimed, then FP64 and mixed-precision
+    windows from the same state (two contexts)."""
+    cfg = geometry.ensemble_workload("ensemble_64", seed=4)
+    out = {"workload": "ensemble_64", "swimmers": len(cfg.swimmers),
+           "filaments": int(sum(g.nfil for g in cfg.swimmers)),
+           "segments": int(sum(g.ns for g in cfg.swimmers)),
+           "blobs": int(sum(g.nblob for g in cfg.swimmers))}
+    for name, prec, k in (("fp64", capi.GMI_FP64, args.ens_steps // 2),
+                          ("mixed", capi.GMI_FP32_KAHAN, args.ens_steps)):
+        sim = capi.Sim(cfg, precision=prec)
+        sim.set_sim_precision(prec)
+        t0 = time.perf_counter()
+        sim.make_body_reference_matrices()
+        setup_s = time.perf_counter() - t0
+        for nt in range(2):
+            assert sim.step(nt)[0] == 0
+        out[name] = _timed_steps(sim, 2, k, f"ensemble {name}")
+        out[name]["setup_s"] = setup_s
+        sim.close()
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    return out
+
+
+def bench_rollout(args):
+    """Learned warm start over a long rollout of the 16 x 20-segment +
+    642-blob swimmer.  Two identical runs take one driving period (300
+    steps) of plain FP64 steps; one records every 5th step (18,880 rows of
+    (features, converged - guess)) and trains the network; then both run
+    `rollout` more steps, one plain, one with the network."""
+    cfg = geometry.sim_workload("step_962", seed=1)
+    plain, learned = capi.Sim(cfg), capi.Sim(cfg)
+    learned.warmstart_init(hidden=128, seed=7)
+    learned.warmstart_configure(apply=False, collect=20000, stride=5)
+    for s in (plain, learned):
+        s.make_body_reference_matrices()
+        for nt in range(300):
+            assert s.step(nt)[0] == 0
+    rows = learned.warmstart_dataset_size()
+    t0 = time.perf_counter()
+    loss = learned.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    learned.warmstart_configure(apply=True, collect=0)
+    k = args.rollout
+    rp = _timed_steps(plain, 300, k, "rollout plain")
+    rl = _timed_steps(learned, 300, k, "rollout learned")
+    dx = float(np.abs(plain.state()["x"] - learned.state()["x"]).max())
+    plain.close()
+    learned.close()
+    return {"workload": "step_962", "warmup_steps": 300, "rollout_steps": k,
+            "plain": rp, "learned": rl,
+            "speedup_learned_vs_plain": rl["steps_per_s"] / rp["steps_per_s"],
+            "final_max_position_diff": dx,
             "warm_start": {"training_rows": rows, "train_iters": 3000, "train_s": train_s,
                            "final_loss_standardised": loss}}
 
@@ -187,6 +264,10 @@ def main():
     ap.add_argument("--step-steps", type=int, default=20)
     ap.add_argument("--step-warmup", type=int, default=300)
     ap.add_argument("--no-step", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=40)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--rollout", type=int, default=1000)
+    ap.add_argument("--no-rollout", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one MI355X; "
@@ -269,6 +350,10 @@ def main():
         line["sim_steps"] = bench_step(args)
         if not args.no_cpu:
             line["sim_steps"]["cpu_baseline"] = cpu_step_baseline(args)
+    if not args.no_ensemble:
+        line["ensemble"] = bench_ensemble(args)
+    if not args.no_rollout:
+        line["rollout"] = bench_rollout(args)
     print(json.dumps(line), flush=True)
 
 

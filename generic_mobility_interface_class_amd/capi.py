@@ -91,6 +91,9 @@ SIGNATURES = [
     ("gmi_warmstart_init", C.c_int, [C.c_void_p, C.c_int, C.c_ulonglong]),
     ("gmi_warmstart_configure", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("gmi_warmstart_dataset_size", C.c_int, [C.c_void_p, _ip]),
+    ("gmi_warmstart_dataset", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float), C.c_int]),
+    ("gmi_warmstart_stats", C.c_int, [C.c_void_p, _ip, _ip]),
     ("gmi_warmstart_train", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
                                       C.c_ulonglong, _dp]),
     ("gmi_warmstart_param_count", C.c_int, [C.c_void_p, _ip]),
@@ -167,6 +170,9 @@ class Context:
         self.Ns = int(ns.sum())
         self.Nb = int(nb.sum())
 
+    def last_error(self):
+        return self.lib.gmi_last_error(self.h).decode()
+
     def check(self, rc):
         if rc != GMI_OK:
             raise GmiError(rc, self.lib.gmi_last_error(self.h).decode())
@@ -225,7 +231,7 @@ class GmiSimParams(C.Structure):
         ("max_broyden_iter", C.c_int), ("gmres_max_iter", C.c_int),
         ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
         ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
-        ("sphere_radius", C.c_double),
+        ("sphere_radius", C.c_double), ("sphere_radii", _dp),
     ]
 
 
@@ -244,7 +250,7 @@ class Sim(Context):
             ptr(a["body_x"]), ptr(a["phases"]), cfg.kb, cfg.kt, cfg.dt, cfg.tol,
             cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
             cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
-            float(a["radius"][0]))
+            float(a["radius"][0]), ptr(a["radius"]))
         self.call("gmi_sim_create", C.byref(self.sp))
         self.nsw = len(a["nfil"])
         self.nfil_total = int(a["nfil"].sum())
@@ -300,6 +306,19 @@ class Sim(Context):
         n = C.c_int(0)
         self.call("gmi_warmstart_dataset_size", C.byref(n))
         return n.value
+
+    def warmstart_dataset(self, rows=None):
+        n = self.warmstart_dataset_size() if rows is None else int(rows)
+        X = np.zeros((n, 27), np.float32)
+        T = np.zeros((n, 6), np.float32)
+        fp = C.POINTER(C.c_float)
+        self.call("gmi_warmstart_dataset", X.ctypes.data_as(fp), T.ctypes.data_as(fp), n)
+        return X, T
+
+    def warmstart_stats(self):
+        a, f = C.c_int(0), C.c_int(0)
+        self.call("gmi_warmstart_stats", C.byref(a), C.byref(f))
+        return a.value, f.value
 
     def warmstart_train(self, iters=2000, batch=1024, lr=1e-3, seed=0):
         loss = C.c_double(0.0)

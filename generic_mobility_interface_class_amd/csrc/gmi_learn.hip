@@ -48,6 +48,8 @@ struct Learn {
   // per-step buffers
   float *FX = nullptr, *FY = nullptr;          // [Ns][nin], [Ns][nout]
   double *G = nullptr;                         // guess snapshot [Ns][6]
+  double *bsnap = nullptr;                     // body (x, u) at the guess [Nsw][6]
+  int applied = 0, fallbacks = 0;              // learned steps, and those redone plainly
   double *corr = nullptr;                      // update vector [nbroy]
   // training / test buffers (bmax rows)
   int bmax = 0;
@@ -210,10 +212,39 @@ __device__ void unknowns(const SimDev &a, int gidx, double *o) {
   }
 }
 
-__global__ void k_snapshot(SimDev a, double *G) {
+__global__ void k_snapshot(SimDev a, double *G, double *bs) {
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gidx >= a.Ns) return;
-  unknowns(a, gidx, G + 6 * size_t(gidx));
+  if (gidx < a.Ns) {
+    unknowns(a, gidx, G + 6 * size_t(gidx));
+  } else if (gidx < a.Ns + a.Nsw) {
+    const int n = gidx - a.Ns;
+    const double *b = a.body + size_t(n) * BREC;
+    for (int k = 0; k < 3; k++) { bs[6 * n + k] = b[BX + k]; bs[6 * n + 3 + k] = b[BU + k]; }
+  }
+}
+
+// update vector (Broyden layout) that takes the current iterate back to the
+// snapshot taken at the extrapolated guess
+__global__ void k_back_to_guess(SimDev a, const double *G, const double *bs, double *delta) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx < a.Ns) {
+    const int f = gidx / a.S, k = gidx % a.S;
+    const int off = 6 * a.S * f + 6 * a.fil_sw[f];
+    double c[6];
+    unknowns(a, gidx, c);
+    for (int q = 0; q < 3; q++) {
+      delta[off + 3 * k + q] = G[6 * size_t(gidx) + q] - c[q];
+      delta[off + 3 * a.S + 3 * k + q] = G[6 * size_t(gidx) + 3 + q] - c[3 + q];
+    }
+  } else if (gidx < a.Ns + a.Nsw) {
+    const int n = gidx - a.Ns;
+    const int off = 6 * a.S * (a.sw_fil0[n] + a.sw_nfil[n]) + 6 * n;
+    const double *b = a.body + size_t(n) * BREC;
+    for (int k = 0; k < 3; k++) {
+      delta[off + k] = bs[6 * n + k] - b[BX + k];
+      delta[off + 3 + k] = bs[6 * n + 3 + k] - b[BU + k];
+    }
+  }
 }
 
 // append (features, converged - guess) rows to the dataset
@@ -337,7 +368,8 @@ int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied) {
   SimDev &d = s->d;
   const double t = (nt + 1) * s->p.dt;
   hipLaunchKernelGGL(k_features, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, t, L->FX);
-  hipLaunchKernelGGL(k_snapshot, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G);
+  hipLaunchKernelGGL(k_snapshot, dim3(nb(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d, L->G,
+                     L->bsnap);
   int rc;
   if ((rc = hip_check(c, hipGetLastError(), "features"))) return rc;
   if (L->apply && L->trained) {
@@ -352,11 +384,24 @@ int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied) {
                        L->FY, L->xs + 2 * L->nin, L->xs + 2 * L->nin + L->nout, L->corr);
     if ((rc = hip_check(c, hipGetLastError(), "correction"))) return rc;
     *applied = true;
+    L->applied++;
   }
   return GMI_OK;
 }
 
 double *learn_correction(SimHost *s) { return s->learn ? s->learn->corr : nullptr; }
+
+// after a learned attempt failed: the update back to the extrapolated guess
+// (into the correction buffer), counted as a fallback
+int learn_back_to_guess(gmi_ctx *c, SimHost *s, double **delta) {
+  Learn *L = s->learn;
+  SimDev &d = s->d;
+  hipLaunchKernelGGL(k_back_to_guess, dim3(nb(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d,
+                     L->G, L->bsnap, L->corr);
+  L->fallbacks++;
+  *delta = L->corr;
+  return hip_check(c, hipGetLastError(), "back to guess");
+}
 
 int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt) {
   Learn *L = s->learn;
@@ -400,6 +445,7 @@ int gmi_warmstart_init(gmi_ctx *c, int hidden, unsigned long long seed) {
   if ((rc = lalloc(c, L, &L->FX, size_t(s->d.Ns) * ni))) return rc;
   if ((rc = lalloc(c, L, &L->FY, size_t(s->d.Ns) * no))) return rc;
   if ((rc = lalloc(c, L, &L->G, size_t(s->d.Ns) * 6))) return rc;
+  if ((rc = lalloc(c, L, &L->bsnap, size_t(s->d.Nsw) * 6))) return rc;
   if ((rc = lalloc(c, L, &L->corr, size_t(s->d.nbroy)))) return rc;
   if ((rc = lalloc(c, L, &L->loss, 1))) return rc;
   // Glorot-uniform weights from splitmix64, zero biases; unit statistics
@@ -476,6 +522,27 @@ int gmi_warmstart_dataset_size(gmi_ctx *c, int *rows) {
   Learn *L = learn_of(c);
   if (!L || !rows) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
   *rows = L->count;
+  return GMI_OK;
+}
+
+int gmi_warmstart_dataset(gmi_ctx *c, float *X, float *T, int rows) {
+  Learn *L = learn_of(c);
+  if (!L || !X || !T || rows < 0) return c ? fail(c, GMI_ERR_ARG, "bad arguments") : GMI_ERR_ARG;
+  if (rows > L->count) return fail(c, GMI_ERR_ARG, "more rows than recorded");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  if ((rc = hip_check(c, hipMemcpy(X, L->DX, sizeof(float) * size_t(rows) * L->nin,
+                                   hipMemcpyDeviceToHost), "dataset X")))
+    return rc;
+  return hip_check(c, hipMemcpy(T, L->DT, sizeof(float) * size_t(rows) * L->nout,
+                                hipMemcpyDeviceToHost), "dataset T");
+}
+
+int gmi_warmstart_stats(gmi_ctx *c, int *applied, int *fallbacks) {
+  Learn *L = learn_of(c);
+  if (!L || !applied || !fallbacks) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
+  *applied = L->applied;
+  *fallbacks = L->fallbacks;
   return GMI_OK;
 }
 

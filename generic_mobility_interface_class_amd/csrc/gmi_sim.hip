@@ -941,7 +941,7 @@ __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
         B[r * n + (c < 3 ? c : 3 * S + c - 3)] = s;
       }
     // sphere-segment RPY corrections (swimmer.cpp:580-650)
-    const double R = a.R, mu = a.mu, as = a.as;
+    const double R = a.sw_R[a.fil_sw[f]], mu = a.mu, as = a.as;
     const d3 bx = ga3(b + BX);
     auto add3 = [&](int r0, int c0, const m3 &M) {
       for (int i = 0; i < 3; i++)
@@ -1536,8 +1536,12 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if (!(p->dt > 0) || !(p->tol > 0) || !(p->gmres_tol > 0) || p->max_broyden_iter < 1 ||
       p->gmres_max_iter < 1 || p->gmres_max_iter > 4000 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
     return fail(c, GMI_ERR_ARG, "bad sim parameters");
+  if (p->sphere_radii)
+    for (int n = 0; n < c->nsw; n++)
+      if (!(p->sphere_radii[n] > 0)) return fail(c, GMI_ERR_ARG, "sphere_radii must be positive");
   SimHost *s = new SimHost();
   s->p = *p;
+  s->p.sphere_radii = nullptr;  // copied to the device below; the caller keeps its array
   s->precision = c->precision;
   c->extra = s;
   c->extra_release = sim_release;
@@ -1549,7 +1553,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   d.maxit = p->gmres_max_iter;
   d.DL = 2.2 * c->as; d.mu = c->mu; d.as = c->as; d.ab = c->ab;
   d.kb = p->kb; d.kt = p->kt; d.dt = p->dt; d.tol = p->tol; d.gamma = p->jacobian_confidence;
-  d.gtol = p->gmres_tol; d.R = p->sphere_radius; d.neuler = p->num_euler_steps;
+  d.gtol = p->gmres_tol; d.neuler = p->num_euler_steps;
   d.camp = p->curvature_amp; d.cper = p->curvature_period;
   d.X = c->Xs; d.Fs = c->Fs; d.Vs = c->Vs; d.Xb = c->Xb; d.Vb = c->Vb; d.Frep = c->Frep;
   int rc;
@@ -1561,6 +1565,8 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &ip, d.Nsw))) return rc; d.sw_nblob = ip;
   if ((rc = dalloc(c, s, &ip, Nf))) return rc; d.fil_sw = ip;
   if ((rc = dalloc(c, s, &ip, d.Nb))) return rc; d.blob_sw = ip;
+  double *swR;
+  if ((rc = dalloc(c, s, &swR, d.Nsw))) return rc; d.sw_R = swR;
   if ((rc = dalloc(c, s, &d.flags, F_N))) return rc;
   if ((rc = dalloc(c, s, &d.Xm1, 3 * ls))) return rc;
   if ((rc = dalloc(c, s, &d.Xm2, 3 * ls))) return rc;
@@ -1661,6 +1667,11 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, const_cast<int *>(d.sw_nblob), nblv.data(), d.Nsw))) return rc;
   if ((rc = upload(c, const_cast<int *>(d.fil_sw), fil_sw.data(), Nf))) return rc;
   if ((rc = upload(c, const_cast<int *>(d.blob_sw), blob_sw.data(), d.Nb))) return rc;
+  {
+    std::vector<double> R(d.Nsw, p->sphere_radius);
+    if (p->sphere_radii) R.assign(p->sphere_radii, p->sphere_radii + d.Nsw);
+    if ((rc = upload(c, const_cast<double *>(d.sw_R), R.data(), d.Nsw))) return rc;
+  }
   if ((rc = upload(c, d.body, body.data(), body.size()))) return rc;
   if ((rc = upload(c, d.blob_ref, bref.data(), bref.size()))) return rc;
   if ((rc = upload(c, d.fil_ref, p->fil_refs, 3 * size_t(Nf)))) return rc;
@@ -1889,44 +1900,81 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
   return GMI_OK;
 }
 
-int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
-  SimHost *s = c ? sim_of(c) : nullptr;
-  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
-  int rc, big = 0, bad = 0, its = 0, total = 0;
-  if ((rc = gmi_initial_guess(c, nt))) return rc;
-  bool corrected = false;
-  if ((rc = learn_after_initial_guess(c, s, nt, &corrected))) return rc;
-  if (corrected) {
-    // network proposal: the extrapolated iterate plus the predicted
-    // correction, applied through the ordinary state update
-    double *corr = learn_correction(s);
-    hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, corr);
-    hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, corr);
-    if ((rc = launch_ok(c, "warm start"))) return rc;
-  }
+// Broyden iterations from the current iterate (cilia_sim_main.cpp:252-314):
+// returns GMI_OK, GMI_ERR_NOT_CONVERGED after max_iter iterations, or
+// GMI_ERR_NOT_FINITE; sets no error message for those two.  Like the
+// reference driver, a non-finite first residual is only caught when
+// `check_first` is set (the learned attempt).
+static int broyden_solve(gmi_ctx *c, SimHost *s, int nt, int max_iter, bool check_first,
+                         int *iters, int *gmres_total) {
+  int rc, big = 0, bad = 0, its = 0;
   if ((rc = gmi_forces_and_torques(c, nt))) return rc;
   if ((rc = compute_velocities(c, s, &its))) return rc;
-  total += its;
+  *gmres_total += its;
   if ((rc = compute_errors(c, s, nt, s->err, &big, &bad))) return rc;
+  *iters = 0;
+  if (bad && check_first) return GMI_ERR_NOT_FINITE;
   if (big && (rc = gmi_prepare_jacobian_inv(c, nt))) return rc;
   s->iter = 0;
-  while (big && s->iter < s->p.max_broyden_iter) {
+  while (big && s->iter < max_iter) {
     if ((rc = gmi_broyden_find_update(c))) return rc;
     if ((rc = gmi_apply_update(c))) return rc;
     if ((rc = gmi_forces_and_torques(c, nt))) return rc;
     if ((rc = compute_velocities(c, s, &its))) return rc;
-    total += its;
+    *gmres_total += its;
     if ((rc = compute_errors(c, s, nt, s->nerr, &big, &bad))) return rc;
     if (bad) {
-      if (broyden_iters) *broyden_iters = s->iter + 1;
-      if (gmres_iters) *gmres_iters = total;
-      return fail(c, GMI_ERR_NOT_FINITE, "Broyden's method diverged");
+      *iters = s->iter + 1;
+      return GMI_ERR_NOT_FINITE;
     }
     if ((rc = gmi_broyden_end_of_iter(c, big))) return rc;
   }
-  if (broyden_iters) *broyden_iters = s->iter;
+  *iters = s->iter;
+  return big ? GMI_ERR_NOT_CONVERGED : GMI_OK;
+}
+
+static void apply_update_vector(gmi_ctx *c, SimHost *s, const double *v) {
+  hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, v);
+  hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, v);
+}
+
+int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  int rc, bi = 0, total = 0, spent = 0;
+  if ((rc = gmi_initial_guess(c, nt))) return rc;
+  bool corrected = false;
+  if ((rc = learn_after_initial_guess(c, s, nt, &corrected))) return rc;
+  rc = GMI_ERR_NOT_CONVERGED;
+  if (corrected) {
+    // network proposal: the extrapolated iterate plus the predicted
+    // correction, applied through the ordinary state update.  The attempt
+    // gets about twice the recent iteration count; if it does not
+    // converge in that, the step restarts from the extrapolated guess.
+    apply_update_vector(c, s, learn_correction(s));
+    if ((rc = launch_ok(c, "warm start"))) return rc;
+    const int cap = std::min(s->p.max_broyden_iter, std::max(4, int(2.0 * s->iters_avg) + 2));
+    rc = broyden_solve(c, s, nt, cap, true, &bi, &total);
+    spent = bi;
+    if (rc != GMI_OK && rc != GMI_ERR_NOT_CONVERGED && rc != GMI_ERR_NOT_FINITE) return rc;
+    if (rc != GMI_OK) {
+      double *back = nullptr;
+      if ((rc = learn_back_to_guess(c, s, &back))) return rc;
+      apply_update_vector(c, s, back);
+      if ((rc = launch_ok(c, "back to guess"))) return rc;
+      rc = GMI_ERR_NOT_CONVERGED;
+    }
+  }
+  if (rc != GMI_OK) {
+    rc = broyden_solve(c, s, nt, s->p.max_broyden_iter, false, &bi, &total);
+    spent += bi;
+  }
+  if (broyden_iters) *broyden_iters = spent;
   if (gmres_iters) *gmres_iters = total;
-  if (big) return fail(c, GMI_ERR_NOT_CONVERGED, "Broyden's method did not converge");
+  if (rc == GMI_ERR_NOT_FINITE) return fail(c, rc, "Broyden's method diverged");
+  if (rc == GMI_ERR_NOT_CONVERGED) return fail(c, rc, "Broyden's method did not converge");
+  if (rc) return rc;
+  s->iters_avg = 0.8 * s->iters_avg + 0.2 * spent;
   return learn_after_convergence(c, s, nt);
 }
 

@@ -19,11 +19,12 @@ enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_
 struct SimDev {
   // sizes
   int S, Nf, Nsw, Ns, Nb, lds, ldb, nbroy, npad, maxit;
-  double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol, R;
+  double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol;
   int neuler;
   double camp, cper;
   // index maps
   const int *sw_fil0, *sw_nfil, *sw_blob0, *sw_nblob, *fil_sw, *blob_sw;
+  const double *sw_R;  // body radius per swimmer (Schur RPY correction)
   // segment SoA
   double *X, *Xm1, *Xm2, *Q, *Qm1, *U, *Um1, *Fint, *Fs, *Vs;
   // filament
@@ -64,6 +65,7 @@ struct SimHost {
   int precision = GMI_FP64;
   int last_gmres = 0;
   int chunk = 6;
+  double iters_avg = 0.0;  // moving average of Broyden iterations per step
   Learn *learn = nullptr;
   // blob tiles for the preconditioner and K products
   BTile *btiles = nullptr;
@@ -82,5 +84,6 @@ void learn_release(SimHost *s);
 int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied);
 int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt);
 double *learn_correction(SimHost *s);
+int learn_back_to_guess(gmi_ctx *c, SimHost *s, double **delta);
 
 }  // namespace gmi

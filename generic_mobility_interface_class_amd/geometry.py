@@ -257,7 +257,7 @@ class SimConfig:
         fil_refs = np.concatenate([(g.filament_directions() * g.base_radius()).reshape(-1)
                                    for g in self.swimmers])
         body_x = np.concatenate([np.asarray(g.centre, float) for g in self.swimmers])
-        radius = np.array([g.R for g in self.swimmers])
+        radius = np.array([g.R for g in self.swimmers], dtype=np.float64)
         return dict(nfil=nfil, nblob=nblob, blob_refs=np.ascontiguousarray(blob_refs),
                     fil_refs=np.ascontiguousarray(fil_refs), body_x=body_x,
                     phases=np.ascontiguousarray(self.phases, dtype=np.float64), radius=radius)
@@ -281,3 +281,47 @@ SIM_WORKLOADS = {
 
 def sim_workload(name, seed=0, **kw):
     return sim_config(seed=seed, **SIM_WORKLOADS[name], **kw)
+
+
+def ensemble_config(n_swimmers=64, seed=0, nfil_min=16, nfil_max=512, nseg=20,
+                    freq_min=8, freq_max=32, spacing=None, **kw):
+    """Batch of hydrodynamically independent swimmers of uneven size.
+
+    Per swimmer (draw order: all NFIL, then all phases):
+      * NFIL = round(exp(U[ln nfil_min, ln nfil_max])), log-uniform;
+      * blobs proportional to NFIL at the density of the 256-filament,
+        4002-blob swimmer (4002 * NFIL / 256), snapped to the nearest geodesic
+        count 10 f^2 + 2 with f in [freq_min, freq_max] (642 .. 10242);
+      * blob-shell radius R = 25 sqrt(Nb / 4002): the blob areal density of
+        the 4002-blob swimmer on R = 25 (and of 642 blobs on R = 10);
+      * centres on the x axis, `spacing` apart (default: 2 x the largest
+        swimmer's reach), so the swimmers never touch;
+      * driving phases U[0, 2 pi) per filament.
+    """
+    rng = Xoshiro256ss(seed)
+    lo, hi = math.log(nfil_min), math.log(nfil_max)
+    nfil = [int(round(math.exp(lo + (hi - lo) * u))) for u in rng.uniform(n_swimmers)]
+    geoms = []
+    for nf in nfil:
+        f = int(round(math.sqrt(max(4002.0 * nf / 256.0 - 2.0, 0.0) / 10.0)))
+        f = min(max(f, freq_min), freq_max)
+        nb = 10 * f * f + 2
+        geoms.append(SwimmerGeometry(nfil=nf, nseg=nseg, nblob=nb, R=25.0 * math.sqrt(nb / 4002.0)))
+    reach = max(g.base_radius() + nseg * g.DL for g in geoms)
+    step = spacing if spacing is not None else 2.0 * reach + 10.0
+    for k, g in enumerate(geoms):
+        g.centre = np.array([k * step, 0.0, 0.0])
+    phases = 2.0 * math.pi * rng.uniform(sum(nfil))
+    return SimConfig(swimmers=geoms, phases=phases, **kw)
+
+
+ENSEMBLE_WORKLOADS = {
+    # 64 swimmers, NFIL log-uniform in [16, 512], 642..10242 blobs
+    "ensemble_64": dict(n_swimmers=64, nfil_min=16, nfil_max=512),
+    # 6 swimmers, NFIL in [2, 12]: small enough for the oracle
+    "ensemble_6": dict(n_swimmers=6, nfil_min=2, nfil_max=12, nseg=10, freq_min=3, freq_max=8),
+}
+
+
+def ensemble_workload(name, seed=0, **kw):
+    return ensemble_config(seed=seed, **ENSEMBLE_WORKLOADS[name], **kw)

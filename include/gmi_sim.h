@@ -45,6 +45,9 @@ typedef struct gmi_sim_params {
   double curvature_period;
   double sphere_radius;       /* body radius of the Schur-complement RPY
                                  correction (swimmer.cpp:580)                */
+  const double *sphere_radii; /* per-swimmer body radius [n_swimmers], or
+                                 NULL for sphere_radius on every swimmer (the
+                                 reference has one R_OVER_L, config.hpp:126)  */
 } gmi_sim_params;
 
 /* swimmers[n].initial_setup for all n + mobility.initialise()
@@ -111,6 +114,14 @@ int gmi_warmstart_init(gmi_ctx *ctx, int hidden, unsigned long long seed);
  * that many rows (0: stop recording). */
 int gmi_warmstart_configure(gmi_ctx *ctx, int apply, int collect_capacity, int collect_stride);
 int gmi_warmstart_dataset_size(gmi_ctx *ctx, int *rows);
+/* Copies the first `rows` recorded rows (raw, not standardised) to the host:
+ * X[rows x 27] features, T[rows x 6] converged - guess.  Rows are in
+ * recording order: per recorded step, segment-major over all filaments. */
+int gmi_warmstart_dataset(gmi_ctx *ctx, float *X, float *T, int rows);
+/* Steps that used the network, and those of them whose learned attempt did
+ * not converge within about twice the recent Broyden iteration count and
+ * were redone from the extrapolated guess (both counted since init). */
+int gmi_warmstart_stats(gmi_ctx *ctx, int *applied, int *fallbacks);
 /* Adam on the MSE of standardised targets; returns the mean loss of the
  * last 20 iterations. */
 int gmi_warmstart_train(gmi_ctx *ctx, int iters, int batch, double lr,

@@ -131,7 +131,7 @@ class OrcSimParams(C.Structure):
         ("max_broyden_iter", C.c_int), ("gmres_max_iter", C.c_int),
         ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
         ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
-        ("sphere_radius", C.c_double),
+        ("sphere_radius", C.c_double), ("sphere_radii", _dp),
     ]
 
 
@@ -184,7 +184,7 @@ class Sim:
             g0.mu, g0.a_seg, g0.a_blob, cfg.kb, cfg.kt, cfg.dt, cfg.tol,
             cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
             cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
-            float(a["radius"][0]))
+            float(a["radius"][0]), _p(a["radius"]))
         self.h = self.lib.orc_sim_create(C.byref(self.p), nthreads or nthreads_default())
         self.Ns = self.lib.orc_sim_num_segments(self.h)
         self.Nb = self.lib.orc_sim_num_blobs(self.h)

@@ -28,6 +28,7 @@ typedef struct orc_sim_params {
   double curvature_amp;     /* preferred-curvature amplitude                */
   double curvature_period;
   double sphere_radius;     /* body radius in the Schur RPY correction      */
+  const double *sphere_radii; /* per swimmer, or NULL: sphere_radius for all */
 } orc_sim_params;
 
 typedef struct orc_sim orc_sim;

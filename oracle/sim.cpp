@@ -54,6 +54,7 @@ struct Swimmer {
   int nfil = 0, nblob = 0;
   int seg0 = 0, blob0 = 0;     // offsets in the global particle arrays
   int err0 = 0;                // offset in the Broyden vectors
+  double R = 0;                // body radius of the Schur RPY correction
   double x[3] = {0, 0, 0}, xm1[3] = {0, 0, 0}, xm2[3] = {0, 0, 0};
   double u[3] = {0, 0, 0}, um1[3] = {0, 0, 0};
   Quat q, qm1;
@@ -491,7 +492,7 @@ bool sw_prepare_jacobian_inv(Swimmer &B, int nt, const orc_sim_params &p, double
   schur.add3(3, 3, dtf * (old - 0.5 * (blc * old) + (1.0 / 12.0) * (blc * blc * old)));
   const M3 Dinv = eye3() + (1.0 / 12.0) * (blc * blc) - 0.5 * blc;
   const M3 DinvT = eye3() + (1.0 / 12.0) * (blc * blc) + 0.5 * blc;
-  const double R = p.sphere_radius;
+  const double R = B.R;
   const double mu = p.mu, as = p.a_seg;
   B.Bblk.assign(B.nfil, Mat());
   for (int i = 0; i < B.nfil; i++) {
@@ -942,6 +943,7 @@ orc_sim *orc_sim_create(const orc_sim_params *p, int nthreads) {
     B.nfil = p->nfil[n];
     B.nblob = p->nblob[n];
     B.seg0 = seg0; B.blob0 = blob0; B.err0 = err0;
+    B.R = p->sphere_radii ? p->sphere_radii[n] : p->sphere_radius;
     B.blob_ref.assign(p->blob_refs + 3 * ref0, p->blob_refs + 3 * (ref0 + B.nblob));
     B.fil_ref.assign(p->fil_refs + 3 * fil0, p->fil_refs + 3 * (fil0 + B.nfil));
     for (int k = 0; k < 3; k++) B.x[k] = B.xm1[k] = B.xm2[k] = p->body_x ? p->body_x[3 * n + k] : 0.0;

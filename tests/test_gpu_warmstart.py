@@ -46,7 +46,8 @@ def rollout(s, steps, nt0=0):
 
 def test_trained_network_cuts_broyden_iterations(gpu):
     """Two identical swimmers run one driving period (300 steps) plainly; B
-    records every 5th step (19,200 rows) and trains.  Over the next 150 steps
+    records steps 5, 10, ..., 295 (59 x 320 segments = 18,880 rows; step 0
+    has no history to learn from) and trains.  Over the next 150 steps
     A stays plain and B uses the network: B needs fewer Broyden iterations
     and both trajectories converge to the same tolerance."""
     cfg, a = sim()
@@ -56,7 +57,7 @@ def test_trained_network_cuts_broyden_iterations(gpu):
     rollout(a, 300)
     rollout(b, 300)
     rows = b.warmstart_dataset_size()
-    assert rows >= 19000
+    assert rows == 59 * 320
     loss = b.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
     print(f"dataset rows {rows}; training loss (standardised MSE) {loss:.4f}", flush=True)
     assert loss < 0.5
