@@ -37,6 +37,17 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
 
+def finite_json(v):
+    """Non-finite floats become null (strict JSON)."""
+    if isinstance(v, dict):
+        return {k: finite_json(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [finite_json(x) for x in v]
+    if isinstance(v, float) and not np.isfinite(v):
+        return None
+    return v
+
+
 def grand_pairs_flops(ns, nb):
     pairs = ns * ns + 2 * ns * nb + nb * nb
     flops = FLOPS["ss"] * ns * ns + (FLOPS["sb"] + FLOPS["bs"]) * ns * nb + FLOPS["bb"] * nb * nb
@@ -93,59 +104,72 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
 def bench_step(args):
     """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer.
 
-    One driving period (300 steps) of plain FP64 steps first, recording
-    every 37th step's (features, converged - guess) rows for the learned
-    warm start; then timed windows of `step_steps` steps each:
+    Four identical runs take one driving period (300 steps) of plain FP64
+    steps; the two learned ones record every 5th step's (features,
+    converged - guess) rows and train the same network.  Then each run
+    steps over the same `step_steps` steps in its own mode:
       fp64            plain FP64 (the reference method)
       learned_mixed   network warm start + mixed precision
       learned_fp64    network warm start, FP64
       mixed           mixed precision, extrapolation warm start
     """
     cfg = geometry.sim_workload(args.step_workload, seed=1)
-    sim = capi.Sim(cfg, precision=capi.GMI_FP64)
-    sim.set_sim_precision(capi.GMI_FP64)
-    sim.warmstart_init(hidden=128, seed=7)
-    stride = max(1, args.step_warmup // 8)
-    sim.warmstart_configure(apply=False, collect=40960, stride=stride)
-    sim.make_body_reference_matrices()
-    nt = 0
+    modes = (("fp64", capi.GMI_FP64, False), ("learned_mixed", capi.GMI_FP32_KAHAN, True),
+             ("learned_fp64", capi.GMI_FP64, True), ("mixed", capi.GMI_FP32_KAHAN, False))
+    stride = 5
+    sims = {}
+    for name, _, learned in modes:
+        sim = capi.Sim(cfg, precision=capi.GMI_FP64)
+        sim.set_sim_precision(capi.GMI_FP64)
+        sim.set_graphs(not args.no_graphs)
+        if learned:
+            sim.warmstart_init(hidden=128, seed=7)
+            sim.warmstart_configure(apply=False, collect=(args.step_warmup // stride) * sim.Ns,
+                                    stride=stride)
+        sim.make_body_reference_matrices()
+        sims[name] = sim
     t0 = time.perf_counter()
-    for _ in range(args.step_warmup):
-        rc, bi, gi = sim.step(nt)
-        assert rc == 0, rc
-        nt += 1
+    for nt in range(args.step_warmup):
+        for name, sim in sims.items():
+            rc, bi, gi = sim.step(nt)
+            assert rc == 0, (name, nt, rc, sim.last_error())
     warm_s = time.perf_counter() - t0
-    print(f"# {args.step_warmup} warm-up steps in {warm_s:.1f} s, "
-          f"{sim.warmstart_dataset_size()} training rows", file=sys.stderr, flush=True)
-    rows = sim.warmstart_dataset_size()
-    sim.warmstart_configure(apply=False, collect=0)
-
-    def window(name, prec, learned):
-        nonlocal nt
+    rows = sims["learned_mixed"].warmstart_dataset_size()
+    print(f"# {args.step_warmup} warm-up steps x {len(sims)} runs in {warm_s:.1f} s, "
+          f"{rows} training rows", file=sys.stderr, flush=True)
+    train = {}
+    for name, prec, learned in modes:
+        sim = sims[name]
+        if learned:
+            t0 = time.perf_counter()
+            loss = sim.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
+            train = {"training_rows": rows, "train_iters": 3000,
+                     "train_s": time.perf_counter() - t0, "final_loss_standardised": loss}
+            sim.warmstart_configure(apply=True, collect=0)
         sim.set_sim_precision(prec)
-        sim.warmstart_configure(apply=learned, collect=0)
-        a0, f0 = sim.warmstart_stats()
-        r = _timed_steps(sim, nt, args.step_steps, f"step {name}")
-        nt += args.step_steps
+    res = {}
+    nt0 = args.step_warmup
+    for name, prec, learned in modes:
+        sim = sims[name]
+        a0, f0 = sim.warmstart_stats() if learned else (0, 0)
+        r = _timed_steps(sim, nt0, args.step_steps, f"step {name}")
         if learned:
             a1, f1 = sim.warmstart_stats()
             r["learned_steps"], r["fallbacks"] = a1 - a0, f1 - f0
-        return r
-
-    res = {"fp64": window("fp64", capi.GMI_FP64, False)}
-    t0 = time.perf_counter()
-    loss = sim.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
-    train_s = time.perf_counter() - t0
-    res["learned_mixed"] = window("learned_mixed", capi.GMI_FP32_KAHAN, True)
-    res["learned_fp64"] = window("learned_fp64", capi.GMI_FP64, True)
-    res["mixed"] = window("mixed", capi.GMI_FP32_KAHAN, False)
-    sim.close()
+        res[name] = r
+    ref = sims["fp64"].state()["x"]
+    for name in ("learned_mixed", "learned_fp64", "mixed"):
+        res[name]["max_position_diff_vs_fp64"] = float(np.abs(sims[name].state()["x"] - ref).max())
+    for sim in sims.values():
+        sim.close()
+    fp64 = res["fp64"]["steps_per_s"]
     return {"workload": args.step_workload, "warmup_steps": args.step_warmup,
-            "steps_per_window": args.step_steps, **res,
-            "speedup_learned_mixed_vs_fp64":
-                res["learned_mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"],
-            "warm_start": {"training_rows": rows, "train_iters": 3000, "train_s": train_s,
-                           "final_loss_standardised": loss}}
+            "steps_per_window": args.step_steps, "window": f"{nt0}-{nt0 + args.step_steps - 1}",
+            **res,
+            "speedup_learned_mixed_vs_fp64": res["learned_mixed"]["steps_per_s"] / fp64,
+            "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / fp64,
+            "speedup_learned_fp64_vs_fp64": res["learned_fp64"]["steps_per_s"] / fp64,
+            "warm_start": train}
 
 
 def _timed_steps(sim, nt0, k, name):
@@ -191,6 +215,7 @@ def bench_ensemble(args):
                           ("mixed", capi.GMI_FP32_KAHAN, args.ens_steps)):
         sim = capi.Sim(cfg, precision=prec)
         sim.set_sim_precision(prec)
+        sim.set_graphs(not args.no_graphs)
         t0 = time.perf_counter()
         sim.make_body_reference_matrices()
         setup_s = time.perf_counter() - t0
@@ -211,6 +236,8 @@ def bench_rollout(args):
     `rollout` more steps, one plain, one with the network."""
     cfg = geometry.sim_workload("step_962", seed=1)
     plain, learned = capi.Sim(cfg), capi.Sim(cfg)
+    for s in (plain, learned):
+        s.set_graphs(not args.no_graphs)
     learned.warmstart_init(hidden=128, seed=7)
     learned.warmstart_configure(apply=False, collect=20000, stride=5)
     for s in (plain, learned):
@@ -261,13 +288,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--step-workload", default="step_9122")
-    ap.add_argument("--step-steps", type=int, default=20)
+    ap.add_argument("--step-steps", type=int, default=40)
     ap.add_argument("--step-warmup", type=int, default=300)
     ap.add_argument("--no-step", action="store_true")
     ap.add_argument("--ens-steps", type=int, default=40)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
     ap.add_argument("--no-rollout", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the Arnoldi kernels eagerly (for kernel-trace profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one MI355X; "
@@ -354,7 +383,7 @@ def main():
         line["ensemble"] = bench_ensemble(args)
     if not args.no_rollout:
         line["rollout"] = bench_rollout(args)
-    print(json.dumps(line), flush=True)
+    print(json.dumps(finite_json(line), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":

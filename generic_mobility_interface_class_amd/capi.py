@@ -21,6 +21,7 @@ GMI_ERR_NOT_FINITE = -4
 GMI_ERR_SINGULAR = -5
 
 GMI_FP64 = 0
+WARMSTART_NIN = 51   # GMI_WARMSTART_NIN
 GMI_FP32_KAHAN = 1
 
 _dp = C.POINTER(C.c_double)
@@ -309,7 +310,7 @@ class Sim(Context):
 
     def warmstart_dataset(self, rows=None):
         n = self.warmstart_dataset_size() if rows is None else int(rows)
-        X = np.zeros((n, 27), np.float32)
+        X = np.zeros((n, WARMSTART_NIN), np.float32)
         T = np.zeros((n, 6), np.float32)
         fp = C.POINTER(C.c_float)
         self.call("gmi_warmstart_dataset", X.ctypes.data_as(fp), T.ctypes.data_as(fp), n)

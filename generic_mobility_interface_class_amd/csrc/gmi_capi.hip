@@ -95,9 +95,14 @@ static int build_list(gmi_ctx *c, int mask, WorkList &wl) {
   for (const TileDesc &T : wl.h_tiles) wl.nsplit_tiles += (T.nsplit > 1);
   if (wl.d_tiles) (void)hipFree(wl.d_tiles);
   if (wl.d_work) (void)hipFree(wl.d_work);
-  wl.d_tiles = nullptr; wl.d_work = nullptr;
+  if (wl.d_cnt) (void)hipFree(wl.d_cnt);
+  wl.d_tiles = nullptr; wl.d_work = nullptr; wl.d_cnt = nullptr;
   if (wl.ntiles == 0) return GMI_OK;
   int rc;
+  if ((rc = hip_check(c, hipMalloc(&wl.d_cnt, sizeof(int) * wl.ntiles), "hipMalloc counters")))
+    return rc;
+  if ((rc = hip_check(c, hipMemset(wl.d_cnt, 0, sizeof(int) * wl.ntiles), "zero counters")))
+    return rc;
   if ((rc = hip_check(c, hipMalloc(&wl.d_tiles, sizeof(TileDesc) * wl.ntiles), "hipMalloc tiles")))
     return rc;
   if ((rc = hip_check(c, hipMalloc(&wl.d_work, sizeof(WorkItem) * wl.nwork), "hipMalloc work")))
@@ -117,7 +122,9 @@ int rebuild_lists(gmi_ctx *c) {
     if ((rc = build_list(c, mask, c->lists[mask]))) return rc;
     slots = std::max(slots, size_t(c->lists[mask].nslots));
   }
-  if ((rc = build_list(c, kSS | kBB, c->barrier_tiles))) return rc;
+  // the barrier visits every source of the target's group: the all-blocks shape
+  if ((rc = build_list(c, kSS | kSB | kBS | kBB, c->barrier_tiles))) return rc;
+  slots = std::max(slots, size_t(c->barrier_tiles.nslots));
   if (slots > c->part_slots) {
     if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
@@ -140,6 +147,7 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
   a.part = c->part;
   a.tiles = wl.d_tiles;
   a.work = wl.d_work;
+  a.tile_cnt = wl.d_cnt;
   a.mask = mask;
   a.accumulate = accumulate;
   a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
@@ -156,9 +164,7 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
     ev_mid = c->prof_events[c->prof_used + 1];
     c->prof_used += 2;
   }
-  return hip_check(c, launch_pair(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0,
-                                  fp32, c->stream, ev_mid),
-                   "pair launch");
+  return hip_check(c, launch_pair(a, wl.nwork, fp32, c->stream, ev_mid), "pair launch");
 }
 
 }  // namespace gmi
@@ -311,9 +317,11 @@ void gmi_destroy(gmi_ctx *c) {
   for (auto &kv : c->lists) {
     if (kv.second.d_tiles) (void)hipFree(kv.second.d_tiles);
     if (kv.second.d_work) (void)hipFree(kv.second.d_work);
+    if (kv.second.d_cnt) (void)hipFree(kv.second.d_cnt);
   }
   if (c->barrier_tiles.d_tiles) (void)hipFree(c->barrier_tiles.d_tiles);
   if (c->barrier_tiles.d_work) (void)hipFree(c->barrier_tiles.d_work);
+  if (c->barrier_tiles.d_cnt) (void)hipFree(c->barrier_tiles.d_cnt);
   for (auto &st : c->stage) {
     if (st.pinned) (void)hipHostFree(st.pinned);
     if (st.dev) (void)hipFree(st.dev);
@@ -372,9 +380,8 @@ int gmi_apply_interparticle_forces(gmi_ctx *c, double K) {
   if (!c) return GMI_ERR_ARG;
   if (!std::isfinite(K)) return fail(c, GMI_ERR_ARG, "barrier K not finite");
   const WorkList &wl = c->barrier_tiles;
-  return hip_check(c, launch_barrier(c->Fs, c->lds, c->Frep, c->ldb, c->Xs, c->Xb,
-                                     wl.d_tiles, wl.ntiles, c->nseg_fil, c->as, c->ab, K,
-                                     c->stream),
+  return hip_check(c, launch_barrier(c->Fs, c->lds, c->Frep, c->ldb, c->Xs, c->Xb, wl, c->part,
+                                     c->nseg_fil, c->as, c->ab, K, c->stream),
                    "barrier launch");
 }
 

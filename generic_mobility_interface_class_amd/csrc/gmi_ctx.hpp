@@ -37,7 +37,7 @@ struct gmi_ctx {
 
   // launch shaping (set by the tuner)
   int target_blocks = 2048;
-  int min_chunk = 64;
+  int min_chunk = 128;
   int list_generation = 0;  // bumped whenever the work lists are rebuilt
 
   std::map<int, gmi::WorkList> lists;

@@ -15,10 +15,19 @@
 // FP32 master weights are rounded to BF16 (round-to-nearest-even) after each
 // update and the forward pass reads only the BF16 copy.
 //
-// Features (nin = 27) per segment, lab frame, at the initial guess:
+// Features (nin = 51) per segment, lab frame, at the initial guess:
 //   x - x_base (3), tangent (3), x - x_{n-1} (3), x_{n-1} - x_{n-2} (3),
 //   u (3), u_{n-1} (3), dt v_prev (3), dt w_prev (3)  [velocities of the
-//   previous converged step], k/(NSEG-1), sin and cos of the driving phase.
+//   previous converged step], k/(NSEG-1), sin and cos of the driving phase,
+//   and the targets (converged - guess) of the previous kLags = 4 steps
+//   (6 each, newest first).
+// The converged multipliers carry step-to-step noise up to what the Broyden
+// tolerance allows, and the quadratic extrapolation amplifies it; the lagged
+// targets let the network filter it (a linear filter of them alone explains
+// about three quarters of the target variance on later steps).
+// For segments k >= 1 the Lie-type correction is left out (target zero, not
+// applied): converged - extrapolated u is ~1e-5 there, far below the Broyden
+// tolerance, while segment 0's clamp multiplier is O(1).
 // Inputs and targets are standardised with statistics of the training set.
 #include "gmi_ctx.hpp"
 #include "gmi_lie.hpp"
@@ -32,7 +41,8 @@
 
 namespace gmi {
 
-constexpr int kNin = 27;
+constexpr int kLags = 4;
+constexpr int kNin = 27 + 6 * kLags;
 constexpr int kNout = 6;
 
 struct Learn {
@@ -49,6 +59,7 @@ struct Learn {
   float *FX = nullptr, *FY = nullptr;          // [Ns][nin], [Ns][nout]
   double *G = nullptr;                         // guess snapshot [Ns][6]
   double *bsnap = nullptr;                     // body (x, u) at the guess [Nsw][6]
+  float *lags = nullptr;                       // [Ns][kLags][6], newest first
   int applied = 0, fallbacks = 0;              // learned steps, and those redone plainly
   double *corr = nullptr;                      // update vector [nbroy]
   // training / test buffers (bmax rows)
@@ -175,7 +186,7 @@ __global__ void k_gather(const float *DX, const float *DT, const int *idx, int B
 __device__ __forceinline__ d3 rowv(const double *p) { return {p[0], p[1], p[2]}; }
 
 // raw features of every segment at the initial guess
-__global__ void k_features(SimDev a, double t, float *FX) {
+__global__ void k_features(SimDev a, double t, const float *lags, float *FX) {
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (gidx >= a.Ns) return;
   const int f = gidx / a.S, k = gidx % a.S;
@@ -197,6 +208,8 @@ __global__ void k_features(SimDev a, double t, float *FX) {
   o[24] = float(a.S > 1 ? double(k) / (a.S - 1) : 0.0);
   o[25] = float(sin(ph));
   o[26] = float(cos(ph));
+  const float *lg = lags + size_t(gidx) * 6 * kLags;
+  for (int q = 0; q < 6 * kLags; q++) o[27 + q] = lg[q];
 }
 
 // the six Broyden unknowns of each segment, current values
@@ -248,16 +261,30 @@ __global__ void k_back_to_guess(SimDev a, const double *G, const double *bs, dou
 }
 
 // append (features, converged - guess) rows to the dataset
+__device__ void target(const SimDev &a, int gidx, const double *G, float *t) {
+  double c[6];
+  unknowns(a, gidx, c);
+  const int nq = (gidx % a.S == 0) ? 6 : 3;  // Lie-type part only for the clamp
+  for (int q = 0; q < 6; q++) t[q] = q < nq ? float(c[q] - G[6 * size_t(gidx) + q]) : 0.0f;
+}
+
 __global__ void k_record(SimDev a, const double *G, const float *FX, float *DX, float *DT,
                          int row0, int cap) {
   const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (gidx >= a.Ns) return;
   const int r = row0 + gidx;
   if (r >= cap) return;
-  double c[6];
-  unknowns(a, gidx, c);
-  for (int q = 0; q < 6; q++) DT[size_t(r) * kNout + q] = float(c[q] - G[6 * size_t(gidx) + q]);
+  target(a, gidx, G, DT + size_t(r) * kNout);
   for (int q = 0; q < kNin; q++) DX[size_t(r) * kNin + q] = FX[size_t(gidx) * kNin + q];
+}
+
+// shift the lag buffer and insert this step's target
+__global__ void k_push_lags(SimDev a, const double *G, float *lags) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= a.Ns) return;
+  float *lg = lags + size_t(gidx) * 6 * kLags;
+  for (int q = 6 * kLags - 1; q >= 6; q--) lg[q] = lg[q - 6];
+  target(a, gidx, G, lg);
 }
 
 __global__ void k_standardise(float *X, int rows, int n, const float *mean, const float *sd) {
@@ -277,7 +304,7 @@ __global__ void k_correction(SimDev a, const float *Y, const float *ym, const fl
     for (int c = 0; c < 3; c++) {
       corr[off + 3 * k + c] = double(ym[c] + ysd[c] * Y[size_t(gidx) * kNout + c]);
       corr[off + 3 * a.S + 3 * k + c] =
-          double(ym[3 + c] + ysd[3 + c] * Y[size_t(gidx) * kNout + 3 + c]);
+          k == 0 ? double(ym[3 + c] + ysd[3 + c] * Y[size_t(gidx) * kNout + 3 + c]) : 0.0;
     }
   } else if (gidx < a.Ns + a.Nsw) {
     const int n = gidx - a.Ns;
@@ -364,12 +391,15 @@ static int mlp_backward(gmi_ctx *c, Learn *L, const float *Xin, const float *T, 
 int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied) {
   Learn *L = s->learn;
   *applied = false;
-  if (!L || (!L->collect && !(L->apply && L->trained)) || nt < 1) return GMI_OK;
+  if (!L || nt < 1) return GMI_OK;
   SimDev &d = s->d;
   const double t = (nt + 1) * s->p.dt;
-  hipLaunchKernelGGL(k_features, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, t, L->FX);
+  // the guess snapshot feeds the lag features, so it is taken every step
   hipLaunchKernelGGL(k_snapshot, dim3(nb(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d, L->G,
                      L->bsnap);
+  if (L->collect || (L->apply && L->trained))
+    hipLaunchKernelGGL(k_features, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, t, L->lags,
+                       L->FX);
   int rc;
   if ((rc = hip_check(c, hipGetLastError(), "features"))) return rc;
   if (L->apply && L->trained) {
@@ -405,11 +435,15 @@ int learn_back_to_guess(gmi_ctx *c, SimHost *s, double **delta) {
 
 int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt) {
   Learn *L = s->learn;
-  if (!L || !L->collect || nt < 1 || L->count >= L->cap || nt % L->stride) return GMI_OK;
+  if (!L || nt < 1) return GMI_OK;
   SimDev &d = s->d;
-  hipLaunchKernelGGL(k_record, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G, L->FX, L->DX,
-                     L->DT, L->count, L->cap);
-  L->count = std::min(L->cap, L->count + d.Ns);
+  if (L->collect && L->count < L->cap && nt % L->stride == 0) {
+    hipLaunchKernelGGL(k_record, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G, L->FX,
+                       L->DX, L->DT, L->count, L->cap);
+    L->count = std::min(L->cap, L->count + d.Ns);
+  }
+  // the lag features follow every converged step while the network exists
+  hipLaunchKernelGGL(k_push_lags, dim3(nb(d.Ns, 256)), dim3(256), 0, c->stream, d, L->G, L->lags);
   return hip_check(c, hipGetLastError(), "record");
 }
 
@@ -446,6 +480,7 @@ int gmi_warmstart_init(gmi_ctx *c, int hidden, unsigned long long seed) {
   if ((rc = lalloc(c, L, &L->FY, size_t(s->d.Ns) * no))) return rc;
   if ((rc = lalloc(c, L, &L->G, size_t(s->d.Ns) * 6))) return rc;
   if ((rc = lalloc(c, L, &L->bsnap, size_t(s->d.Nsw) * 6))) return rc;
+  if ((rc = lalloc(c, L, &L->lags, size_t(s->d.Ns) * 6 * kLags))) return rc;
   if ((rc = lalloc(c, L, &L->corr, size_t(s->d.nbroy)))) return rc;
   if ((rc = lalloc(c, L, &L->loss, 1))) return rc;
   // Glorot-uniform weights from splitmix64, zero biases; unit statistics

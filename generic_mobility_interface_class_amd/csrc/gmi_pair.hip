@@ -419,42 +419,38 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
       sweep<R, BB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
     }
   }
-  if (tid >= T.nt) return;
-  double o[6];
-  sum.get(o);
-  if (T.nsplit == 1) {
-    if (seg_t) {
-      double *V = a.Vs + T.t0 + tid;
-      for (int c = 0; c < 6; ++c)
-        V[c * a.lds] = a.accumulate ? V[c * a.lds] + o[c] : o[c];
-    } else {
-      double *V = a.Vb + T.t0 + tid;
-      for (int c = 0; c < 3; ++c)
-        V[c * a.ldb] = a.accumulate ? V[c * a.ldb] + o[c] : o[c];
-    }
-  } else {
-    double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
-    const int nc = seg_t ? 6 : 3;
-    for (int c = 0; c < nc; ++c) P[c * kBlock] = o[c];
-  }
-}
-
-// Deterministic reduction of split partials: fixed split order.
-__global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
-  if (a.skip && *a.skip) return;
-  const TileDesc T = a.tiles[blockIdx.x];
-  const int tid = threadIdx.x;
-  if (T.nsplit == 1 || tid >= T.nt) return;
-  const bool seg_t = (T.species == 0);
+  const bool live = tid < T.nt;
   const int nc = seg_t ? 6 : 3;
   double *V = seg_t ? a.Vs : a.Vb;
   const int ld = seg_t ? a.lds : a.ldb;
+  if (T.nsplit == 1) {
+    if (!live) return;
+    double o[6];
+    sum.get(o);
+    for (int c = 0; c < nc; ++c) {
+      double *v = V + c * ld + T.t0 + tid;
+      *v = a.accumulate ? *v + o[c] : o[c];
+    }
+    return;
+  }
+  if (live) {
+    double o[6];
+    sum.get(o);
+    double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
+    for (int c = 0; c < nc; ++c) P[c * kBlock] = o[c];
+  }
+  // the tile's last split to finish sums all partials in split order
+  if (!last_arrival(a.tile_cnt + w.tile, T.nsplit) || !live) return;
+  // split order per component; the loads of several splits are in flight
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double *P = a.part + size_t(T.part0) * 6 * kBlock + tid;
+#pragma unroll 4
+  for (int q = 0; q < T.nsplit; ++q)
+    for (int c = 0; c < 6; ++c)
+      if (c < nc) s[c] += P[size_t(q) * 6 * kBlock + c * kBlock];
   for (int c = 0; c < nc; ++c) {
-    double s = 0.0;
-    for (int q = 0; q < T.nsplit; ++q)
-      s += a.part[size_t(T.part0 + q) * 6 * kBlock + c * kBlock + tid];
     double *v = V + c * ld + T.t0 + tid;
-    *v = a.accumulate ? *v + s : s;
+    *v = a.accumulate ? *v + s[c] : s[c];
   }
 }
 
@@ -475,27 +471,43 @@ __device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, do
   }
 }
 
+// Work item = (target tile, source split) of the all-sources work list; the
+// tile's last split to finish sums the split partials in split order.
 __global__ __launch_bounds__(kBlock) void k_barrier(
     double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
     const double *__restrict__ Xs, const double *__restrict__ Xb,
-    const TileDesc *__restrict__ tiles, int nseg_fil, double as, double ab,
-    double K) {
+    const TileDesc *__restrict__ tiles, const WorkItem *__restrict__ work,
+    double *__restrict__ part, int *__restrict__ tile_cnt, int nseg_fil, double as,
+    double ab, double K) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
-  const TileDesc T = tiles[blockIdx.x];
+  const WorkItem w = work[blockIdx.x];
+  const TileDesc T = tiles[w.tile];
   const int tid = threadIdx.x;
   const bool seg_t = (T.species == 0);
   const int i = T.t0 + (tid < T.nt ? tid : 0);
   const double *X = seg_t ? Xs : Xb;
   const int ld = seg_t ? lds : ldb;
   const double xi = X[i], yi = X[ld + i], zi = X[2 * ld + i];
+  // excluded sources: a segment itself and its neighbours in the same
+  // filament (:1040) -- the index range [lo, hi]; a blob itself (:1120)
+  int lo = i, hi = i;
+  if (seg_t) {
+    if (i % nseg_fil != 0) lo = i - 1;
+    if ((i + 1) % nseg_fil != 0) hi = i + 1;
+  }
+  const long L = long(T.nsseg) + T.nsblob;
+  const int beg = int(L * w.split / T.nsplit);
+  const int end = int(L * (w.split + 1) / T.nsplit);
   double fx = 0, fy = 0, fz = 0;
-  const int fili = i / nseg_fil;
   for (int pass = 0; pass < 2; ++pass) {
     const bool src_seg = (pass == 0);
     const double *S = src_seg ? Xs : Xb;
     const int sld = src_seg ? lds : ldb;
-    const int j0 = src_seg ? T.sseg0 : T.sblob0;
-    const int j1 = j0 + (src_seg ? T.nsseg : T.nsblob);
+    const int r0 = src_seg ? beg : max(beg, T.nsseg) - T.nsseg;
+    const int r1 = src_seg ? min(end, T.nsseg) : end - T.nsseg;
+    const int j0 = (src_seg ? T.sseg0 : T.sblob0) + r0;
+    const int j1 = (src_seg ? T.sseg0 : T.sblob0) + r1;
+    const bool same = (src_seg == seg_t);
     const double asum = src_seg ? (seg_t ? 2.0 * as : as + ab) : (seg_t ? as + ab : 2.0 * ab);
     for (int base = j0; base < j1; base += kTile) {
       const int n = min(kTile, j1 - base);
@@ -506,18 +518,35 @@ __global__ __launch_bounds__(kBlock) void k_barrier(
       __syncthreads();
       for (int jj = 0; jj < n; ++jj) {
         const int j = base + jj;
-        // exclusions: adjacent segments of one filament (:1040), a blob itself (:1120)
-        if (src_seg && seg_t && fili == j / nseg_fil && abs(i - j) <= 1) continue;
-        if (!src_seg && !seg_t && j == i) continue;
+        if (same && j >= lo && j <= hi) continue;
         barrier_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], asum, K, fx, fy, fz);
       }
     }
   }
-  if (tid >= T.nt) return;
+  const bool live = tid < T.nt;
+  if (T.nsplit == 1) {
+    if (!live) return;
+    if (seg_t) {
+      Fs[i] += fx; Fs[lds + i] += fy; Fs[2 * lds + i] += fz;
+    } else {
+      Frep[i] = fx; Frep[ldb + i] = fy; Frep[2 * ldb + i] = fz;
+    }
+    return;
+  }
+  if (live) {
+    double *P = part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
+    P[0] = fx; P[kBlock] = fy; P[2 * kBlock] = fz;
+  }
+  if (!last_arrival(tile_cnt + w.tile, T.nsplit) || !live) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  const double *P = part + size_t(T.part0) * 6 * kBlock + tid;
+#pragma unroll 4
+  for (int q = 0; q < T.nsplit; ++q)
+    for (int c = 0; c < 3; ++c) f[c] += P[size_t(q) * 6 * kBlock + c * kBlock];
   if (seg_t) {
-    Fs[i] += fx; Fs[lds + i] += fy; Fs[2 * lds + i] += fz;
+    Fs[i] += f[0]; Fs[lds + i] += f[1]; Fs[2 * lds + i] += f[2];
   } else {
-    Frep[i] = fx; Frep[ldb + i] = fy; Frep[2 * ldb + i] = fz;
+    Frep[i] = f[0]; Frep[ldb + i] = f[1]; Frep[2 * ldb + i] = f[2];
   }
 }
 
@@ -538,30 +567,28 @@ __global__ void k_soa_to_aos(double *__restrict__ dst,
 
 }  // namespace
 
-hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce,
-                       bool fp32, hipStream_t s, hipEvent_t ev_mid) {
+hipError_t launch_pair(const PairArgs &a, int nwork, bool fp32, hipStream_t s,
+                       hipEvent_t ev_mid) {
   if (nwork <= 0) return hipSuccess;
+  if (!a.tile_cnt) return hipErrorInvalidValue;
   if (fp32)
     hipLaunchKernelGGL(k_pair<float>, dim3(nwork), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_pair<double>, dim3(nwork), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
-  if (ntiles_reduce > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
-    e = hipGetLastError();
-  }
+  if (ev_mid) e = hipEventRecord(ev_mid, s);
   return e;
 }
 
 hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb,
-                          const double *Xs, const double *Xb,
-                          const TileDesc *tiles, int ntiles, int nseg_fil,
-                          double as, double ab, double K, hipStream_t s) {
-  if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_barrier, dim3(ntiles), dim3(kBlock), 0, s, Fs, lds,
-                     Frep, ldb, Xs, Xb, tiles, nseg_fil, as, ab, K);
+                          const double *Xs, const double *Xb, const WorkList &wl,
+                          double *part, int nseg_fil, double as, double ab, double K,
+                          hipStream_t s) {
+  if (wl.nwork <= 0) return hipSuccess;
+  if (!wl.d_cnt) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_barrier, dim3(wl.nwork), dim3(kBlock), 0, s, Fs, lds, Frep, ldb, Xs, Xb,
+                     wl.d_tiles, wl.d_work, part, wl.d_cnt, nseg_fil, as, ab, K);
   return hipGetLastError();
 }
 

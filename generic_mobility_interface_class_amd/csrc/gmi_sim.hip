@@ -351,21 +351,20 @@ __device__ void tile_kt(const SimDev &a, const BTile &T, const double *v, double
   }
 }
 
-__global__ __launch_bounds__(256) void k_tile_kt(SimDev a, const BTile *tiles, const double *v,
-                                                 double *part, const int *flags) {
+// preconditioner, -K^T in per tile; the swimmer's last tile to finish sums
+// the tile partials (tile order) and solves the body rows
+// (mobility_solver.cpp:651-717): out_U = (K^T K)^-1 b - N in_U
+__global__ __launch_bounds__(256) void k_precond_kt_body(SimDev a, const BTile *tiles,
+                                                         const int *sw_tile0, const int *sw_ntile,
+                                                         int *sw_cnt, double *part,
+                                                         const double *in, double *out,
+                                                         double *ou_all, const int *flags) {
   if (flags && flags[F_DONE]) return;
   __shared__ double sh[256];
   const BTile T = tiles[blockIdx.x];
-  tile_kt(a, T, v, part + 6 * blockIdx.x, sh);
-}
-
-// preconditioner, body rows (mobility_solver.cpp:651-717): thread per swimmer
-__global__ void k_precond_body(SimDev a, const int *sw_tile0, const int *sw_ntile,
-                               const double *part, const double *in, double *out, double *ou_all,
-                               const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.Nsw) return;
+  tile_kt(a, T, in, part + 6 * blockIdx.x, sh);
+  const int n = T.sw;
+  if (!last_arrival(sw_cnt + n, sw_ntile[n]) || threadIdx.x != 0) return;
   const double *b = a.body + size_t(n) * BREC;
   const m3 Qm = rotm(ga4(b + BQ));
   double bb[6] = {0, 0, 0, 0, 0, 0};
@@ -403,10 +402,12 @@ __global__ __launch_bounds__(256) void k_precond_blobs(SimDev a, const BTile *ti
 }
 
 // system_matrix_mult K products (mobility_solver.cpp:905-964), blob rows:
-// out_lambda = M_bb z - (U + W x d); with the tile's -K^T z partials
-__global__ __launch_bounds__(256) void k_kprod_blobs(SimDev a, const BTile *tiles, const double *z,
-                                                     const double *vb, double *out, double *part,
-                                                     const int *flags) {
+// out_lambda = M_bb z - (U + W x d); with the tile's -K^T z partials, summed
+// in tile order by the swimmer's last tile into the body rows out_U = -K^T z
+__global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, const int *sw_tile0,
+                                               const int *sw_ntile, int *sw_cnt, const double *z,
+                                               const double *vb, double *out, double *part,
+                                               const int *flags) {
   if (flags && flags[F_DONE]) return;
   __shared__ double sh[256];
   const BTile T = tiles[blockIdx.x];
@@ -421,94 +422,157 @@ __global__ __launch_bounds__(256) void k_kprod_blobs(SimDev a, const BTile *tile
     st3(out, ld, m, ld3(vb, ld, m) - (V + cross3(W, d)));
   }
   tile_kt(a, T, z, part + 6 * blockIdx.x, sh);
-}
-
-// body rows of the K products: out_U = -K^T z (sum of the swimmer's tiles)
-__global__ void k_kprod_body(SimDev a, const int *sw_tile0, const int *sw_ntile,
-                             const double *part, double *out, const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.Nsw) return;
+  const int n = T.sw;
+  if (!last_arrival(sw_cnt + n, sw_ntile[n]) || threadIdx.x != 0) return;
   double bb[6] = {0, 0, 0, 0, 0, 0};
   for (int t = sw_tile0[n]; t < sw_tile0[n] + sw_ntile[n]; t++)
     for (int k = 0; k < 6; k++) bb[k] += part[6 * t + k];
-  const int vid = 3 * a.ldb + 6 * n;
-  for (int k = 0; k < 6; k++) out[vid + k] = bb[k];
+  for (int k = 0; k < 6; k++) out[vid + k] = bb[k];  // vid: swimmer n's U rows
 }
 
 // -------------------------------------------------------------------------
 // vector kernels (GMRES / Broyden)
 
-// h[i] = <Q_i, w>, i < k (one workgroup per column)
-__global__ __launch_bounds__(256) void k_dots(const double *Qb, size_t ldq, int k, const double *w,
-                                              int n, double *h, const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  __shared__ double sh[256];
-  const int i = blockIdx.x;
-  if (i >= k) return;
-  const double *q = Qb + ldq * i;
-  double s = 0;
-  for (int e = threadIdx.x; e < n; e += 256) s += q[e] * w[e];
-  s = block_sum<256>(s, sh);
-  if (threadIdx.x == 0) h[i] = s;
+// Dot products with many columns, chunked over the rows: each workgroup
+// takes kChunk consecutive rows, writes one partial per column, and the
+// workgroup that finishes last sums the partials in chunk order (fixed
+// order: the result does not depend on scheduling).
+constexpr int kChunk = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// w -= sum_i h[i] Q_i; with part_sq, each workgroup also stores the sum of
-// squares of its updated elements (the new basis vector's norm, in pieces)
-__global__ __launch_bounds__(256) void k_sub(const double *Qb, size_t ldq, int k, const double *h,
-                                             double *w, int n, const int *flags,
-                                             double *part_sq) {
+// partials of this workgroup's rows (values in sw) with columns j < k:
+// part[j * nchunk + chunk]; wave w takes the columns j = w (mod 4)
+__device__ void chunk_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
+                           double *part, int nchunk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < k; j += 4) {
+    const double *q = Qb + ldq * j + e0;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = lane + 64 * r;
+      if (e0 + i < n) acc += q[i] * sw[i];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) part[size_t(j) * nchunk + blockIdx.x] = acc;
+  }
+}
+
+// true in the workgroup that arrives last; every workgroup's earlier global
+// writes are visible to it.  It resets the counter for the next launch.
+__device__ bool last_group(int *cnt) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = atomicAdd(cnt, 1);
+    last = prev == int(gridDim.x) - 1;
+    if (last) atomicExch(cnt, 0);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  return true;
+}
+
+// h[j] = sum over chunks of part[j][chunk], in chunk order
+__device__ void reduce_parts(const double *part, int k, int nchunk, double *h) {
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    const double *p = part + size_t(j) * nchunk;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += p[c];
+    h[j] = s;
+  }
+}
+
+// h[j] = <Q_j, w>, j < k.  Grid: ceil(n / kChunk) workgroups of 256.
+__global__ __launch_bounds__(256) void k_pdots(const double *Qb, size_t ldq, int k,
+                                               const double *w, int n, double *part, int *cnt,
+                                               double *h, const int *flags) {
   if (flags && flags[F_DONE]) return;
+  __shared__ double sw[kChunk];
+  const int e0 = blockIdx.x * kChunk, e = e0 + threadIdx.x;
+  sw[threadIdx.x] = e < n ? w[e] : 0.0;
+  __syncthreads();
+  chunk_dots(Qb, ldq, k, sw, e0, n, part, gridDim.x);
+  if (!last_group(cnt)) return;
+  reduce_parts(part, k, gridDim.x, h);
+}
+
+// Gram-Schmidt pass 1 update w -= sum_j h1[j] Q_j, then pass 2 dots
+// h2[j] = <Q_j, w> of the updated vector
+__global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq, int k,
+                                                   const double *h1, double *w, int n,
+                                                   double *part, int *cnt, double *h2,
+                                                   const int *flags) {
+  if (flags[F_DONE]) return;
+  __shared__ double sw[kChunk];
+  const int e0 = blockIdx.x * kChunk, e = e0 + threadIdx.x;
+  double s = 0.0;
+  if (e < n) {
+    s = w[e];
+    for (int j = 0; j < k; j++) s -= h1[j] * Qb[ldq * j + e];
+    w[e] = s;
+  }
+  sw[threadIdx.x] = s;
+  __syncthreads();
+  chunk_dots(Qb, ldq, k, sw, e0, n, part, gridDim.x);
+  if (!last_group(cnt)) return;
+  reduce_parts(part, k, gridDim.x, h2);
+}
+
+// Gram-Schmidt pass 2 update w -= sum_j h2[j] Q_j; the last workgroup sums
+// the squared norm over chunks and takes the Givens step of Arnoldi
+// iteration `it` (mobility_solver.cpp:1183-1234).  H column = h1 + h2 (the
+// two passes together); the division by the norm is k_scale_by.
+__global__ __launch_bounds__(256) void k_psub_givens(const double *Qb, size_t ldq, int it,
+                                                     const double *h1, const double *h2,
+                                                     double *w, int n, double *part, int *cnt,
+                                                     double *H, int ldh, double *cs, double *sn,
+                                                     double *beta, int *flags, double *scal,
+                                                     double tol, int maxit, double switch_rel) {
+  if (flags[F_DONE]) return;
   __shared__ double sh[256];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.x * kChunk + threadIdx.x;
   double s2 = 0.0;
   if (e < n) {
     double s = w[e];
-    for (int i = 0; i < k; i++) s -= h[i] * Qb[ldq * i + e];
+    for (int j = 0; j < it; j++) s -= h2[j] * Qb[ldq * j + e];
     w[e] = s;
     s2 = s * s;
   }
-  if (part_sq) {
-    s2 = block_sum<256>(s2, sh);
-    if (threadIdx.x == 0) part_sq[blockIdx.x] = s2;
-  }
-}
-
-// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
-// new vector's norm comes from the partial sums of squares of the second
-// Gram-Schmidt sweep (fixed order); the division by it is k_scale_by.
-__global__ __launch_bounds__(256) void k_givens(const double *part_sq, int nparts, int it,
-                                                const double *h1, const double *h2, double *H,
-                                                int ldh, double *cs, double *sn, double *beta,
-                                                int *flags, double *scal, double tol, int maxit,
-                                                double switch_rel) {
-  if (flags[F_DONE]) return;
-  __shared__ double sh[256];
-  double s = 0.0;
-  for (int p = threadIdx.x; p < nparts; p += 256) s += part_sq[p];
-  s = block_sum<256>(s, sh);
+  s2 = block_sum<256>(s2, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s2;
+  if (!last_group(cnt)) return;
+  double t = 0.0;
+  for (int p = threadIdx.x; p < int(gridDim.x); p += 256) t += part[p];
+  t = block_sum<256>(t, sh);
   if (threadIdx.x != 0) return;
-  const double qn = sqrt(s);
+  const double qn = sqrt(t);
   scal[3] = qn;
   const int c = it - 1;
   auto h = [&](int i, int j) -> double & { return H[size_t(i) * ldh + j]; };
   for (int i = 0; i < it; i++) h(i, c) = h1[i] + h2[i];
   h(it, c) = qn;
   for (int i = 1; i < it; i++) {
-    const double t = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
+    const double tt = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
     h(i, c) = cs[i - 1] * h(i, c) - sn[i - 1] * h(i - 1, c);
-    h(i - 1, c) = t;
+    h(i - 1, c) = tt;
   }
   if (h(it, c) == 0.0) {
     cs[c] = 1.0; sn[c] = 0.0;
   } else if (fabs(h(it, c)) > fabs(h(c, c))) {
-    const double t = h(c, c) / h(it, c);
-    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + t * t);
-    cs[c] = t * sn[c];
+    const double tt = h(c, c) / h(it, c);
+    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + tt * tt);
+    cs[c] = tt * sn[c];
   } else {
-    const double t = h(it, c) / h(c, c);
-    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + t * t);
-    sn[c] = t * cs[c];
+    const double tt = h(it, c) / h(c, c);
+    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + tt * tt);
+    sn[c] = tt * cs[c];
   }
   h(c, c) = cs[c] * h(c, c) + sn[c] * h(it, c);
   h(it, c) = 0.0;
@@ -1275,36 +1339,31 @@ static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, int mo
   pa.Xs = c->Xs; pa.Fs = c->Fs; pa.lds = c->lds;
   pa.Xb = c->Xb; pa.Fb = z; pa.ldb = c->ldb;
   pa.Vs = c->Vs; pa.Vb = c->Vb;
-  pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work;
+  pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work; pa.tile_cnt = wl.d_cnt;
   pa.mask = kBB; pa.accumulate = 0;
   pa.a_s = c->as; pa.a_b = c->ab; pa.mu = c->mu;
-  const int nred = wl.nsplit_tiles ? wl.ntiles : 0;
   int rc;
   if (mode == 2) {
     pa.skip = d.flags + F_SKIP64;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, false, c->stream), "pair bb64")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, false, c->stream), "pair bb64")))
       return rc;
     pa.skip = d.flags + F_SKIP32;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, true, c->stream), "pair bb32")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, true, c->stream), "pair bb32")))
       return rc;
   } else {
     pa.skip = done;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, mode == 1, c->stream), "pair bb")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, mode == 1, c->stream), "pair bb")))
       return rc;
   }
-  hipLaunchKernelGGL(k_kprod_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, z,
-                     c->Vb, out, s->tpart, done);
-  hipLaunchKernelGGL(k_kprod_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
-                     s->sw_tile0, s->sw_ntile, s->tpart, out, done);
+  hipLaunchKernelGGL(k_kprod, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->sw_ntile, s->swcnt, z, c->Vb, out, s->tpart, done);
   return launch_ok(c, "kprod");
 }
 
 static int precond(gmi_ctx *c, SimHost *s, const double *in, double *out, const int *skip) {
   const SimDev &d = s->d;
-  hipLaunchKernelGGL(k_tile_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, in,
-                     s->tpart, skip);
-  hipLaunchKernelGGL(k_precond_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
-                     s->sw_tile0, s->sw_ntile, s->tpart, in, out, s->ou, skip);
+  hipLaunchKernelGGL(k_precond_kt_body, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->sw_ntile, s->swcnt + d.Nsw, s->tpart, in, out, s->ou, skip);
   hipLaunchKernelGGL(k_precond_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, in,
                      s->ou, out, skip);
   return launch_ok(c, "precond");
@@ -1338,7 +1397,9 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
     return hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x");
   }
   if (std::fabs(b0) <= tol * nrhs) return GMI_OK;
-  if (std::fabs(b0) > nrhs) {
+  // a warm start worse than zero (mobility_solver.cpp:1060-1156 keeps the
+  // previous solution) -- or a non-finite one -- is replaced by zero
+  if (!std::isfinite(b0) || std::fabs(b0) > nrhs) {
     if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
       return rc;
     if ((rc = hip_check(c, hipMemcpyAsync(r, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
@@ -1357,7 +1418,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, d.flags);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
-  // scal[0] = ||rhs|| is read by k_givens
+  // scal[0] = ||rhs|| is read by k_psub_givens
   const int ldh = s->d.maxit + 1;
   // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
   // says the solve has converged, so a chunk needs no host decisions
@@ -1368,17 +1429,14 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
       double *w = s->Qk + ldq * it;
       if ((r = precond(c, s, qprev, s->z, d.flags))) return r;
       if ((r = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags))) return r;
-      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h1,
-                         d.flags);
-      hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w, n,
-                         d.flags, nullptr);
-      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->h2,
-                         d.flags);
-      hipLaunchKernelGGL(k_sub, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h2, w, n,
-                         d.flags, s->part);
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, nb, it, s->h1, s->h2,
-                         s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
-                         switch_rel);
+      // CGS2: dots, update + second dots, update + norm + Givens
+      hipLaunchKernelGGL(k_pdots, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->pp,
+                         s->cnt, s->h1, d.flags);
+      hipLaunchKernelGGL(k_psub_dots, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w,
+                         n, s->pp, s->cnt + 1, s->h2, d.flags);
+      hipLaunchKernelGGL(k_psub_givens, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1,
+                         s->h2, w, n, s->pp, s->cnt + 2, s->H, ldh, s->cs, s->sn, s->beta, d.flags,
+                         s->scal, tol, maxit, switch_rel);
       hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, w, n, s->scal, 3);
       if ((r = launch_ok(c, "arnoldi"))) return r;
     }
@@ -1531,6 +1589,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   for (int n = 0; n < c->nsw; n++) {
     if (p->nfil[n] < 1 || p->nfil[n] * S != c->nseg[n])
       return fail(c, GMI_ERR_ARG, "nfil*NSEG must match the context's segment counts");
+    if (c->nblob[n] < 1) return fail(c, GMI_ERR_ARG, "every swimmer needs a blob body");
     Nf += p->nfil[n];
   }
   if (!(p->dt > 0) || !(p->tol > 0) || !(p->gmres_tol > 0) || p->max_broyden_iter < 1 ||
@@ -1605,6 +1664,11 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &s->tmp, np))) return rc;
   if ((rc = dalloc(c, s, &s->scal, 16))) return rc;
   if ((rc = dalloc(c, s, &s->part, nblk(d.npad, 256)))) return rc;
+  // chunk partials of the column dot products (GMRES basis, Broyden D)
+  if ((rc = dalloc(c, s, &s->pp, std::max(size_t(maxit + 1) * nblk(d.npad, kChunk),
+                                          size_t(p->max_broyden_iter) * nblk(d.nbroy, kChunk)))))
+    return rc;
+  if ((rc = dalloc(c, s, &s->cnt, 8))) return rc;
   const size_t nbr = size_t(d.nbroy);
   if ((rc = dalloc(c, s, &s->err, nbr))) return rc;
   if ((rc = dalloc(c, s, &s->nerr, nbr))) return rc;
@@ -1703,6 +1767,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, i1, nt.data(), d.Nsw))) return rc;
   s->sw_tile0 = i0;
   s->sw_ntile = i1;
+  if ((rc = dalloc(c, s, &s->swcnt, 2 * size_t(d.Nsw)))) return rc;
   if ((rc = dalloc(c, s, &s->tpart, 6 * std::max<size_t>(bt.size(), 1)))) return rc;
   if ((rc = dalloc(c, s, &s->ou, 6 * size_t(d.Nsw)))) return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
@@ -1858,8 +1923,8 @@ int gmi_broyden_find_update(gmi_ctx *c) {
   if ((rc = jacobian_inv_mult(c, s, s->err, s->upd))) return rc;
   const int n = d.nbroy;
   if (s->iter > 0)
-    hipLaunchKernelGGL(k_dots, dim3(s->iter), dim3(256), 0, c->stream, s->D, size_t(n), s->iter,
-                       s->err, n, s->bd, nullptr);
+    hipLaunchKernelGGL(k_pdots, dim3(nblk(n, kChunk)), dim3(256), 0, c->stream, s->D, size_t(n),
+                       s->iter, s->err, n, s->pp, s->cnt + 3, s->bd, nullptr);
   hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, s->upd,
                      s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n);
   return launch_ok(c, "find_update");
@@ -1884,8 +1949,8 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
     const int it = s->iter;
     double *Cit = s->C + size_t(n) * it, *Dit = s->D + size_t(n) * it;
     if (it > 0)
-      hipLaunchKernelGGL(k_dots, dim3(it), dim3(256), 0, c->stream, s->D, size_t(n), it, s->nerr,
-                         n, s->bd, nullptr);
+      hipLaunchKernelGGL(k_pdots, dim3(nblk(n, kChunk)), dim3(256), 0, c->stream, s->D, size_t(n),
+                         it, s->nerr, n, s->pp, s->cnt + 3, s->bd, nullptr);
     hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
                        s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n);
     hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Dit, 1.0, s->nerr,

@@ -56,7 +56,8 @@ struct SimHost {
   std::vector<void *> owned;
   double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
          *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
-         *tmp = nullptr, *scal = nullptr, *part = nullptr;
+         *tmp = nullptr, *scal = nullptr, *part = nullptr, *pp = nullptr;
+  int *cnt = nullptr;  // arrival counters of the chunked dot kernels (zero between launches)
   double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
          *bd = nullptr;
   int *h_flags = nullptr;
@@ -71,6 +72,7 @@ struct SimHost {
   BTile *btiles = nullptr;
   int nbtiles = 0;
   int *sw_tile0 = nullptr, *sw_ntile = nullptr;
+  int *swcnt = nullptr;  // per-swimmer tile arrival counters [2][Nsw] (K products, preconditioner)
   double *tpart = nullptr, *ou = nullptr;
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
   bool use_graphs = true;

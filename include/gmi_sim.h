@@ -102,10 +102,15 @@ int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
 
 /* ---- learned warm start (no reference counterpart: the reference's warm
  * starts are the extrapolations of segment.cpp:53-73, filament.cpp:908-1016,
- * rigid_body.cpp:251-271).  A per-segment MLP (27 -> hidden -> hidden -> 6,
- * tanh, BF16 weights, FP32 accumulation) corrects each segment's six
- * extrapolated Broyden unknowns before the first residual of a step.
- * Broyden/GMRES still converge to the same tolerances afterwards. */
+ * rigid_body.cpp:251-271).  A per-segment MLP (51 -> hidden -> hidden -> 6,
+ * tanh, BF16 weights, FP32 accumulation) corrects the extrapolated Broyden
+ * unknowns before the first residual of a step: each segment's multiplier
+ * triple, and segment 0's clamp multiplier.  Its inputs are 27 state
+ * features and the corrections that would have been exact on the previous
+ * 4 steps.  Broyden/GMRES still converge to the same tolerances afterwards;
+ * an attempt that does not converge in about twice the recent iteration
+ * count is redone from the extrapolated guess. */
+#define GMI_WARMSTART_NIN 51
 /* Creates the network with Glorot-uniform weights from `seed`. */
 int gmi_warmstart_init(gmi_ctx *ctx, int hidden, unsigned long long seed);
 /* apply: use the trained network in gmi_step (steps nt >= 1);
@@ -115,7 +120,7 @@ int gmi_warmstart_init(gmi_ctx *ctx, int hidden, unsigned long long seed);
 int gmi_warmstart_configure(gmi_ctx *ctx, int apply, int collect_capacity, int collect_stride);
 int gmi_warmstart_dataset_size(gmi_ctx *ctx, int *rows);
 /* Copies the first `rows` recorded rows (raw, not standardised) to the host:
- * X[rows x 27] features, T[rows x 6] converged - guess.  Rows are in
+ * X[rows x 51] features, T[rows x 6] converged - guess.  Rows are in
  * recording order: per recorded step, segment-major over all filaments. */
 int gmi_warmstart_dataset(gmi_ctx *ctx, float *X, float *T, int rows);
 /* Steps that used the network, and those of them whose learned attempt did
@@ -130,7 +135,7 @@ int gmi_warmstart_param_count(gmi_ctx *ctx, int *n);
 /* FP32 master parameters [W1, b1, W2, b2, W3, b3] (row-major, W as
  * inputs x outputs); get and/or set (set re-rounds the BF16 copy). */
 int gmi_warmstart_params(gmi_ctx *ctx, float *get, const float *set);
-/* Forward pass on B standardised rows X[B x 27] -> Y[B x 6]. */
+/* Forward pass on B standardised rows X[B x 51] -> Y[B x 6]. */
 int gmi_warmstart_forward(gmi_ctx *ctx, const float *X, int B, float *Y);
 /* MSE loss of B rows and its gradient w.r.t. the parameters (through the
  * BF16 weights), grad[param_count]. */

@@ -10,7 +10,7 @@ are extrapolations, segment.cpp:53-73, filament.cpp:908-1016).
 """
 import numpy as np
 
-NIN, NOUT = 27, 6
+NIN, NOUT = 51, 6      # 27 state features + 4 lags x 6 previous targets
 
 
 def bf16_round(w):
