@@ -95,14 +95,9 @@ static int build_list(gmi_ctx *c, int mask, WorkList &wl) {
   for (const TileDesc &T : wl.h_tiles) wl.nsplit_tiles += (T.nsplit > 1);
   if (wl.d_tiles) (void)hipFree(wl.d_tiles);
   if (wl.d_work) (void)hipFree(wl.d_work);
-  if (wl.d_cnt) (void)hipFree(wl.d_cnt);
-  wl.d_tiles = nullptr; wl.d_work = nullptr; wl.d_cnt = nullptr;
+  wl.d_tiles = nullptr; wl.d_work = nullptr;
   if (wl.ntiles == 0) return GMI_OK;
   int rc;
-  if ((rc = hip_check(c, hipMalloc(&wl.d_cnt, sizeof(int) * wl.ntiles), "hipMalloc counters")))
-    return rc;
-  if ((rc = hip_check(c, hipMemset(wl.d_cnt, 0, sizeof(int) * wl.ntiles), "zero counters")))
-    return rc;
   if ((rc = hip_check(c, hipMalloc(&wl.d_tiles, sizeof(TileDesc) * wl.ntiles), "hipMalloc tiles")))
     return rc;
   if ((rc = hip_check(c, hipMalloc(&wl.d_work, sizeof(WorkItem) * wl.nwork), "hipMalloc work")))
@@ -147,7 +142,6 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
   a.part = c->part;
   a.tiles = wl.d_tiles;
   a.work = wl.d_work;
-  a.tile_cnt = wl.d_cnt;
   a.mask = mask;
   a.accumulate = accumulate;
   a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
@@ -164,7 +158,9 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
     ev_mid = c->prof_events[c->prof_used + 1];
     c->prof_used += 2;
   }
-  return hip_check(c, launch_pair(a, wl.nwork, fp32, c->stream, ev_mid), "pair launch");
+  return hip_check(c, launch_pair(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, fp32, c->stream,
+                                  ev_mid),
+                   "pair launch");
 }
 
 }  // namespace gmi
@@ -317,11 +313,9 @@ void gmi_destroy(gmi_ctx *c) {
   for (auto &kv : c->lists) {
     if (kv.second.d_tiles) (void)hipFree(kv.second.d_tiles);
     if (kv.second.d_work) (void)hipFree(kv.second.d_work);
-    if (kv.second.d_cnt) (void)hipFree(kv.second.d_cnt);
   }
   if (c->barrier_tiles.d_tiles) (void)hipFree(c->barrier_tiles.d_tiles);
   if (c->barrier_tiles.d_work) (void)hipFree(c->barrier_tiles.d_work);
-  if (c->barrier_tiles.d_cnt) (void)hipFree(c->barrier_tiles.d_cnt);
   for (auto &st : c->stage) {
     if (st.pinned) (void)hipHostFree(st.pinned);
     if (st.dev) (void)hipFree(st.dev);

@@ -23,7 +23,7 @@ struct TileDesc {
   int t0, nt;         // target range in the species array
   int sseg0, nsseg;   // segment sources of the group
   int sblob0, nsblob; // blob sources of the group
-  int nsplit;         // source splits (>1: the last split to finish reduces)
+  int nsplit;         // source splits (>1: partials reduced afterwards)
   int part0;          // first partial slot of this tile
 };
 
@@ -44,42 +44,21 @@ struct PairArgs {
   int accumulate;           // 0: V = M F, 1: V += M F
   double a_s, a_b, mu;
   const int *skip = nullptr; // if non-null and *skip != 0 the launch is a no-op
-  int *tile_cnt = nullptr;   // per-tile arrival counters (WorkList::d_cnt)
 };
 
 // Work list of one pair launch (built once per mask).
 struct WorkList {
   TileDesc *d_tiles = nullptr;
   WorkItem *d_work = nullptr;
-  int *d_cnt = nullptr;      // per-tile split arrival counters, zero between launches
   int ntiles = 0, nwork = 0, nslots = 0, nsplit_tiles = 0;
   std::vector<TileDesc> h_tiles;
 };
 
-// Cross-workgroup arrival: true in the workgroup that arrives last of
-// `total` (every arrival's earlier global writes are then visible to it);
-// that workgroup resets the counter for the next launch.  Called by every
-// thread of the workgroup; no waiting, so no workgroup can block another.
-__device__ inline bool last_arrival(int *cnt, int total) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = atomicAdd(cnt, 1);
-    last = prev == total - 1;
-    if (last) atomicExch(cnt, 0);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  return true;
-}
-
-// Launchers (gmi_pair.hip).  Split partials are reduced inside the pair
-// kernel by the last split of each tile (fixed split order).  ev_mid
-// (optional) is recorded right after the pair kernel.
-hipError_t launch_pair(const PairArgs &a, int nwork, bool fp32, hipStream_t s,
-                       hipEvent_t ev_mid = nullptr);
+// Launchers (gmi_pair.hip).  Split partials are summed in split order by
+// k_reduce over the first ntiles_reduce tiles (0: the caller sums them).
+// ev_mid (optional) is recorded between the pair kernel and the reduction.
+hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp32,
+                       hipStream_t s, hipEvent_t ev_mid = nullptr);
 // Barrier over the all-sources work list `wl` (split partials in `part`).
 hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb,
                           const double *Xs, const double *Xb, const WorkList &wl,

@@ -439,9 +439,19 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
     double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
     for (int c = 0; c < nc; ++c) P[c * kBlock] = o[c];
   }
-  // the tile's last split to finish sums all partials in split order
-  if (!last_arrival(a.tile_cnt + w.tile, T.nsplit) || !live) return;
-  // split order per component; the loads of several splits are in flight
+}
+
+// Deterministic reduction of split partials (split order), one workgroup per
+// tile.  Several splits' loads are in flight at once.
+__global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
+  if (a.skip && *a.skip) return;
+  const TileDesc T = a.tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (T.nsplit == 1 || tid >= T.nt) return;
+  const bool seg_t = (T.species == 0);
+  const int nc = seg_t ? 6 : 3;
+  double *V = seg_t ? a.Vs : a.Vb;
+  const int ld = seg_t ? a.lds : a.ldb;
   double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const double *P = a.part + size_t(T.part0) * 6 * kBlock + tid;
 #pragma unroll 4
@@ -471,14 +481,13 @@ __device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, do
   }
 }
 
-// Work item = (target tile, source split) of the all-sources work list; the
-// tile's last split to finish sums the split partials in split order.
+// Work item = (target tile, source split) of the all-sources work list;
+// split partials are summed by k_barrier_reduce.
 __global__ __launch_bounds__(kBlock) void k_barrier(
     double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
     const double *__restrict__ Xs, const double *__restrict__ Xb,
     const TileDesc *__restrict__ tiles, const WorkItem *__restrict__ work,
-    double *__restrict__ part, int *__restrict__ tile_cnt, int nseg_fil, double as,
-    double ab, double K) {
+    double *__restrict__ part, int nseg_fil, double as, double ab, double K) {
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
   const WorkItem w = work[blockIdx.x];
   const TileDesc T = tiles[w.tile];
@@ -537,13 +546,21 @@ __global__ __launch_bounds__(kBlock) void k_barrier(
     double *P = part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
     P[0] = fx; P[kBlock] = fy; P[2 * kBlock] = fz;
   }
-  if (!last_arrival(tile_cnt + w.tile, T.nsplit) || !live) return;
+}
+
+__global__ __launch_bounds__(kBlock) void k_barrier_reduce(
+    double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
+    const TileDesc *__restrict__ tiles, const double *__restrict__ part) {
+  const TileDesc T = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (T.nsplit == 1 || tid >= T.nt) return;
+  const int i = T.t0 + tid;
   double f[3] = {0.0, 0.0, 0.0};
   const double *P = part + size_t(T.part0) * 6 * kBlock + tid;
 #pragma unroll 4
   for (int q = 0; q < T.nsplit; ++q)
     for (int c = 0; c < 3; ++c) f[c] += P[size_t(q) * 6 * kBlock + c * kBlock];
-  if (seg_t) {
+  if (T.species == 0) {
     Fs[i] += f[0]; Fs[lds + i] += f[1]; Fs[2 * lds + i] += f[2];
   } else {
     Frep[i] = f[0]; Frep[ldb + i] = f[1]; Frep[2 * ldb + i] = f[2];
@@ -567,17 +584,20 @@ __global__ void k_soa_to_aos(double *__restrict__ dst,
 
 }  // namespace
 
-hipError_t launch_pair(const PairArgs &a, int nwork, bool fp32, hipStream_t s,
-                       hipEvent_t ev_mid) {
+hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp32,
+                       hipStream_t s, hipEvent_t ev_mid) {
   if (nwork <= 0) return hipSuccess;
-  if (!a.tile_cnt) return hipErrorInvalidValue;
   if (fp32)
     hipLaunchKernelGGL(k_pair<float>, dim3(nwork), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_pair<double>, dim3(nwork), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (ev_mid) e = hipEventRecord(ev_mid, s);
+  if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+  if (ntiles_reduce > 0) {
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
+    e = hipGetLastError();
+  }
   return e;
 }
 
@@ -586,9 +606,12 @@ hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb,
                           double *part, int nseg_fil, double as, double ab, double K,
                           hipStream_t s) {
   if (wl.nwork <= 0) return hipSuccess;
-  if (!wl.d_cnt) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_barrier, dim3(wl.nwork), dim3(kBlock), 0, s, Fs, lds, Frep, ldb, Xs, Xb,
-                     wl.d_tiles, wl.d_work, part, wl.d_cnt, nseg_fil, as, ab, K);
+                     wl.d_tiles, wl.d_work, part, nseg_fil, as, ab, K);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || wl.nsplit_tiles == 0) return e;
+  hipLaunchKernelGGL(k_barrier_reduce, dim3(wl.ntiles), dim3(kBlock), 0, s, Fs, lds, Frep, ldb,
+                     wl.d_tiles, part);
   return hipGetLastError();
 }
 

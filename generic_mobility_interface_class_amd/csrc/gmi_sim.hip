@@ -328,20 +328,18 @@ __global__ void k_assemble_rhs(SimDev a, double *rhs) {
 }
 
 // Blob tiles: up to 256 consecutive blobs of one swimmer (tile table built
-// at creation), so the per-blob parts of the preconditioner and of the K
-// products run on many workgroups; per-swimmer sums are reduced in tile
-// order (deterministic).
-// part[t][0..6) = -sum over the tile's blobs of (v_m, d_m x v_m)  (-K^T v)
-__device__ void tile_kt(const SimDev &a, const BTile &T, const double *v, double *part,
-                        double *sh) {
-  const double *b = a.body + size_t(T.sw) * BREC;
-  const m3 Qm = rotm(ga4(b + BQ));
-  const int ld = a.ldb;
+// at creation; the same tiles as the blob-blob pair work list), so the
+// per-blob parts of the preconditioner and of the K products run on many
+// workgroups.  Per-swimmer sums over tiles are taken in tile order by each
+// consumer (deterministic, no cross-workgroup synchronisation inside a
+// kernel).
+
+// -(sum over the tile's blobs of (v_m, d_m x v_m)) = the tile's share of -K^T v
+__device__ void tile_kt_vals(const m3 &Qm, const SimDev &a, int m, bool live, d3 l,
+                             double *part, double *sh) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  if (threadIdx.x < T.n) {
-    const int m = T.m0 + threadIdx.x;
-    const d3 l = ld3(v, ld, m);
-    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, ld, m)), l);
+  if (live) {
+    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, a.ldb, m)), l);
     acc[0] = -l.x; acc[1] = -l.y; acc[2] = -l.z;
     acc[3] = -c.x; acc[4] = -c.y; acc[5] = -c.z;
   }
@@ -351,228 +349,288 @@ __device__ void tile_kt(const SimDev &a, const BTile &T, const double *v, double
   }
 }
 
-// preconditioner, -K^T in per tile; the swimmer's last tile to finish sums
-// the tile partials (tile order) and solves the body rows
-// (mobility_solver.cpp:651-717): out_U = (K^T K)^-1 b - N in_U
-__global__ __launch_bounds__(256) void k_precond_kt_body(SimDev a, const BTile *tiles,
-                                                         const int *sw_tile0, const int *sw_ntile,
-                                                         int *sw_cnt, double *part,
-                                                         const double *in, double *out,
-                                                         double *ou_all, const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  __shared__ double sh[256];
-  const BTile T = tiles[blockIdx.x];
-  tile_kt(a, T, in, part + 6 * blockIdx.x, sh);
-  const int n = T.sw;
-  if (!last_arrival(sw_cnt + n, sw_ntile[n]) || threadIdx.x != 0) return;
+// Rotated body matrices of every swimmer for one rigid-body solve (the body
+// orientation is fixed during it): rot[n] = [Q N_ref Q^T | Q (K^T K)^-1 Q^T]
+__global__ void k_rotate_body(SimDev a, double *rot) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
   const double *b = a.body + size_t(n) * BREC;
   const m3 Qm = rotm(ga4(b + BQ));
-  double bb[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = sw_tile0[n]; t < sw_tile0[n] + sw_ntile[n]; t++)
-    for (int k = 0; k < 6; k++) bb[k] += part[6 * t + k];
-  double bm[36], kk[36];
-  rotate66(b + BN, Qm, bm);
-  rotate66(b + BK, Qm, kk);
-  const int vid = 3 * a.ldb + 6 * n;
-  for (int r = 0; r < 6; r++) {
-    double s1 = 0, s2 = 0;
-    for (int k = 0; k < 6; k++) { s1 += kk[6 * r + k] * bb[k]; s2 += bm[6 * r + k] * in[vid + k]; }
-    const double o = s1 - s2;
-    out[vid + r] = o;
-    ou_all[6 * n + r] = o;
-  }
+  rotate66(b + BN, Qm, rot + 72 * size_t(n));
+  rotate66(b + BK, Qm, rot + 72 * size_t(n) + 36);
 }
 
-// preconditioner, blob rows: out = M^-1 (in + U + W x d)
-__global__ __launch_bounds__(256) void k_precond_blobs(SimDev a, const BTile *tiles,
-                                                       const double *in, const double *ou_all,
-                                                       double *out, const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  const BTile T = tiles[blockIdx.x];
-  if (threadIdx.x >= T.n) return;
-  const int m = T.m0 + threadIdx.x;
-  const double *b = a.body + size_t(T.sw) * BREC;
-  const m3 Qm = rotm(ga4(b + BQ));
-  const double *ou = ou_all + 6 * T.sw;
-  const d3 V = {ou[0], ou[1], ou[2]}, W = {ou[3], ou[4], ou[5]};
-  const int ld = a.ldb;
-  const double Minv = 6.0 * kPi * a.mu * a.ab;
-  const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
-  st3(out, ld, m, Minv * (ld3(in, ld, m) + V + cross3(W, d)));
-}
-
-// system_matrix_mult K products (mobility_solver.cpp:905-964), blob rows:
-// out_lambda = M_bb z - (U + W x d); with the tile's -K^T z partials, summed
-// in tile order by the swimmer's last tile into the body rows out_U = -K^T z
-__global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, const int *sw_tile0,
-                                               const int *sw_ntile, int *sw_cnt, const double *z,
-                                               const double *vb, double *out, double *part,
-                                               const int *flags) {
+// Preconditioner pass 1 (mobility_solver.cpp:651-717): the tile's share of
+// -K^T in.  `in` is the Krylov vector of the previous Arnoldi step, still
+// unnormalised: its norm scal[3] (from k_givens) is divided out here and the
+// normalised vector written back (each tile owns its blob rows, the
+// swimmer's first tile its 6 body rows).  norm_slot < 0: `in` is used as is.
+__global__ __launch_bounds__(256) void k_precond_kt(SimDev a, const BTile *tiles,
+                                                    const int *sw_tile0, double *in,
+                                                    const double *scal, int norm_slot,
+                                                    double *part, const int *flags) {
   if (flags && flags[F_DONE]) return;
   __shared__ double sh[256];
   const BTile T = tiles[blockIdx.x];
   const double *b = a.body + size_t(T.sw) * BREC;
   const m3 Qm = rotm(ga4(b + BQ));
-  const int ld = a.ldb;
-  const int vid = 3 * ld + 6 * T.sw;
-  if (threadIdx.x < T.n) {
-    const int m = T.m0 + threadIdx.x;
-    const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
-    const d3 d = mv(Qm, ld3(a.blob_ref, ld, m));
-    st3(out, ld, m, ld3(vb, ld, m) - (V + cross3(W, d)));
+  const double nrm = norm_slot >= 0 ? scal[norm_slot] : 1.0;
+  const bool live = threadIdx.x < T.n;
+  const int m = T.m0 + threadIdx.x;
+  d3 l = {0, 0, 0};
+  if (live) {
+    l = ld3(in, a.ldb, m);
+    if (norm_slot >= 0) {
+      l = {l.x / nrm, l.y / nrm, l.z / nrm};
+      st3(in, a.ldb, m, l);
+    }
   }
-  tile_kt(a, T, z, part + 6 * blockIdx.x, sh);
-  const int n = T.sw;
-  if (!last_arrival(sw_cnt + n, sw_ntile[n]) || threadIdx.x != 0) return;
-  double bb[6] = {0, 0, 0, 0, 0, 0};
-  for (int t = sw_tile0[n]; t < sw_tile0[n] + sw_ntile[n]; t++)
-    for (int k = 0; k < 6; k++) bb[k] += part[6 * t + k];
-  for (int k = 0; k < 6; k++) out[vid + k] = bb[k];  // vid: swimmer n's U rows
+  if (norm_slot >= 0 && blockIdx.x == sw_tile0[T.sw] && threadIdx.x < 6)
+    in[3 * a.ldb + 6 * T.sw + threadIdx.x] /= nrm;
+  tile_kt_vals(Qm, a, m, live, l, part + 6 * blockIdx.x, sh);
+}
+
+// Preconditioner pass 2: each tile sums its swimmer's -K^T in partials (tile
+// order), solves the body rows out_U = (K^T K)^-1 b - N in_U, and writes its
+// blobs out = M^-1 (in + U + W x d); the swimmer's first tile writes out_U.
+// Also the tile's share of -K^T out, which the K products need next.
+__global__ __launch_bounds__(256) void k_precond_blobs(SimDev a, const BTile *tiles,
+                                                       const int *sw_tile0, const int *sw_ntile,
+                                                       const double *part, const double *rot,
+                                                       const double *in, double *out,
+                                                       double *part_out, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  __shared__ double sb[6], so[6];
+  const BTile T = tiles[blockIdx.x];
+  const int n = T.sw, t = threadIdx.x;
+  const int vid = 3 * a.ldb + 6 * n;
+  if (t < 6) {
+    double s = 0.0;
+    for (int q = sw_tile0[n]; q < sw_tile0[n] + sw_ntile[n]; q++) s += part[6 * q + t];
+    sb[t] = s;
+  }
+  __syncthreads();
+  if (t < 6) {
+    const double *bm = rot + 72 * size_t(n), *kk = bm + 36;
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < 6; k++) { s1 += kk[6 * t + k] * sb[k]; s2 += bm[6 * t + k] * in[vid + k]; }
+    so[t] = s1 - s2;
+    if (blockIdx.x == sw_tile0[n]) out[vid + t] = s1 - s2;
+  }
+  __syncthreads();
+  const double *b = a.body + size_t(n) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const bool live = t < T.n;
+  const int m = T.m0 + t;
+  d3 z = {0, 0, 0};
+  if (live) {
+    const d3 V = {so[0], so[1], so[2]}, W = {so[3], so[4], so[5]};
+    const double Minv = 6.0 * kPi * a.mu * a.ab;
+    const d3 d = mv(Qm, ld3(a.blob_ref, a.ldb, m));
+    z = Minv * (ld3(in, a.ldb, m) + V + cross3(W, d));
+    st3(out, a.ldb, m, z);
+  }
+  tile_kt_vals(Qm, a, m, live, z, part_out + 6 * blockIdx.x, sh);
+}
+
+// system_matrix_mult K products (mobility_solver.cpp:905-964): blob rows
+// out = M_bb z - (U + W x d), with M_bb z summed here from the pair kernel's
+// split partials (split order) when the tile was split; body rows
+// out_U = -K^T z from the tile shares of k_precond_blobs (tile order),
+// written by the swimmer's first tile.
+__global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, const int *sw_tile0,
+                                               const int *sw_ntile, const TileDesc *ptiles,
+                                               const double *ppart, const double *z,
+                                               const double *vb, double *out,
+                                               const double *zpart, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const BTile T = tiles[blockIdx.x];
+  const int n = T.sw, t = threadIdx.x;
+  const int vid = 3 * a.ldb + 6 * n;
+  if (blockIdx.x == sw_tile0[n] && t < 6) {
+    double s = 0.0;
+    for (int q = sw_tile0[n]; q < sw_tile0[n] + sw_ntile[n]; q++) s += zpart[6 * q + t];
+    out[vid + t] = s;
+  }
+  if (t >= T.n) return;
+  const int m = T.m0 + t;
+  const TileDesc P = ptiles[blockIdx.x];
+  d3 v;
+  if (P.nsplit == 1) {
+    v = ld3(vb, a.ldb, m);
+  } else {
+    double s[3] = {0.0, 0.0, 0.0};
+    const double *pp = ppart + size_t(P.part0) * 6 * kBlock + t;
+#pragma unroll 4
+    for (int q = 0; q < P.nsplit; ++q)
+      for (int c = 0; c < 3; ++c) s[c] += pp[size_t(q) * 6 * kBlock + c * kBlock];
+    v = {s[0], s[1], s[2]};
+  }
+  const double *b = a.body + size_t(n) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
+  const d3 d = mv(Qm, ld3(a.blob_ref, a.ldb, m));
+  st3(out, a.ldb, m, v - (V + cross3(W, d)));
 }
 
 // -------------------------------------------------------------------------
 // vector kernels (GMRES / Broyden)
 
-// Dot products with many columns, chunked over the rows: each workgroup
-// takes kChunk consecutive rows, writes one partial per column, and the
-// workgroup that finishes last sums the partials in chunk order (fixed
-// order: the result does not depend on scheduling).
-constexpr int kChunk = 256;
+// Dot products with many columns, chunked over the rows: workgroup b takes
+// R consecutive rows (R a multiple of 256, at most 128 workgroups until R
+// reaches kMaxRows) and writes one partial per column, part[j * nb + b];
+// each consumer sums a column's partials in workgroup order.  No
+// cross-workgroup synchronisation inside a kernel: the kernel boundary makes
+// the partials visible.
+constexpr int kMaxDotBlocks = 128;
+constexpr int kMaxRows = 6144;     // rows per workgroup (LDS: R doubles, + kMaxCols)
+constexpr int kMaxCols = 1024;     // Krylov / Broyden columns held in LDS
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-// partials of this workgroup's rows (values in sw) with columns j < k:
-// part[j * nchunk + chunk]; wave w takes the columns j = w (mod 4)
-__device__ void chunk_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
-                           double *part, int nchunk) {
+// partials of rows [e0, e0+R) (values in sw) with columns j < k; wave w
+// takes the columns j = w (mod 4)
+__device__ void col_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
+                         int R, double *part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rmax = min(R, n - e0);
   for (int j = wave; j < k; j += 4) {
     const double *q = Qb + ldq * j + e0;
     double acc = 0.0;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int i = lane + 64 * r;
-      if (e0 + i < n) acc += q[i] * sw[i];
-    }
+    for (int r = lane; r < rmax; r += 64) acc += q[r] * sw[r];
     acc = wave_sum(acc);
-    if (lane == 0) part[size_t(j) * nchunk + blockIdx.x] = acc;
+    if (lane == 0) part[size_t(j) * gridDim.x + blockIdx.x] = acc;
   }
 }
 
-// true in the workgroup that arrives last; every workgroup's earlier global
-// writes are visible to it.  It resets the counter for the next launch.
-__device__ bool last_group(int *cnt) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = atomicAdd(cnt, 1);
-    last = prev == int(gridDim.x) - 1;
-    if (last) atomicExch(cnt, 0);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  return true;
-}
-
-// h[j] = sum over chunks of part[j][chunk], in chunk order
-__device__ void reduce_parts(const double *part, int k, int nchunk, double *h) {
+// h[j] = sum over workgroups of part[j][b] (workgroup order), j < k, into
+// LDS (and, from workgroup 0, into global hout)
+__device__ void col_sums(const double *part, int k, int nb, double *hs, double *hout) {
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
-    const double *p = part + size_t(j) * nchunk;
+    const double *p = part + size_t(j) * nb;
     double s = 0.0;
-    for (int c = 0; c < nchunk; c++) s += p[c];
-    h[j] = s;
+#pragma unroll 4
+    for (int b = 0; b < nb; b++) s += p[b];
+    hs[j] = s;
+    if (hout && blockIdx.x == 0) hout[j] = s;
   }
 }
 
-// h[j] = <Q_j, w>, j < k.  Grid: ceil(n / kChunk) workgroups of 256.
+// partials of h[j] = <Q_j, w>, j < k
 __global__ __launch_bounds__(256) void k_pdots(const double *Qb, size_t ldq, int k,
-                                               const double *w, int n, double *part, int *cnt,
-                                               double *h, const int *flags) {
+                                               const double *w, int n, int R, double *part,
+                                               const int *flags) {
   if (flags && flags[F_DONE]) return;
-  __shared__ double sw[kChunk];
-  const int e0 = blockIdx.x * kChunk, e = e0 + threadIdx.x;
-  sw[threadIdx.x] = e < n ? w[e] : 0.0;
+  extern __shared__ double sw[];
+  const int e0 = blockIdx.x * R;
+  for (int r = threadIdx.x; r < R; r += 256) sw[r] = (e0 + r < n) ? w[e0 + r] : 0.0;
   __syncthreads();
-  chunk_dots(Qb, ldq, k, sw, e0, n, part, gridDim.x);
-  if (!last_group(cnt)) return;
-  reduce_parts(part, k, gridDim.x, h);
+  col_dots(Qb, ldq, k, sw, e0, n, R, part);
 }
 
-// Gram-Schmidt pass 1 update w -= sum_j h1[j] Q_j, then pass 2 dots
-// h2[j] = <Q_j, w> of the updated vector
+// h[j] = sum of the partials (one workgroup per 256 columns)
+__global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= k) return;
+  const double *p = part + size_t(j) * nb;
+  double s = 0.0;
+#pragma unroll 4
+  for (int b = 0; b < nb; b++) s += p[b];
+  h[j] = s;
+}
+
+// Gram-Schmidt pass 1: h1 = sum of the pass-1 partials; w -= sum_j h1[j] Q_j;
+// then the partials of the pass-2 dots <Q_j, w>
 __global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq, int k,
-                                                   const double *h1, double *w, int n,
-                                                   double *part, int *cnt, double *h2,
+                                                   const double *part_in, double *h1, double *w,
+                                                   int n, int R, double *part_out,
                                                    const int *flags) {
   if (flags[F_DONE]) return;
-  __shared__ double sw[kChunk];
-  const int e0 = blockIdx.x * kChunk, e = e0 + threadIdx.x;
-  double s = 0.0;
-  if (e < n) {
-    s = w[e];
-    for (int j = 0; j < k; j++) s -= h1[j] * Qb[ldq * j + e];
-    w[e] = s;
-  }
-  sw[threadIdx.x] = s;
+  __shared__ double hs[kMaxCols];
+  extern __shared__ double sw[];
+  col_sums(part_in, k, gridDim.x, hs, h1);
   __syncthreads();
-  chunk_dots(Qb, ldq, k, sw, e0, n, part, gridDim.x);
-  if (!last_group(cnt)) return;
-  reduce_parts(part, k, gridDim.x, h2);
+  const int e0 = blockIdx.x * R;
+  for (int r = threadIdx.x; r < R; r += 256) {
+    const int e = e0 + r;
+    double v = 0.0;
+    if (e < n) {
+      v = w[e];
+      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
+      w[e] = v;
+    }
+    sw[r] = v;
+  }
+  __syncthreads();
+  col_dots(Qb, ldq, k, sw, e0, n, R, part_out);
 }
 
-// Gram-Schmidt pass 2 update w -= sum_j h2[j] Q_j; the last workgroup sums
-// the squared norm over chunks and takes the Givens step of Arnoldi
-// iteration `it` (mobility_solver.cpp:1183-1234).  H column = h1 + h2 (the
-// two passes together); the division by the norm is k_scale_by.
-__global__ __launch_bounds__(256) void k_psub_givens(const double *Qb, size_t ldq, int it,
-                                                     const double *h1, const double *h2,
-                                                     double *w, int n, double *part, int *cnt,
-                                                     double *H, int ldh, double *cs, double *sn,
-                                                     double *beta, int *flags, double *scal,
-                                                     double tol, int maxit, double switch_rel) {
+// Gram-Schmidt pass 2: h2 = sum of the pass-2 partials; w -= sum_j h2[j] Q_j;
+// the workgroup's share of |w|^2
+__global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq, int k,
+                                                   const double *part_in, double *h2, double *w,
+                                                   int n, int R, double *norm_part,
+                                                   const int *flags) {
   if (flags[F_DONE]) return;
+  __shared__ double hs[kMaxCols];
   __shared__ double sh[256];
-  const int e = blockIdx.x * kChunk + threadIdx.x;
+  col_sums(part_in, k, gridDim.x, hs, h2);
+  __syncthreads();
+  const int e0 = blockIdx.x * R;
   double s2 = 0.0;
-  if (e < n) {
-    double s = w[e];
-    for (int j = 0; j < it; j++) s -= h2[j] * Qb[ldq * j + e];
-    w[e] = s;
-    s2 = s * s;
+  for (int r = threadIdx.x; r < R; r += 256) {
+    const int e = e0 + r;
+    if (e < n) {
+      double v = w[e];
+      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
+      w[e] = v;
+      s2 += v * v;
+    }
   }
   s2 = block_sum<256>(s2, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s2;
-  if (!last_group(cnt)) return;
-  double t = 0.0;
-  for (int p = threadIdx.x; p < int(gridDim.x); p += 256) t += part[p];
-  t = block_sum<256>(t, sh);
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
+}
+
+// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
+// new vector's norm is the sum of the workgroup shares; H column = h1 + h2
+// (the two Gram-Schmidt passes).  The division by the norm happens in the
+// next iteration's k_precond_kt.
+__global__ __launch_bounds__(256) void k_givens(const double *norm_part, int nparts, int it,
+                                                const double *h1, const double *h2, double *H,
+                                                int ldh, double *cs, double *sn, double *beta,
+                                                int *flags, double *scal, double tol, int maxit,
+                                                double switch_rel) {
+  if (flags[F_DONE]) return;
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += norm_part[p];
+  s = block_sum<256>(s, sh);
   if (threadIdx.x != 0) return;
-  const double qn = sqrt(t);
+  const double qn = sqrt(s);
   scal[3] = qn;
   const int c = it - 1;
   auto h = [&](int i, int j) -> double & { return H[size_t(i) * ldh + j]; };
   for (int i = 0; i < it; i++) h(i, c) = h1[i] + h2[i];
   h(it, c) = qn;
   for (int i = 1; i < it; i++) {
-    const double tt = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
+    const double t = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
     h(i, c) = cs[i - 1] * h(i, c) - sn[i - 1] * h(i - 1, c);
-    h(i - 1, c) = tt;
+    h(i - 1, c) = t;
   }
   if (h(it, c) == 0.0) {
     cs[c] = 1.0; sn[c] = 0.0;
   } else if (fabs(h(it, c)) > fabs(h(c, c))) {
-    const double tt = h(c, c) / h(it, c);
-    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + tt * tt);
-    cs[c] = tt * sn[c];
+    const double t = h(c, c) / h(it, c);
+    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + t * t);
+    cs[c] = t * sn[c];
   } else {
-    const double tt = h(it, c) / h(c, c);
-    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + tt * tt);
-    sn[c] = tt * cs[c];
+    const double t = h(it, c) / h(c, c);
+    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + t * t);
+    sn[c] = t * cs[c];
   }
   h(c, c) = cs[c] * h(c, c) + sn[c] * h(it, c);
   h(it, c) = 0.0;
@@ -964,6 +1022,86 @@ __global__ __launch_bounds__(256) void k_gj_invert(SimDev a) {
   }
 }
 
+// The same Gauss-Jordan inverse with the matrix held in LDS (n*n doubles,
+// 115 KB at NSEG = 20; dynamic shared memory), 1024 threads: thread
+// (column j = t % 128, row group t / 128) updates rows j-group, +8, ...
+// Used when the device's per-workgroup LDS holds the matrix.
+constexpr int kGjThreads = 1024;
+
+__global__ __launch_bounds__(kGjThreads) void k_gj_invert_lds(SimDev a) {
+  extern __shared__ double lds_gj[];
+  __shared__ int sp;
+  __shared__ double spv;
+  const int f = blockIdx.x;
+  const int n = 6 * a.S;
+  const int t = threadIdx.x;
+  double *A = lds_gj;
+  double *colk = A + size_t(n) * n;
+  int *piv = reinterpret_cast<int *>(colk + n);
+  const double *G = a.Jw + size_t(f) * n * n;
+  for (int e = t; e < n * n; e += kGjThreads) A[e] = G[e];
+  for (int i = t; i < n; i += kGjThreads) piv[i] = i;
+  const int jc = t & 127, rg = t >> 7;
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    // pivot: first row of maximal |A(i,k)|, i >= k (wave 0)
+    if (t < 64) {
+      double best = -1.0;
+      int bi = n;
+      for (int i = k + t; i < n; i += 64) {
+        const double v = fabs(A[size_t(i) * n + k]);
+        if (v > best) { best = v; bi = i; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(best, o, 64);
+        const int i2 = __shfl_xor(bi, o, 64);
+        if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+      }
+      if (t == 0) { sp = bi; spv = best; }
+    }
+    __syncthreads();
+    const int p = sp;
+    if (!(spv > 0.0)) {
+      if (t == 0) atomicOr(&a.flags[F_SING], 1);
+      return;
+    }
+    if (p != k) {
+      for (int j = t; j < n; j += kGjThreads) {
+        const double x = A[size_t(k) * n + j];
+        A[size_t(k) * n + j] = A[size_t(p) * n + j];
+        A[size_t(p) * n + j] = x;
+      }
+      if (t == 0) { const int x = piv[k]; piv[k] = piv[p]; piv[p] = x; }
+    }
+    __syncthreads();
+    const double inv = 1.0 / A[size_t(k) * n + k];
+    for (int i = t; i < n; i += kGjThreads) colk[i] = A[size_t(i) * n + k];
+    __syncthreads();
+    for (int j = t; j < n; j += kGjThreads)
+      A[size_t(k) * n + j] = (j == k ? 1.0 : A[size_t(k) * n + j]) * inv;
+    __syncthreads();
+    if (jc < n) {
+      const double akj = A[size_t(k) * n + jc];
+      for (int i = rg; i < n; i += kGjThreads / 128) {
+        if (i == k) continue;
+        const double base = (jc == k) ? 0.0 : A[size_t(i) * n + jc];
+        A[size_t(i) * n + jc] = base - colk[i] * akj;
+      }
+    }
+    __syncthreads();
+  }
+  double *Out = a.Jinv + size_t(f) * n * n;
+  for (int e = t; e < n * n; e += kGjThreads) {
+    const int i = e / n, j = e % n;
+    Out[size_t(i) * n + piv[j]] = A[e];
+  }
+}
+
+inline size_t gj_lds_bytes(int S) {
+  const size_t n = 6 * size_t(S);
+  return n * n * sizeof(double) + n * sizeof(double) + n * sizeof(int);
+}
+
 // Schur-complement blocks of one filament (swimmer.cpp:526-680):
 // Bblk = B J^-1 (6 x 6S) and BA = Bblk A (6 x 6).
 __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
@@ -1328,44 +1466,65 @@ static int read_flags(gmi_ctx *c, SimHost *s) {
 
 static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
 
+// Row split of the column dot products: nb workgroups of R rows.
+struct DotShape {
+  int nb, R;
+};
+static DotShape dot_shape(int n) {
+  int per = (nblk(n, 256) + kMaxDotBlocks - 1) / kMaxDotBlocks;  // 256-row chunks per workgroup
+  per = std::max(1, std::min(per, kMaxRows / 256));
+  const int R = per * 256;
+  return {nblk(n, R), R};
+}
+
 // system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964).
 // mode 0: FP64 pairs, 1: FP32 pairs, 2: both launched, each gated by its
 // device flag (F_SKIP64 / F_SKIP32) so the precision switch needs no sync.
-static int sys_mult(gmi_ctx *c, SimHost *s, const double *z, double *out, int mode,
-                    const int *done) {
+// The tile shares of -K^T z are in s->zpart when zpart_ready (z came from
+// the preconditioner); otherwise they are computed first.
+static int sys_mult(gmi_ctx *c, SimHost *s, double *z, double *out, int mode, const int *done,
+                    bool zpart_ready) {
   SimDev &d = s->d;
+  if (!zpart_ready)
+    hipLaunchKernelGGL(k_precond_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                       s->sw_tile0, z, s->scal, -1, s->zpart, done);
   PairArgs pa;
   const WorkList &wl = c->lists[kBB];
   pa.Xs = c->Xs; pa.Fs = c->Fs; pa.lds = c->lds;
   pa.Xb = c->Xb; pa.Fb = z; pa.ldb = c->ldb;
   pa.Vs = c->Vs; pa.Vb = c->Vb;
-  pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work; pa.tile_cnt = wl.d_cnt;
+  pa.part = c->part; pa.tiles = wl.d_tiles; pa.work = wl.d_work;
   pa.mask = kBB; pa.accumulate = 0;
   pa.a_s = c->as; pa.a_b = c->ab; pa.mu = c->mu;
   int rc;
+  // split partials are summed by k_kprod (its tiles are the list's tiles)
   if (mode == 2) {
     pa.skip = d.flags + F_SKIP64;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, false, c->stream), "pair bb64")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, false, c->stream), "pair bb64")))
       return rc;
     pa.skip = d.flags + F_SKIP32;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, true, c->stream), "pair bb32")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, true, c->stream), "pair bb32")))
       return rc;
   } else {
     pa.skip = done;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, mode == 1, c->stream), "pair bb")))
+    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, mode == 1, c->stream), "pair bb")))
       return rc;
   }
   hipLaunchKernelGGL(k_kprod, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                     s->sw_tile0, s->sw_ntile, s->swcnt, z, c->Vb, out, s->tpart, done);
+                     s->sw_tile0, s->sw_ntile, wl.d_tiles, c->part, z, c->Vb, out, s->zpart,
+                     done);
   return launch_ok(c, "kprod");
 }
 
-static int precond(gmi_ctx *c, SimHost *s, const double *in, double *out, const int *skip) {
+// apply_preconditioner: out = P^-1 in.  norm_slot >= 0: `in` is first
+// divided by scal[norm_slot] in place (the previous Arnoldi vector's norm).
+static int precond(gmi_ctx *c, SimHost *s, double *in, double *out, const int *skip,
+                   int norm_slot) {
   const SimDev &d = s->d;
-  hipLaunchKernelGGL(k_precond_kt_body, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                     s->sw_tile0, s->sw_ntile, s->swcnt + d.Nsw, s->tpart, in, out, s->ou, skip);
-  hipLaunchKernelGGL(k_precond_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, in,
-                     s->ou, out, skip);
+  hipLaunchKernelGGL(k_precond_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, in, s->scal, norm_slot, s->tpart, skip);
+  hipLaunchKernelGGL(k_precond_blobs, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->sw_ntile, s->tpart, s->rot, in, out, s->zpart, skip);
   return launch_ok(c, "precond");
 }
 
@@ -1383,7 +1542,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   *its = 0;
   // r = rhs - A x ; norms
   double *r = s->Qk;
-  if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr))) return rc;
+  if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr, false))) return rc;
   hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
   if ((rc = launch_ok(c, "gmres init"))) return rc;
@@ -1418,26 +1577,32 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, d.flags);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
-  // scal[0] = ||rhs|| is read by k_psub_givens
+  // scal[0] = ||rhs|| is read by k_givens
   const int ldh = s->d.maxit + 1;
   // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
   // says the solve has converged, so a chunk needs no host decisions
+  const DotShape ds = dot_shape(n);
+  const size_t shm = sizeof(double) * ds.R;
+  double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
   auto enqueue = [&](int it0, int it1) -> int {
     int r;
     for (int it = it0; it <= it1; it++) {
       double *qprev = s->Qk + ldq * (it - 1);
       double *w = s->Qk + ldq * it;
-      if ((r = precond(c, s, qprev, s->z, d.flags))) return r;
-      if ((r = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags))) return r;
-      // CGS2: dots, update + second dots, update + norm + Givens
-      hipLaunchKernelGGL(k_pdots, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, w, n, s->pp,
-                         s->cnt, s->h1, d.flags);
-      hipLaunchKernelGGL(k_psub_dots, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1, w,
-                         n, s->pp, s->cnt + 1, s->h2, d.flags);
-      hipLaunchKernelGGL(k_psub_givens, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, it, s->h1,
-                         s->h2, w, n, s->pp, s->cnt + 2, s->H, ldh, s->cs, s->sn, s->beta, d.flags,
-                         s->scal, tol, maxit, switch_rel);
-      hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, w, n, s->scal, 3);
+      // the previous vector is normalised by the preconditioner's first
+      // kernel (the first one was scaled by beta0 already)
+      if ((r = precond(c, s, qprev, s->z, d.flags, it >= 2 ? 3 : -1))) return r;
+      if ((r = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags, true))) return r;
+      // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
+      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
+                         ds.R, pp1, d.flags);
+      hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
+                         s->h1, w, n, ds.R, pp2, d.flags);
+      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
+                         s->h2, w, n, ds.R, s->part, d.flags);
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
+                         s->h2, s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
+                         switch_rel);
       if ((r = launch_ok(c, "arnoldi"))) return r;
     }
     return GMI_OK;
@@ -1485,7 +1650,7 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
   hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, d.flags, s->y,
                      s->tmp, n);
   if ((rc = launch_ok(c, "gmres solution"))) return rc;
-  if ((rc = precond(c, s, s->tmp, s->z, nullptr))) return rc;
+  if ((rc = precond(c, s, s->tmp, s->z, nullptr, -1))) return rc;
   hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
   if ((rc = launch_ok(c, "gmres update"))) return rc;
   *its = nits;
@@ -1496,8 +1661,11 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
 // then the true residual with the FP64 operator; if it misses the tolerance
 // GMRES restarts from the current iterate (iterative refinement, FP64 outer).
 static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
-  if (s->precision == GMI_FP64) return gmres(c, s, false, s->p.gmres_tol, its);
   SimDev &d = s->d;
+  hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
+  int rc0;
+  if ((rc0 = launch_ok(c, "rotate body"))) return rc0;
+  if (s->precision == GMI_FP64) return gmres(c, s, false, s->p.gmres_tol, its);
   const int n = d.npad, nb = nblk(n, 256);
   int rc, total = 0;
   for (int cycle = 0; cycle < 4; cycle++) {
@@ -1505,7 +1673,7 @@ static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
     if ((rc = gmres(c, s, true, s->p.gmres_tol, &it))) return rc;
     total += it;
     // true residual with the FP64 operator
-    if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr))) return rc;
+    if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr, false))) return rc;
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->z, 1.0, s->rhs, -1.0,
                        s->tmp, n);
     hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5);
@@ -1593,7 +1761,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
     Nf += p->nfil[n];
   }
   if (!(p->dt > 0) || !(p->tol > 0) || !(p->gmres_tol > 0) || p->max_broyden_iter < 1 ||
-      p->gmres_max_iter < 1 || p->gmres_max_iter > 4000 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
+      p->gmres_max_iter < 1 || p->gmres_max_iter > kMaxCols - 1 || !(p->sphere_radius > 0) || !(p->curvature_period > 0))
     return fail(c, GMI_ERR_ARG, "bad sim parameters");
   if (p->sphere_radii)
     for (int n = 0; n < c->nsw; n++)
@@ -1665,10 +1833,11 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &s->scal, 16))) return rc;
   if ((rc = dalloc(c, s, &s->part, nblk(d.npad, 256)))) return rc;
   // chunk partials of the column dot products (GMRES basis, Broyden D)
-  if ((rc = dalloc(c, s, &s->pp, std::max(size_t(maxit + 1) * nblk(d.npad, kChunk),
-                                          size_t(p->max_broyden_iter) * nblk(d.nbroy, kChunk)))))
-    return rc;
-  if ((rc = dalloc(c, s, &s->cnt, 8))) return rc;
+  // column dot partials: two halves (Gram-Schmidt passes 1 and 2)
+  s->pp_half = std::max(size_t(maxit + 1) * dot_shape(d.npad).nb,
+                        size_t(p->max_broyden_iter) * dot_shape(d.nbroy).nb);
+  if ((rc = dalloc(c, s, &s->pp, 2 * s->pp_half))) return rc;
+  if ((rc = dalloc(c, s, &s->rot, 72 * size_t(d.Nsw)))) return rc;
   const size_t nbr = size_t(d.nbroy);
   if ((rc = dalloc(c, s, &s->err, nbr))) return rc;
   if ((rc = dalloc(c, s, &s->nerr, nbr))) return rc;
@@ -1767,9 +1936,32 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, i1, nt.data(), d.Nsw))) return rc;
   s->sw_tile0 = i0;
   s->sw_ntile = i1;
-  if ((rc = dalloc(c, s, &s->swcnt, 2 * size_t(d.Nsw)))) return rc;
+  // Gauss-Jordan in LDS when one workgroup can hold the filament matrix
+  {
+    int lds_max = 0;
+    if ((rc = hip_check(c, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                                 c->device), "LDS size")))
+      return rc;
+    const size_t need = gj_lds_bytes(d.S) + 64;  // + the kernel's static shared scalars
+    s->gj_lds = need <= size_t(lds_max);
+    if (s->gj_lds && need > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(k_gj_invert_lds),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(gj_lds_bytes(d.S))) != hipSuccess) {
+      (void)hipGetLastError();
+      s->gj_lds = false;
+    }
+  }
   if ((rc = dalloc(c, s, &s->tpart, 6 * std::max<size_t>(bt.size(), 1)))) return rc;
-  if ((rc = dalloc(c, s, &s->ou, 6 * size_t(d.Nsw)))) return rc;
+  if ((rc = dalloc(c, s, &s->zpart, 6 * std::max<size_t>(bt.size(), 1)))) return rc;
+  // k_kprod reads the blob-blob work list's tile i as blob tile i
+  {
+    const WorkList &wl = c->lists[kBB];
+    bool same = wl.ntiles == int(bt.size());
+    for (int i = 0; same && i < wl.ntiles; i++)
+      same = wl.h_tiles[i].t0 == bt[i].m0 && wl.h_tiles[i].nt == bt[i].n;
+    if (!same) return fail(c, GMI_ERR_STATE, "blob tiles differ from the blob-blob work list");
+  }
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
@@ -1904,7 +2096,11 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   const double t = (nt + 1) * s->p.dt;
   hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
   hipLaunchKernelGGL(k_fil_jacobian, dim3(d.Nf), dim3(64), 0, c->stream, d, nt, t);
-  hipLaunchKernelGGL(k_gj_invert, dim3(d.Nf), dim3(256), 0, c->stream, d);
+  if (s->gj_lds)
+    hipLaunchKernelGGL(k_gj_invert_lds, dim3(d.Nf), dim3(kGjThreads), gj_lds_bytes(d.S), c->stream,
+                       d);
+  else
+    hipLaunchKernelGGL(k_gj_invert, dim3(d.Nf), dim3(256), 0, c->stream, d);
   hipLaunchKernelGGL(k_schur_fil, dim3(d.Nf), dim3(256), 0, c->stream, d, nt);
   hipLaunchKernelGGL(k_schur_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, nt);
   int rc;
@@ -1922,9 +2118,13 @@ int gmi_broyden_find_update(gmi_ctx *c) {
   int rc;
   if ((rc = jacobian_inv_mult(c, s, s->err, s->upd))) return rc;
   const int n = d.nbroy;
-  if (s->iter > 0)
-    hipLaunchKernelGGL(k_pdots, dim3(nblk(n, kChunk)), dim3(256), 0, c->stream, s->D, size_t(n),
-                       s->iter, s->err, n, s->pp, s->cnt + 3, s->bd, nullptr);
+  if (s->iter > 0) {
+    const DotShape ds = dot_shape(n);
+    hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
+                       size_t(n), s->iter, s->err, n, ds.R, s->pp, nullptr);
+    hipLaunchKernelGGL(k_colsum, dim3(nblk(s->iter, 256)), dim3(256), 0, c->stream, s->pp,
+                       s->iter, ds.nb, s->bd);
+  }
   hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, s->upd,
                      s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n);
   return launch_ok(c, "find_update");
@@ -1948,9 +2148,13 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
     if ((rc = jacobian_inv_mult(c, s, s->nerr, s->upd))) return rc;
     const int it = s->iter;
     double *Cit = s->C + size_t(n) * it, *Dit = s->D + size_t(n) * it;
-    if (it > 0)
-      hipLaunchKernelGGL(k_pdots, dim3(nblk(n, kChunk)), dim3(256), 0, c->stream, s->D, size_t(n),
-                         it, s->nerr, n, s->pp, s->cnt + 3, s->bd, nullptr);
+    if (it > 0) {
+      const DotShape ds = dot_shape(n);
+      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
+                         size_t(n), it, s->nerr, n, ds.R, s->pp, nullptr);
+      hipLaunchKernelGGL(k_colsum, dim3(nblk(it, 256)), dim3(256), 0, c->stream, s->pp, it, ds.nb,
+                         s->bd);
+    }
     hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
                        s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n);
     hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Dit, 1.0, s->nerr,

@@ -57,7 +57,8 @@ struct SimHost {
   double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
          *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
          *tmp = nullptr, *scal = nullptr, *part = nullptr, *pp = nullptr;
-  int *cnt = nullptr;  // arrival counters of the chunked dot kernels (zero between launches)
+  size_t pp_half = 0;  // column dot partials: pp[0, pp_half) and pp[pp_half, 2 pp_half)
+  double *rot = nullptr;  // rotated N_ref and (K^T K)^-1 per swimmer [Nsw][72]
   double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
          *bd = nullptr;
   int *h_flags = nullptr;
@@ -67,13 +68,13 @@ struct SimHost {
   int last_gmres = 0;
   int chunk = 6;
   double iters_avg = 0.0;  // moving average of Broyden iterations per step
+  bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS
   Learn *learn = nullptr;
   // blob tiles for the preconditioner and K products
   BTile *btiles = nullptr;
   int nbtiles = 0;
   int *sw_tile0 = nullptr, *sw_ntile = nullptr;
-  int *swcnt = nullptr;  // per-swimmer tile arrival counters [2][Nsw] (K products, preconditioner)
-  double *tpart = nullptr, *ou = nullptr;
+  double *tpart = nullptr, *zpart = nullptr;  // tile shares of -K^T v (preconditioner, K products)
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
   bool use_graphs = true;
   int graph_gen = -1;

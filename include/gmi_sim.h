@@ -38,7 +38,8 @@ typedef struct gmi_sim_params {
   double jacobian_confidence; /* JACOBIAN_CONFIDENCE_FACTOR (config.hpp:140) */
   double gmres_tol;           /* LINEAR_SYSTEM_TOL (config.hpp:154)          */
   int max_broyden_iter;       /* MAX_BROYDEN_ITER (config.hpp:142)           */
-  int gmres_max_iter;         /* MAX_LINEAR_SYSTEM_ITER (config.hpp:153)     */
+  int gmres_max_iter;         /* MAX_LINEAR_SYSTEM_ITER (config.hpp:153),
+                                 at most 1023                                */
   int num_euler_steps;        /* NUM_EULER_STEPS (config.hpp:167)            */
   double barrier_K;           /* REPULSIVE_FORCE_FACTOR*END_FORCE_MAGNITUDE  */
   double curvature_amp;       /* preferred curvature amplitude (strain_twist) */
