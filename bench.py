@@ -127,6 +127,7 @@ def bench_step(args):
             sim.warmstart_configure(apply=False, collect=(args.step_warmup // stride) * sim.Ns,
                                     stride=stride)
         sim.make_body_reference_matrices()
+        shape = sim.autotune_blocks(capi.GMI_BLOCK_BB, reps=10)
         sims[name] = sim
     t0 = time.perf_counter()
     for nt in range(args.step_warmup):
@@ -165,6 +166,7 @@ def bench_step(args):
     fp64 = res["fp64"]["steps_per_s"]
     return {"workload": args.step_workload, "warmup_steps": args.step_warmup,
             "steps_per_window": args.step_steps, "window": f"{nt0}-{nt0 + args.step_steps - 1}",
+            "blob_blob_launch_shape": list(shape),
             **res,
             "speedup_learned_mixed_vs_fp64": res["learned_mixed"]["steps_per_s"] / fp64,
             "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / fp64,
@@ -218,6 +220,7 @@ def bench_ensemble(args):
         sim.set_graphs(not args.no_graphs)
         t0 = time.perf_counter()
         sim.make_body_reference_matrices()
+        sim.autotune_blocks(capi.GMI_BLOCK_BB, reps=5)
         setup_s = time.perf_counter() - t0
         for nt in range(2):
             assert sim.step(nt)[0] == 0
@@ -242,6 +245,7 @@ def bench_rollout(args):
     learned.warmstart_configure(apply=False, collect=20000, stride=5)
     for s in (plain, learned):
         s.make_body_reference_matrices()
+        s.autotune_blocks(capi.GMI_BLOCK_BB, reps=10)
         for nt in range(300):
             assert s.step(nt)[0] == 0
     rows = learned.warmstart_dataset_size()

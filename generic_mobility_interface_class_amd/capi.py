@@ -21,6 +21,7 @@ GMI_ERR_NOT_FINITE = -4
 GMI_ERR_SINGULAR = -5
 
 GMI_FP64 = 0
+GMI_BLOCK_SS, GMI_BLOCK_SB, GMI_BLOCK_BS, GMI_BLOCK_BB = 1, 2, 4, 8
 WARMSTART_NIN = 51   # GMI_WARMSTART_NIN
 GMI_FP32_KAHAN = 1
 
@@ -72,6 +73,7 @@ SIGNATURES = [
     ("gmi_profile_end", C.c_int, [C.c_void_p, _dp, _ip]),
     ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gmi_autotune", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
+    ("gmi_autotune_blocks", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),
     # gmi_sim.h
     ("gmi_sim_create", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gmi_make_body_reference_matrices", C.c_int, [C.c_void_p]),
@@ -221,6 +223,11 @@ class Context:
     def autotune(self, reps=3):
         t, ch = C.c_int(0), C.c_int(0)
         self.call("gmi_autotune", int(reps), C.byref(t), C.byref(ch))
+        return t.value, ch.value
+
+    def autotune_blocks(self, mask, reps=10):
+        t, ch = C.c_int(0), C.c_int(0)
+        self.call("gmi_autotune_blocks", int(mask), int(reps), C.byref(t), C.byref(ch))
         return t.value, ch.value
 
 

@@ -462,29 +462,28 @@ int gmi_set_launch_shape(gmi_ctx *c, int target_blocks, int min_chunk);
 // problem shape for the life of the process.
 namespace {
 struct TuneKey {
-  int Ns, Nb, nsw, interact, precision;
+  int Ns, Nb, nsw, interact, precision, mask;
   bool operator<(const TuneKey &o) const {
-    return std::tie(Ns, Nb, nsw, interact, precision) <
-           std::tie(o.Ns, o.Nb, o.nsw, o.interact, o.precision);
+    return std::tie(Ns, Nb, nsw, interact, precision, mask) <
+           std::tie(o.Ns, o.Nb, o.nsw, o.interact, o.precision, o.mask);
   }
 };
 std::map<TuneKey, std::pair<int, int>> g_tuned;
-}  // namespace
 
-int gmi_autotune(gmi_ctx *c, int reps, int *best_target, int *best_chunk) {
-  if (!c || reps < 1) return GMI_ERR_ARG;
-  const TuneKey key{c->Ns, c->Nb, c->nsw, int(c->interact), c->precision};
+// Times `reps` launches of the `mask` blocks for each launch shape of a grid
+// and keeps the fastest (cached per sizes, precision and mask).
+int tune_shape(gmi_ctx *c, int mask, int reps, int *best_target, int *best_chunk) {
+  const TuneKey key{c->Ns, c->Nb, c->nsw, int(c->interact), c->precision, mask};
   int rc;
   auto it = g_tuned.find(key);
   if (it == g_tuned.end()) {
-    const int targets[] = {512, 1024, 2048, 4096, 8192};
-    const int chunks[] = {256, 512, 1024, 2048};
+    const int targets[] = {256, 512, 1024, 2048, 4096, 8192};
+    const int chunks[] = {64, 128, 256, 512, 1024, 2048};
     double best = 1e300;
     std::pair<int, int> arg{c->target_blocks, c->min_chunk};
     hipEvent_t e0, e1;
     if ((rc = hip_check(c, hipEventCreate(&e0), "event"))) return rc;
     if ((rc = hip_check(c, hipEventCreate(&e1), "event"))) return rc;
-    const int mask = kSS | kSB | kBS | kBB;
     const bool fp32 = c->precision == GMI_FP32_KAHAN;
     for (int tb : targets)
       for (int ch : chunks) {
@@ -510,6 +509,18 @@ int gmi_autotune(gmi_ctx *c, int reps, int *best_target, int *best_chunk) {
   if (best_target) *best_target = it->second.first;
   if (best_chunk) *best_chunk = it->second.second;
   return GMI_OK;
+}
+}  // namespace
+
+int gmi_autotune(gmi_ctx *c, int reps, int *best_target, int *best_chunk) {
+  if (!c || reps < 1) return GMI_ERR_ARG;
+  return tune_shape(c, kSS | kSB | kBS | kBB, reps, best_target, best_chunk);
+}
+
+int gmi_autotune_blocks(gmi_ctx *c, int mask, int reps, int *best_target, int *best_chunk) {
+  if (!c) return GMI_ERR_ARG;
+  if (reps < 1 || mask < 1 || mask > 15) return fail(c, GMI_ERR_ARG, "mask in [1, 15], reps >= 1");
+  return tune_shape(c, mask, reps, best_target, best_chunk);
 }
 
 int gmi_set_launch_shape(gmi_ctx *c, int target_blocks, int min_chunk) {

@@ -80,6 +80,11 @@ __device__ bool gj_small(double *A, int n) {
   return true;
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <int NT>
 __device__ double block_sum(double v, double *sh) {
   const int t = threadIdx.x;
@@ -335,6 +340,7 @@ __global__ void k_assemble_rhs(SimDev a, double *rhs) {
 // kernel).
 
 // -(sum over the tile's blobs of (v_m, d_m x v_m)) = the tile's share of -K^T v
+// (wave sums, then the 4 waves in order; sh holds 24 doubles)
 __device__ void tile_kt_vals(const m3 &Qm, const SimDev &a, int m, bool live, d3 l,
                              double *part, double *sh) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -343,10 +349,15 @@ __device__ void tile_kt_vals(const m3 &Qm, const SimDev &a, int m, bool live, d3
     acc[0] = -l.x; acc[1] = -l.y; acc[2] = -l.z;
     acc[3] = -c.x; acc[4] = -c.y; acc[5] = -c.z;
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int k = 0; k < 6; k++) {
-    const double r = block_sum<256>(acc[k], sh);
-    if (threadIdx.x == 0) part[k] = r;
+    const double r = wave_sum(acc[k]);
+    if (lane == 0) sh[6 * wave + k] = r;
   }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    part[threadIdx.x] = ((sh[threadIdx.x] + sh[6 + threadIdx.x]) + sh[12 + threadIdx.x]) +
+                        sh[18 + threadIdx.x];
 }
 
 // Rotated body matrices of every swimmer for one rigid-body solve (the body
@@ -487,11 +498,6 @@ constexpr int kMaxDotBlocks = 128;
 constexpr int kMaxRows = 6144;     // rows per workgroup (LDS: R doubles, + kMaxCols)
 constexpr int kMaxCols = 1024;     // Krylov / Broyden columns held in LDS
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // partials of rows [e0, e0+R) (values in sw) with columns j < k; wave w
 // takes the columns j = w (mod 4)
 __device__ void col_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
@@ -532,6 +538,19 @@ __global__ __launch_bounds__(256) void k_pdots(const double *Qb, size_t ldq, int
   col_dots(Qb, ldq, k, sw, e0, n, R, part);
 }
 
+// h[j] = sum of the partials, one workgroup per column (large n: many
+// partials per column)
+__global__ __launch_bounds__(256) void k_colsum_wide(const double *part, int nb, double *h,
+                                                     const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  const double *p = part + size_t(blockIdx.x) * nb;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) h[blockIdx.x] = s;
+}
+
 // h[j] = sum of the partials (one workgroup per 256 columns)
 __global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h) {
   const int j = blockIdx.x * 256 + threadIdx.x;
@@ -545,14 +564,18 @@ __global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int n
 
 // Gram-Schmidt pass 1: h1 = sum of the pass-1 partials; w -= sum_j h1[j] Q_j;
 // then the partials of the pass-2 dots <Q_j, w>
+// (h_ready: h1 was summed by k_colsum_wide already)
 __global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq, int k,
                                                    const double *part_in, double *h1, double *w,
                                                    int n, int R, double *part_out,
-                                                   const int *flags) {
+                                                   const int *flags, int h_ready) {
   if (flags[F_DONE]) return;
   __shared__ double hs[kMaxCols];
   extern __shared__ double sw[];
-  col_sums(part_in, k, gridDim.x, hs, h1);
+  if (h_ready)
+    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h1[j];
+  else
+    col_sums(part_in, k, gridDim.x, hs, h1);
   __syncthreads();
   const int e0 = blockIdx.x * R;
   for (int r = threadIdx.x; r < R; r += 256) {
@@ -574,11 +597,14 @@ __global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq,
 __global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq, int k,
                                                    const double *part_in, double *h2, double *w,
                                                    int n, int R, double *norm_part,
-                                                   const int *flags) {
+                                                   const int *flags, int h_ready) {
   if (flags[F_DONE]) return;
   __shared__ double hs[kMaxCols];
   __shared__ double sh[256];
-  col_sums(part_in, k, gridDim.x, hs, h2);
+  if (h_ready)
+    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h2[j];
+  else
+    col_sums(part_in, k, gridDim.x, hs, h2);
   __syncthreads();
   const int e0 = blockIdx.x * R;
   double s2 = 0.0;
@@ -606,36 +632,46 @@ __global__ __launch_bounds__(256) void k_givens(const double *norm_part, int npa
                                                 double switch_rel) {
   if (flags[F_DONE]) return;
   __shared__ double sh[256];
+  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
   double s = 0.0;
   for (int p = threadIdx.x; p < nparts; p += 256) s += norm_part[p];
+  // the new column and the previous rotations, staged in LDS
+  for (int i = threadIdx.x; i < it; i += 256) {
+    hc[i] = h1[i] + h2[i];
+    if (i < it - 1) { scs[i] = cs[i]; ssn[i] = sn[i]; }
+  }
   s = block_sum<256>(s, sh);
-  if (threadIdx.x != 0) return;
-  const double qn = sqrt(s);
-  scal[3] = qn;
   const int c = it - 1;
-  auto h = [&](int i, int j) -> double & { return H[size_t(i) * ldh + j]; };
-  for (int i = 0; i < it; i++) h(i, c) = h1[i] + h2[i];
-  h(it, c) = qn;
-  for (int i = 1; i < it; i++) {
-    const double t = cs[i - 1] * h(i - 1, c) + sn[i - 1] * h(i, c);
-    h(i, c) = cs[i - 1] * h(i, c) - sn[i - 1] * h(i - 1, c);
-    h(i - 1, c) = t;
+  if (threadIdx.x == 0) {
+    const double qn = sqrt(s);
+    scal[3] = qn;
+    hc[it] = qn;
+    for (int i = 1; i < it; i++) {
+      const double t = scs[i - 1] * hc[i - 1] + ssn[i - 1] * hc[i];
+      hc[i] = scs[i - 1] * hc[i] - ssn[i - 1] * hc[i - 1];
+      hc[i - 1] = t;
+    }
+    double csc, snc;
+    if (hc[it] == 0.0) {
+      csc = 1.0; snc = 0.0;
+    } else if (fabs(hc[it]) > fabs(hc[c])) {
+      const double t = hc[c] / hc[it];
+      snc = (2.0 * double(hc[it] > 0) - 1.0) / sqrt(1.0 + t * t);
+      csc = t * snc;
+    } else {
+      const double t = hc[it] / hc[c];
+      csc = (2.0 * double(hc[c] > 0) - 1.0) / sqrt(1.0 + t * t);
+      snc = t * csc;
+    }
+    cs[c] = csc; sn[c] = snc;
+    hc[c] = csc * hc[c] + snc * hc[it];
+    hc[it] = 0.0;
+    beta[it] = -snc * beta[c];
+    beta[c] = csc * beta[c];
   }
-  if (h(it, c) == 0.0) {
-    cs[c] = 1.0; sn[c] = 0.0;
-  } else if (fabs(h(it, c)) > fabs(h(c, c))) {
-    const double t = h(c, c) / h(it, c);
-    sn[c] = (2.0 * double(h(it, c) > 0) - 1.0) / sqrt(1.0 + t * t);
-    cs[c] = t * sn[c];
-  } else {
-    const double t = h(it, c) / h(c, c);
-    cs[c] = (2.0 * double(h(c, c) > 0) - 1.0) / sqrt(1.0 + t * t);
-    sn[c] = t * cs[c];
-  }
-  h(c, c) = cs[c] * h(c, c) + sn[c] * h(it, c);
-  h(it, c) = 0.0;
-  beta[it] = -sn[c] * beta[c];
-  beta[c] = cs[c] * beta[c];
+  __syncthreads();
+  for (int i = threadIdx.x; i <= it; i += 256) H[size_t(i) * ldh + c] = hc[i];
+  if (threadIdx.x != 0) return;
   const double rel = fabs(beta[it]) / scal[0];
   if (rel <= tol || it == maxit) {
     flags[F_DONE] = 1;
@@ -1467,14 +1503,19 @@ static int read_flags(gmi_ctx *c, SimHost *s) {
 static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
 
 // Row split of the column dot products: nb workgroups of R rows.
+// Up to kMaxDotBlocks workgroups every consumer workgroup re-sums a column's
+// partials itself; beyond that (wide) a k_colsum_wide launch sums them once
+// and up to ~1024 workgroups share the rows.
 struct DotShape {
   int nb, R;
+  bool wide;
 };
 static DotShape dot_shape(int n) {
-  int per = (nblk(n, 256) + kMaxDotBlocks - 1) / kMaxDotBlocks;  // 256-row chunks per workgroup
-  per = std::max(1, std::min(per, kMaxRows / 256));
+  const int chunks = nblk(n, 256);
+  if (chunks <= kMaxDotBlocks) return {chunks, 256, false};
+  const int per = std::min((chunks + 1023) / 1024, kMaxRows / 256);  // 256-row chunks per workgroup
   const int R = per * 256;
-  return {nblk(n, R), R};
+  return {nblk(n, R), R, true};
 }
 
 // system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964).
@@ -1596,10 +1637,16 @@ static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
       // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
       hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
                          ds.R, pp1, d.flags);
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
+                           d.flags);
       hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
-                         s->h1, w, n, ds.R, pp2, d.flags);
+                         s->h1, w, n, ds.R, pp2, d.flags, int(ds.wide));
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
+                           d.flags);
       hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
-                         s->h2, w, n, ds.R, s->part, d.flags);
+                         s->h2, w, n, ds.R, s->part, d.flags, int(ds.wide));
       hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
                          s->h2, s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
                          switch_rel);
@@ -2122,8 +2169,12 @@ int gmi_broyden_find_update(gmi_ctx *c) {
     const DotShape ds = dot_shape(n);
     hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
                        size_t(n), s->iter, s->err, n, ds.R, s->pp, nullptr);
-    hipLaunchKernelGGL(k_colsum, dim3(nblk(s->iter, 256)), dim3(256), 0, c->stream, s->pp,
-                       s->iter, ds.nb, s->bd);
+    if (ds.wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(s->iter), dim3(256), 0, c->stream, s->pp, ds.nb,
+                         s->bd, nullptr);
+    else
+      hipLaunchKernelGGL(k_colsum, dim3(nblk(s->iter, 256)), dim3(256), 0, c->stream, s->pp,
+                         s->iter, ds.nb, s->bd);
   }
   hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, s->upd,
                      s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n);
@@ -2152,8 +2203,12 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
       const DotShape ds = dot_shape(n);
       hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
                          size_t(n), it, s->nerr, n, ds.R, s->pp, nullptr);
-      hipLaunchKernelGGL(k_colsum, dim3(nblk(it, 256)), dim3(256), 0, c->stream, s->pp, it, ds.nb,
-                         s->bd);
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, s->pp, ds.nb, s->bd,
+                           nullptr);
+      else
+        hipLaunchKernelGGL(k_colsum, dim3(nblk(it, 256)), dim3(256), 0, c->stream, s->pp, it,
+                           ds.nb, s->bd);
     }
     hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
                        s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n);

@@ -128,6 +128,14 @@ int gmi_set_launch_shape(gmi_ctx *ctx, int target_blocks, int min_chunk);
  * launch shapes (reps launches each), applies the fastest and reports it.
  * The choice is cached per (sizes, precision) for the process. */
 int gmi_autotune(gmi_ctx *ctx, int reps, int *best_target_blocks, int *best_min_chunk);
+/* The same for the launches of one block subset: mask is an OR of GMI_BLOCK_*
+ * (GMI_BLOCK_BB alone is the operator of the rigid-body GMRES). */
+#define GMI_BLOCK_SS 1 /* segment <- segment (Mss) */
+#define GMI_BLOCK_SB 2 /* segment <- blob (Msb)    */
+#define GMI_BLOCK_BS 4 /* blob <- segment (Mbs)    */
+#define GMI_BLOCK_BB 8 /* blob <- blob (Mbb)       */
+int gmi_autotune_blocks(gmi_ctx *ctx, int mask, int reps, int *best_target_blocks,
+                        int *best_min_chunk);
 
 #ifdef __cplusplus
 }
