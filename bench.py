@@ -37,6 +37,18 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
 
+def measured_traffic():
+    """HBM bytes per FP32 grand-apply launch at 46,722 particles from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy of known
+    size), recorded next to the profiles; None if absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_pair_apply_46722.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def finite_json(v):
     """Non-finite floats become null (strict JSON)."""
     if isinstance(v, dict):
@@ -299,6 +311,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
     ap.add_argument("--no-rollout", action="store_true")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="only the 46,722-particle apply (no CPU baseline, no time steps)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device (a transfer of known size for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the Arnoldi kernels eagerly (for kernel-trace profiling)")
     args = ap.parse_args()
@@ -306,6 +323,17 @@ def main():
         print("bench.py measures one GPU only: every box here has one MI355X; "
               "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
         sys.exit(2)
+
+    if args.apply_only:
+        args.no_cpu = args.no_step = args.no_ensemble = args.no_rollout = True
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {src.numel() * 4} bytes read and written",
+              file=sys.stderr, flush=True)
+        del src, dst
 
     g = geometry.workload_geometry(args.workload)
     xs, xb = g.positions()
@@ -358,12 +386,15 @@ def main():
             "blocks": "ss+sb+bs+bb", "precision": "FP32 pairs, Kahan cross-tile, FP64 reduce",
         },
         "roofline": {
-            "bound": "valu",
+            # compute-bound: the published FP32 peak (the same figure for the
+            # vector units this kernel runs on and for the matrix cores)
+            "bound": "mfma",
+            "units": "VALU FP32 (pair sums are not a contraction; no MFMA)",
             "achieved": round(achieved32, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved32 / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "k_pair<float>",
             "kernel_ms": r32["kernel_ms"],
             "launch_shape": r32["launch_shape"],
