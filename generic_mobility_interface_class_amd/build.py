@@ -20,6 +20,7 @@ HIP_SOURCES = [
     "gmi_capi.hip",
     "gmi_sim.hip",
     "gmi_learn.hip",
+    "gmi_io.hip",
 ]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would

@@ -97,6 +97,11 @@ SIGNATURES = [
     ("gmi_warmstart_dataset", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
                                         C.POINTER(C.c_float), C.c_int]),
     ("gmi_warmstart_stats", C.c_int, [C.c_void_p, _ip, _ip]),
+    ("gmi_sim_write_par", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int]),
+    ("gmi_sim_write_reference_positions", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("gmi_sim_write_data", C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+    ("gmi_sim_write_backup", C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int]),
+    ("gmi_sim_read_backup", C.c_int, [C.c_void_p, C.c_char_p, _ip, _ip]),
     ("gmi_warmstart_train", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double,
                                       C.c_ulonglong, _dp]),
     ("gmi_warmstart_param_count", C.c_int, [C.c_void_p, _ip]),
@@ -299,6 +304,25 @@ class Sim(Context):
 
     def set_graphs(self, use_graphs=True, chunk=6):
         self.call("gmi_sim_set_graphs", int(bool(use_graphs)), int(chunk))
+
+    # ---- output and checkpoint files (reference formats) --------------
+    def write_par(self, name, total_time_steps, plot_frequency):
+        self.call("gmi_sim_write_par", str(name).encode(), int(total_time_steps),
+                  int(plot_frequency))
+
+    def write_reference_positions(self, name):
+        self.call("gmi_sim_write_reference_positions", str(name).encode())
+
+    def write_data(self, nt, name):
+        self.call("gmi_sim_write_data", int(nt), str(name).encode())
+
+    def write_backup(self, nt, name, extended=True):
+        self.call("gmi_sim_write_backup", int(nt), str(name).encode(), int(bool(extended)))
+
+    def read_backup(self, name):
+        nt, exact = C.c_int(0), C.c_int(0)
+        self.call("gmi_sim_read_backup", str(name).encode(), C.byref(nt), C.byref(exact))
+        return nt.value, bool(exact.value)
 
     # ---- learned warm start ------------------------------------------
     def warmstart_init(self, hidden=128, seed=0):

@@ -421,6 +421,23 @@ int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied) {
 
 double *learn_correction(SimHost *s) { return s->learn ? s->learn->corr : nullptr; }
 
+// the lag features (host copy; empty without a network) for checkpoints
+int learn_lags(gmi_ctx *c, SimHost *s, std::vector<float> *get, const std::vector<float> *set) {
+  Learn *L = s->learn;
+  const size_t n = L ? size_t(s->d.Ns) * 6 * kLags : 0;
+  if (get) {
+    get->assign(n, 0.0f);
+    if (n && hipMemcpy(get->data(), L->lags, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(c, GMI_ERR_HIP, "lags D2H");
+  }
+  if (set) {
+    if (set->size() != n) return fail(c, GMI_ERR_ARG, "lag features of another size");
+    if (n && hipMemcpy(L->lags, set->data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, GMI_ERR_HIP, "lags H2D");
+  }
+  return GMI_OK;
+}
+
 // after a learned attempt failed: the update back to the extrapolated guess
 // (into the correction buffer), counted as a fallback
 int learn_back_to_guess(gmi_ctx *c, SimHost *s, double **delta) {
@@ -651,8 +668,12 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
   float *SX = nullptr, *ST = nullptr;
   if ((rc = lalloc(c, L, &SX, size_t(N) * ni))) return rc;
   if ((rc = lalloc(c, L, &ST, size_t(N) * no))) return rc;
-  hipMemcpyAsync(SX, L->DX, sizeof(float) * N * ni, hipMemcpyDeviceToDevice, c->stream);
-  hipMemcpyAsync(ST, L->DT, sizeof(float) * N * no, hipMemcpyDeviceToDevice, c->stream);
+  if ((rc = hip_check(c, hipMemcpyAsync(SX, L->DX, sizeof(float) * N * ni, hipMemcpyDeviceToDevice,
+                                        c->stream), "copy dataset")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(ST, L->DT, sizeof(float) * N * no, hipMemcpyDeviceToDevice,
+                                        c->stream), "copy dataset")))
+    return rc;
   hipLaunchKernelGGL(k_standardise, dim3(nb(N * ni, 256)), dim3(256), 0, c->stream, SX, N, ni,
                      L->xs, L->xs + ni);
   hipLaunchKernelGGL(k_standardise, dim3(nb(N * no, 256)), dim3(256), 0, c->stream, ST, N, no,
@@ -680,7 +701,9 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
     int *h = hidx + (it & 1) * batch;
     (void)hipEventSynchronize(ev);  // the pinned half written two iterations ago is consumed
     for (int i = 0; i < batch; i++) h[i] = int(next() % uint64_t(N));
-    hipMemcpyAsync(L->idx, h, sizeof(int) * batch, hipMemcpyHostToDevice, c->stream);
+    if ((rc = hip_check(c, hipMemcpyAsync(L->idx, h, sizeof(int) * batch, hipMemcpyHostToDevice,
+                                          c->stream), "batch indices")))
+      break;
     hipLaunchKernelGGL(k_gather, dim3(nb(batch * (ni + no), 256)), dim3(256), 0, c->stream, SX, ST,
                        L->idx, batch, ni, no, L->Xb, L->Tb);
     if ((rc = mlp_backward(c, L, L->Xb, L->Tb, batch))) break;
@@ -691,8 +714,10 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
     (void)hipEventRecord(ev, c->stream);
     if (it >= iters - 20) {
       float lf = 0.f;
-      hipMemcpyAsync(&lf, L->loss, sizeof(float), hipMemcpyDeviceToHost, c->stream);
-      (void)hipStreamSynchronize(c->stream);
+      if ((rc = hip_check(c, hipMemcpyAsync(&lf, L->loss, sizeof(float), hipMemcpyDeviceToHost,
+                                            c->stream), "loss")))
+        break;
+      if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) break;
       lsum += lf;
       lcount++;
     }

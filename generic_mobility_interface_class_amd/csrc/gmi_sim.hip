@@ -1815,7 +1815,20 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
       if (!(p->sphere_radii[n] > 0)) return fail(c, GMI_ERR_ARG, "sphere_radii must be positive");
   SimHost *s = new SimHost();
   s->p = *p;
-  s->p.sphere_radii = nullptr;  // copied to the device below; the caller keeps its array
+  // the caller's arrays need not outlive this call: keep copies of what is
+  // read later (blob and filament references), drop the other pointers
+  {
+    size_t nbr = 0, nfr = 0;
+    for (int n = 0; n < c->nsw; n++) { nbr += 3 * size_t(c->nblob[n]); nfr += 3 * size_t(p->nfil[n]); }
+    s->blob_refs_h.assign(p->blob_refs, p->blob_refs + nbr);
+    s->fil_refs_h.assign(p->fil_refs, p->fil_refs + nfr);
+    s->p.blob_refs = s->blob_refs_h.data();
+    s->p.fil_refs = s->fil_refs_h.data();
+    s->p.nfil = nullptr;
+    s->p.body_x = nullptr;
+    s->p.fil_phase = nullptr;
+    s->p.sphere_radii = nullptr;
+  }
   s->precision = c->precision;
   c->extra = s;
   c->extra_release = sim_release;

@@ -50,7 +50,8 @@ struct BTile {
 
 struct SimHost {
   SimDev d;
-  gmi_sim_params p;
+  gmi_sim_params p;          // pointer members re-pointed at the copies below
+  std::vector<double> blob_refs_h, fil_refs_h;
   std::vector<int> nfil;
   // device buffers owned
   std::vector<void *> owned;

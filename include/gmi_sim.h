@@ -101,6 +101,37 @@ int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
  * graph and replayed (no reference counterpart).  Default: graphs on, 6. */
 int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
 
+/* ---- output and checkpoint files in the reference's text formats.  `name`
+ * is SIMULATION_NAME (a path prefix); files as in config.hpp:208-215. ---- */
+/* <name>.par (cilia_sim_main.cpp:79-116); the format describes identical
+ * swimmers, so uneven batches are refused. */
+int gmi_sim_write_par(gmi_ctx *ctx, const char *name, int total_time_steps,
+                      int plot_frequency_in_steps);
+/* <name>_blob_references.dat and <name>_fil_references.dat of swimmer 0
+ * (swimmer.cpp:885-900, rigid_body.cpp:383-400, cilia_sim_main.cpp:155). */
+int gmi_sim_write_reference_positions(gmi_ctx *ctx, const char *name);
+/* Appends one line per file after step nt (cilia_sim_main.cpp:346-366):
+ * _seg_states.dat and _body_states.dat start with nt+1 (filament.cpp:
+ * 1577-1591, rigid_body.cpp:408-412); _body_vels.dat, _blob_forces.dat,
+ * _seg_vels.dat and _seg_forces.dat start with nt (mobility_solver.cpp:
+ * 1775-1895).  Values "%.10e " (std::scientific, precision 10). */
+int gmi_sim_write_data(gmi_ctx *ctx, int nt, const char *name);
+/* Overwrites <name>.backup after step nt (cilia_sim_main.cpp:368-381,
+ * swimmer.cpp:914-925, filament.cpp:1593-1650, rigid_body.cpp:414-432).
+ * extended != 0 also writes <name>.backup_gmi (no reference counterpart):
+ * the same state at full precision plus the GMRES warm start, the previous
+ * velocities and the learned warm start's lag features, from which a resumed
+ * run repeats the original bit for bit. */
+int gmi_sim_write_backup(gmi_ctx *ctx, int nt, const char *name, int extended);
+/* Loads <name>.backup into the sim (cilia_sim_main.cpp:29-71, filament.cpp:
+ * 84-163, rigid_body.cpp:28-56; q_m1 = q; the GMRES warm start is left as
+ * it is, i.e. the last solve's solution, as in a resumed reference run), then
+ * <name>.backup_gmi if present (*exact = 1).  *nt_of_backup is the stored
+ * step count: continue with gmi_step(ctx, *nt_of_backup, ...) as the
+ * reference does for INITIAL_CONDITIONS_TYPE 1 (cilia_sim_main.cpp:193-197).
+ * The sizes must match this sim's. */
+int gmi_sim_read_backup(gmi_ctx *ctx, const char *name, int *nt_of_backup, int *exact);
+
 /* ---- learned warm start (no reference counterpart: the reference's warm
  * starts are the extrapolations of segment.cpp:53-73, filament.cpp:908-1016,
  * rigid_body.cpp:251-271).  A per-segment MLP (51 -> hidden -> hidden -> 6,

@@ -165,6 +165,11 @@ def _sim_lib():
         lib.orc_sim_test_errors.argtypes = [vp, C.c_int, _dp]
         lib.orc_sim_test_initial_guess.argtypes = [vp, C.c_int]
         lib.orc_sim_test_initial_guess.restype = None
+        lib.orc_sim_backup_size.argtypes = [vp]
+        lib.orc_sim_get_backup.argtypes = [vp, _dp]
+        lib.orc_sim_get_backup.restype = None
+        lib.orc_sim_set_backup.argtypes = [vp, _dp]
+        lib.orc_sim_set_backup.restype = None
         lib._sim_ready = True
     return lib
 
@@ -197,6 +202,16 @@ class Sim:
         if self.h:
             self.lib.orc_sim_destroy(self.h)
             self.h = None
+
+    def backup_values(self):
+        out = np.zeros(self.lib.orc_sim_backup_size(self.h))
+        self.lib.orc_sim_get_backup(self.h, _p(out))
+        return out
+
+    def set_backup(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.lib.orc_sim_backup_size(self.h)
+        self.lib.orc_sim_set_backup(self.h, _p(v))
 
     def __del__(self):
         try:

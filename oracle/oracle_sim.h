@@ -44,6 +44,9 @@ void orc_sim_get_state(const orc_sim *s, double *seg_x, double *seg_q, double *s
 void orc_sim_get_mobility(const orc_sim *s, double *v_segs, double *v_bodies, double *f_blobs,
                           double *f_segs, double *body_ref66);
 void orc_sim_get_broyden(const orc_sim *s, double *err);
+int orc_sim_backup_size(const orc_sim *s);
+void orc_sim_get_backup(const orc_sim *s, double *out);
+void orc_sim_set_backup(orc_sim *s, const double *in);
 int orc_sim_test_compute_velocities(orc_sim *s);
 void orc_sim_test_forces(orc_sim *s);
 int orc_sim_test_prepare_jacobian(orc_sim *s, int nt);

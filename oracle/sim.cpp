@@ -1098,6 +1098,73 @@ void orc_sim_get_state(const orc_sim *s, double *seg_x, double *seg_q, double *s
   }
 }
 
+// Backup record in the reference's order (no text): per swimmer its
+// filaments (filament.cpp:1629-1648: per segment q, x, xm1, xm2, u, um1;
+// tether, clamp and link multipliers with two histories), then its body
+// (rigid_body.cpp:414-424: q, x, xm1, xm2, u, um1); swimmer.cpp:914-925.
+}  // extern "C"
+namespace {
+template <typename S_, typename F>
+void backup_walk(S_ *s, F &&io) {
+  const int S = s->S;
+  for (auto &B : s->sw) {
+    for (auto &Fl : B.fil) {
+      for (int j = 0; j < S; j++) {
+        auto &g = Fl.seg[j];
+        io(g.q.s);
+        for (int k = 0; k < 3; k++) io(g.q.v[k]);
+        for (int k = 0; k < 3; k++) io(g.x[k]);
+        for (int k = 0; k < 3; k++) io(g.xm1[k]);
+        for (int k = 0; k < 3; k++) io(g.xm2[k]);
+        for (int k = 0; k < 3; k++) io(g.u[k]);
+        for (int k = 0; k < 3; k++) io(g.um1[k]);
+      }
+      for (int k = 0; k < 3; k++) io(Fl.teth[k]);
+      for (int k = 0; k < 3; k++) io(Fl.tethm1[k]);
+      for (int k = 0; k < 3; k++) io(Fl.tethm2[k]);
+      for (int k = 0; k < 3; k++) io(Fl.clamp[k]);
+      for (int k = 0; k < 3; k++) io(Fl.clampm1[k]);
+      for (int k = 0; k < 3; k++) io(Fl.clampm2[k]);
+      for (int n = 0; n < S - 1; n++) {
+        for (int k = 0; k < 3; k++) io(Fl.lam[3 * n + k]);
+        for (int k = 0; k < 3; k++) io(Fl.lamm1[3 * n + k]);
+        for (int k = 0; k < 3; k++) io(Fl.lamm2[3 * n + k]);
+      }
+    }
+    io(B.q.s);
+    for (int k = 0; k < 3; k++) io(B.q.v[k]);
+    for (int k = 0; k < 3; k++) io(B.x[k]);
+    for (int k = 0; k < 3; k++) io(B.xm1[k]);
+    for (int k = 0; k < 3; k++) io(B.xm2[k]);
+    for (int k = 0; k < 3; k++) io(B.u[k]);
+    for (int k = 0; k < 3; k++) io(B.um1[k]);
+  }
+}
+}  // namespace
+extern "C" {
+
+int orc_sim_backup_size(const orc_sim *s) {
+  int n = 0;
+  backup_walk(s, [&](const double &) { n++; });
+  return n;
+}
+
+void orc_sim_get_backup(const orc_sim *s, double *out) {
+  size_t i = 0;
+  backup_walk(s, [&](const double &v) { out[i++] = v; });
+}
+
+// The reader (filament.cpp:84-163, rigid_body.cpp:28-56): q_m1 = q.
+void orc_sim_set_backup(orc_sim *s, const double *in) {
+  size_t i = 0;
+  backup_walk(s, [&](double &v) { v = in[i++]; });
+  for (auto &B : s->sw) {
+    B.qm1 = B.q;
+    for (auto &Fl : B.fil)
+      for (auto &g : Fl.seg) g.qm1 = g.q;
+  }
+}
+
 void orc_sim_get_mobility(const orc_sim *s, double *v_segs, double *v_bodies, double *f_blobs,
                           double *f_segs, double *body_ref66 /* Nref then KTKinv, per swimmer */) {
   if (v_segs) std::copy(s->vs.begin(), s->vs.end(), v_segs);
