@@ -21,6 +21,7 @@ HIP_SOURCES = [
     "gmi_sim.hip",
     "gmi_learn.hip",
     "gmi_io.hip",
+    "gmi_seed.hip",
 ]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would

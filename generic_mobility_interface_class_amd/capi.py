@@ -22,6 +22,7 @@ GMI_ERR_SINGULAR = -5
 
 GMI_FP64 = 0
 GMI_BLOCK_SS, GMI_BLOCK_SB, GMI_BLOCK_BS, GMI_BLOCK_BB = 1, 2, 4, 8
+GMI_SEED_UNIFORM, GMI_SEED_EQUATORIAL, GMI_SEED_PLATY = 0, 1, 2
 WARMSTART_NIN = 51   # GMI_WARMSTART_NIN
 GMI_FP32_KAHAN = 1
 
@@ -97,6 +98,7 @@ SIGNATURES = [
     ("gmi_warmstart_dataset", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
                                         C.POINTER(C.c_float), C.c_int]),
     ("gmi_warmstart_stats", C.c_int, [C.c_void_p, _ip, _ip]),
+    ("gmi_seed_sphere", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _ip]),
     ("gmi_sim_write_par", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int]),
     ("gmi_sim_write_reference_positions", C.c_int, [C.c_void_p, C.c_char_p]),
     ("gmi_sim_write_data", C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
@@ -229,6 +231,15 @@ class Context:
         t, ch = C.c_int(0), C.c_int(0)
         self.call("gmi_autotune", int(reps), C.byref(t), C.byref(ch))
         return t.value, ch.value
+
+    def seed_sphere(self, n, kind=0, r_over_l=1.0, max_iterations=200000):
+        """Unit-sphere seeding as the reference does (gmi_seed_sphere);
+        returns (points [n, 3], iterations)."""
+        out = np.zeros(3 * int(n))
+        it = C.c_int(0)
+        self.call("gmi_seed_sphere", int(n), int(kind), float(r_over_l), int(max_iterations),
+                  ptr(out), C.byref(it))
+        return out.reshape(-1, 3), it.value
 
     def autotune_blocks(self, mask, reps=10):
         t, ch = C.c_int(0), C.c_int(0)

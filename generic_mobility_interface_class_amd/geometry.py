@@ -19,6 +19,7 @@ workloads here use closed-form geometry instead:
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -152,6 +153,8 @@ class SwimmerGeometry:
     a_blob: float = 1.0
     mu: float = 1.0
     centre: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    blob_unit: np.ndarray = None   # unit-sphere blob directions (default: geodesic)
+    fil_unit: np.ndarray = None    # unit filament base directions (default: Fibonacci)
 
     @property
     def DL(self) -> float:   # config.hpp:204
@@ -162,9 +165,14 @@ class SwimmerGeometry:
         return self.nfil * self.nseg
 
     def blob_references(self) -> np.ndarray:
+        if self.blob_unit is not None:
+            return np.asarray(self.blob_unit, dtype=np.float64).reshape(-1, 3) * self.R
         return geodesic_sphere(geodesic_freq_for(self.nblob), self.R)
 
     def filament_directions(self) -> np.ndarray:
+        if self.fil_unit is not None:
+            u = np.asarray(self.fil_unit, dtype=np.float64).reshape(-1, 3)
+            return u / np.linalg.norm(u, axis=1, keepdims=True)
         return fibonacci_sphere(self.nfil)
 
     def base_radius(self) -> float:
@@ -325,3 +333,63 @@ ENSEMBLE_WORKLOADS = {
 
 def ensemble_workload(name, seed=0, **kw):
     return ensemble_config(seed=seed, **ENSEMBLE_WORKLOADS[name], **kw)
+
+
+# ---------------------------------------------------------------- reference seeding
+
+SEED_KINDS = {"uniform": 0, "equatorial": 1, "platy": 2}
+
+
+def seed_file_name(n, kind):
+    """The reference's cache file of a seeding (seeding.cu:121,352-356)."""
+    return {"uniform": f"sphere{n}.seed", "equatorial": f"sphere{n}_equatorial.seed",
+            "platy": f"sphere{n}_platy.seed"}[kind]
+
+
+def read_seed_file(path, n):
+    """Unit-sphere points from a seed file (rigid_body.cpp:214-218,
+    swimmer.cpp:188-194: 3N whitespace-separated numbers)."""
+    vals = np.array([float(t) for t in open(path).read().split()])
+    if vals.size < 3 * n:
+        raise ValueError(f"{path}: {vals.size} values, need {3 * n}")
+    return vals[:3 * n].reshape(n, 3)
+
+
+def write_seed_file(path, points, per_line):
+    """As the reference writes it: "%g " after every value (blob seeds,
+    seeding.cu:121-126) or "%g \n" (filament seeds, seeding.cu:361-365)."""
+    sep = " \n" if per_line else " "
+    with open(path, "w") as f:
+        f.write("".join(("%g" % v) + sep for v in np.ravel(points)))
+
+
+def reference_seed(ctx, n, kind="uniform", r_over_l=1.0, cache_dir=None, per_line=False):
+    """Unit-sphere points of the reference's seeding.  With cache_dir, an
+    existing sphere<N>*.seed file there is read (6 significant digits, as
+    the reference reads its cache); otherwise the library seeds on the GPU
+    (gmi_seed_sphere via `ctx`, a capi.Context) and, with cache_dir, writes
+    the file (the platy kind has no cache file in the reference)."""
+    path = os.path.join(cache_dir, seed_file_name(n, kind)) if cache_dir else None
+    if path and kind != "platy" and os.path.exists(path):
+        return read_seed_file(path, n)
+    if ctx is None:
+        raise RuntimeError("no seed file and no device context to compute it")
+    pts, _ = ctx.seed_sphere(n, SEED_KINDS[kind], r_over_l)
+    if path and kind != "platy":
+        write_seed_file(path, pts, per_line)
+    return pts
+
+
+def reference_swimmer(ctx, nfil, nseg, nblob, r_over_l=1.0, a_seg=1.0, a_blob=1.4,
+                      seeding="uniform", cache_dir=None, centre=None):
+    """A swimmer laid out as the reference lays out a spherical body: blobs
+    on a sphere of radius R_OVER_L * L from the uniform seeding
+    (rigid_body.cpp:186-226), filament bases from the chosen seeding at
+    R_OVER_L * L + 1.05 (DL/2 + RBLOB) (swimmer.cpp:171-206).  Defaults are
+    the reference's RSEG = 1, RBLOB = 1.4 (config.hpp:102-103)."""
+    L = nseg * 2.2 * a_seg
+    blobs = reference_seed(ctx, nblob, "uniform", cache_dir=cache_dir, per_line=False)
+    fils = reference_seed(ctx, nfil, seeding, r_over_l, cache_dir=cache_dir, per_line=True)
+    return SwimmerGeometry(nfil=nfil, nseg=nseg, nblob=nblob, R=r_over_l * L, a_seg=a_seg,
+                           a_blob=a_blob, centre=np.zeros(3) if centre is None else centre,
+                           blob_unit=blobs, fil_unit=fils)

@@ -101,6 +101,22 @@ int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
  * graph and replayed (no reference counterpart).  Default: graphs on, 6. */
 int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
 
+/* ---- seeding points on the unit sphere (seeding.cu:13-389) ------------- */
+#define GMI_SEED_UNIFORM 0    /* UNIFORM_SEEDING: spiral start, then the
+                                 repulsion relaxation (seeding.cu:13-130,
+                                 232-375; cuda_functions.cu:1159-1250), on
+                                 the GPU for any N                          */
+#define GMI_SEED_EQUATORIAL 1 /* EQUATORIAL_SEEDING band (seeding.cu:140-175) */
+#define GMI_SEED_PLATY 2      /* PLATY_SEEDING band + ring (seeding.cu:177-230) */
+/* Fills unit_xyz[3n] on the ctx's device.  r_over_l (R_OVER_L) is used by
+ * the band seedings only.  The relaxation stops when no coordinate moves by
+ * more than 1e-4 (*iterations reports how many it took) or returns
+ * GMI_ERR_NOT_CONVERGED after max_iterations.  The reference caches the
+ * result as sphere<N>.seed; generic_mobility_interface_class_amd.geometry
+ * reads and writes those files. */
+int gmi_seed_sphere(gmi_ctx *ctx, int n, int kind, double r_over_l, int max_iterations,
+                    double *unit_xyz, int *iterations);
+
 /* ---- output and checkpoint files in the reference's text formats.  `name`
  * is SIMULATION_NAME (a path prefix); files as in config.hpp:208-215. ---- */
 /* <name>.par (cilia_sim_main.cpp:79-116); the format describes identical

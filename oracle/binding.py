@@ -34,6 +34,9 @@ def load():
             getattr(lib, fn).argtypes = [_dp, _dp, _dp]
             getattr(lib, fn).restype = None
         lib.orc_invert.argtypes = [_dp, i]
+        lib.orc_seed_uniform.argtypes = [i, i, i, _dp]
+        lib.orc_seed_band.argtypes = [i, i, d, _dp]
+        lib.orc_seed_band.restype = None
         for f in (lib.orc_Mss, lib.orc_Mbb, lib.orc_Msb, lib.orc_Mbs,
                   lib.orc_barrier, lib.orc_ss_block):
             f.restype = None
@@ -117,6 +120,19 @@ def ss_block(xi, xj, a, mu=1.0):
     load().orc_ss_block(_p(out), _p(np.ascontiguousarray(xi, float)),
                         _p(np.ascontiguousarray(xj, float)), a, mu)
     return out.reshape(6, 6)
+
+
+def seed_uniform(n, order=1, max_it=200000):
+    """Reference uniform seeding (seed.cpp); returns (points [n, 3], iterations)."""
+    X = np.zeros(3 * n)
+    it = load().orc_seed_uniform(int(n), int(order), int(max_it), _p(X))
+    return X.reshape(-1, 3), it
+
+
+def seed_band(n, platy, r_over_l):
+    X = np.zeros(3 * n)
+    load().orc_seed_band(int(n), int(bool(platy)), float(r_over_l), _p(X))
+    return X.reshape(-1, 3)
 
 
 # ---------------------------------------------------------------- time step

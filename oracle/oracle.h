@@ -47,6 +47,12 @@ void orc_barrier(double *f_segs, double *f_blobs_rep, const double *xs, int ns,
 void orc_ss_block(double *out, const double *xi, const double *xj, double a,
                   double mu);
 
+/* Unit-sphere seeding (seed.cpp): uniform repulsion (order 1: the
+ * reference's GPU summation, 0: its CPU pair loop), returns iterations or -1;
+ * and the equatorial / platy bands. */
+int orc_seed_uniform(int N, int order, int max_it, double *X);
+void orc_seed_band(int N, int platy, double r_over_l, double *X);
+
 #ifdef __cplusplus
 }
 #endif
