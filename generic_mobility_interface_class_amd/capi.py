@@ -75,6 +75,7 @@ SIGNATURES = [
     ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gmi_autotune", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     ("gmi_autotune_blocks", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),
+    ("gmi_set_wall", C.c_int, [C.c_void_p, C.c_int]),
     # gmi_sim.h
     ("gmi_sim_create", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gmi_make_body_reference_matrices", C.c_int, [C.c_void_p]),
@@ -219,8 +220,10 @@ class Context:
     def velocities(self):
         vs = np.zeros(6 * self.Ns)
         vb = np.zeros(3 * self.Nb)
-        self.call("gmi_copy_segment_velocities_to_host", ptr(vs))
-        self.call("gmi_copy_blob_velocities_to_host", ptr(vb))
+        if self.Ns:
+            self.call("gmi_copy_segment_velocities_to_host", ptr(vs))
+        if self.Nb:
+            self.call("gmi_copy_blob_velocities_to_host", ptr(vb))
         self.call("gmi_wait_for_device")
         return vs.reshape(-1, 6), vb.reshape(-1, 3)
 
@@ -231,6 +234,9 @@ class Context:
         t, ch = C.c_int(0), C.c_int(0)
         self.call("gmi_autotune", int(reps), C.byref(t), C.byref(ch))
         return t.value, ch.value
+
+    def set_wall(self, wall=True):
+        self.call("gmi_set_wall", int(bool(wall)))
 
     def seed_sphere(self, n, kind=0, r_over_l=1.0, max_iterations=200000):
         """Unit-sphere seeding as the reference does (gmi_seed_sphere);

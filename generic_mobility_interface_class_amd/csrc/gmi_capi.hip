@@ -158,6 +158,10 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
     ev_mid = c->prof_events[c->prof_used + 1];
     c->prof_used += 2;
   }
+  if (c->wall)
+    return hip_check(c, launch_pair_wall(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, c->stream,
+                                         ev_mid),
+                     "wall pair launch");
   return hip_check(c, launch_pair(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, fp32, c->stream,
                                   ev_mid),
                    "pair launch");
@@ -384,6 +388,15 @@ int gmi_wait_for_device(gmi_ctx *c) {
   int rc;
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
   return flush_pending(c);
+}
+
+int gmi_set_wall(gmi_ctx *c, int wall) {
+  if (!c) return GMI_ERR_ARG;
+  if (wall && c->Nb > 0)
+    return fail(c, GMI_ERR_STATE, "the wall-bounded mobility has segments only (no blobs)");
+  if (wall && c->extra) return fail(c, GMI_ERR_STATE, "no wall in a time-stepping context");
+  c->wall = wall != 0;
+  return GMI_OK;
 }
 
 int gmi_evaluate_segment_segment_mobility(gmi_ctx *c) {

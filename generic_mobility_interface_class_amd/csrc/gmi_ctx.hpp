@@ -36,6 +36,7 @@ struct gmi_ctx {
   size_t part_slots = 0;
 
   // launch shaping (set by the tuner)
+  bool wall = false;  // segment-segment mobility above a no-slip wall at z = 0
   int target_blocks = 2048;
   int min_chunk = 128;
   int list_generation = 0;  // bumped whenever the work lists are rebuilt

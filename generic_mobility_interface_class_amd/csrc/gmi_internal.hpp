@@ -59,6 +59,9 @@ struct WorkList {
 // ev_mid (optional) is recorded between the pair kernel and the reduction.
 hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp32,
                        hipStream_t s, hipEvent_t ev_mid = nullptr);
+// Segment-segment mobility above a no-slip wall at z = 0 (FP64; kSS lists).
+hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,
+                            hipEvent_t ev_mid = nullptr);
 // Barrier over the all-sources work list `wl` (split partials in `part`).
 hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb,
                           const double *Xs, const double *Xb, const WorkList &wl,

@@ -441,6 +441,159 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Segment-segment mobility above a no-slip plane wall at z = 0 (Swan & Brady,
+// as the reference evaluates it: cuda_functions.cu:8-242, its sign
+// convention for the rotation-translation wall terms included).  FP64 only.
+// Positions are divided by a and the four blocks scaled by 1/(mu a),
+// 1/(mu a^2), 1/(mu a^2), 1/(mu a^3) -- the reference's expression when
+// RSEG = MU = 1.  Pairs j != i use the far-field RPY form at any separation.
+struct WallAcc {
+  double vt[3], vr[3], wt[3], wr[3];  // v from forces / torques, w from forces / torques
+};
+
+__device__ __forceinline__ void wall_self(double zi, const double *f, const double *t,
+                                          WallAcc &A) {
+  constexpr double FF = 1.0 / (6.0 * kPi), TF = 1.0 / (8.0 * kPi);
+  const double hm1 = 1.0 / zi, hm3 = hm1 * hm1 * hm1, hm4 = hm3 * hm1, hm5 = hm4 * hm1;
+  const double par = FF - (9.0 * hm1 - 2.0 * hm3 + hm5) * (FF / 16.0);
+  const double perp = FF - (9.0 * hm1 - 4.0 * hm3 + hm5) * (FF / 8.0);
+  const double c = (3.0 * FF / 32.0) * hm4;
+  A.vt[0] += par * f[0]; A.vt[1] += par * f[1]; A.vt[2] += perp * f[2];
+  A.vr[0] += c * t[1]; A.vr[1] -= c * t[0];
+  const double rpar = TF - (15.0 * FF / 64.0) * hm3, rperp = TF - (3.0 * FF / 32.0) * hm3;
+  A.wr[0] += rpar * t[0]; A.wr[1] += rpar * t[1]; A.wr[2] += rperp * t[2];
+  A.wt[0] -= c * f[1]; A.wt[1] += c * f[0];
+}
+
+__device__ __forceinline__ void wall_pair(double dx, double dy, double dz, double zi, double zj,
+                                          const double *f, const double *t, WallAcc &A) {
+  constexpr double FF = 1.0 / (6.0 * kPi), TF = 1.0 / (8.0 * kPi);
+  // unbounded RPY, far-field form
+  double rm1 = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  const double rx = dx * rm1, ry = dy * rm1, rz = dz * rm1;
+  const double rf = rx * f[0] + ry * f[1] + rz * f[2];
+  const double rt = rx * t[0] + ry * t[1] + rz * t[2];
+  double a0 = TF * rm1;
+  const double b1 = 1.0 + (2.0 / 3.0) * rm1 * rm1, b2 = 1.0 - 2.0 * rm1 * rm1;
+  A.vt[0] += a0 * (b1 * f[0] + b2 * rf * rx);
+  A.vt[1] += a0 * (b1 * f[1] + b2 * rf * ry);
+  A.vt[2] += a0 * (b1 * f[2] + b2 * rf * rz);
+  a0 = 0.5 * TF * rm1 * rm1 * rm1;
+  A.wr[0] += a0 * (3.0 * rt * rx - t[0]);
+  A.wr[1] += a0 * (3.0 * rt * ry - t[1]);
+  A.wr[2] += a0 * (3.0 * rt * rz - t[2]);
+  a0 = TF * rm1 * rm1;
+  A.vr[0] += a0 * (t[1] * rz - t[2] * ry);
+  A.vr[1] += a0 * (t[2] * rx - t[0] * rz);
+  A.vr[2] += a0 * (t[0] * ry - t[1] * rx);
+  A.wt[0] += a0 * (f[1] * rz - f[2] * ry);
+  A.wt[1] += a0 * (f[2] * rx - f[0] * rz);
+  A.wt[2] += a0 * (f[0] * ry - f[1] * rx);
+  // wall corrections: the image of source j at (xj, yj, -zj)
+  const double sz = zi + zj;
+  rm1 = 1.0 / sqrt(dx * dx + dy * dy + sz * sz);
+  const double rm2 = rm1 * rm1, rm3 = rm2 * rm1, rm4 = rm3 * rm1, rm5 = rm4 * rm1;
+  double ex = dx * rm1, ey = dy * rm1;
+  const double ez = sz * rm1, ez2 = ez * ez;
+  double h = zj / sz;
+  const double ef = ex * f[0] + ey * f[1] + ez * f[2];
+  const double et = ex * t[0] + ey * t[1] + ez * t[2];
+  {
+    const double c0 = -(0.75 * rm1 * (1.0 + 2.0 * h * (1.0 - h) * ez2) +
+                        0.5 * (rm3 * (1.0 - 3.0 * ez2) - rm5 * (1.0 - 5.0 * ez2)));
+    const double c1 = -(0.75 * rm1 * (1.0 - 6.0 * h * (1.0 - h) * ez2) -
+                        1.5 * rm3 * (1.0 - 5.0 * ez2) + 2.5 * rm5 * (1.0 - 7.0 * ez2)) * ef;
+    const double c2 = ez * f[2] * (1.5 * h * rm1 * (1.0 - 6.0 * (1.0 - h) * ez2) -
+                                   3.0 * rm3 * (1.0 - 5.0 * ez2) + 5.0 * rm5 * (2.0 - 7.0 * ez2));
+    const double c3 = (1.5 * h * rm1 - 5.0 * rm5) * ef;
+    const double c4 = -((3.0 * h * h * rm1 + 3.0 * rm3) * ez2 + (2.0 - 15.0 * ez2) * rm5) * f[2];
+    A.vt[0] += FF * (c0 * f[0] + (c1 + c2) * ex);
+    A.vt[1] += FF * (c0 * f[1] + (c1 + c2) * ey);
+    A.vt[2] += FF * (c0 * f[2] + (c1 + c2 + c3) * ez + c4);
+  }
+  {
+    const double c0 = rm3 * (0.375 - 2.25 * ez2);
+    const double c1 = -1.125 * rm3 * et;
+    const double c2 = 2.25 * ez * rm3 * et;
+    const double c3 = 2.25 * rm3 * (ex * t[1] - ey * t[0]);
+    A.wr[0] += FF * (c0 * t[0] + c1 * ex - c3 * ey);
+    A.wr[1] += FF * (c0 * t[1] + c1 * ey + c3 * ex);
+    A.wr[2] += FF * (c0 * t[2] + c1 * ez + c2);
+  }
+  const double q0 = 0.75 * rm2;
+  {
+    const double c1 = f[2] * (9.0 * h * ez2 * rm2 + (1.5 - 15.0 * ez2) * rm4);
+    const double c2 = -ez * ef * (4.5 * h * rm2 - 7.5 * rm4);
+    const double c3 = -1.5 * ez * (h * rm2 - rm4);
+    A.wt[0] -= FF * (q0 * (f[1] * ez - f[2] * ey) - (c1 + c2) * ey + c3 * f[1]);
+    A.wt[1] -= FF * (q0 * (f[2] * ex - f[0] * ez) + (c1 + c2) * ex - c3 * f[0]);
+    A.wt[2] -= FF * q0 * (f[0] * ey - f[1] * ex);
+  }
+  {  // the rotation-translation block of the pair (j, i), transposed
+    h = 1.0 - h;
+    ex = -ex;
+    ey = -ey;
+    const double ect = ex * t[1] - ey * t[0];
+    const double c1 = (9.0 * h * ez2 * rm2 + (1.5 - 15.0 * ez2) * rm4) * ect;
+    const double c2 = -ez * (4.5 * h * rm2 - 7.5 * rm4) * ect;
+    const double c3 = -1.5 * ez * (h * rm2 - rm4);
+    A.vr[0] -= FF * (q0 * (t[2] * ey - t[1] * ez) + c2 * ex - c3 * t[1]);
+    A.vr[1] -= FF * (q0 * (t[0] * ez - t[2] * ex) + c2 * ey + c3 * t[0]);
+    A.vr[2] -= FF * (q0 * (t[1] * ex - t[0] * ey) + c2 * ez + c1);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
+  __shared__ double sx[kTile], sy[kTile], sz[kTile], sf[6][kTile];
+  if (a.skip && *a.skip) return;
+  const WorkItem w = a.work[blockIdx.x];
+  const TileDesc T = a.tiles[w.tile];
+  const int tid = threadIdx.x;
+  const double inv_a = 1.0 / a.a_s;
+  const int ti = T.t0 + (tid < T.nt ? tid : 0);
+  const double xi = a.Xs[ti] * inv_a, yi = a.Xs[a.lds + ti] * inv_a, zi = a.Xs[2 * a.lds + ti] * inv_a;
+  const int ns = T.nsseg;
+  const int beg = int(long(ns) * w.split / T.nsplit);
+  const int end = int(long(ns) * (w.split + 1) / T.nsplit);
+  WallAcc A;
+  for (int k = 0; k < 3; k++) A.vt[k] = A.vr[k] = A.wt[k] = A.wr[k] = 0.0;
+  for (int base = T.sseg0 + beg; base < T.sseg0 + end; base += kTile) {
+    const int n = min(kTile, T.sseg0 + end - base);
+    __syncthreads();
+    if (tid < n) {
+      const int j = base + tid;
+      sx[tid] = a.Xs[j] * inv_a; sy[tid] = a.Xs[a.lds + j] * inv_a; sz[tid] = a.Xs[2 * a.lds + j] * inv_a;
+      for (int c = 0; c < 6; c++) sf[c][tid] = a.Fs[c * a.lds + j];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < n; jj++) {
+      const double f[3] = {sf[0][jj], sf[1][jj], sf[2][jj]};
+      const double t[3] = {sf[3][jj], sf[4][jj], sf[5][jj]};
+      if (base + jj == ti)
+        wall_self(zi, f, t, A);
+      else
+        wall_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], zi, sz[jj], f, t, A);
+    }
+  }
+  if (tid >= T.nt) return;
+  const double s1 = 1.0 / (a.mu * a.a_s), s2 = s1 * inv_a, s3 = s2 * inv_a;
+  double o[6];
+  for (int k = 0; k < 3; k++) {
+    o[k] = s1 * A.vt[k] + s2 * A.vr[k];
+    o[3 + k] = s2 * A.wt[k] + s3 * A.wr[k];
+  }
+  if (T.nsplit == 1) {
+    for (int c = 0; c < 6; ++c) {
+      double *v = a.Vs + c * a.lds + T.t0 + tid;
+      *v = a.accumulate ? *v + o[c] : o[c];
+    }
+  } else {
+    double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
+    for (int c = 0; c < 6; ++c) P[c * kBlock] = o[c];
+  }
+}
+
 // Deterministic reduction of split partials (split order), one workgroup per
 // tile.  Several splits' loads are in flight at once.
 __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
@@ -591,6 +744,20 @@ hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp3
     hipLaunchKernelGGL(k_pair<float>, dim3(nwork), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_pair<double>, dim3(nwork), dim3(kBlock), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+  if (ntiles_reduce > 0) {
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,
+                            hipEvent_t ev_mid) {
+  if (nwork <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pair_wall, dim3(nwork), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;

@@ -119,6 +119,15 @@ int gmi_profile_begin(gmi_ctx *ctx);
  * number of launches since gmi_profile_begin(), and stops recording. */
 int gmi_profile_end(gmi_ctx *ctx, double *pair_ms, int *launches);
 
+/* Segment-segment mobility above a no-slip plane wall at z = 0 (the
+ * reference's INFINITE_PLANE_WALL build: Swan & Brady, cuda_functions.cu:
+ * 8-242).  wall != 0 switches gmi_evaluate_segment_segment_mobility (and the
+ * grand apply, which then has no other blocks) to it; FP64 whatever the
+ * context precision.  The reference's wall build has no bodies: contexts
+ * with blobs, or with a time-stepping sim, are refused.  Segments must lie
+ * above the wall (z > 0). */
+int gmi_set_wall(gmi_ctx *ctx, int wall);
+
 /* ---- launch shaping (no reference counterpart; the reference launches
  * ceil(N/64) blocks of 64, rpy_mobility_solver.cu:249-305) --------------- */
 /* Pair launches aim at `target_blocks` workgroups by splitting each target

@@ -34,6 +34,8 @@ def load():
             getattr(lib, fn).argtypes = [_dp, _dp, _dp]
             getattr(lib, fn).restype = None
         lib.orc_invert.argtypes = [_dp, i]
+        lib.orc_Mss_wall.argtypes = [_dp, _dp, _dp, i, i, i, d, d, i]
+        lib.orc_Mss_wall.restype = None
         lib.orc_seed_uniform.argtypes = [i, i, i, _dp]
         lib.orc_seed_band.argtypes = [i, i, d, _dp]
         lib.orc_seed_band.restype = None
@@ -63,6 +65,16 @@ def Mss(xs, fs, a, mu=1.0, rows=None, nthreads=None):
     i0, i1 = rows if rows else (0, n)
     V = np.zeros((n, 6))
     load().orc_Mss(_p(V), _p(fs), _p(xs), n, i0, i1, a, mu, nthreads or nthreads_default())
+    return V
+
+
+def Mss_wall(xs, fs, a, mu=1.0, rows=None, nthreads=None):
+    """Segment-segment mobility above the wall z = 0 (orc_Mss_wall)."""
+    xs, fs = _c(xs, 3), _c(fs, 6)
+    n = xs.shape[0]
+    i0, i1 = rows if rows else (0, n)
+    V = np.zeros((n, 6))
+    load().orc_Mss_wall(_p(V), _p(fs), _p(xs), n, i0, i1, a, mu, nthreads or nthreads_default())
     return V
 
 

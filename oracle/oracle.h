@@ -47,6 +47,11 @@ void orc_barrier(double *f_segs, double *f_blobs_rep, const double *xs, int ns,
 void orc_ss_block(double *out, const double *xi, const double *xj, double a,
                   double mu);
 
+/* Segment-segment mobility above a no-slip wall at z = 0 (Swan & Brady;
+ * cuda_functions.cu:8-242), V[6 ns] rows i0..i1. */
+void orc_Mss_wall(double *V, const double *F, const double *X, int ns, int i0, int i1,
+                  double a, double mu, int nthreads);
+
 /* Unit-sphere seeding (seed.cpp): uniform repulsion (order 1: the
  * reference's GPU summation, 0: its CPU pair loop), returns iterations or -1;
  * and the equatorial / platy bands. */

@@ -284,3 +284,119 @@ extern "C" void orc_ss_block(double *out, const double *xi, const double *xj,
       out[6 * p + (q + 3)] = c * cross_e(p, q);                   // v <- tau
     }
 }
+
+// ---------------------------------------------------------------------------
+// Segment-segment mobility above a no-slip plane wall at z = 0 (Swan & Brady
+// 2007, as the reference evaluates it in cuda_functions.cu:8-242, including
+// its sign convention for the rotation-translation wall terms).  The
+// reference assumes RSEG = MU = 1; here positions are divided by a and the
+// blocks scaled by 1/(mu a) (tt), 1/(mu a^2) (tr, rt) and 1/(mu a^3) (rr),
+// which is the same expression for a = mu = 1.  Pairs j != i use the
+// far-field RPY form at any separation, as the reference does.
+extern "C" void orc_Mss_wall(double *V, const double *F, const double *X, int ns, int i0, int i1,
+                             double a, double mu, int nthreads) {
+  const double FF = 1.0 / (6.0 * kPi);   // ForceFactor
+  const double TF = 1.0 / (8.0 * kPi);   // TorqueFactor
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (int i = i0; i < i1; i++) {
+    // v = vt (from forces) + vr (from torques); w = wt (forces) + wr (torques)
+    double vt[3] = {0, 0, 0}, vr[3] = {0, 0, 0}, wt[3] = {0, 0, 0}, wr[3] = {0, 0, 0};
+    const double xi = X[3 * i] / a, yi = X[3 * i + 1] / a, zi = X[3 * i + 2] / a;
+    for (int j = 0; j < ns; j++) {
+      const double *f = &F[6 * j], *t = &F[6 * j + 3];
+      const double xj = X[3 * j] / a, yj = X[3 * j + 1] / a, zj = X[3 * j + 2] / a;
+      if (j == i) {
+        const double hm1 = 1.0 / zi, hm3 = hm1 * hm1 * hm1, hm4 = hm3 * hm1, hm5 = hm4 * hm1;
+        const double par = FF - (9.0 * hm1 - 2.0 * hm3 + hm5) * FF / 16.0;
+        const double perp = FF - (9.0 * hm1 - 4.0 * hm3 + hm5) * FF / 8.0;
+        const double c = 3.0 * FF / 32.0 * hm4;
+        vt[0] += par * f[0]; vt[1] += par * f[1]; vt[2] += perp * f[2];
+        vr[0] += c * t[1]; vr[1] -= c * t[0];
+        const double rpar = TF - 15.0 * FF / 64.0 * hm3;
+        const double rperp = TF - 3.0 * FF / 32.0 * hm3;
+        wr[0] += rpar * t[0]; wr[1] += rpar * t[1]; wr[2] += rperp * t[2];
+        wt[0] -= c * f[1]; wt[1] += c * f[0];
+        continue;
+      }
+      // unbounded RPY, far-field form
+      const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+      double rm1 = 1.0 / std::sqrt(dx * dx + dy * dy + dz * dz);
+      const double rx = dx * rm1, ry = dy * rm1, rz = dz * rm1;
+      const double rf = rx * f[0] + ry * f[1] + rz * f[2];
+      const double rt = rx * t[0] + ry * t[1] + rz * t[2];
+      double A = TF * rm1, B = 1.0 + 2.0 / 3.0 * rm1 * rm1, C = 1.0 - 2.0 * rm1 * rm1;
+      vt[0] += A * (B * f[0] + C * rf * rx);
+      vt[1] += A * (B * f[1] + C * rf * ry);
+      vt[2] += A * (B * f[2] + C * rf * rz);
+      A = 0.5 * TF * rm1 * rm1 * rm1;
+      wr[0] += A * (3.0 * rt * rx - t[0]);
+      wr[1] += A * (3.0 * rt * ry - t[1]);
+      wr[2] += A * (3.0 * rt * rz - t[2]);
+      A = TF * rm1 * rm1;
+      vr[0] += A * (t[1] * rz - t[2] * ry);
+      vr[1] += A * (t[2] * rx - t[0] * rz);
+      vr[2] += A * (t[0] * ry - t[1] * rx);
+      wt[0] += A * (f[1] * rz - f[2] * ry);
+      wt[1] += A * (f[2] * rx - f[0] * rz);
+      wt[2] += A * (f[0] * ry - f[1] * rx);
+      // wall corrections: image at (xj, yj, -zj)
+      const double sz = zi + zj;
+      rm1 = 1.0 / std::sqrt(dx * dx + dy * dy + sz * sz);
+      const double rm2 = rm1 * rm1, rm3 = rm2 * rm1, rm4 = rm3 * rm1, rm5 = rm4 * rm1;
+      double ex = dx * rm1, ey = dy * rm1;
+      const double ez = sz * rm1;
+      double h = zj / sz;
+      const double ef = ex * f[0] + ey * f[1] + ez * f[2];
+      const double et = ex * t[0] + ey * t[1] + ez * t[2];
+      const double ez2 = ez * ez;
+      {  // tt
+        const double a0 = -(0.75 * rm1 * (1.0 + 2.0 * h * (1.0 - h) * ez2) +
+                            0.5 * (rm3 * (1.0 - 3.0 * ez2) - rm5 * (1.0 - 5.0 * ez2)));
+        const double a1 = -(0.75 * rm1 * (1.0 - 6.0 * h * (1.0 - h) * ez2) -
+                            1.5 * rm3 * (1.0 - 5.0 * ez2) + 2.5 * rm5 * (1.0 - 7.0 * ez2)) * ef;
+        const double a2 = ez * f[2] * (1.5 * h * rm1 * (1.0 - 6.0 * (1.0 - h) * ez2) -
+                                       3.0 * rm3 * (1.0 - 5.0 * ez2) + 5.0 * rm5 * (2.0 - 7.0 * ez2));
+        const double a3 = (1.5 * h * rm1 - 5.0 * rm5) * ef;
+        const double a4 = -((3.0 * h * h * rm1 + 3.0 * rm3) * ez2 + (2.0 - 15.0 * ez2) * rm5) * f[2];
+        vt[0] += FF * (a0 * f[0] + (a1 + a2) * ex);
+        vt[1] += FF * (a0 * f[1] + (a1 + a2) * ey);
+        vt[2] += FF * (a0 * f[2] + (a1 + a2 + a3) * ez + a4);
+      }
+      {  // rr
+        const double a0 = rm3 * (0.375 - 2.25 * ez2);
+        const double a1 = -1.125 * rm3 * et;
+        const double a2 = 2.25 * ez * rm3 * et;
+        const double a3 = 2.25 * rm3 * (ex * t[1] - ey * t[0]);
+        wr[0] += FF * (a0 * t[0] + a1 * ex - a3 * ey);
+        wr[1] += FF * (a0 * t[1] + a1 * ey + a3 * ex);
+        wr[2] += FF * (a0 * t[2] + a1 * ez + a2);
+      }
+      const double q0 = 0.75 * rm2;
+      {  // rt (angular velocity from forces), the reference's sign (:197-204)
+        const double a1 = f[2] * (9.0 * h * ez2 * rm2 + (1.5 - 15.0 * ez2) * rm4);
+        const double a2 = -ez * ef * (4.5 * h * rm2 - 7.5 * rm4);
+        const double a3 = -1.5 * ez * (h * rm2 - rm4);
+        wt[0] -= FF * (q0 * (f[1] * ez - f[2] * ey) - (a1 + a2) * ey + a3 * f[1]);
+        wt[1] -= FF * (q0 * (f[2] * ex - f[0] * ez) + (a1 + a2) * ex - a3 * f[0]);
+        wt[2] -= FF * q0 * (f[0] * ey - f[1] * ex);
+      }
+      {  // tr: the rt block of the pair (j, i) transposed (:206-224)
+        h = 1.0 - h;
+        ex = -ex;
+        ey = -ey;
+        const double ect = ex * t[1] - ey * t[0];
+        const double a1 = (9.0 * h * ez2 * rm2 + (1.5 - 15.0 * ez2) * rm4) * ect;
+        const double a2 = -ez * (4.5 * h * rm2 - 7.5 * rm4) * ect;
+        const double a3 = -1.5 * ez * (h * rm2 - rm4);
+        vr[0] -= FF * (q0 * (t[2] * ey - t[1] * ez) + a2 * ex - a3 * t[1]);
+        vr[1] -= FF * (q0 * (t[0] * ez - t[2] * ex) + a2 * ey + a3 * t[0]);
+        vr[2] -= FF * (q0 * (t[1] * ex - t[0] * ey) + a2 * ez + a1);
+      }
+    }
+    const double s1 = 1.0 / (mu * a), s2 = s1 / a, s3 = s2 / a;
+    for (int k = 0; k < 3; k++) {
+      V[6 * i + k] = s1 * vt[k] + s2 * vr[k];
+      V[6 * i + 3 + k] = s2 * wt[k] + s3 * wr[k];
+    }
+  }
+}
