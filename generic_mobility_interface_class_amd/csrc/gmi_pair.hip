@@ -137,6 +137,12 @@ template <typename R> struct Acc {
 };
 
 // ---- per-pair kernels ------------------------------------------------------
+// Every lane evaluates the far forms; the overlap forms run only when some
+// lane of the wave has an overlapping pair (a wave-uniform branch, so the
+// common case needs no per-pair execution-mask changes), and are selected
+// per lane.  The far values of an overlapping pair may be infinite (r = 0);
+// the select discards them.
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
 // ss: 6x6 equal radius.
 template <typename R>
@@ -146,16 +152,18 @@ __device__ __forceinline__ void pair_ss(R dx, R dy, R dz, R fx, R fy, R fz,
   const R r2 = dx * dx + dy * dy + dz * dz;
   const R fd = fx * dx + fy * dy + fz * dz;
   const R td = tx * dx + ty * dy + tz * dz;
-  R At, Bt, Ar, Br, g;
-  if (__builtin_expect(r2 > k.far2, 1)) {
-    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-    At = ri * (k.c8 + k.c8s3 * ri2);
-    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-    Ar = -k.c16 * ri3;
-    Br = k.c16x3 * ri3 * ri2 * td;
-    g = k.c8 * ri3;
-  } else {
-    near_equal(r2, fd, td, k, At, Bt, Ar, Br, g);
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+  R At = ri * (k.c8 + k.c8s3 * ri2);
+  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  R Ar = -k.c16 * ri3;
+  R Br = k.c16x3 * ri3 * ri2 * td;
+  R g = k.c8 * ri3;
+  const bool nr = !(r2 > k.far2);
+  if (__builtin_expect(wave_any(nr), 0)) {
+    R At2, Bt2, Ar2, Br2, g2;
+    near_equal(r2, fd, td, k, At2, Bt2, Ar2, Br2, g2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Ar = nr ? Ar2 : Ar; Br = nr ? Br2 : Br;
+    g = nr ? g2 : g;
   }
   o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
   o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
@@ -171,14 +179,15 @@ __device__ __forceinline__ void pair_sb(R dx, R dy, R dz, R fx, R fy, R fz,
                                         const PC<R> &k, Acc<R> &o) {
   const R r2 = dx * dx + dy * dy + dz * dz;
   const R fd = fx * dx + fy * dy + fz * dz;
-  R At, Bt, g;
-  if (__builtin_expect(r2 > k.far2, 1)) {
-    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-    At = ri * (k.c8 + k.c8s3 * ri2);
-    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-    g = k.c8 * ri3;
-  } else {
-    near_unequal(r2, fd, k, At, Bt, g);
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+  R At = ri * (k.c8 + k.c8s3 * ri2);
+  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  R g = k.c8 * ri3;
+  const bool nr = !(r2 > k.far2);
+  if (__builtin_expect(wave_any(nr), 0)) {
+    R At2, Bt2, g2;
+    near_unequal(r2, fd, k, At2, Bt2, g2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
   }
   o.vx += At * fx + Bt * dx;
   o.vy += At * fy + Bt * dy;
@@ -195,14 +204,15 @@ __device__ __forceinline__ void pair_bs(R dx, R dy, R dz, R fx, R fy, R fz,
                                         Acc<R> &o) {
   const R r2 = dx * dx + dy * dy + dz * dz;
   const R fd = fx * dx + fy * dy + fz * dz;
-  R At, Bt, g;
-  if (__builtin_expect(r2 > k.far2, 1)) {
-    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-    At = ri * (k.c8 + k.c8s3 * ri2);
-    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-    g = k.c8 * ri3;
-  } else {
-    near_unequal(r2, fd, k, At, Bt, g);
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+  R At = ri * (k.c8 + k.c8s3 * ri2);
+  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  R g = k.c8 * ri3;
+  const bool nr = !(r2 > k.far2);
+  if (__builtin_expect(wave_any(nr), 0)) {
+    R At2, Bt2, g2;
+    near_unequal(r2, fd, k, At2, Bt2, g2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
   }
   o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
   o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
@@ -215,14 +225,14 @@ __device__ __forceinline__ void pair_bb(R dx, R dy, R dz, R fx, R fy, R fz,
                                         const PC<R> &k, Acc<R> &o) {
   const R r2 = dx * dx + dy * dy + dz * dz;
   const R fd = fx * dx + fy * dy + fz * dz;
-  R At, Bt;
-  if (__builtin_expect(r2 > k.far2, 1)) {
-    const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-    At = ri * (k.c8 + k.c8s3 * ri2);
-    Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-  } else {
-    R Ar, Br, g;
-    near_equal(r2, fd, R(0), k, At, Bt, Ar, Br, g);
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+  R At = ri * (k.c8 + k.c8s3 * ri2);
+  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  const bool nr = !(r2 > k.far2);
+  if (__builtin_expect(wave_any(nr), 0)) {
+    R At2, Bt2, Ar, Br, g;
+    near_equal(r2, fd, R(0), k, At2, Bt2, Ar, Br, g);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
   o.vx += At * fx + Bt * dx;
   o.vy += At * fy + Bt * dy;
