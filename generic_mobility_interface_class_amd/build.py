@@ -22,6 +22,7 @@ HIP_SOURCES = [
     "gmi_learn.hip",
     "gmi_io.hip",
     "gmi_seed.hip",
+    "gmi_cilia.hip",
 ]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would

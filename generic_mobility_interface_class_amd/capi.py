@@ -115,6 +115,23 @@ SIGNATURES = [
     ("gmi_warmstart_loss_grad", C.c_int, [C.c_void_p, C.POINTER(C.c_float),
                                           C.POINTER(C.c_float), C.c_int, _dp,
                                           C.POINTER(C.c_float)]),
+    # gmi_cilia.h
+    ("gmi_cilia_create", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("gmi_cilia_step", C.c_int, [C.c_void_p, C.c_int, _ip]),
+    ("gmi_cilia_initial_guess", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_cilia_compute_velocities", C.c_int, [C.c_void_p, C.c_int, _ip]),
+    ("gmi_cilia_integrate_bodies", C.c_int, [C.c_void_p]),
+    ("gmi_cilia_end_of_step", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_cilia_get_state", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    ("gmi_cilia_get_solution", C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    ("gmi_cilia_generalised_forces", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_cilia_segment_velocities", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_cilia_system_size", C.c_int, [C.c_void_p, _ip]),
+    ("gmi_cilia_rhs", C.c_int, [C.c_void_p, _dp]),
+    ("gmi_cilia_apply_operator", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("gmi_cilia_apply_preconditioner", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("gmi_cilia_set_precision", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_cilia_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
 ]
 
 GMI_ERR_NOT_CONVERGED = -6
@@ -400,3 +417,114 @@ class Sim(Context):
         self.call("gmi_warmstart_loss_grad", X.ctypes.data_as(fp), T.ctypes.data_as(fp),
                   X.shape[0], C.byref(loss), g.ctypes.data_as(fp))
         return loss.value, g
+
+
+# ---- prescribed cilia (gmi_cilia.h) ------------------------------------------
+
+GMI_BEAT_BUILD, GMI_BEAT_FULFORD_BLAKE, GMI_BEAT_CORAL_LARVAE = 0, 1, 2
+
+
+class GmiCiliaParams(C.Structure):
+    _fields_ = [
+        ("nfil", _ip), ("blob_refs", _dp), ("fil_refs", _dp), ("body_x", _dp), ("phase", _dp),
+        ("omega0", _dp), ("r_over_l", C.c_double), ("dt", C.c_double), ("beat", C.c_int),
+        ("amplitude", C.c_double), ("recovery_window", C.c_double),
+        ("effective_stroke", C.c_double), ("zero_velocity_avoidance", C.c_double),
+        ("dynamic_phase", C.c_int), ("fixed_body", C.c_int), ("body_velocity", _dp),
+        ("gen_force_refs", _dp), ("n_gen_force_refs", C.c_int), ("gmres_tol", C.c_double),
+        ("gmres_max_iter", C.c_int),
+    ]
+
+
+def _optp(a):
+    return ptr(a) if a is not None else None
+
+
+class Cilia(Context):
+    """Prescribed-cilia time stepper for a geometry.CiliaConfig (gmi_cilia.h)."""
+
+    def __init__(self, cfg, precision=GMI_FP64, device=0):
+        a = cfg.arrays()
+        super().__init__(a["nfil"] * cfg.nseg, a["nblob"], cfg.nseg, mu=cfg.mu,
+                         a_seg=cfg.a_seg, a_blob=cfg.a_blob, precision=precision,
+                         interact=cfg.interact, device=device)
+        self._a = a
+        ng = 0 if a["gen_force_refs"] is None else a["gen_force_refs"].size
+        self.cp = GmiCiliaParams(
+            a["nfil"].ctypes.data_as(_ip), _optp(a["blob_refs"]), ptr(a["fil_refs"]),
+            _optp(a["body_x"]), _optp(a["phase"]), _optp(a["omega0"]), cfg.r_over_l, cfg.dt,
+            cfg.beat, cfg.amplitude, cfg.recovery_window, cfg.effective_stroke,
+            cfg.zero_velocity_avoidance, int(bool(cfg.dynamic_phase)), int(bool(cfg.fixed_body)),
+            _optp(a["body_velocity"]), _optp(a["gen_force_refs"]), ng, cfg.gmres_tol,
+            cfg.gmres_max_iter)
+        self.call("gmi_cilia_create", C.byref(self.cp))
+        self.cfg = cfg
+        n = C.c_int(0)
+        self.call("gmi_cilia_system_size", C.byref(n))
+        self.n = n.value
+
+    def step(self, nt):
+        it = C.c_int(0)
+        rc = self.lib.gmi_cilia_step(self.h, int(nt), C.byref(it))
+        return rc, it.value
+
+    def initial_guess(self, nt=0):
+        self.call("gmi_cilia_initial_guess", int(nt))
+
+    def compute_velocities(self, nt=0):
+        it = C.c_int(0)
+        rc = self.lib.gmi_cilia_compute_velocities(self.h, int(nt), C.byref(it))
+        return rc, it.value
+
+    def integrate_bodies(self):
+        self.call("gmi_cilia_integrate_bodies")
+
+    def end_of_step(self, nt=0):
+        self.call("gmi_cilia_end_of_step", int(nt))
+
+    def state(self):
+        c = self.cfg
+        x = np.zeros(3 * c.ns); ph = np.zeros(c.nf); sp = np.zeros(c.nf)
+        body = np.zeros(7 * c.n_swimmers); k = np.zeros(3 * c.ns)
+        self.call("gmi_cilia_get_state", ptr(x), ptr(ph), ptr(sp), ptr(body), ptr(k))
+        return dict(x=x.reshape(-1, 3), phase=ph, speed=sp, body=body.reshape(-1, 7),
+                    vel_dir=k.reshape(-1, 3))
+
+    def solution(self):
+        c = self.cfg
+        fs = np.zeros(3 * c.ns); fb = np.zeros(3 * c.nb); vb = np.zeros(6 * c.n_swimmers)
+        self.call("gmi_cilia_get_solution", ptr(fs), ptr(fb) if c.nb else None, ptr(vb))
+        return dict(f_segs=fs.reshape(-1, 3), f_blobs=fb.reshape(-1, 3), v_bodies=vb.reshape(-1, 6))
+
+    def generalised_forces(self):
+        out = np.zeros(self.cfg.nf)
+        self.call("gmi_cilia_generalised_forces", ptr(out))
+        return out
+
+    def segment_velocities(self):
+        out = np.zeros(3 * self.cfg.ns)
+        self.call("gmi_cilia_segment_velocities", ptr(out))
+        return out.reshape(-1, 3)
+
+    def rhs(self):
+        out = np.zeros(self.n)
+        self.call("gmi_cilia_rhs", ptr(out))
+        return out
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.n)
+        self.call("gmi_cilia_apply_operator", ptr(x), ptr(out))
+        return out
+
+    def apply_preconditioner(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.n)
+        self.call("gmi_cilia_apply_preconditioner", ptr(x), ptr(out))
+        return out
+
+    def set_cilia_precision(self, p):
+        self.call("gmi_cilia_set_precision", int(p))
+
+    def set_graphs(self, use_graphs=True, chunk=6):
+        self.call("gmi_cilia_set_graphs", int(bool(use_graphs)), int(chunk))

@@ -145,6 +145,7 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
   a.mask = mask;
   a.accumulate = accumulate;
   a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
+  a.tt_only = c->tt_only ? 1 : 0;
   hipEvent_t ev_mid = nullptr;
   if (c->profiling && wl.nwork > 0) {
     int rc;

@@ -2,6 +2,11 @@
 #pragma once
 #include "gmi_internal.hpp"
 
+namespace gmi {
+// what gmi_ctx::extra holds
+enum ExtraKind : int { kExtraNone = 0, kExtraSim = 1, kExtraCilia = 2 };
+}  // namespace gmi
+
 struct gmi_ctx {
   // pinned/device staging slots for host copies
   enum { kStXs, kStFs, kStXb, kStFb, kStRep, kStVb, kStVs, kNumStage };
@@ -37,6 +42,7 @@ struct gmi_ctx {
 
   // launch shaping (set by the tuner)
   bool wall = false;  // segment-segment mobility above a no-slip wall at z = 0
+  bool tt_only = false;  // translation-only pair blocks (prescribed cilia)
   int target_blocks = 2048;
   int min_chunk = 128;
   int list_generation = 0;  // bumped whenever the work lists are rebuilt
@@ -55,10 +61,12 @@ struct gmi_ctx {
   // hook for subsystems that own further device memory
   void (*extra_release)(gmi_ctx *) = nullptr;
   void *extra = nullptr;
+  int extra_kind = gmi::kExtraNone;
   void release_extra() {
     if (extra_release) extra_release(this);
     extra_release = nullptr;
     extra = nullptr;
+    extra_kind = gmi::kExtraNone;
   }
 };
 

@@ -44,6 +44,8 @@ struct PairArgs {
   int accumulate;           // 0: V = M F, 1: V += M F
   double a_s, a_b, mu;
   const int *skip = nullptr; // if non-null and *skip != 0 the launch is a no-op
+  int tt_only = 0;           // 1: translations only -- segments carry forces
+                             // (3 rows of Fs) and get velocities (3 rows of Vs)
 };
 
 // Work list of one pair launch (built once per mask).

@@ -34,7 +34,9 @@ int learn_lags(gmi_ctx *c, SimHost *s, std::vector<float> *get, const std::vecto
 
 namespace {
 
-SimHost *sim_of(gmi_ctx *c) { return c ? static_cast<SimHost *>(c->extra) : nullptr; }
+SimHost *sim_of(gmi_ctx *c) {
+  return (c && c->extra_kind == kExtraSim) ? static_cast<SimHost *>(c->extra) : nullptr;
+}
 
 // Host copy of the device state, SoA as on the device.
 struct HostState {
@@ -77,6 +79,9 @@ int download(gmi_ctx *c, SimHost *s, HostState &h) {
 int upload(gmi_ctx *c, SimHost *s, const HostState &h) {
   const SimDev &d = s->d;
   int rc;
+  // the copies below run on the null stream, which does not order with the
+  // context's non-blocking stream
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   if ((rc = h2d_vec(c, d.X, h.X))) return rc;
   if ((rc = h2d_vec(c, d.Xm1, h.Xm1))) return rc;
   if ((rc = h2d_vec(c, d.Xm2, h.Xm2))) return rc;

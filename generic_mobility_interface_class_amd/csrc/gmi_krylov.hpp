@@ -1,0 +1,547 @@
+// gmi_krylov.hpp -- device GMRES shared by the rigid-body solve
+// (gmi_sim.hip) and the prescribed-cilia saddle-point solve (gmi_cilia.hip).
+//
+// Right-preconditioned GMRES (mobility_solver.cpp:970-1354) with the Krylov
+// basis, Hessenberg matrix and Givens rotations resident in HBM.  Arnoldi
+// uses classical Gram-Schmidt applied twice (CGS2): one step is two batched
+// dot-product sweeps over all previous columns instead of k dependent ones
+// (same basis to rounding).  The host only reads the convergence flag once
+// per chunk of Arnoldi steps; a chunk can be captured once as a HIP graph
+// and replayed by later solves.
+//
+// An operator type Op supplies
+//   int mult(double *z, double *out, int mode, const int *done, bool zpart_ready)
+//       out = A z; mode 0: FP64 pairs, 1: FP32 pairs, 2: both launched, each
+//       gated by its device flag (F_SKIP64 / F_SKIP32); zpart_ready: z came
+//       straight from precond() (which may leave per-tile shares behind);
+//   int precond(double *in, double *out, const int *skip, int norm_slot)
+//       out = P^-1 in; norm_slot >= 0: `in` is first divided in place by
+//       scal[norm_slot] (the previous Arnoldi vector's norm).
+// Every kernel either operator launches must return at once when `done`
+// (the F_DONE flag) is set, so that a chunk needs no host decisions.
+#pragma once
+#include "gmi_ctx.hpp"
+#include "../../include/gmi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <utility>
+
+namespace gmi {
+
+// device flags
+enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
+       F_N = 8 };
+
+// Device buffers and host state of one GMRES solver.
+struct KrylovState {
+  double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr,
+         *sn = nullptr, *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr,
+         *z = nullptr, *tmp = nullptr, *scal = nullptr, *part = nullptr, *pp = nullptr;
+  size_t pp_half = 0;     // column dot partials: pp[0, pp_half) and pp[pp_half, 2 pp_half)
+  int *flags = nullptr;   // device flags (F_*)
+  int *h_flags = nullptr; // pinned copies
+  double *h_scal = nullptr;
+  int cols = 0;           // Arnoldi steps the basis holds (H is (cols+1) x (cols+1))
+  int chunk = 6;          // Arnoldi steps per host check
+  bool use_graphs = true;
+  int graph_gen = -1;
+  // captured Arnoldi chunks, keyed by (mixed, first iteration)
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  void drop_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+};
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int NT>
+__device__ double block_sum(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Dot products with many columns, chunked over the rows: workgroup b takes
+// R consecutive rows (R a multiple of 256, at most 128 workgroups until R
+// reaches kMaxRows) and writes one partial per column, part[j * nb + b];
+// each consumer sums a column's partials in workgroup order.  No
+// cross-workgroup synchronisation inside a kernel: the kernel boundary makes
+// the partials visible.
+constexpr int kMaxDotBlocks = 128;
+constexpr int kMaxRows = 6144;     // rows per workgroup (LDS: R doubles, + kMaxCols)
+constexpr int kMaxCols = 1024;     // Krylov / Broyden columns held in LDS
+
+// partials of rows [e0, e0+R) (values in sw) with columns j < k; wave w
+// takes the columns j = w (mod 4)
+__device__ void col_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
+                         int R, double *part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rmax = min(R, n - e0);
+  for (int j = wave; j < k; j += 4) {
+    const double *q = Qb + ldq * j + e0;
+    double acc = 0.0;
+    for (int r = lane; r < rmax; r += 64) acc += q[r] * sw[r];
+    acc = wave_sum(acc);
+    if (lane == 0) part[size_t(j) * gridDim.x + blockIdx.x] = acc;
+  }
+}
+
+// h[j] = sum over workgroups of part[j][b] (workgroup order), j < k, into
+// LDS (and, from workgroup 0, into global hout)
+__device__ void col_sums(const double *part, int k, int nb, double *hs, double *hout) {
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    const double *p = part + size_t(j) * nb;
+    double s = 0.0;
+#pragma unroll 4
+    for (int b = 0; b < nb; b++) s += p[b];
+    hs[j] = s;
+    if (hout && blockIdx.x == 0) hout[j] = s;
+  }
+}
+
+// partials of h[j] = <Q_j, w>, j < k
+__global__ __launch_bounds__(256) void k_pdots(const double *Qb, size_t ldq, int k,
+                                               const double *w, int n, int R, double *part,
+                                               const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  extern __shared__ double sw[];
+  const int e0 = blockIdx.x * R;
+  for (int r = threadIdx.x; r < R; r += 256) sw[r] = (e0 + r < n) ? w[e0 + r] : 0.0;
+  __syncthreads();
+  col_dots(Qb, ldq, k, sw, e0, n, R, part);
+}
+
+// h[j] = sum of the partials, one workgroup per column (large n: many
+// partials per column)
+__global__ __launch_bounds__(256) void k_colsum_wide(const double *part, int nb, double *h,
+                                                     const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[256];
+  const double *p = part + size_t(blockIdx.x) * nb;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) h[blockIdx.x] = s;
+}
+
+// h[j] = sum of the partials (one workgroup per 256 columns)
+__global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= k) return;
+  const double *p = part + size_t(j) * nb;
+  double s = 0.0;
+#pragma unroll 4
+  for (int b = 0; b < nb; b++) s += p[b];
+  h[j] = s;
+}
+
+// Gram-Schmidt pass 1: h1 = sum of the pass-1 partials; w -= sum_j h1[j] Q_j;
+// then the partials of the pass-2 dots <Q_j, w>
+// (h_ready: h1 was summed by k_colsum_wide already)
+__global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq, int k,
+                                                   const double *part_in, double *h1, double *w,
+                                                   int n, int R, double *part_out,
+                                                   const int *flags, int h_ready) {
+  if (flags[F_DONE]) return;
+  __shared__ double hs[kMaxCols];
+  extern __shared__ double sw[];
+  if (h_ready)
+    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h1[j];
+  else
+    col_sums(part_in, k, gridDim.x, hs, h1);
+  __syncthreads();
+  const int e0 = blockIdx.x * R;
+  for (int r = threadIdx.x; r < R; r += 256) {
+    const int e = e0 + r;
+    double v = 0.0;
+    if (e < n) {
+      v = w[e];
+      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
+      w[e] = v;
+    }
+    sw[r] = v;
+  }
+  __syncthreads();
+  col_dots(Qb, ldq, k, sw, e0, n, R, part_out);
+}
+
+// Gram-Schmidt pass 2: h2 = sum of the pass-2 partials; w -= sum_j h2[j] Q_j;
+// the workgroup's share of |w|^2
+__global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq, int k,
+                                                   const double *part_in, double *h2, double *w,
+                                                   int n, int R, double *norm_part,
+                                                   const int *flags, int h_ready) {
+  if (flags[F_DONE]) return;
+  __shared__ double hs[kMaxCols];
+  __shared__ double sh[256];
+  if (h_ready)
+    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h2[j];
+  else
+    col_sums(part_in, k, gridDim.x, hs, h2);
+  __syncthreads();
+  const int e0 = blockIdx.x * R;
+  double s2 = 0.0;
+  for (int r = threadIdx.x; r < R; r += 256) {
+    const int e = e0 + r;
+    if (e < n) {
+      double v = w[e];
+      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
+      w[e] = v;
+      s2 += v * v;
+    }
+  }
+  s2 = block_sum<256>(s2, sh);
+  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
+}
+
+// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
+// new vector's norm is the sum of the workgroup shares; H column = h1 + h2
+// (the two Gram-Schmidt passes).  The division by the norm happens in the
+// next iteration's k_precond_kt.
+__global__ __launch_bounds__(256) void k_givens(const double *norm_part, int nparts, int it,
+                                                const double *h1, const double *h2, double *H,
+                                                int ldh, double *cs, double *sn, double *beta,
+                                                int *flags, double *scal, double tol, int maxit,
+                                                double switch_rel) {
+  if (flags[F_DONE]) return;
+  __shared__ double sh[256];
+  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
+  double s = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) s += norm_part[p];
+  // the new column and the previous rotations, staged in LDS
+  for (int i = threadIdx.x; i < it; i += 256) {
+    hc[i] = h1[i] + h2[i];
+    if (i < it - 1) { scs[i] = cs[i]; ssn[i] = sn[i]; }
+  }
+  s = block_sum<256>(s, sh);
+  const int c = it - 1;
+  if (threadIdx.x == 0) {
+    const double qn = sqrt(s);
+    scal[3] = qn;
+    hc[it] = qn;
+    for (int i = 1; i < it; i++) {
+      const double t = scs[i - 1] * hc[i - 1] + ssn[i - 1] * hc[i];
+      hc[i] = scs[i - 1] * hc[i] - ssn[i - 1] * hc[i - 1];
+      hc[i - 1] = t;
+    }
+    double csc, snc;
+    if (hc[it] == 0.0) {
+      csc = 1.0; snc = 0.0;
+    } else if (fabs(hc[it]) > fabs(hc[c])) {
+      const double t = hc[c] / hc[it];
+      snc = (2.0 * double(hc[it] > 0) - 1.0) / sqrt(1.0 + t * t);
+      csc = t * snc;
+    } else {
+      const double t = hc[it] / hc[c];
+      csc = (2.0 * double(hc[c] > 0) - 1.0) / sqrt(1.0 + t * t);
+      snc = t * csc;
+    }
+    cs[c] = csc; sn[c] = snc;
+    hc[c] = csc * hc[c] + snc * hc[it];
+    hc[it] = 0.0;
+    beta[it] = -snc * beta[c];
+    beta[c] = csc * beta[c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i <= it; i += 256) H[size_t(i) * ldh + c] = hc[i];
+  if (threadIdx.x != 0) return;
+  const double rel = fabs(beta[it]) / scal[0];
+  if (rel <= tol || it == maxit) {
+    flags[F_DONE] = 1;
+    flags[F_ITS] = it;
+    flags[F_SKIP64] = 1;
+    flags[F_SKIP32] = 1;
+  } else if (rel <= switch_rel) {
+    // relaxed Krylov: once the residual has dropped below switch_rel the
+    // operator may carry a relative error up to tol / rel, so the FP32 pair
+    // kernel replaces the FP64 one for the remaining Arnoldi steps
+    flags[F_SKIP64] = 1;
+    flags[F_SKIP32] = 0;
+  }
+}
+
+__global__ void k_gmres_reset(int *flags) {
+  flags[F_DONE] = 0;
+  flags[F_ITS] = 0;
+  flags[F_SKIP64] = 0;
+  flags[F_SKIP32] = 1;
+}
+
+// back substitution for y (mobility_solver.cpp:1236-1257) by one wave:
+// column sweep from the last unknown, each lane updating its rows, which
+// subtracts the terms in the reference's order (j from it-1 down to i+1)
+__global__ __launch_bounds__(64) void k_backsub(const double *H, int ldh, const double *beta,
+                                                const int *flags, double *y) {
+  extern __shared__ double rb[];
+  const int it = flags[F_ITS];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < it; i += 64) rb[i] = beta[i];
+  __syncthreads();
+  for (int j = it - 1; j >= 0; j--) {
+    const double yj = rb[j] / H[size_t(j) * ldh + j];
+    for (int i = lane; i < j; i += 64) rb[i] -= H[size_t(i) * ldh + j] * yj;
+    if (lane == 0) y[j] = yj;
+    __syncthreads();
+  }
+}
+
+__global__ void k_combine(const double *Qb, size_t ldq, const int *flags, const double *y,
+                          double *t, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int it = flags[F_ITS];
+  double s = 0;
+  for (int i = 0; i < it; i++) s += y[i] * Qb[ldq * i + e];
+  t[e] = s;
+}
+
+// out = a*x + b*y  (elementwise)
+__global__ void k_axpby(double *out, double a, const double *x, double b, const double *y, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = a * x[e] + (y ? b * y[e] : 0.0);
+}
+
+// scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup)
+__global__ __launch_bounds__(1024) void k_norms(const double *x, const double *y, int n,
+                                                double *scal, int o0, int o1) {
+  __shared__ double sh[1024];
+  for (int pass = 0; pass < 2; pass++) {
+    const double *v = pass ? y : x;
+    if (!v) break;
+    double s = 0;
+    for (int e = threadIdx.x; e < n; e += 1024) s += v[e] * v[e];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) scal[pass ? o1 : o0] = sqrt(sh[0]);
+    __syncthreads();
+  }
+}
+
+__global__ void k_scale_by(double *x, int n, const double *scal, int o) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] /= scal[o];
+}
+
+__global__ void k_set_ints(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+// Relative error of the FP32-Kahan pair operator against FP64 (measured
+// max-norm error ~6e-7 on 46.7k particles, bounded here by 1e-6).
+constexpr double kFp32OperatorError = 1e-6;
+
+static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
+
+static int krylov_read_flags(gmi_ctx *c, KrylovState *s) {
+  int rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->h_flags, s->flags, sizeof(int) * F_N,
+                                        hipMemcpyDeviceToHost, c->stream), "flags D2H")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+// Row split of the column dot products: nb workgroups of R rows.
+// Up to kMaxDotBlocks workgroups every consumer workgroup re-sums a column's
+// partials itself; beyond that (wide) a k_colsum_wide launch sums them once
+// and up to ~1024 workgroups share the rows.
+struct DotShape {
+  int nb, R;
+  bool wide;
+};
+static DotShape dot_shape(int n) {
+  const int chunks = nblk(n, 256);
+  if (chunks <= kMaxDotBlocks) return {chunks, 256, false};
+  const int per = std::min((chunks + 1023) / 1024, kMaxRows / 256);  // 256-row chunks per workgroup
+  const int R = per * 256;
+  return {nblk(n, R), R, true};
+}
+
+// GMRES with right preconditioning (mobility_solver.cpp:970-1354) on s->rhs,
+// warm-started from s->x; CGS2 Arnoldi.  mixed: relaxed precision (FP64
+// operator until the residual drops below switch_rel, FP32 after).
+// n: vector length (padding rows stay zero); maxit: iteration cap.
+// Returns iterations via *its.
+template <class Op>
+int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixed, double tol,
+                 int *its) {
+  const size_t ldq = size_t(n);
+  const int nb = nblk(n, 256);
+  int rc;
+  *its = 0;
+  // r = rhs - A x ; norms
+  double *r = s->Qk;
+  if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
+  if ((rc = launch_ok(c, "gmres init"))) return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal, s->scal, 2 * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream), "scal D2H")))
+    return rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  const double nrhs = s->h_scal[0];
+  double b0 = s->h_scal[1];
+  if (nrhs == 0.0) {
+    return hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x");
+  }
+  if (std::fabs(b0) <= tol * nrhs) return GMI_OK;
+  // a warm start worse than zero (mobility_solver.cpp:1060-1156 keeps the
+  // previous solution) -- or a non-finite one -- is replaced by zero
+  if (!std::isfinite(b0) || std::fabs(b0) > nrhs) {
+    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
+      return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(r, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                          c->stream), "copy rhs")))
+      return rc;
+    b0 = nrhs;
+  }
+  s->h_scal[2] = b0;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->scal + 2, s->h_scal + 2, sizeof(double),
+                                        hipMemcpyHostToDevice, c->stream), "beta0")))
+    return rc;
+  if ((rc = hip_check(c, hipMemcpyAsync(s->beta, s->h_scal + 2, sizeof(double),
+                                        hipMemcpyHostToDevice, c->stream), "beta0")))
+    return rc;
+  hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, r, n, s->scal, 2);
+  hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, s->flags);
+  if ((rc = launch_ok(c, "gmres start"))) return rc;
+  const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
+  // scal[0] = ||rhs|| is read by k_givens
+  const int ldh = s->cols + 1;
+  // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
+  // says the solve has converged, so a chunk needs no host decisions
+  const DotShape ds = dot_shape(n);
+  const size_t shm = sizeof(double) * ds.R;
+  double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
+  auto enqueue = [&](int it0, int it1) -> int {
+    int r;
+    for (int it = it0; it <= it1; it++) {
+      double *qprev = s->Qk + ldq * (it - 1);
+      double *w = s->Qk + ldq * it;
+      // the previous vector is normalised by the preconditioner's first
+      // kernel (the first one was scaled by beta0 already)
+      if ((r = op.precond(qprev, s->z, s->flags, it >= 2 ? 3 : -1))) return r;
+      if ((r = op.mult(s->z, w, mixed ? 2 : 0, s->flags, true))) return r;
+      // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
+      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
+                         ds.R, pp1, s->flags);
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
+                           s->flags);
+      hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
+                         s->h1, w, n, ds.R, pp2, s->flags, int(ds.wide));
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
+                           s->flags);
+      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
+                         s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
+                         s->h2, s->H, ldh, s->cs, s->sn, s->beta, s->flags, s->scal, tol, maxit,
+                         switch_rel);
+      if ((r = launch_ok(c, "arnoldi"))) return r;
+    }
+    return GMI_OK;
+  };
+  if (s->graph_gen != c->list_generation) {
+    s->drop_graphs();
+    s->graph_gen = c->list_generation;
+  }
+  int it = 1;
+  int done = 0;
+  while (!done) {
+    const int last = std::min(maxit, it + s->chunk - 1);
+    if (s->use_graphs) {
+      // one captured graph per (precision mode, chunk start); replayed by
+      // every later solve of this context
+      const std::pair<int, int> key{mixed ? 1 : 0, it};
+      auto g = s->graphs.find(key);
+      if (g == s->graphs.end()) {
+        hipGraph_t graph;
+        if ((rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                            "begin capture")))
+          return rc;
+        const int erc = enqueue(it, last);
+        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (erc) return erc;
+        if ((rc = hip_check(c, ec, "end capture"))) return rc;
+        hipGraphExec_t exec;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if ((rc = hip_check(c, ei, "instantiate"))) return rc;
+        g = s->graphs.emplace(key, exec).first;
+      }
+      if ((rc = hip_check(c, hipGraphLaunch(g->second, c->stream), "graph launch"))) return rc;
+    } else {
+      if ((rc = enqueue(it, last))) return rc;
+    }
+    it = last + 1;
+    if ((rc = krylov_read_flags(c, s))) return rc;
+    done = s->h_flags[F_DONE];
+    if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
+  }
+  const int nits = s->h_flags[F_ITS];
+  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->cols + 1), c->stream,
+                     s->H, ldh, s->beta, s->flags, s->y);
+  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, s->flags, s->y,
+                     s->tmp, n);
+  if ((rc = launch_ok(c, "gmres solution"))) return rc;
+  if ((rc = op.precond(s->tmp, s->z, nullptr, -1))) return rc;
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
+  if ((rc = launch_ok(c, "gmres update"))) return rc;
+  *its = nits;
+  return GMI_OK;
+}
+
+// Linear solve.  FP64: plain GMRES.  Mixed: relaxed-precision GMRES, then
+// the true residual with the FP64 operator; if it misses the tolerance GMRES
+// restarts from the current iterate (iterative refinement, FP64 outer).
+template <class Op>
+int krylov_solve(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, int precision, double tol,
+                 int *its) {
+  if (precision == GMI_FP64) return krylov_gmres(c, s, op, n, maxit, false, tol, its);
+  const int nb = nblk(n, 256);
+  int rc, total = 0;
+  for (int cycle = 0; cycle < 4; cycle++) {
+    int it = 0;
+    if ((rc = krylov_gmres(c, s, op, n, maxit, true, tol, &it))) return rc;
+    total += it;
+    // true residual with the FP64 operator
+    if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->z, 1.0, s->rhs, -1.0,
+                       s->tmp, n);
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5);
+    if ((rc = launch_ok(c, "refinement residual"))) return rc;
+    if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal + 4, s->scal + 4, 2 * sizeof(double),
+                                          hipMemcpyDeviceToHost, c->stream), "scal")))
+      return rc;
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+    const double nb0 = s->h_scal[4], nr = s->h_scal[5];
+    if (nb0 == 0.0 || nr <= tol * nb0) break;
+  }
+  *its = total;
+  return GMI_OK;
+}
+
+}  // namespace
+}  // namespace gmi

@@ -425,6 +425,8 @@ double *learn_correction(SimHost *s) { return s->learn ? s->learn->corr : nullpt
 int learn_lags(gmi_ctx *c, SimHost *s, std::vector<float> *get, const std::vector<float> *set) {
   Learn *L = s->learn;
   const size_t n = L ? size_t(s->d.Ns) * 6 * kLags : 0;
+  // null-stream copies: order them after the context stream's work
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, GMI_ERR_HIP, "sync");
   if (get) {
     get->assign(n, 0.0f);
     if (n && hipMemcpy(get->data(), L->lags, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
@@ -469,14 +471,15 @@ int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt) {
 using namespace gmi;
 
 static Learn *learn_of(gmi_ctx *c) {
-  if (!c || !c->extra) return nullptr;
+  if (!c || !c->extra || c->extra_kind != kExtraSim) return nullptr;
   return static_cast<SimHost *>(c->extra)->learn;
 }
 
 extern "C" {
 
 int gmi_warmstart_init(gmi_ctx *c, int hidden, unsigned long long seed) {
-  if (!c || !c->extra) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (!c || !c->extra || c->extra_kind != kExtraSim)
+    return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   if (hidden < 1 || hidden > 1024) return fail(c, GMI_ERR_ARG, "hidden width out of range");
   SimHost *s = static_cast<SimHost *>(c->extra);
   if (s->learn) learn_release(s);
@@ -547,8 +550,10 @@ int gmi_warmstart_params(gmi_ctx *c, float *get, const float *set) {
     hipLaunchKernelGGL(k_quantize, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->pq, L->np);
     if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   }
-  if (get)
+  if (get) {
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
     return hip_check(c, hipMemcpy(get, L->p, sizeof(float) * L->np, hipMemcpyDeviceToHost), "get");
+  }
   return GMI_OK;
 }
 

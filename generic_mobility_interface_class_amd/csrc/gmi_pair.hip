@@ -219,7 +219,30 @@ __device__ __forceinline__ void pair_bs(R dx, R dy, R dz, R fx, R fy, R fz,
   o.vz += At * fz + Bt * dz + g * (tx * dy - ty * dx);
 }
 
-// bb: blob v <- blob f, equal radius.
+// translation only, unequal radii: segment v <- blob f or blob v <- segment f
+// (the tt parts of sb / bs; the prescribed-cilia operator has no torques and
+// no angular velocities, cuda_functions.cu:646,808)
+template <typename R>
+__device__ __forceinline__ void pair_ut(R dx, R dy, R dz, R fx, R fy, R fz,
+                                        const PC<R> &k, Acc<R> &o) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R fd = fx * dx + fy * dy + fz * dz;
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
+  R At = ri * (k.c8 + k.c8s3 * ri2);
+  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  const bool nr = !(r2 > k.far2);
+  if (__builtin_expect(wave_any(nr), 0)) {
+    R At2, Bt2, g;
+    near_unequal(r2, fd, k, At2, Bt2, g);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
+  }
+  o.vx += At * fx + Bt * dx;
+  o.vy += At * fy + Bt * dy;
+  o.vz += At * fz + Bt * dz;
+}
+
+// bb: blob v <- blob f, equal radius (also segment v <- segment f when only
+// translations couple).
 template <typename R>
 __device__ __forceinline__ void pair_bb(R dx, R dy, R dz, R fx, R fy, R fz,
                                         const PC<R> &k, Acc<R> &o) {
@@ -334,7 +357,7 @@ template <> struct Sum<float> {
   }
 };
 
-enum Kind { SS, SB, BS, BB };
+enum Kind { SS, SB, BS, BB, UT };
 
 // Walk sources [j0, j1) of one species in LDS tiles and accumulate into sum.
 template <typename R, int K>
@@ -375,6 +398,8 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
       else if (K == BS)
         pair_bs<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], lds.tx[j],
                    lds.ty[j], lds.tz[j], k, acc);
+      else if (K == UT)
+        pair_ut<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
       else
         pair_bb<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
     }
@@ -413,16 +438,25 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
   if (seg_t) {
     if (s1 > s0) {
       const PC<R> k = make_pc<R>(a.a_s, a.a_s, a.mu);
-      sweep<R, SS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+      if (a.tt_only)
+        sweep<R, BB>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+      else
+        sweep<R, SS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
     }
     if (b1 > b0) {
       const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
-      sweep<R, SB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+      if (a.tt_only)
+        sweep<R, UT>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+      else
+        sweep<R, SB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
     }
   } else {
     if (s1 > s0) {
       const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
-      sweep<R, BS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+      if (a.tt_only)
+        sweep<R, UT>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+      else
+        sweep<R, BS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
     }
     if (b1 > b0) {
       const PC<R> k = make_pc<R>(a.a_b, a.a_b, a.mu);
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
     }
   }
   const bool live = tid < T.nt;
-  const int nc = seg_t ? 6 : 3;
+  const int nc = (seg_t && !a.tt_only) ? 6 : 3;
   double *V = seg_t ? a.Vs : a.Vb;
   const int ld = seg_t ? a.lds : a.ldb;
   if (T.nsplit == 1) {
@@ -612,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
   const int tid = threadIdx.x;
   if (T.nsplit == 1 || tid >= T.nt) return;
   const bool seg_t = (T.species == 0);
-  const int nc = seg_t ? 6 : 3;
+  const int nc = (seg_t && !a.tt_only) ? 6 : 3;
   double *V = seg_t ? a.Vs : a.Vb;
   const int ld = seg_t ? a.lds : a.ldb;
   double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};

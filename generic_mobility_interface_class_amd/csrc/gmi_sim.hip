@@ -20,6 +20,7 @@
 #include "gmi_ctx.hpp"
 #include "gmi_lie.hpp"
 #include "gmi_sim_state.hpp"
+#include "gmi_krylov.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -78,25 +79,6 @@ __device__ bool gj_small(double *A, int n) {
     for (int i = 0; i < n; i++) B[i * n + piv[j]] = A[i * n + j];
   for (int k = 0; k < n * n; k++) A[k] = B[k];
   return true;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-template <int NT>
-__device__ double block_sum(double v, double *sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
 }
 
 // -------------------------------------------------------------------------
@@ -487,276 +469,6 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
 
 // -------------------------------------------------------------------------
 // vector kernels (GMRES / Broyden)
-
-// Dot products with many columns, chunked over the rows: workgroup b takes
-// R consecutive rows (R a multiple of 256, at most 128 workgroups until R
-// reaches kMaxRows) and writes one partial per column, part[j * nb + b];
-// each consumer sums a column's partials in workgroup order.  No
-// cross-workgroup synchronisation inside a kernel: the kernel boundary makes
-// the partials visible.
-constexpr int kMaxDotBlocks = 128;
-constexpr int kMaxRows = 6144;     // rows per workgroup (LDS: R doubles, + kMaxCols)
-constexpr int kMaxCols = 1024;     // Krylov / Broyden columns held in LDS
-
-// partials of rows [e0, e0+R) (values in sw) with columns j < k; wave w
-// takes the columns j = w (mod 4)
-__device__ void col_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
-                         int R, double *part) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rmax = min(R, n - e0);
-  for (int j = wave; j < k; j += 4) {
-    const double *q = Qb + ldq * j + e0;
-    double acc = 0.0;
-    for (int r = lane; r < rmax; r += 64) acc += q[r] * sw[r];
-    acc = wave_sum(acc);
-    if (lane == 0) part[size_t(j) * gridDim.x + blockIdx.x] = acc;
-  }
-}
-
-// h[j] = sum over workgroups of part[j][b] (workgroup order), j < k, into
-// LDS (and, from workgroup 0, into global hout)
-__device__ void col_sums(const double *part, int k, int nb, double *hs, double *hout) {
-  for (int j = threadIdx.x; j < k; j += blockDim.x) {
-    const double *p = part + size_t(j) * nb;
-    double s = 0.0;
-#pragma unroll 4
-    for (int b = 0; b < nb; b++) s += p[b];
-    hs[j] = s;
-    if (hout && blockIdx.x == 0) hout[j] = s;
-  }
-}
-
-// partials of h[j] = <Q_j, w>, j < k
-__global__ __launch_bounds__(256) void k_pdots(const double *Qb, size_t ldq, int k,
-                                               const double *w, int n, int R, double *part,
-                                               const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  extern __shared__ double sw[];
-  const int e0 = blockIdx.x * R;
-  for (int r = threadIdx.x; r < R; r += 256) sw[r] = (e0 + r < n) ? w[e0 + r] : 0.0;
-  __syncthreads();
-  col_dots(Qb, ldq, k, sw, e0, n, R, part);
-}
-
-// h[j] = sum of the partials, one workgroup per column (large n: many
-// partials per column)
-__global__ __launch_bounds__(256) void k_colsum_wide(const double *part, int nb, double *h,
-                                                     const int *flags) {
-  if (flags && flags[F_DONE]) return;
-  __shared__ double sh[256];
-  const double *p = part + size_t(blockIdx.x) * nb;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
-  s = block_sum<256>(s, sh);
-  if (threadIdx.x == 0) h[blockIdx.x] = s;
-}
-
-// h[j] = sum of the partials (one workgroup per 256 columns)
-__global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= k) return;
-  const double *p = part + size_t(j) * nb;
-  double s = 0.0;
-#pragma unroll 4
-  for (int b = 0; b < nb; b++) s += p[b];
-  h[j] = s;
-}
-
-// Gram-Schmidt pass 1: h1 = sum of the pass-1 partials; w -= sum_j h1[j] Q_j;
-// then the partials of the pass-2 dots <Q_j, w>
-// (h_ready: h1 was summed by k_colsum_wide already)
-__global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq, int k,
-                                                   const double *part_in, double *h1, double *w,
-                                                   int n, int R, double *part_out,
-                                                   const int *flags, int h_ready) {
-  if (flags[F_DONE]) return;
-  __shared__ double hs[kMaxCols];
-  extern __shared__ double sw[];
-  if (h_ready)
-    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h1[j];
-  else
-    col_sums(part_in, k, gridDim.x, hs, h1);
-  __syncthreads();
-  const int e0 = blockIdx.x * R;
-  for (int r = threadIdx.x; r < R; r += 256) {
-    const int e = e0 + r;
-    double v = 0.0;
-    if (e < n) {
-      v = w[e];
-      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
-      w[e] = v;
-    }
-    sw[r] = v;
-  }
-  __syncthreads();
-  col_dots(Qb, ldq, k, sw, e0, n, R, part_out);
-}
-
-// Gram-Schmidt pass 2: h2 = sum of the pass-2 partials; w -= sum_j h2[j] Q_j;
-// the workgroup's share of |w|^2
-__global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq, int k,
-                                                   const double *part_in, double *h2, double *w,
-                                                   int n, int R, double *norm_part,
-                                                   const int *flags, int h_ready) {
-  if (flags[F_DONE]) return;
-  __shared__ double hs[kMaxCols];
-  __shared__ double sh[256];
-  if (h_ready)
-    for (int j = threadIdx.x; j < k; j += 256) hs[j] = h2[j];
-  else
-    col_sums(part_in, k, gridDim.x, hs, h2);
-  __syncthreads();
-  const int e0 = blockIdx.x * R;
-  double s2 = 0.0;
-  for (int r = threadIdx.x; r < R; r += 256) {
-    const int e = e0 + r;
-    if (e < n) {
-      double v = w[e];
-      for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
-      w[e] = v;
-      s2 += v * v;
-    }
-  }
-  s2 = block_sum<256>(s2, sh);
-  if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
-}
-
-// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
-// new vector's norm is the sum of the workgroup shares; H column = h1 + h2
-// (the two Gram-Schmidt passes).  The division by the norm happens in the
-// next iteration's k_precond_kt.
-__global__ __launch_bounds__(256) void k_givens(const double *norm_part, int nparts, int it,
-                                                const double *h1, const double *h2, double *H,
-                                                int ldh, double *cs, double *sn, double *beta,
-                                                int *flags, double *scal, double tol, int maxit,
-                                                double switch_rel) {
-  if (flags[F_DONE]) return;
-  __shared__ double sh[256];
-  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
-  double s = 0.0;
-  for (int p = threadIdx.x; p < nparts; p += 256) s += norm_part[p];
-  // the new column and the previous rotations, staged in LDS
-  for (int i = threadIdx.x; i < it; i += 256) {
-    hc[i] = h1[i] + h2[i];
-    if (i < it - 1) { scs[i] = cs[i]; ssn[i] = sn[i]; }
-  }
-  s = block_sum<256>(s, sh);
-  const int c = it - 1;
-  if (threadIdx.x == 0) {
-    const double qn = sqrt(s);
-    scal[3] = qn;
-    hc[it] = qn;
-    for (int i = 1; i < it; i++) {
-      const double t = scs[i - 1] * hc[i - 1] + ssn[i - 1] * hc[i];
-      hc[i] = scs[i - 1] * hc[i] - ssn[i - 1] * hc[i - 1];
-      hc[i - 1] = t;
-    }
-    double csc, snc;
-    if (hc[it] == 0.0) {
-      csc = 1.0; snc = 0.0;
-    } else if (fabs(hc[it]) > fabs(hc[c])) {
-      const double t = hc[c] / hc[it];
-      snc = (2.0 * double(hc[it] > 0) - 1.0) / sqrt(1.0 + t * t);
-      csc = t * snc;
-    } else {
-      const double t = hc[it] / hc[c];
-      csc = (2.0 * double(hc[c] > 0) - 1.0) / sqrt(1.0 + t * t);
-      snc = t * csc;
-    }
-    cs[c] = csc; sn[c] = snc;
-    hc[c] = csc * hc[c] + snc * hc[it];
-    hc[it] = 0.0;
-    beta[it] = -snc * beta[c];
-    beta[c] = csc * beta[c];
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i <= it; i += 256) H[size_t(i) * ldh + c] = hc[i];
-  if (threadIdx.x != 0) return;
-  const double rel = fabs(beta[it]) / scal[0];
-  if (rel <= tol || it == maxit) {
-    flags[F_DONE] = 1;
-    flags[F_ITS] = it;
-    flags[F_SKIP64] = 1;
-    flags[F_SKIP32] = 1;
-  } else if (rel <= switch_rel) {
-    // relaxed Krylov: once the residual has dropped below switch_rel the
-    // operator may carry a relative error up to tol / rel, so the FP32 pair
-    // kernel replaces the FP64 one for the remaining Arnoldi steps
-    flags[F_SKIP64] = 1;
-    flags[F_SKIP32] = 0;
-  }
-}
-
-__global__ void k_gmres_reset(int *flags) {
-  flags[F_DONE] = 0;
-  flags[F_ITS] = 0;
-  flags[F_SKIP64] = 0;
-  flags[F_SKIP32] = 1;
-}
-
-// back substitution for y (mobility_solver.cpp:1236-1257) by one wave:
-// column sweep from the last unknown, each lane updating its rows, which
-// subtracts the terms in the reference's order (j from it-1 down to i+1)
-__global__ __launch_bounds__(64) void k_backsub(const double *H, int ldh, const double *beta,
-                                                const int *flags, double *y) {
-  extern __shared__ double rb[];
-  const int it = flags[F_ITS];
-  const int lane = threadIdx.x;
-  for (int i = lane; i < it; i += 64) rb[i] = beta[i];
-  __syncthreads();
-  for (int j = it - 1; j >= 0; j--) {
-    const double yj = rb[j] / H[size_t(j) * ldh + j];
-    for (int i = lane; i < j; i += 64) rb[i] -= H[size_t(i) * ldh + j] * yj;
-    if (lane == 0) y[j] = yj;
-    __syncthreads();
-  }
-}
-
-__global__ void k_combine(const double *Qb, size_t ldq, const int *flags, const double *y,
-                          double *t, int n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int it = flags[F_ITS];
-  double s = 0;
-  for (int i = 0; i < it; i++) s += y[i] * Qb[ldq * i + e];
-  t[e] = s;
-}
-
-// out = a*x + b*y  (elementwise)
-__global__ void k_axpby(double *out, double a, const double *x, double b, const double *y, int n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) out[e] = a * x[e] + (y ? b * y[e] : 0.0);
-}
-
-// scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup)
-__global__ __launch_bounds__(1024) void k_norms(const double *x, const double *y, int n,
-                                                double *scal, int o0, int o1) {
-  __shared__ double sh[1024];
-  for (int pass = 0; pass < 2; pass++) {
-    const double *v = pass ? y : x;
-    if (!v) break;
-    double s = 0;
-    for (int e = threadIdx.x; e < n; e += 1024) s += v[e] * v[e];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) scal[pass ? o1 : o0] = sqrt(sh[0]);
-    __syncthreads();
-  }
-}
-
-__global__ void k_scale_by(double *x, int n, const double *scal, int o) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) x[e] /= scal[o];
-}
-
-__global__ void k_set_ints(int *p, int n, int v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 // copy U of every swimmer into column `col` of N_ref
 __global__ void k_store_nref_col(SimDev a, const double *x, int col) {
@@ -1443,28 +1155,19 @@ __global__ void k_fil_update(SimDev a, const double *upd) {
   robot_arm(a, f);
 }
 
-inline int nblk(int n, int b) { return (n + b - 1) / b; }
-
-// Relative error of the FP32-Kahan pair operator against FP64 (measured
-// max-norm error ~6e-7 on 46.7k particles, bounded here by 1e-6).
-constexpr double kFp32OperatorError = 1e-6;
-
 }  // namespace
 
 // ==========================================================================
 // host side
 
-static SimHost *sim_of(gmi_ctx *c) { return static_cast<SimHost *>(c->extra); }
-
-static void drop_graphs(SimHost *s) {
-  for (auto &kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
-  s->graphs.clear();
+static SimHost *sim_of(gmi_ctx *c) {
+  return c->extra_kind == kExtraSim ? static_cast<SimHost *>(c->extra) : nullptr;
 }
 
 static void sim_release(gmi_ctx *c) {
   SimHost *s = sim_of(c);
   if (!s) return;
-  drop_graphs(s);
+  s->drop_graphs();
   learn_release(s);
   for (void *p : s->owned) (void)hipFree(p);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
@@ -1492,31 +1195,7 @@ static int upload(gmi_ctx *c, T *dst, const T *src, size_t n) {
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
-static int read_flags(gmi_ctx *c, SimHost *s) {
-  int rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->h_flags, s->d.flags, sizeof(int) * F_N,
-                                        hipMemcpyDeviceToHost, c->stream), "flags D2H")))
-    return rc;
-  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
-}
-
-static int launch_ok(gmi_ctx *c, const char *what) { return hip_check(c, hipGetLastError(), what); }
-
-// Row split of the column dot products: nb workgroups of R rows.
-// Up to kMaxDotBlocks workgroups every consumer workgroup re-sums a column's
-// partials itself; beyond that (wide) a k_colsum_wide launch sums them once
-// and up to ~1024 workgroups share the rows.
-struct DotShape {
-  int nb, R;
-  bool wide;
-};
-static DotShape dot_shape(int n) {
-  const int chunks = nblk(n, 256);
-  if (chunks <= kMaxDotBlocks) return {chunks, 256, false};
-  const int per = std::min((chunks + 1023) / 1024, kMaxRows / 256);  // 256-row chunks per workgroup
-  const int R = per * 256;
-  return {nblk(n, R), R, true};
-}
+static int read_flags(gmi_ctx *c, SimHost *s) { return krylov_read_flags(c, s); }
 
 // system_matrix_mult: out = A z (A as in mobility_solver.cpp:905-964).
 // mode 0: FP64 pairs, 1: FP32 pairs, 2: both launched, each gated by its
@@ -1569,171 +1248,28 @@ static int precond(gmi_ctx *c, SimHost *s, double *in, double *out, const int *s
   return launch_ok(c, "precond");
 }
 
-// GMRES with right preconditioning (mobility_solver.cpp:970-1354) on s->rhs,
-// warm-started from s->x; CGS2 Arnoldi.  mixed: relaxed precision (FP64
-// operator until the residual drops below switch_rel, FP32 after).
-// Returns iterations via *its.
-static int gmres(gmi_ctx *c, SimHost *s, bool mixed, double tol, int *its) {
-  SimDev &d = s->d;
-  const int n = d.npad;
-  const size_t ldq = size_t(n);
-  const int maxit = std::min(s->d.maxit, 3 * d.Nb + 6 * d.Nsw);
-  const int nb = nblk(n, 256);
-  int rc;
-  *its = 0;
-  // r = rhs - A x ; norms
-  double *r = s->Qk;
-  if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr, false))) return rc;
-  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
-  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
-  if ((rc = launch_ok(c, "gmres init"))) return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal, s->scal, 2 * sizeof(double),
-                                        hipMemcpyDeviceToHost, c->stream), "scal D2H")))
-    return rc;
-  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
-  const double nrhs = s->h_scal[0];
-  double b0 = s->h_scal[1];
-  if (nrhs == 0.0) {
-    return hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x");
+// The rigid-body system as the Krylov engine's operator.
+struct SimOp {
+  gmi_ctx *c;
+  SimHost *s;
+  int mult(double *z, double *out, int mode, const int *done, bool zpart_ready) {
+    return sys_mult(c, s, z, out, mode, done, zpart_ready);
   }
-  if (std::fabs(b0) <= tol * nrhs) return GMI_OK;
-  // a warm start worse than zero (mobility_solver.cpp:1060-1156 keeps the
-  // previous solution) -- or a non-finite one -- is replaced by zero
-  if (!std::isfinite(b0) || std::fabs(b0) > nrhs) {
-    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
-      return rc;
-    if ((rc = hip_check(c, hipMemcpyAsync(r, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                          c->stream), "copy rhs")))
-      return rc;
-    b0 = nrhs;
+  int precond(double *in, double *out, const int *skip, int norm_slot) {
+    return gmi::precond(c, s, in, out, skip, norm_slot);
   }
-  s->h_scal[2] = b0;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->scal + 2, s->h_scal + 2, sizeof(double),
-                                        hipMemcpyHostToDevice, c->stream), "beta0")))
-    return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->beta, s->h_scal + 2, sizeof(double),
-                                        hipMemcpyHostToDevice, c->stream), "beta0")))
-    return rc;
-  hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, r, n, s->scal, 2);
-  hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, d.flags);
-  if ((rc = launch_ok(c, "gmres start"))) return rc;
-  const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
-  // scal[0] = ||rhs|| is read by k_givens
-  const int ldh = s->d.maxit + 1;
-  // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
-  // says the solve has converged, so a chunk needs no host decisions
-  const DotShape ds = dot_shape(n);
-  const size_t shm = sizeof(double) * ds.R;
-  double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
-  auto enqueue = [&](int it0, int it1) -> int {
-    int r;
-    for (int it = it0; it <= it1; it++) {
-      double *qprev = s->Qk + ldq * (it - 1);
-      double *w = s->Qk + ldq * it;
-      // the previous vector is normalised by the preconditioner's first
-      // kernel (the first one was scaled by beta0 already)
-      if ((r = precond(c, s, qprev, s->z, d.flags, it >= 2 ? 3 : -1))) return r;
-      if ((r = sys_mult(c, s, s->z, w, mixed ? 2 : 0, d.flags, true))) return r;
-      // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
-      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
-                         ds.R, pp1, d.flags);
-      if (ds.wide)
-        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
-                           d.flags);
-      hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
-                         s->h1, w, n, ds.R, pp2, d.flags, int(ds.wide));
-      if (ds.wide)
-        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
-                           d.flags);
-      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
-                         s->h2, w, n, ds.R, s->part, d.flags, int(ds.wide));
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
-                         s->h2, s->H, ldh, s->cs, s->sn, s->beta, d.flags, s->scal, tol, maxit,
-                         switch_rel);
-      if ((r = launch_ok(c, "arnoldi"))) return r;
-    }
-    return GMI_OK;
-  };
-  if (s->graph_gen != c->list_generation) {
-    drop_graphs(s);
-    s->graph_gen = c->list_generation;
-  }
-  int it = 1;
-  int done = 0;
-  while (!done) {
-    const int last = std::min(maxit, it + s->chunk - 1);
-    if (s->use_graphs) {
-      // one captured graph per (precision mode, chunk start); replayed by
-      // every later solve of this context
-      const std::pair<int, int> key{mixed ? 1 : 0, it};
-      auto g = s->graphs.find(key);
-      if (g == s->graphs.end()) {
-        hipGraph_t graph;
-        if ((rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
-                            "begin capture")))
-          return rc;
-        const int erc = enqueue(it, last);
-        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (erc) return erc;
-        if ((rc = hip_check(c, ec, "end capture"))) return rc;
-        hipGraphExec_t exec;
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if ((rc = hip_check(c, ei, "instantiate"))) return rc;
-        g = s->graphs.emplace(key, exec).first;
-      }
-      if ((rc = hip_check(c, hipGraphLaunch(g->second, c->stream), "graph launch"))) return rc;
-    } else {
-      if ((rc = enqueue(it, last))) return rc;
-    }
-    it = last + 1;
-    if ((rc = read_flags(c, s))) return rc;
-    done = s->h_flags[F_DONE];
-    if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
-  }
-  const int nits = s->h_flags[F_ITS];
-  hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->d.maxit + 1), c->stream,
-                     s->H, ldh, s->beta, d.flags, s->y);
-  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, d.flags, s->y,
-                     s->tmp, n);
-  if ((rc = launch_ok(c, "gmres solution"))) return rc;
-  if ((rc = precond(c, s, s->tmp, s->z, nullptr, -1))) return rc;
-  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
-  if ((rc = launch_ok(c, "gmres update"))) return rc;
-  *its = nits;
-  return GMI_OK;
-}
+};
 
-// Rigid-body solve.  FP64: plain GMRES.  Mixed: relaxed-precision GMRES,
-// then the true residual with the FP64 operator; if it misses the tolerance
-// GMRES restarts from the current iterate (iterative refinement, FP64 outer).
+// Rigid-body solve (mobility_solver.cpp:970-1354): the body matrices are
+// rotated once, then GMRES on the rigid-body operator.
 static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
   SimDev &d = s->d;
   hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
-  int rc0;
-  if ((rc0 = launch_ok(c, "rotate body"))) return rc0;
-  if (s->precision == GMI_FP64) return gmres(c, s, false, s->p.gmres_tol, its);
-  const int n = d.npad, nb = nblk(n, 256);
-  int rc, total = 0;
-  for (int cycle = 0; cycle < 4; cycle++) {
-    int it = 0;
-    if ((rc = gmres(c, s, true, s->p.gmres_tol, &it))) return rc;
-    total += it;
-    // true residual with the FP64 operator
-    if ((rc = sys_mult(c, s, s->x, s->tmp, 0, nullptr, false))) return rc;
-    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->z, 1.0, s->rhs, -1.0,
-                       s->tmp, n);
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5);
-    if ((rc = launch_ok(c, "refinement residual"))) return rc;
-    if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal + 4, s->scal + 4, 2 * sizeof(double),
-                                          hipMemcpyDeviceToHost, c->stream), "scal")))
-      return rc;
-    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
-    const double nb0 = s->h_scal[4], nr = s->h_scal[5];
-    if (nb0 == 0.0 || nr <= s->p.gmres_tol * nb0) break;
-  }
-  *its = total;
-  return GMI_OK;
+  int rc;
+  if ((rc = launch_ok(c, "rotate body"))) return rc;
+  SimOp op{c, s};
+  return krylov_solve(c, s, op, d.npad, std::min(d.maxit, 3 * d.Nb + 6 * d.Nsw), s->precision,
+                      s->p.gmres_tol, its);
 }
 
 // read_positions_and_forces (mobility_solver.cpp:132-192)
@@ -1833,6 +1369,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   s->precision = c->precision;
   c->extra = s;
   c->extra_release = sim_release;
+  c->extra_kind = kExtraSim;
   SimDev &d = s->d;
   d.S = S; d.Nf = Nf; d.Nsw = c->nsw; d.Ns = c->Ns; d.Nb = c->Nb;
   d.lds = c->lds; d.ldb = c->ldb;
@@ -1856,6 +1393,8 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   double *swR;
   if ((rc = dalloc(c, s, &swR, d.Nsw))) return rc; d.sw_R = swR;
   if ((rc = dalloc(c, s, &d.flags, F_N))) return rc;
+  s->flags = d.flags;
+  s->cols = d.maxit;
   if ((rc = dalloc(c, s, &d.Xm1, 3 * ls))) return rc;
   if ((rc = dalloc(c, s, &d.Xm2, 3 * ls))) return rc;
   if ((rc = dalloc(c, s, &d.Q, 4 * ls))) return rc;
@@ -2032,7 +1571,7 @@ int gmi_sim_set_graphs(gmi_ctx *c, int use_graphs, int chunk) {
   if (chunk < 1 || chunk > 64) return fail(c, GMI_ERR_ARG, "chunk must be in [1, 64]");
   int rc;
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
-  drop_graphs(s);
+  s->drop_graphs();
   s->use_graphs = use_graphs != 0;
   s->chunk = chunk;
   return GMI_OK;

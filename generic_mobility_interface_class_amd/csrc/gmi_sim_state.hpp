@@ -4,6 +4,7 @@
 #include <map>
 #include <utility>
 #include "../../include/gmi_sim.h"
+#include "gmi_krylov.hpp"
 
 namespace gmi {
 
@@ -12,9 +13,6 @@ enum {
   BX = 0, BXM1 = 3, BXM2 = 6, BU = 9, BUM1 = 12, BQ = 15, BQM1 = 19, BQI = 23,
   BF = 32, BN = 41, BK = 77, BS = 113, BREC = 152
 };
-// device flags
-enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
-       F_N = 8 };
 
 struct SimDev {
   // sizes
@@ -48,26 +46,20 @@ struct BTile {
   int sw, m0, n;
 };
 
-struct SimHost {
+// GMRES state (x, rhs, basis, flags, graphs) comes from KrylovState.
+struct SimHost : KrylovState {
   SimDev d;
   gmi_sim_params p;          // pointer members re-pointed at the copies below
   std::vector<double> blob_refs_h, fil_refs_h;
   std::vector<int> nfil;
   // device buffers owned
   std::vector<void *> owned;
-  double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr, *sn = nullptr,
-         *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr, *z = nullptr,
-         *tmp = nullptr, *scal = nullptr, *part = nullptr, *pp = nullptr;
-  size_t pp_half = 0;  // column dot partials: pp[0, pp_half) and pp[pp_half, 2 pp_half)
   double *rot = nullptr;  // rotated N_ref and (K^T K)^-1 per swimmer [Nsw][72]
   double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
          *bd = nullptr;
-  int *h_flags = nullptr;
-  double *h_scal = nullptr;
   int iter = 0;
   int precision = GMI_FP64;
   int last_gmres = 0;
-  int chunk = 6;
   double iters_avg = 0.0;  // moving average of Broyden iterations per step
   bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS
   Learn *learn = nullptr;
@@ -76,10 +68,6 @@ struct SimHost {
   int nbtiles = 0;
   int *sw_tile0 = nullptr, *sw_ntile = nullptr;
   double *tpart = nullptr, *zpart = nullptr;  // tile shares of -K^T v (preconditioner, K products)
-  // captured Arnoldi chunks, keyed by (mixed, first iteration)
-  bool use_graphs = true;
-  int graph_gen = -1;
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 };
 
 

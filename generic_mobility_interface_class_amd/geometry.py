@@ -393,3 +393,111 @@ def reference_swimmer(ctx, nfil, nseg, nblob, r_over_l=1.0, a_seg=1.0, a_blob=1.
     return SwimmerGeometry(nfil=nfil, nseg=nseg, nblob=nblob, R=r_over_l * L, a_seg=a_seg,
                            a_blob=a_blob, centre=np.zeros(3) if centre is None else centre,
                            blob_unit=blobs, fil_unit=fils)
+
+
+# ---------------------------------------------------------------------------
+# Prescribed cilia (the reference's CILIA_TYPE 3 build)
+
+BEAT_BUILD, BEAT_FULFORD_BLAKE, BEAT_CORAL_LARVAE = 0, 1, 2
+
+
+@dataclass
+class CiliaConfig:
+    """Inputs of a prescribed-cilia run (include/gmi_cilia.h).
+
+    Defaults are the reference's (config.hpp:21-47, 76, 150-154): coral-larva
+    beat, dynamic phases, free bodies, DT = 1/STEPS_PER_PERIOD with 300
+    steps per period, GMRES tolerance 1e-10 within 150 iterations; the
+    build-a-beat parameters are its SCALED_BEAT_AMPLITUDE 2.0, windows 0.25
+    and 0.25, zero-velocity avoidance 0.05.
+    """
+    nseg: int
+    nfil: np.ndarray          # per swimmer
+    nblob: np.ndarray         # per swimmer
+    blob_refs: np.ndarray     # body-frame blob positions, all swimmers [Nb, 3]
+    fil_refs: np.ndarray      # body-frame filament bases, all swimmers [Nf, 3]
+    body_x: np.ndarray        # initial centres [Nsw, 3]
+    phase: np.ndarray = None  # initial phases [Nf] (None: metachronal start)
+    omega0: np.ndarray = None # natural frequencies [Nf] (None: 2 pi)
+    r_over_l: float = 1.0
+    mu: float = 1.0
+    a_seg: float = 1.0
+    a_blob: float = 1.4
+    dt: float = 1.0 / 300.0
+    beat: int = BEAT_CORAL_LARVAE
+    amplitude: float = 2.0
+    recovery_window: float = 0.25
+    effective_stroke: float = 0.25
+    zero_velocity_avoidance: float = 0.05
+    dynamic_phase: bool = True
+    fixed_body: bool = False
+    body_velocity: np.ndarray = None   # [Nsw, 6] for fixed bodies (None: zero)
+    gen_force_refs: np.ndarray = None
+    gmres_tol: float = 1e-10
+    gmres_max_iter: int = 150
+    interact: bool = True
+
+    @property
+    def n_swimmers(self):
+        return len(self.nfil)
+
+    @property
+    def nf(self):
+        return int(np.sum(self.nfil))
+
+    @property
+    def ns(self):
+        return self.nf * self.nseg
+
+    @property
+    def nb(self):
+        return int(np.sum(self.nblob))
+
+    def system_size(self):
+        return (3 * self.ns + 3 * self.nb + (0 if self.fixed_body else 6 * self.n_swimmers)
+                + (self.nf if self.dynamic_phase else 0))
+
+    def arrays(self):
+        """Flat contiguous arrays shared by the library and the oracle."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        return dict(nfil=np.ascontiguousarray(self.nfil, dtype=np.int32),
+                    nblob=np.ascontiguousarray(self.nblob, dtype=np.int32),
+                    blob_refs=f(self.blob_refs), fil_refs=f(self.fil_refs),
+                    body_x=f(self.body_x), phase=f(self.phase), omega0=f(self.omega0),
+                    body_velocity=f(self.body_velocity), gen_force_refs=f(self.gen_force_refs))
+
+
+def cilia_config(nfil, nseg, nblob, n_swimmers=1, r_over_l=1.0, a_seg=1.0, a_blob=1.4,
+                 **kw):
+    """Spherical swimmers laid out as the reference lays out a prescribed-
+    cilia swimmer: blobs on a sphere of radius R_OVER_L DL NSEG
+    (rigid_body.cpp:184), filament bases at R_OVER_L L + 1.05 (DL/2 + RBLOB)
+    with L = DL (NSEG - 1) (swimmer.cpp:54-60,171), bodies spaced
+    3 (R + DL NSEG) apart along x (rigid_body.cpp:188-192).  Blob directions
+    are a geodesic sphere and filament directions a Fibonacci sphere (the
+    reference's repulsive seeding is gmi_seed_sphere)."""
+    DL = 2.2 * a_seg
+    Rb = r_over_l * DL * nseg
+    Rf = r_over_l * DL * (nseg - 1) + 1.05 * (0.5 * DL + a_blob)
+    blobs = geodesic_sphere(geodesic_freq_for(nblob), Rb)
+    if blobs.shape[0] != nblob:
+        raise ValueError(f"nblob must be a geodesic count 10 f^2 + 2, got {nblob}")
+    fils = fibonacci_sphere(nfil) * Rf
+    spacing = 3.0 * (Rb + DL * nseg)
+    body_x = np.array([[n * spacing, 0.0, 0.0] for n in range(n_swimmers)])
+    return CiliaConfig(nseg=nseg, nfil=np.full(n_swimmers, nfil), nblob=np.full(n_swimmers, nblob),
+                       blob_refs=np.tile(blobs, (n_swimmers, 1)),
+                       fil_refs=np.tile(fils, (n_swimmers, 1)), body_x=body_x,
+                       r_over_l=r_over_l, a_seg=a_seg, a_blob=a_blob, **kw)
+
+
+CILIA_WORKLOADS = {
+    # 32 cilia x 20 segments on a 642-blob sphere (CPU-runnable)
+    "cilia_1282": dict(nfil=32, nseg=20, nblob=642),
+    # 512 cilia x 20 segments on a 4002-blob sphere
+    "cilia_14242": dict(nfil=512, nseg=20, nblob=4002),
+}
+
+
+def cilia_workload(name, **kw):
+    return cilia_config(**CILIA_WORKLOADS[name], **kw)

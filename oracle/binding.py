@@ -359,3 +359,181 @@ def invert(A):
     if rc != 0:
         raise np.linalg.LinAlgError("singular")
     return A
+
+
+# ---- prescribed cilia (oracle_cilia.h) -------------------------------------
+
+class OrcCiliaParams(C.Structure):
+    _fields_ = [
+        ("n_swimmers", C.c_int), ("nfil", C.POINTER(C.c_int)), ("nblob", C.POINTER(C.c_int)),
+        ("nseg", C.c_int), ("interact", C.c_int), ("blob_refs", _dp), ("fil_refs", _dp),
+        ("body_x", _dp), ("phase", _dp), ("omega0", _dp), ("r_over_l", C.c_double),
+        ("mu", C.c_double), ("a_seg", C.c_double), ("a_blob", C.c_double), ("dt", C.c_double),
+        ("beat", C.c_int), ("amplitude", C.c_double), ("recovery_window", C.c_double),
+        ("effective_stroke", C.c_double), ("zero_velocity_avoidance", C.c_double),
+        ("dynamic_phase", C.c_int), ("fixed_body", C.c_int), ("body_velocity", _dp),
+        ("gen_force_refs", _dp), ("n_gen_force_refs", C.c_int), ("gmres_tol", C.c_double),
+        ("gmres_max_iter", C.c_int),
+    ]
+
+
+def _cilia_lib():
+    lib = load()
+    if getattr(lib, "_cilia_ready", False):
+        return lib
+    i, v, cp = C.c_int, C.c_void_p, C.POINTER(C.c_int)
+    sig = [
+        ("orc_cilia_create", v, [C.POINTER(OrcCiliaParams), i]),
+        ("orc_cilia_destroy", None, [v]),
+        ("orc_cilia_system_size", i, [v]),
+        ("orc_cilia_step", i, [v, i, cp]),
+        ("orc_cilia_initial_guess", None, [v, i]),
+        ("orc_cilia_compute_velocities", i, [v, i, cp]),
+        ("orc_cilia_integrate_bodies", None, [v]),
+        ("orc_cilia_end_of_step", None, [v, i]),
+        ("orc_cilia_rhs", None, [v, _dp]),
+        ("orc_cilia_apply_operator", None, [v, _dp, _dp]),
+        ("orc_cilia_apply_preconditioner", None, [v, _dp, _dp]),
+        ("orc_cilia_precon_matrices", None, [v, _dp]),
+        ("orc_cilia_get_state", None, [v, _dp, _dp, _dp, _dp, _dp, _dp]),
+        ("orc_cilia_get_solution", None, [v, _dp, _dp, _dp]),
+        ("orc_cilia_generalised_forces", None, [v, _dp]),
+        ("orc_cilia_segment_velocities", None, [v, _dp]),
+        ("orc_cilia_interp_gen_force", C.c_double, [v, C.c_double, C.c_double]),
+    ]
+    for name, res, args in sig:
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    lib._cilia_ready = True
+    return lib
+
+
+def _opt(a):
+    return _p(a) if a is not None else None
+
+
+class Cilia:
+    """Oracle prescribed-cilia stepper for a geometry.CiliaConfig."""
+
+    def __init__(self, cfg, nthreads=None):
+        self.lib = _cilia_lib()
+        a = cfg.arrays()
+        self._keep = a
+        ng = 0 if a["gen_force_refs"] is None else a["gen_force_refs"].size
+        self.p = OrcCiliaParams(
+            cfg.n_swimmers, a["nfil"].ctypes.data_as(C.POINTER(C.c_int)),
+            a["nblob"].ctypes.data_as(C.POINTER(C.c_int)), cfg.nseg, int(bool(cfg.interact)),
+            _p(a["blob_refs"]), _p(a["fil_refs"]), _p(a["body_x"]), _opt(a["phase"]),
+            _opt(a["omega0"]), cfg.r_over_l, cfg.mu, cfg.a_seg, cfg.a_blob, cfg.dt, cfg.beat,
+            cfg.amplitude, cfg.recovery_window, cfg.effective_stroke,
+            cfg.zero_velocity_avoidance, int(bool(cfg.dynamic_phase)), int(bool(cfg.fixed_body)),
+            _opt(a["body_velocity"]), _opt(a["gen_force_refs"]), ng, cfg.gmres_tol,
+            cfg.gmres_max_iter)
+        self.h = self.lib.orc_cilia_create(C.byref(self.p), nthreads or nthreads_default())
+        if not self.h:
+            raise ValueError("orc_cilia_create rejected the configuration")
+        self.cfg = cfg
+        self.n = self.lib.orc_cilia_system_size(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.orc_cilia_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nt):
+        it = C.c_int(0)
+        rc = self.lib.orc_cilia_step(self.h, nt, C.byref(it))
+        return rc, it.value
+
+    def initial_guess(self, nt=0):
+        self.lib.orc_cilia_initial_guess(self.h, nt)
+
+    def compute_velocities(self, nt=0):
+        it = C.c_int(0)
+        rc = self.lib.orc_cilia_compute_velocities(self.h, nt, C.byref(it))
+        return rc, it.value
+
+    def integrate_bodies(self):
+        self.lib.orc_cilia_integrate_bodies(self.h)
+
+    def end_of_step(self, nt=0):
+        self.lib.orc_cilia_end_of_step(self.h, nt)
+
+    def rhs(self):
+        out = np.zeros(self.n)
+        self.lib.orc_cilia_rhs(self.h, _p(out))
+        return out
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.n)
+        self.lib.orc_cilia_apply_operator(self.h, _p(x), _p(out))
+        return out
+
+    def apply_preconditioner(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.n)
+        self.lib.orc_cilia_apply_preconditioner(self.h, _p(x), _p(out))
+        return out
+
+    def precon_matrices(self):
+        out = np.zeros(36 * self.cfg.n_swimmers)
+        self.lib.orc_cilia_precon_matrices(self.h, _p(out))
+        return out.reshape(-1, 6, 6)
+
+    def state(self):
+        c = self.cfg
+        x = np.zeros(3 * c.ns); ph = np.zeros(c.nf); sp = np.zeros(c.nf)
+        body = np.zeros(7 * c.n_swimmers); k = np.zeros(3 * c.ns); fq = np.zeros(4 * c.nf)
+        self.lib.orc_cilia_get_state(self.h, _p(x), _p(ph), _p(sp), _p(body), _p(k), _p(fq))
+        return dict(x=x.reshape(-1, 3), phase=ph, speed=sp, body=body.reshape(-1, 7),
+                    vel_dir=k.reshape(-1, 3), fil_q=fq.reshape(-1, 4))
+
+    def solution(self):
+        c = self.cfg
+        fs = np.zeros(3 * c.ns); fb = np.zeros(3 * c.nb); vb = np.zeros(6 * c.n_swimmers)
+        self.lib.orc_cilia_get_solution(self.h, _p(fs), _p(fb), _p(vb))
+        return dict(f_segs=fs.reshape(-1, 3), f_blobs=fb.reshape(-1, 3), v_bodies=vb.reshape(-1, 6))
+
+    def generalised_forces(self):
+        out = np.zeros(self.cfg.nf)
+        self.lib.orc_cilia_generalised_forces(self.h, _p(out))
+        return out
+
+    def segment_velocities(self):
+        out = np.zeros(3 * self.cfg.ns)
+        self.lib.orc_cilia_segment_velocities(self.h, _p(out))
+        return out.reshape(-1, 3)
+
+    def interp_gen_force(self, phase, omega0=2.0 * np.pi):
+        return self.lib.orc_cilia_interp_gen_force(self.h, float(phase), float(omega0))
+
+
+def cilia_reference_gen_forces(cfg, steps_per_period, nthreads=None):
+    """The reference generalised forces of a beat: one filament on the
+    configuration's body, phase 0, phase speed 2 pi, no phase dynamics, for
+    one period (config.hpp:259-264, cilia_sim_main.cpp:184-246): vel_dir . f
+    after each step's solve."""
+    import dataclasses
+    one = dataclasses.replace(
+        cfg, nfil=np.array([1]), nblob=np.array([cfg.nblob[0]]),
+        blob_refs=np.asarray(cfg.blob_refs).reshape(-1, 3)[:cfg.nblob[0]],
+        fil_refs=np.asarray(cfg.fil_refs).reshape(-1, 3)[:1], body_x=np.zeros((1, 3)),
+        phase=np.zeros(1), omega0=np.array([2.0 * np.pi]), dt=1.0 / steps_per_period,
+        dynamic_phase=False, gen_force_refs=None, body_velocity=None)
+    o = Cilia(one, nthreads)
+    out = np.zeros(steps_per_period)
+    for nt in range(steps_per_period):
+        o.initial_guess(nt)
+        rc, _ = o.compute_velocities(nt)
+        assert rc == 0
+        out[nt] = o.generalised_forces()[0]
+        o.integrate_bodies()
+        o.end_of_step(nt)
+    return out
