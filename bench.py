@@ -20,6 +20,7 @@ host cores on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -116,24 +117,40 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
 def bench_step(args):
     """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer.
 
-    Four identical runs take one driving period (300 steps) of plain FP64
-    steps; the two learned ones record every 5th step's (features,
-    converged - guess) rows and train the same network.  Then each run
-    steps over the same `step_steps` steps in its own mode:
-      fp64            plain FP64 (the reference method)
-      learned_mixed   network warm start + mixed precision
-      learned_fp64    network warm start, FP64
-      mixed           mixed precision, extrapolation warm start
+    Five identical runs take one driving period (300 steps) of FP64 steps;
+    the two learned ones record every 5th step's (features, converged -
+    guess) rows and train the same network.  Then each run steps over the
+    same `step_steps` steps in its own mode:
+      fp64            plain FP64 (the reference method: extrapolated Broyden
+                      iterate, previous solution as the GMRES guess)
+      learned_mixed   network Broyden warm start + GMRES guess fitted on the
+                      last 16 right-hand sides + mixed precision
+      learned_fp64    network + fitted GMRES guess, FP64
+      fitted_fp64     fitted GMRES guess only, FP64
+      mixed           mixed precision only
+    The fitted guess is on from the first step in the runs that use it.
     """
     cfg = geometry.sim_workload(args.step_workload, seed=1)
-    modes = (("fp64", capi.GMI_FP64, False), ("learned_mixed", capi.GMI_FP32_KAHAN, True),
-             ("learned_fp64", capi.GMI_FP64, True), ("mixed", capi.GMI_FP32_KAHAN, False))
+    modes = (("fp64", capi.GMI_FP64, False, False, False),
+             ("learned_mixed", capi.GMI_FP32_KAHAN, True, True, False),
+             ("learned_fp64", capi.GMI_FP64, True, True, False),
+             ("fitted_fp64", capi.GMI_FP64, False, True, False),
+             ("mixed", capi.GMI_FP32_KAHAN, False, False, False),
+             ("inverse_fp64", capi.GMI_FP64, False, False, True),
+             ("learned_inverse_mixed", capi.GMI_FP32_KAHAN, True, False, True))
     stride = 5
     sims = {}
-    for name, _, learned in modes:
+    setup = {}
+    for name, _, learned, fitted, inverse in modes:
         sim = capi.Sim(cfg, precision=capi.GMI_FP64)
         sim.set_sim_precision(capi.GMI_FP64)
         sim.set_graphs(not args.no_graphs)
+        if fitted:
+            sim.set_gmres_history(16)
+        if inverse:
+            t0 = time.perf_counter()
+            sim.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+            setup[name] = time.perf_counter() - t0
         if learned:
             sim.warmstart_init(hidden=128, seed=7)
             sim.warmstart_configure(apply=False, collect=(args.step_warmup // stride) * sim.Ns,
@@ -151,7 +168,7 @@ def bench_step(args):
     print(f"# {args.step_warmup} warm-up steps x {len(sims)} runs in {warm_s:.1f} s, "
           f"{rows} training rows", file=sys.stderr, flush=True)
     train = {}
-    for name, prec, learned in modes:
+    for name, prec, learned, _, _ in modes:
         sim = sims[name]
         if learned:
             t0 = time.perf_counter()
@@ -162,7 +179,7 @@ def bench_step(args):
         sim.set_sim_precision(prec)
     res = {}
     nt0 = args.step_warmup
-    for name, prec, learned in modes:
+    for name, prec, learned, _, _ in modes:
         sim = sims[name]
         a0, f0 = sim.warmstart_stats() if learned else (0, 0)
         r = _timed_steps(sim, nt0, args.step_steps, f"step {name}")
@@ -171,7 +188,10 @@ def bench_step(args):
             r["learned_steps"], r["fallbacks"] = a1 - a0, f1 - f0
         res[name] = r
     ref = sims["fp64"].state()["x"]
-    for name in ("learned_mixed", "learned_fp64", "mixed"):
+    for name in setup:
+        res[name]["preconditioner_setup_s"] = setup[name]
+    for name in ("learned_mixed", "learned_fp64", "fitted_fp64", "mixed", "inverse_fp64",
+                 "learned_inverse_mixed"):
         res[name]["max_position_diff_vs_fp64"] = float(np.abs(sims[name].state()["x"] - ref).max())
     for sim in sims.values():
         sim.close()
@@ -183,6 +203,10 @@ def bench_step(args):
             "speedup_learned_mixed_vs_fp64": res["learned_mixed"]["steps_per_s"] / fp64,
             "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / fp64,
             "speedup_learned_fp64_vs_fp64": res["learned_fp64"]["steps_per_s"] / fp64,
+            "speedup_fitted_fp64_vs_fp64": res["fitted_fp64"]["steps_per_s"] / fp64,
+            "speedup_inverse_fp64_vs_fp64": res["inverse_fp64"]["steps_per_s"] / fp64,
+            "speedup_learned_inverse_mixed_vs_fp64":
+                res["learned_inverse_mixed"]["steps_per_s"] / fp64,
             "warm_start": train}
 
 
@@ -279,6 +303,95 @@ def bench_rollout(args):
                            "final_loss_standardised": loss}}
 
 
+def _cilia_table(cfg, steps, use_graphs=True):
+    """Reference generalised forces of one filament of `cfg`'s body, computed
+    by the device as the reference computes them (fixed phase speed 2 pi from
+    phase 0 over one period, vel_dir . f after each solve)."""
+    one = dataclasses.replace(cfg, nfil=np.array([1]),
+                              fil_refs=np.asarray(cfg.fil_refs).reshape(-1, 3)[:1],
+                              phase=np.zeros(1), dynamic_phase=False, dt=1.0 / steps)
+    g = capi.Cilia(one)
+    g.set_graphs(use_graphs)
+    out = np.zeros(steps)
+    for nt in range(steps):
+        g.initial_guess(nt)
+        assert g.compute_velocities(nt)[0] == 0
+        out[nt] = g.generalised_forces()[0]
+        g.integrate_bodies()
+        g.end_of_step(nt)
+    g.close()
+    return out
+
+
+def _cilia_window(g, nt0, k, name):
+    g.call("gmi_wait_for_device")
+    g.call("gmi_timer_start")
+    its = 0
+    for nt in range(nt0, nt0 + k):
+        rc, it = g.step(nt)
+        assert rc == 0, g.last_error()
+        its += it
+    ms = capi.C.c_double(0.0)
+    g.call("gmi_timer_stop", capi.C.byref(ms))
+    r = dict(steps=f"{nt0}-{nt0 + k - 1}", steps_per_s=k / (ms.value * 1e-3),
+             ms_per_step=ms.value / k, gmres_iters_per_step=its / k)
+    print(f"# {name}: {r['ms_per_step']:.2f} ms/step, {its / k:.1f} GMRES it/step",
+          file=sys.stderr, flush=True)
+    return r
+
+
+def bench_cilia(args):
+    """Prescribed-cilia steps (coral-larva beat, dynamic phases, free body,
+    the reference's defaults): the 32-cilium swimmer (1,282 particles) and
+    the 512-cilium swimmer (14,242 particles).  Per workload: the reference
+    generalised-force table (one period, 300 steps) from the device, 3
+    untimed steps, then FP64 and mixed-precision windows from fresh starts."""
+    out = {}
+    names = [args.cilia_workload] if args.cilia_workload else ["cilia_1282", "cilia_14242"]
+    for name in names:
+        k = args.cilia_steps
+        base = geometry.cilia_workload(name)
+        t0 = time.perf_counter()
+        table = _cilia_table(base, args.cilia_table_steps, not args.no_graphs)
+        table_s = time.perf_counter() - t0
+        cfg = dataclasses.replace(base, gen_force_refs=table)
+        r = {"segments": cfg.ns, "blobs": cfg.nb, "filaments": cfg.nf,
+             "system_size": cfg.system_size(), "table_s": table_s}
+        for prec_name, prec in (("fp64", capi.GMI_FP64), ("mixed", capi.GMI_FP32_KAHAN)):
+            g = capi.Cilia(cfg, precision=prec)
+            g.set_graphs(not args.no_graphs)
+            for nt in range(3):
+                assert g.step(nt)[0] == 0
+            r[prec_name] = _cilia_window(g, 3, k, f"{name} {prec_name}")
+            g.close()
+        r["speedup_mixed_vs_fp64"] = r["mixed"]["steps_per_s"] / r["fp64"]["steps_per_s"]
+        out[name] = r
+    if not args.no_cpu and "cilia_1282" in out:
+        out["cilia_1282"]["cpu_baseline"] = cpu_cilia_baseline()
+    return out
+
+
+def cpu_cilia_baseline():
+    """Oracle (CPU FP64) prescribed-cilia step of the 1,282-particle swimmer
+    (bounded sample: one step after two, table from the oracle itself)."""
+    from oracle import binding as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    base = geometry.cilia_workload("cilia_1282")
+    one = dataclasses.replace(base, nfil=np.array([1]),
+                              fil_refs=np.asarray(base.fil_refs).reshape(-1, 3)[:1],
+                              phase=np.zeros(1), dynamic_phase=False)
+    table = orc.cilia_reference_gen_forces(one, 60, nthreads=threads)
+    o = orc.Cilia(dataclasses.replace(base, gen_force_refs=table), nthreads=threads)
+    for nt in range(2):
+        o.step(nt)
+    t0 = time.perf_counter()
+    rc, it = o.step(2)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"step 2 of cilia_1282 ({it} GMRES iterations, {dt:.2f} s; pair sums on "
+                      f"{threads} threads, the rest single-threaded; 60-entry table)"}
+
+
 def cpu_step_baseline(args):
     """Oracle (CPU FP64) time step on the same swimmer: one step after the
     body reference matrices (bounded sample)."""
@@ -311,6 +424,14 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
     ap.add_argument("--no-rollout", action="store_true")
+    ap.add_argument("--cilia-steps", type=int, default=20)
+    ap.add_argument("--cilia-workload", default=None,
+                    help="one of geometry.CILIA_WORKLOADS (default: both)")
+    ap.add_argument("--cilia-table-steps", type=int, default=300,
+                    help="entries of the reference generalised-force table (steps per period)")
+    ap.add_argument("--no-cilia", action="store_true")
+    ap.add_argument("--cilia-only", action="store_true",
+                    help="only the prescribed-cilia steps (diagnostic line, e.g. for profiling)")
     ap.add_argument("--apply-only", action="store_true",
                     help="only the 46,722-particle apply (no CPU baseline, no time steps)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -324,8 +445,11 @@ def main():
               "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
         sys.exit(2)
 
+    if args.cilia_only:
+        print(json.dumps(finite_json({"cilia": bench_cilia(args)}), allow_nan=False), flush=True)
+        return
     if args.apply_only:
-        args.no_cpu = args.no_step = args.no_ensemble = args.no_rollout = True
+        args.no_cpu = args.no_step = args.no_ensemble = args.no_rollout = args.no_cilia = True
     if args.copy_calibration:
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -418,6 +542,8 @@ def main():
         line["ensemble"] = bench_ensemble(args)
     if not args.no_rollout:
         line["rollout"] = bench_rollout(args)
+    if not args.no_cilia:
+        line["cilia"] = bench_cilia(args)
     print(json.dumps(finite_json(line), allow_nan=False), flush=True)
 
 

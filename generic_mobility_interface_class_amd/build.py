@@ -23,7 +23,10 @@ HIP_SOURCES = [
     "gmi_io.hip",
     "gmi_seed.hip",
     "gmi_cilia.hip",
+    "gmi_dense.hip",
 ]
+# rocBLAS supplies the DGEMMs of the one-time body-frame inversion (gmi_dense.hip)
+LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would
 # otherwise form v_pk_* from float code.
@@ -45,7 +48,7 @@ def hipcc() -> str:
 def build_library(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs, *LINK_FLAGS]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -23,6 +23,7 @@ GMI_ERR_SINGULAR = -5
 GMI_FP64 = 0
 GMI_BLOCK_SS, GMI_BLOCK_SB, GMI_BLOCK_BS, GMI_BLOCK_BB = 1, 2, 4, 8
 GMI_SEED_UNIFORM, GMI_SEED_EQUATORIAL, GMI_SEED_PLATY = 0, 1, 2
+GMI_PRECOND_SELF_MOBILITY, GMI_PRECOND_BODY_INVERSE = 0, 1
 WARMSTART_NIN = 51   # GMI_WARMSTART_NIN
 GMI_FP32_KAHAN = 1
 
@@ -93,6 +94,8 @@ SIGNATURES = [
     ("gmi_get_broyden", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("gmi_sim_set_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gmi_sim_set_gmres_history", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_sim_set_body_preconditioner", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_warmstart_init", C.c_int, [C.c_void_p, C.c_int, C.c_ulonglong]),
     ("gmi_warmstart_configure", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("gmi_warmstart_dataset_size", C.c_int, [C.c_void_p, _ip]),
@@ -338,6 +341,16 @@ class Sim(Context):
 
     def set_graphs(self, use_graphs=True, chunk=6):
         self.call("gmi_sim_set_graphs", int(bool(use_graphs)), int(chunk))
+
+    def set_body_preconditioner(self, kind=GMI_PRECOND_BODY_INVERSE):
+        """Rigid-body GMRES preconditioner: the reference's self-mobility
+        diagonal, or the exact body-frame block inverse (kept in HBM)."""
+        self.call("gmi_sim_set_body_preconditioner", int(kind))
+
+    def set_gmres_history(self, m=16):
+        """GMRES initial guess fitted on the last m right-hand sides (0: the
+        reference's previous solution)."""
+        self.call("gmi_sim_set_gmres_history", int(m))
 
     # ---- output and checkpoint files (reference formats) --------------
     def write_par(self, name, total_time_steps, plot_frequency):

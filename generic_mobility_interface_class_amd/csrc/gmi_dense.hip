@@ -1,0 +1,479 @@
+// gmi_dense.hip -- exact body-frame preconditioner of the rigid-body solve.
+//
+// In the body frame the rigid-body system of one swimmer,
+//   [ M  -K ] [f]   [r_b]     M: blob-blob mobility (3 nb x 3 nb, SPD)
+//   [-K^T 0 ] [U] = [r_c]     K: U = (V, W) -> V + W x ref_j,
+// has a fixed matrix: blob positions relative to the body and K depend only
+// on the body-frame blob references.  So M^-1 is computed once, kept in HBM
+// (3 nb x 3 nb doubles per swimmer: 1.15 GB at 4,002 blobs) with
+// G = M^-1 K and S^-1 = (K^T M^-1 K)^-1, and the block inverse
+//   w = M^-1 r_b,  U = -S^-1 (r_c + K^T w),  f = w + G U
+// serves as the GMRES preconditioner in place of the reference's
+// self-mobility diagonal (mobility_solver.cpp:651-727).  GMRES keeps its
+// operator and tolerance; the preconditioner only changes how fast it gets
+// there.  The reference has no counterpart.
+//
+// Set-up: M is assembled with the RPY blob-blob forms (cuda_functions.cu:
+// 478-517) at the body-frame references, then inverted in place by block
+// Gauss-Jordan without pivoting (M is symmetric positive definite): per
+// 128-wide block column, the diagonal block is inverted in LDS by one
+// workgroup and the rank-128 updates of the whole matrix are rocBLAS DGEMMs.
+// Application: one GEMV per swimmer (M^-1 is symmetric, so row i is the
+// contiguous column i of the column-major store) between rotations into and
+// out of the body frame.
+#include "gmi_internal.hpp"
+#include "gmi_ctx.hpp"
+#include "gmi_lie.hpp"
+#include "gmi_sim_state.hpp"
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace gmi {
+
+struct DensePrec {
+  rocblas_handle blas = nullptr;
+  std::vector<size_t> moff_h;   // per swimmer: first element of its matrix
+  std::vector<int> voff_h;      // per swimmer: first row of its body-frame vector
+  int nrows = 0, maxn = 0;      // total rows (sum 3 nb), largest 3 nb
+  double *Minv = nullptr;       // per swimmer column-major 3nb x 3nb
+  double *G = nullptr;          // per swimmer column-major 3nb x 6
+  double *Sinv = nullptr;       // [Nsw][36] row-major
+  double *t = nullptr, *w = nullptr;  // body-frame blob part of the input, M^-1 t
+  double *tc = nullptr;         // body-frame U rows of the input [Nsw][6]
+  double *part = nullptr;       // per blob tile: shares of K^T w
+  size_t *moff = nullptr;
+  int *voff = nullptr;
+  std::vector<void *> owned;
+};
+
+namespace {
+
+constexpr double kPiD = 3.14159265358979323846264338327950288;
+constexpr int kDiag = 128;  // Gauss-Jordan block width
+
+// M at the body-frame references of one swimmer, column-major (element
+// (row, col) at row + col * n), rows/cols ordered (component, blob):
+// row = c * nb + i.  Thread = blob i, blockIdx.y = blob j.
+__global__ __launch_bounds__(64) void k_dense_assemble(const double *__restrict__ bref, int ldb,
+                                                       int b0, int nb, double a, double mu,
+                                                       double *__restrict__ M) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y;
+  if (i >= nb) return;
+  const size_t n = size_t(3) * nb;
+  const double dx = bref[b0 + i] - bref[b0 + j];
+  const double dy = bref[ldb + b0 + i] - bref[ldb + b0 + j];
+  const double dz = bref[2 * ldb + b0 + i] - bref[2 * ldb + b0 + j];
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double c8 = 1.0 / (8.0 * kPiD * mu);
+  double A, B;
+  if (r2 > 4.0 * a * a) {
+    const double ri = 1.0 / sqrt(r2), ri2 = ri * ri;
+    A = ri * (c8 + c8 * (2.0 * a * a / 3.0) * ri2);
+    B = ri2 * ri * (c8 - c8 * 2.0 * a * a * ri2);
+  } else {
+    const double r = sqrt(r2), c6a = 1.0 / (6.0 * kPiD * mu * a);
+    const double b = 3.0 * r / (32.0 * a);
+    A = (1.0 - 3.0 * b) * c6a;
+    B = r > 0.0 ? b * c6a / r2 : 0.0;
+  }
+  const double d[3] = {dx, dy, dz};
+  for (int c = 0; c < 3; c++)
+    for (int e = 0; e < 3; e++)
+      M[size_t(c) * nb + i + (size_t(e) * nb + j) * n] = (c == e ? A : 0.0) + B * d[c] * d[e];
+}
+
+// In-LDS Gauss-Jordan inverse (no pivoting: the diagonal blocks of the
+// Schur complements of an SPD matrix are SPD) of the kb x kb diagonal block
+// of A (column-major, ld n) at (k0, k0) into D (column-major, ld kb).
+__global__ __launch_bounds__(256) void k_dense_diag_inv(const double *__restrict__ A, size_t n,
+                                                        int k0, int kb, double *__restrict__ D,
+                                                        int *flags) {
+  extern __shared__ double L[];  // kb x kb column-major
+  __shared__ double colk[kDiag];
+  const int t = threadIdx.x;
+  for (int e = t; e < kb * kb; e += 256) {
+    const int r = e % kb, c = e / kb;
+    L[e] = A[size_t(k0 + r) + size_t(k0 + c) * n];
+  }
+  __syncthreads();
+  for (int k = 0; k < kb; k++) {
+    const double p = L[k + k * kb];
+    if (!(p > 0.0)) {
+      if (t == 0) flags[F_SING] = 1;
+      return;
+    }
+    const double inv = 1.0 / p;
+    // column k of the other rows before the update
+    for (int r = t; r < kb; r += 256) colk[r] = L[r + k * kb];
+    __syncthreads();
+    // row k scaled: L(k, j) = L(k, j) / p, L(k, k) = 1 / p
+    for (int j = t; j < kb; j += 256) L[k + j * kb] = (j == k ? 1.0 : L[k + j * kb]) * inv;
+    __syncthreads();
+    // other rows: L(i, j) -= colk(i) L(k, j); column k: L(i, k) = -colk(i) / p
+    for (int e = t; e < kb * kb; e += 256) {
+      const int r = e % kb, c = e / kb;
+      if (r == k) continue;
+      const double base = (c == k) ? 0.0 : L[e];
+      L[e] = base - colk[r] * L[k + c * kb];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < kb * kb; e += 256) D[e] = L[e];
+}
+
+// Gauss-Jordan fix-ups of block column k0: A[K, :] <- T (kb x n, ld kb) and
+// A[K, K] <- D.
+__global__ void k_dense_fix_rows(double *__restrict__ A, size_t n, int k0, int kb,
+                                 const double *__restrict__ T, const double *__restrict__ D) {
+  const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= size_t(kb) * n) return;
+  const int r = int(e % kb);
+  const size_t c = e / kb;
+  const bool inK = c >= size_t(k0) && c < size_t(k0 + kb);
+  A[k0 + r + c * n] = inK ? D[r + (c - k0) * kb] : T[e];
+}
+
+// K of one swimmer at its body-frame references, column-major 3nb x 6:
+// column e < 3 is e_e on every blob, column 3 + e is e_e x ref_j.
+__global__ void k_dense_k(const double *__restrict__ bref, int ldb, int b0, int nb,
+                          double *__restrict__ K) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const size_t n = size_t(3) * nb;
+  const double r[3] = {bref[b0 + j], bref[ldb + b0 + j], bref[2 * ldb + b0 + j]};
+  for (int e = 0; e < 3; e++)
+    for (int c = 0; c < 3; c++) {
+      K[size_t(c) * nb + j + size_t(e) * n] = (c == e) ? 1.0 : 0.0;
+      // (e_e x r)_c
+      const double ex[3] = {e == 0 ? 1.0 : 0.0, e == 1 ? 1.0 : 0.0, e == 2 ? 1.0 : 0.0};
+      const double cr[3] = {ex[1] * r[2] - ex[2] * r[1], ex[2] * r[0] - ex[0] * r[2],
+                            ex[0] * r[1] - ex[1] * r[0]};
+      K[size_t(c) * nb + j + size_t(3 + e) * n] = cr[c];
+    }
+}
+
+// S^-1 from S (column-major 6x6, symmetric) by Gauss-Jordan with pivoting.
+__global__ void k_dense_sinv(const double *__restrict__ S, double *__restrict__ Sinv, int *flags) {
+  double A[36];
+  int piv[6];
+  for (int k = 0; k < 36; k++) A[k] = S[(k % 6) * 6 + k / 6];  // to row-major
+  for (int i = 0; i < 6; i++) piv[i] = i;
+  for (int k = 0; k < 6; k++) {
+    int p = k;
+    double best = fabs(A[6 * k + k]);
+    for (int i = k + 1; i < 6; i++)
+      if (fabs(A[6 * i + k]) > best) { best = fabs(A[6 * i + k]); p = i; }
+    if (best == 0.0) { flags[F_SING] = 1; return; }
+    if (p != k) {
+      for (int j = 0; j < 6; j++) { const double x = A[6 * k + j]; A[6 * k + j] = A[6 * p + j]; A[6 * p + j] = x; }
+      const int x = piv[k]; piv[k] = piv[p]; piv[p] = x;
+    }
+    const double inv = 1.0 / A[6 * k + k];
+    A[6 * k + k] = 1.0;
+    for (int j = 0; j < 6; j++) A[6 * k + j] *= inv;
+    for (int i = 0; i < 6; i++) {
+      if (i == k) continue;
+      const double g = A[6 * i + k];
+      A[6 * i + k] = 0.0;
+      for (int j = 0; j < 6; j++) A[6 * i + j] -= g * A[6 * k + j];
+    }
+  }
+  for (int j = 0; j < 6; j++)
+    for (int i = 0; i < 6; i++) Sinv[6 * i + piv[j]] = A[6 * i + j];
+}
+
+// ---- application ----------------------------------------------------------
+
+// Per blob tile: normalise `in` (norm_slot >= 0) and rotate into the body
+// frame: blob rows to t (per swimmer, (component, blob) order), U rows to tc.
+__global__ __launch_bounds__(256) void k_dense_in(SimDev a, const BTile *tiles, const int *sw_tile0,
+                                                  const int *voff, double *in, const double *scal,
+                                                  int norm_slot, double *t, double *tc,
+                                                  const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const BTile T = tiles[blockIdx.x];
+  const int n = T.sw, th = threadIdx.x;
+  const m3 Qm = rotm(ga4(a.body + size_t(n) * BREC + BQ));
+  const double nrm = norm_slot >= 0 ? scal[norm_slot] : 1.0;
+  const int b0 = a.sw_blob0[n], nb = a.sw_nblob[n];
+  if (th < T.n) {
+    const int m = T.m0 + th;
+    d3 v = ld3(in, a.ldb, m);
+    if (norm_slot >= 0) {
+      v = {v.x / nrm, v.y / nrm, v.z / nrm};
+      st3(in, a.ldb, m, v);
+    }
+    const d3 u = mtv(Qm, v);
+    double *tt = t + voff[n] + (m - b0);
+    tt[0] = u.x; tt[nb] = u.y; tt[2 * nb] = u.z;
+  }
+  if (blockIdx.x == sw_tile0[n] && th < 2) {
+    double *p = in + 3 * a.ldb + 6 * n + 3 * th;
+    d3 v = ga3(p);
+    if (norm_slot >= 0) {
+      v = {v.x / nrm, v.y / nrm, v.z / nrm};
+      sa3(p, v);
+    }
+    sa3(tc + 6 * n + 3 * th, mtv(Qm, v));
+  }
+}
+
+// w = M^-1 t per swimmer (blockIdx.y): one wave per row, reading the
+// contiguous column of the symmetric inverse.
+__global__ __launch_bounds__(256) void k_dense_gemv(const double *__restrict__ Minv,
+                                                    const size_t *__restrict__ moff,
+                                                    const int *__restrict__ voff,
+                                                    const int *__restrict__ sw_nblob,
+                                                    const double *__restrict__ t,
+                                                    double *__restrict__ w, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  const int n = blockIdx.y;
+  const int rows = 3 * sw_nblob[n];
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const double *col = Minv + moff[n] + size_t(row) * rows;
+  const double *tv = t + voff[n];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int c = lane;
+  for (; c + 192 < rows; c += 256) {
+    s0 += col[c] * tv[c];
+    s1 += col[c + 64] * tv[c + 64];
+    s2 += col[c + 128] * tv[c + 128];
+    s3 += col[c + 192] * tv[c + 192];
+  }
+  for (; c < rows; c += 64) s0 += col[c] * tv[c];
+  double s = (s0 + s1) + (s2 + s3);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) w[voff[n] + row] = s;
+}
+
+// Per blob tile: shares of K^T w = (sum w_j, sum ref_j x w_j).
+__global__ __launch_bounds__(256) void k_dense_kt(SimDev a, const BTile *tiles, const int *voff,
+                                                  const double *w, double *part,
+                                                  const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sh[4 * 6];
+  const BTile T = tiles[blockIdx.x];
+  const int n = T.sw, th = threadIdx.x;
+  const int b0 = a.sw_blob0[n], nb = a.sw_nblob[n];
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  if (th < T.n) {
+    const int m = T.m0 + th;
+    const double *wv = w + voff[n] + (m - b0);
+    const d3 v = {wv[0], wv[nb], wv[2 * nb]};
+    const d3 cr = cross3(ld3(a.blob_ref, a.ldb, m), v);
+    c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = cr.x; c[4] = cr.y; c[5] = cr.z;
+  }
+  const int lane = th & 63, wv = th >> 6;
+  for (int k = 0; k < 6; k++) {
+    const double r = wave_sum(c[k]);
+    if (lane == 0) sh[4 * k + wv] = r;
+  }
+  __syncthreads();
+  if (th < 6) part[6 * blockIdx.x + th] = ((sh[4 * th] + sh[4 * th + 1]) + sh[4 * th + 2]) + sh[4 * th + 3];
+}
+
+// Per blob tile: U = -S^-1 (r_c + K^T w) from the swimmer's shares (tile
+// order), f = w + G U, and out = R f (blob rows), R U (U rows, first tile).
+__global__ __launch_bounds__(256) void k_dense_out(SimDev a, const BTile *tiles, const int *sw_tile0,
+                                                   const int *sw_ntile, const int *voff,
+                                                   const double *w, const double *G,
+                                                   const double *Sinv, const double *tc,
+                                                   const double *part, double *out,
+                                                   const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double sU[6];
+  const BTile T = tiles[blockIdx.x];
+  const int n = T.sw, th = threadIdx.x;
+  if (th < 6) {
+    double s = 0.0;
+    for (int q = sw_tile0[n]; q < sw_tile0[n] + sw_ntile[n]; q++) s += part[6 * q + th];
+    sU[th] = tc[6 * n + th] + s;
+  }
+  __syncthreads();
+  double U[6];
+  for (int i = 0; i < 6; i++) {
+    double s = 0.0;
+    for (int k = 0; k < 6; k++) s += Sinv[36 * n + 6 * i + k] * sU[k];
+    U[i] = -s;
+  }
+  const m3 Qm = rotm(ga4(a.body + size_t(n) * BREC + BQ));
+  const int b0 = a.sw_blob0[n], nb = a.sw_nblob[n];
+  const size_t rows = size_t(3) * nb;
+  if (th < T.n) {
+    const int m = T.m0 + th;
+    const int jl = m - b0;
+    const double *g = G + size_t(6) * voff[n];
+    double f[3];
+    for (int c = 0; c < 3; c++) {
+      const size_t row = size_t(c) * nb + jl;
+      double s = w[voff[n] + row];
+      for (int e = 0; e < 6; e++) s += g[row + e * rows] * U[e];
+      f[c] = s;
+    }
+    st3(out, a.ldb, m, mv(Qm, d3{f[0], f[1], f[2]}));
+  }
+  if (blockIdx.x == sw_tile0[n] && th < 2) {
+    const d3 u = {U[3 * th], U[3 * th + 1], U[3 * th + 2]};
+    sa3(out + 3 * a.ldb + 6 * n + 3 * th, mv(Qm, u));
+  }
+}
+
+inline int nb_of(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+void dense_release(SimHost *s) {
+  DensePrec *D = s->dense;
+  if (!D) return;
+  for (void *p : D->owned) (void)hipFree(p);
+  if (D->blas) (void)rocblas_destroy_handle(D->blas);
+  delete D;
+  s->dense = nullptr;
+}
+
+static int dmalloc(gmi_ctx *c, DensePrec *D, void **p, size_t bytes) {
+  int rc = hip_check(c, hipMalloc(p, std::max<size_t>(bytes, 8)), "hipMalloc dense preconditioner");
+  if (!rc) D->owned.push_back(*p);
+  return rc;
+}
+
+static int blas_check(gmi_ctx *c, rocblas_status st, const char *what) {
+  if (st == rocblas_status_success) return GMI_OK;
+  return fail(c, GMI_ERR_HIP, std::string(what) + ": rocBLAS status " + std::to_string(int(st)));
+}
+
+int dense_setup(gmi_ctx *c, SimHost *s) {
+  const SimDev &d = s->d;
+  DensePrec *D = new DensePrec();
+  s->dense = D;
+  int rc;
+  auto bail = [&](int code) {
+    dense_release(s);
+    return code;
+  };
+  if ((rc = blas_check(c, rocblas_create_handle(&D->blas), "rocblas_create_handle")) ||
+      (rc = blas_check(c, rocblas_set_stream(D->blas, c->stream), "rocblas_set_stream")))
+    return bail(rc);
+  D->moff_h.resize(d.Nsw);
+  D->voff_h.resize(d.Nsw);
+  size_t mtot = 0;
+  int vtot = 0, maxn = 0;
+  for (int n = 0; n < d.Nsw; n++) {
+    const int rows = 3 * c->nblob[n];
+    D->moff_h[n] = mtot;
+    D->voff_h[n] = vtot;
+    mtot += size_t(rows) * rows;
+    vtot += rows;
+    maxn = std::max(maxn, rows);
+  }
+  D->nrows = vtot;
+  D->maxn = maxn;
+  if ((rc = dmalloc(c, D, (void **)&D->Minv, sizeof(double) * mtot)) ||
+      (rc = dmalloc(c, D, (void **)&D->G, sizeof(double) * 6 * size_t(vtot))) ||
+      (rc = dmalloc(c, D, (void **)&D->Sinv, sizeof(double) * 36 * d.Nsw)) ||
+      (rc = dmalloc(c, D, (void **)&D->t, sizeof(double) * vtot)) ||
+      (rc = dmalloc(c, D, (void **)&D->w, sizeof(double) * vtot)) ||
+      (rc = dmalloc(c, D, (void **)&D->tc, sizeof(double) * 6 * d.Nsw)) ||
+      (rc = dmalloc(c, D, (void **)&D->part, sizeof(double) * 6 * std::max(1, s->nbtiles))) ||
+      (rc = dmalloc(c, D, (void **)&D->moff, sizeof(size_t) * d.Nsw)) ||
+      (rc = dmalloc(c, D, (void **)&D->voff, sizeof(int) * d.Nsw)))
+    return bail(rc);
+  // Gauss-Jordan workspaces of the largest swimmer
+  double *P = nullptr, *T = nullptr, *Dd = nullptr, *K = nullptr, *S = nullptr;
+  if ((rc = dmalloc(c, D, (void **)&P, sizeof(double) * size_t(maxn) * kDiag)) ||
+      (rc = dmalloc(c, D, (void **)&T, sizeof(double) * size_t(maxn) * kDiag)) ||
+      (rc = dmalloc(c, D, (void **)&Dd, sizeof(double) * kDiag * kDiag)) ||
+      (rc = dmalloc(c, D, (void **)&K, sizeof(double) * size_t(maxn) * 6)) ||
+      (rc = dmalloc(c, D, (void **)&S, sizeof(double) * 36)))
+    return bail(rc);
+  if ((rc = hip_check(c, hipMemcpyAsync(D->moff, D->moff_h.data(), sizeof(size_t) * d.Nsw,
+                                        hipMemcpyHostToDevice, c->stream), "upload")) ||
+      (rc = hip_check(c, hipMemcpyAsync(D->voff, D->voff_h.data(), sizeof(int) * d.Nsw,
+                                        hipMemcpyHostToDevice, c->stream), "upload")))
+    return bail(rc);
+  if ((rc = hip_check(c, hipFuncSetAttribute(reinterpret_cast<const void *>(k_dense_diag_inv),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(sizeof(double) * kDiag * kDiag)),
+                      "diagonal-block LDS")))
+    return bail(rc);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
+  const double one = 1.0, zero = 0.0, minus = -1.0;
+  for (int n = 0; n < d.Nsw; n++) {
+    const int nb = c->nblob[n], rows = 3 * nb;
+    const size_t ld = size_t(rows);
+    double *A = D->Minv + D->moff_h[n];
+    hipLaunchKernelGGL(k_dense_assemble, dim3(nb_of(nb, 64), nb), dim3(64), 0, c->stream,
+                       d.blob_ref, d.ldb, c->blob_off[n], nb, d.ab, d.mu, A);
+    if ((rc = launch_ok(c, "assemble"))) return bail(rc);
+    for (int k0 = 0; k0 < rows; k0 += kDiag) {
+      const int kb = std::min(kDiag, rows - k0);
+      hipLaunchKernelGGL(k_dense_diag_inv, dim3(1), dim3(256), sizeof(double) * kb * kb,
+                         c->stream, A, ld, k0, kb, Dd, d.flags);
+      // T = D A[K, :]  (kb x rows)
+      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
+                                            rocblas_operation_none, kb, rows, kb, &one, Dd, kb,
+                                            A + k0, rows, &zero, T, kb), "dgemm T")))
+        return bail(rc);
+      // P = A[:, K]
+      if ((rc = hip_check(c, hipMemcpyAsync(P, A + size_t(k0) * ld, sizeof(double) * ld * kb,
+                                            hipMemcpyDeviceToDevice, c->stream), "copy P")))
+        return bail(rc);
+      // A -= P T
+      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
+                                            rocblas_operation_none, rows, rows, kb, &minus, P,
+                                            rows, T, kb, &one, A, rows), "dgemm update")))
+        return bail(rc);
+      // A[:, K] = -P D, then A[K, :] = T and A[K, K] = D
+      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
+                                            rocblas_operation_none, rows, kb, kb, &minus, P,
+                                            rows, Dd, kb, &zero, A + size_t(k0) * ld, rows),
+                           "dgemm column")))
+        return bail(rc);
+      hipLaunchKernelGGL(k_dense_fix_rows, dim3(nb_of(kb * rows, 256)), dim3(256), 0, c->stream,
+                         A, ld, k0, kb, T, Dd);
+      if ((rc = launch_ok(c, "gauss-jordan"))) return bail(rc);
+    }
+    // G = M^-1 K, S = K^T G
+    hipLaunchKernelGGL(k_dense_k, dim3(nb_of(nb, 256)), dim3(256), 0, c->stream, d.blob_ref,
+                       d.ldb, c->blob_off[n], nb, K);
+    double *Gn = D->G + size_t(6) * D->voff_h[n];
+    if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none, rocblas_operation_none,
+                                          rows, 6, rows, &one, A, rows, K, rows, &zero, Gn, rows),
+                         "dgemm G")) ||
+        (rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_transpose,
+                                          rocblas_operation_none, 6, 6, rows, &one, K, rows, Gn,
+                                          rows, &zero, S, 6), "dgemm S")))
+      return bail(rc);
+    hipLaunchKernelGGL(k_dense_sinv, dim3(1), dim3(1), 0, c->stream, S, D->Sinv + 36 * n, d.flags);
+    if ((rc = launch_ok(c, "schur"))) return bail(rc);
+  }
+  int sing = 0;
+  if ((rc = hip_check(c, hipMemcpyAsync(&sing, d.flags + F_SING, sizeof(int), hipMemcpyDeviceToHost,
+                                        c->stream), "flags")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
+    return bail(rc);
+  if (sing) return bail(fail(c, GMI_ERR_SINGULAR, "blob mobility not positive definite"));
+  return GMI_OK;
+}
+
+int dense_apply(gmi_ctx *c, SimHost *s, double *in, double *out, const int *skip, int norm_slot) {
+  const SimDev &d = s->d;
+  DensePrec *D = s->dense;
+  hipLaunchKernelGGL(k_dense_in, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, D->voff, in, s->scal, norm_slot, D->t, D->tc, skip);
+  hipLaunchKernelGGL(k_dense_gemv, dim3(nb_of(D->maxn, 4), d.Nsw), dim3(256), 0, c->stream,
+                     D->Minv, D->moff, D->voff, d.sw_nblob, D->t, D->w, skip);
+  hipLaunchKernelGGL(k_dense_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, D->voff,
+                     D->w, D->part, skip);
+  hipLaunchKernelGGL(k_dense_out, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->sw_ntile, D->voff, D->w, D->G, D->Sinv, D->tc, D->part, out,
+                     skip);
+  return hip_check(c, hipGetLastError(), "dense preconditioner");
+}
+
+}  // namespace gmi

@@ -88,16 +88,34 @@ constexpr int kMaxCols = 1024;     // Krylov / Broyden columns held in LDS
 
 // partials of rows [e0, e0+R) (values in sw) with columns j < k; wave w
 // takes the columns j = w (mod 4)
+// Columns go in batches of kDotBatch per wave so that each lane keeps that
+// many independent loads in flight (a column's sum order is unchanged: lane
+// partials over its rows, then the wave sum).
+constexpr int kDotBatch = 8;
 __device__ void col_dots(const double *Qb, size_t ldq, int k, const double *sw, int e0, int n,
                          int R, double *part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rmax = min(R, n - e0);
-  for (int j = wave; j < k; j += 4) {
-    const double *q = Qb + ldq * j + e0;
-    double acc = 0.0;
-    for (int r = lane; r < rmax; r += 64) acc += q[r] * sw[r];
-    acc = wave_sum(acc);
-    if (lane == 0) part[size_t(j) * gridDim.x + blockIdx.x] = acc;
+  for (int j0 = wave; j0 < k; j0 += 4 * kDotBatch) {
+    double acc[kDotBatch];
+    const double *q[kDotBatch];
+#pragma unroll
+    for (int c = 0; c < kDotBatch; c++) {
+      acc[c] = 0.0;
+      // past the last column a batch re-reads column k-1 and stores nothing
+      q[c] = Qb + ldq * min(j0 + 4 * c, k - 1) + e0;
+    }
+    for (int r = lane; r < rmax; r += 64) {
+      const double s = sw[r];
+#pragma unroll
+      for (int c = 0; c < kDotBatch; c++) acc[c] += q[c][r] * s;
+    }
+#pragma unroll
+    for (int c = 0; c < kDotBatch; c++) {
+      const double a = wave_sum(acc[c]);
+      const int j = j0 + 4 * c;
+      if (lane == 0 && j < k) part[size_t(j) * gridDim.x + blockIdx.x] = a;
+    }
   }
 }
 
@@ -171,6 +189,7 @@ __global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq,
     double v = 0.0;
     if (e < n) {
       v = w[e];
+#pragma unroll 8
       for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
       w[e] = v;
     }
@@ -200,6 +219,7 @@ __global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq,
     const int e = e0 + r;
     if (e < n) {
       double v = w[e];
+#pragma unroll 8
       for (int j = 0; j < k; j++) v -= hs[j] * Qb[ldq * j + e];
       w[e] = v;
       s2 += v * v;

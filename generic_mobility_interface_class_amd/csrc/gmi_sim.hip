@@ -31,6 +31,7 @@ namespace gmi {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int kHistMax = 16;  // right-hand sides kept for the fitted GMRES guess
 
 __device__ __forceinline__ d3 rowvec(const double *p) { return {p[0], p[1], p[2]}; }
 
@@ -469,6 +470,171 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
 
 // -------------------------------------------------------------------------
 // vector kernels (GMRES / Broyden)
+
+// ---- history-fitted GMRES initial guess ------------------------------------
+// In the body frame the rigid-body system matrix is fixed: blob positions
+// and K depend only on the body-frame references.  A new right-hand side,
+// rotated into the body frame, is fitted by least squares onto the last m
+// right-hand sides of the swimmer, and the same combination of their
+// solutions, rotated back, is the GMRES initial guess.  Vectors keep the
+// system's layout (blob rows SoA [3][ldb], then 6 body rows per swimmer);
+// slot i of the history is Hb + i*ldh (right-hand side) and Hx + i*ldh
+// (solution).
+
+// Per tile: the body-frame right-hand side into bnew, and the tile's shares
+// of bnew . Hb[i] (i < count) and of bnew . bnew (slot kHistMax).
+__global__ __launch_bounds__(256) void k_hist_dots(SimDev a, const BTile *tiles,
+                                                   const int *sw_tile0, const double *rhs,
+                                                   double *bnew, const double *Hb, size_t ldh,
+                                                   int count, double *dpart) {
+  __shared__ double sh[4 * (kHistMax + 1)];
+  const BTile T = tiles[blockIdx.x];
+  const int t = threadIdx.x;
+  const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
+  double acc[kHistMax + 1];
+#pragma unroll
+  for (int i = 0; i <= kHistMax; i++) acc[i] = 0.0;
+  auto add = [&](const d3 v, int o, int ld) {  // o: row offset, ld: component stride
+#pragma unroll
+    for (int i = 0; i < kHistMax; i++)
+      if (i < count) {
+        const double *h = Hb + ldh * i + o;
+        acc[i] += v.x * h[0] + v.y * h[ld] + v.z * h[2 * ld];
+      }
+    acc[kHistMax] += dot3(v, v);
+  };
+  if (t < T.n) {
+    const int m = T.m0 + t;
+    const d3 v = mtv(Qm, ld3(rhs, a.ldb, m));
+    st3(bnew, a.ldb, m, v);
+    add(v, m, a.ldb);
+  }
+  if (blockIdx.x == sw_tile0[T.sw] && t < 2) {
+    const int o = 3 * a.ldb + 6 * T.sw + 3 * t;
+    const d3 v = mtv(Qm, ga3(rhs + o));
+    sa3(bnew + o, v);
+    add(v, o, 1);
+  }
+  const int lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int i = 0; i <= kHistMax; i++) {
+    const double r = wave_sum(acc[i]);
+    if (lane == 0) sh[4 * i + w] = r;
+  }
+  __syncthreads();
+  if (t <= kHistMax)
+    dpart[(kHistMax + 1) * size_t(blockIdx.x) + t] =
+        ((sh[4 * t] + sh[4 * t + 1]) + sh[4 * t + 2]) + sh[4 * t + 3];
+}
+
+// Per swimmer (one thread): g = sum of the tile shares (tile order), then
+// (G + lambda I) c = g by Cholesky, lambda = 1e-14 trace(G) / count.
+// A failed factorisation leaves c = 0 (no guess).  g and bnew . bnew are
+// kept for the Gram update after the solve.
+__global__ void k_hist_solve(SimDev a, const int *sw_tile0, const int *sw_ntile,
+                             const double *dpart, const double *G, double *Gw, int count,
+                             double *coef, double *gsave) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.Nsw) return;
+  constexpr int M = kHistMax;
+  double g[M + 1];
+  for (int i = 0; i <= M; i++) {
+    double s = 0.0;
+    for (int q = sw_tile0[n]; q < sw_tile0[n] + sw_ntile[n]; q++) s += dpart[(M + 1) * size_t(q) + i];
+    g[i] = s;
+    gsave[(M + 1) * size_t(n) + i] = s;
+  }
+  const double *Gn = G + size_t(n) * M * M;
+  double *L = Gw + size_t(n) * M * M;
+  double tr = 0.0;
+  for (int i = 0; i < count; i++) tr += Gn[M * i + i];
+  const double lam = 1e-14 * tr / count;
+  bool ok = tr > 0.0;
+  for (int j = 0; j < count && ok; j++) {
+    double d = Gn[M * j + j] + lam;
+    for (int k = 0; k < j; k++) d -= L[M * j + k] * L[M * j + k];
+    if (!(d > 0.0)) { ok = false; break; }
+    const double ljj = sqrt(d);
+    L[M * j + j] = ljj;
+    for (int i = j + 1; i < count; i++) {
+      double e = Gn[M * i + j];
+      for (int k = 0; k < j; k++) e -= L[M * i + k] * L[M * j + k];
+      L[M * i + j] = e / ljj;
+    }
+  }
+  double c[M];
+  for (int i = 0; i < M; i++) c[i] = 0.0;
+  if (ok) {
+    for (int i = 0; i < count; i++) {  // L y = g
+      double e = g[i];
+      for (int k = 0; k < i; k++) e -= L[M * i + k] * c[k];
+      c[i] = e / L[M * i + i];
+    }
+    for (int i = count - 1; i >= 0; i--) {  // L^T c = y
+      double e = c[i];
+      for (int k = i + 1; k < count; k++) e -= L[M * k + i] * c[k];
+      c[i] = e / L[M * i + i];
+    }
+  }
+  for (int i = 0; i < M; i++) coef[M * size_t(n) + i] = ok ? c[i] : 0.0;
+}
+
+// Per tile: x = R sum_i c_i Hx[i] (the guess, rotated back to the lab frame).
+__global__ __launch_bounds__(256) void k_hist_combine(SimDev a, const BTile *tiles,
+                                                      const int *sw_tile0, const double *Hx,
+                                                      size_t ldh, int count, const double *coef,
+                                                      double *x) {
+  const BTile T = tiles[blockIdx.x];
+  const int t = threadIdx.x;
+  const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
+  const double *c = coef + size_t(kHistMax) * T.sw;
+  auto comb = [&](int o, int ld) -> d3 {
+    d3 v = {0.0, 0.0, 0.0};
+    for (int i = 0; i < count; i++) {
+      const double *h = Hx + ldh * i + o;
+      v = v + c[i] * d3{h[0], h[ld], h[2 * ld]};
+    }
+    return mv(Qm, v);
+  };
+  if (t < T.n) st3(x, a.ldb, T.m0 + t, comb(T.m0 + t, a.ldb));
+  if (blockIdx.x == sw_tile0[T.sw] && t < 2) {
+    const int o = 3 * a.ldb + 6 * T.sw + 3 * t;
+    sa3(x + o, comb(o, 1));
+  }
+}
+
+// After the solve: slot `head` <- (bnew, R^T x), and the swimmer's Gram
+// matrix row and column of that slot from the saved dots.
+__global__ __launch_bounds__(256) void k_hist_push(SimDev a, const BTile *tiles,
+                                                   const int *sw_tile0, const double *bnew,
+                                                   const double *x, double *Hb, double *Hx,
+                                                   size_t ldh, int head, int count_before,
+                                                   double *G, const double *gsave) {
+  const BTile T = tiles[blockIdx.x];
+  const int t = threadIdx.x;
+  const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
+  double *hb = Hb + ldh * head, *hx = Hx + ldh * head;
+  if (t < T.n) {
+    const int m = T.m0 + t;
+    st3(hb, a.ldb, m, ld3(bnew, a.ldb, m));
+    st3(hx, a.ldb, m, mtv(Qm, ld3(x, a.ldb, m)));
+  }
+  if (blockIdx.x == sw_tile0[T.sw]) {
+    if (t < 2) {
+      const int o = 3 * a.ldb + 6 * T.sw + 3 * t;
+      sa3(hb + o, ga3(bnew + o));
+      sa3(hx + o, mtv(Qm, ga3(x + o)));
+    }
+    if (t == 0) {
+      constexpr int M = kHistMax;
+      double *Gn = G + size_t(T.sw) * M * M;
+      const double *g = gsave + (M + 1) * size_t(T.sw);
+      for (int i = 0; i < count_before; i++)
+        if (i != head) Gn[M * head + i] = Gn[M * i + head] = g[i];
+      Gn[M * head + head] = g[M];
+    }
+  }
+}
 
 // copy U of every swimmer into column `col` of N_ref
 __global__ void k_store_nref_col(SimDev a, const double *x, int col) {
@@ -1169,6 +1335,7 @@ static void sim_release(gmi_ctx *c) {
   if (!s) return;
   s->drop_graphs();
   learn_release(s);
+  dense_release(s);
   for (void *p : s->owned) (void)hipFree(p);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
   if (s->h_scal) (void)hipHostFree(s->h_scal);
@@ -1253,9 +1420,11 @@ struct SimOp {
   gmi_ctx *c;
   SimHost *s;
   int mult(double *z, double *out, int mode, const int *done, bool zpart_ready) {
-    return sys_mult(c, s, z, out, mode, done, zpart_ready);
+    // the body-frame preconditioner leaves no K^T shares behind
+    return sys_mult(c, s, z, out, mode, done, zpart_ready && !s->dense);
   }
   int precond(double *in, double *out, const int *skip, int norm_slot) {
+    if (s->dense) return dense_apply(c, s, in, out, skip, norm_slot);
     return gmi::precond(c, s, in, out, skip, norm_slot);
   }
 };
@@ -1267,9 +1436,39 @@ static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
   hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
   int rc;
   if ((rc = launch_ok(c, "rotate body"))) return rc;
+  const size_t ldh = size_t(d.npad);
+  if (s->hist_m > 0) {
+    // the fitted initial guess replaces the previous solution once the
+    // history holds a right-hand side
+    hipLaunchKernelGGL(k_hist_dots, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                       s->sw_tile0, s->rhs, s->hist_bnew, s->hist_b, ldh, s->hist_count,
+                       s->hist_part);
+    if (s->hist_count > 0) {
+      hipLaunchKernelGGL(k_hist_solve, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
+                         s->sw_tile0, s->sw_ntile, s->hist_part, s->hist_G, s->hist_Gw,
+                         s->hist_count, s->hist_coef, s->hist_gsave);
+      hipLaunchKernelGGL(k_hist_combine, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                         s->sw_tile0, s->hist_x, ldh, s->hist_count, s->hist_coef, s->x);
+    } else {
+      hipLaunchKernelGGL(k_hist_solve, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
+                         s->sw_tile0, s->sw_ntile, s->hist_part, s->hist_G, s->hist_Gw, 0,
+                         s->hist_coef, s->hist_gsave);
+    }
+    if ((rc = launch_ok(c, "fitted guess"))) return rc;
+  }
   SimOp op{c, s};
-  return krylov_solve(c, s, op, d.npad, std::min(d.maxit, 3 * d.Nb + 6 * d.Nsw), s->precision,
-                      s->p.gmres_tol, its);
+  if ((rc = krylov_solve(c, s, op, d.npad, std::min(d.maxit, 3 * d.Nb + 6 * d.Nsw), s->precision,
+                         s->p.gmres_tol, its)))
+    return rc;
+  if (s->hist_m > 0) {
+    hipLaunchKernelGGL(k_hist_push, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                       s->sw_tile0, s->hist_bnew, s->x, s->hist_b, s->hist_x, ldh, s->hist_head,
+                       s->hist_count, s->hist_G, s->hist_gsave);
+    if ((rc = launch_ok(c, "history push"))) return rc;
+    s->hist_count = std::min(s->hist_count + 1, s->hist_m);
+    s->hist_head = (s->hist_head + 1) % s->hist_m;
+  }
+  return GMI_OK;
 }
 
 // read_positions_and_forces (mobility_solver.cpp:132-192)
@@ -1574,6 +1773,42 @@ int gmi_sim_set_graphs(gmi_ctx *c, int use_graphs, int chunk) {
   s->drop_graphs();
   s->use_graphs = use_graphs != 0;
   s->chunk = chunk;
+  return GMI_OK;
+}
+
+int gmi_sim_set_body_preconditioner(gmi_ctx *c, int kind) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (kind != GMI_PRECOND_SELF_MOBILITY && kind != GMI_PRECOND_BODY_INVERSE)
+    return fail(c, GMI_ERR_ARG, "unknown preconditioner");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  s->drop_graphs();  // captured chunks hold the old preconditioner's kernels
+  dense_release(s);
+  if (kind == GMI_PRECOND_BODY_INVERSE) return dense_setup(c, s);
+  return GMI_OK;
+}
+
+int gmi_sim_set_gmres_history(gmi_ctx *c, int m) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (m < 0 || m > kHistMax) return fail(c, GMI_ERR_ARG, "history length must be in [0, 16]");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  if (m > 0 && !s->hist_b) {
+    const size_t n = size_t(s->d.npad), M = kHistMax, Nsw = size_t(s->d.Nsw);
+    if ((rc = dalloc(c, s, &s->hist_b, M * n)) || (rc = dalloc(c, s, &s->hist_x, M * n)) ||
+        (rc = dalloc(c, s, &s->hist_bnew, n)) || (rc = dalloc(c, s, &s->hist_G, Nsw * M * M)) ||
+        (rc = dalloc(c, s, &s->hist_Gw, Nsw * M * M)) ||
+        (rc = dalloc(c, s, &s->hist_part, size_t(s->nbtiles) * (M + 1))) ||
+        (rc = dalloc(c, s, &s->hist_coef, Nsw * M)) ||
+        (rc = dalloc(c, s, &s->hist_gsave, Nsw * (M + 1))))
+      return rc;
+  }
+  // a new length starts an empty history
+  s->hist_m = m;
+  s->hist_count = 0;
+  s->hist_head = 0;
   return GMI_OK;
 }
 

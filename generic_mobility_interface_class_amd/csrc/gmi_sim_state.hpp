@@ -39,7 +39,8 @@ struct SimDev {
 };
 
 
-struct Learn;  // learned warm start (gmi_learn.hip)
+struct Learn;      // learned warm start (gmi_learn.hip)
+struct DensePrec;  // exact body-frame preconditioner (gmi_dense.hip)
 
 // up to 256 consecutive blobs of one swimmer
 struct BTile {
@@ -61,8 +62,13 @@ struct SimHost : KrylovState {
   int precision = GMI_FP64;
   int last_gmres = 0;
   double iters_avg = 0.0;  // moving average of Broyden iterations per step
+  // history-fitted GMRES initial guess (0: the previous solution)
+  int hist_m = 0, hist_count = 0, hist_head = 0;
+  double *hist_b = nullptr, *hist_x = nullptr, *hist_bnew = nullptr, *hist_G = nullptr,
+         *hist_Gw = nullptr, *hist_part = nullptr, *hist_coef = nullptr, *hist_gsave = nullptr;
   bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS
   Learn *learn = nullptr;
+  DensePrec *dense = nullptr;  // null: the reference's self-mobility preconditioner
   // blob tiles for the preconditioner and K products
   BTile *btiles = nullptr;
   int nbtiles = 0;
@@ -76,6 +82,10 @@ void learn_release(SimHost *s);
 int learn_after_initial_guess(gmi_ctx *c, SimHost *s, int nt, bool *applied);
 int learn_after_convergence(gmi_ctx *c, SimHost *s, int nt);
 double *learn_correction(SimHost *s);
+// exact body-frame preconditioner (gmi_dense.hip)
+int dense_setup(gmi_ctx *c, SimHost *s);
+int dense_apply(gmi_ctx *c, SimHost *s, double *in, double *out, const int *skip, int norm_slot);
+void dense_release(SimHost *s);
 int learn_back_to_guess(gmi_ctx *c, SimHost *s, double **delta);
 
 }  // namespace gmi

@@ -100,6 +100,27 @@ int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
  * convergence checks; with use_graphs each chunk is captured once as a HIP
  * graph and replayed (no reference counterpart).  Default: graphs on, 6. */
 int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
+/* GMRES initial guess of the rigid-body solve.  m = 0 (default): the
+ * previous solution, as the reference (mobility_solver.cpp:1060-1156).
+ * m in [1, 16]: in the body frame, where the system matrix is fixed, the new
+ * right-hand side is fitted by least squares onto the swimmer's last m
+ * right-hand sides, and the same combination of their solutions is the
+ * guess.  GMRES still iterates to the same tolerance.  No reference
+ * counterpart: a warm start of this build. */
+int gmi_sim_set_gmres_history(gmi_ctx *ctx, int m);
+
+/* Preconditioner of the rigid-body GMRES. */
+#define GMI_PRECOND_SELF_MOBILITY 0 /* the reference's: M ~ I/(6 pi mu a)
+                                       (mobility_solver.cpp:651-727)         */
+#define GMI_PRECOND_BODY_INVERSE 1  /* the exact block inverse in the body
+                                       frame, where the matrix is fixed: M^-1
+                                       (3 nb x 3 nb doubles per swimmer, kept
+                                       in HBM), M^-1 K and (K^T M^-1 K)^-1,
+                                       computed once by block Gauss-Jordan */
+/* Selects the preconditioner (set-up cost and memory for BODY_INVERSE:
+ * 2 (3 nb)^3 flops and 8 (3 nb)^2 bytes per swimmer).  GMRES keeps its
+ * operator and tolerance.  No reference counterpart beyond the default. */
+int gmi_sim_set_body_preconditioner(gmi_ctx *ctx, int kind);
 
 /* ---- seeding points on the unit sphere (seeding.cu:13-389) ------------- */
 #define GMI_SEED_UNIFORM 0    /* UNIFORM_SEEDING: spiral start, then the

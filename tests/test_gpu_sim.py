@@ -87,6 +87,65 @@ def test_trajectory_ten_steps(gpu):
         assert dx <= DL * cfg.tol and du <= cfg.tol
 
 
+def test_fitted_gmres_guess_keeps_the_trajectory(gpu):
+    """The history-fitted GMRES initial guess changes where GMRES starts,
+    not what it converges to: the trajectory follows the oracle within the
+    Broyden tolerance, with far fewer GMRES iterations than the
+    previous-solution warm start."""
+    cfg, o, g = pair()
+    plain = capi.Sim(cfg)
+    for s in (o, g, plain):
+        s.make_body_reference_matrices()
+    g.set_gmres_history(16)
+    DL = cfg.swimmers[0].DL
+    its_fit = its_plain = 0
+    for nt in range(12):
+        ro, rg, rp = o.step(nt), g.step(nt), plain.step(nt)
+        so, sg = o.state(), g.state()
+        dx = np.abs(sg["x"] - so["x"]).max()
+        du = np.abs(sg["u"] - so["u"]).max()
+        print(f"step {nt}: oracle {ro} fitted {rg} plain {rp} |dx| {dx:.2e} |du| {du:.2e}",
+              flush=True)
+        assert ro[0] == 0 and rg[0] == 0 and rp[0] == 0
+        assert abs(ro[1] - rg[1]) <= 1
+        assert dx <= DL * cfg.tol and du <= cfg.tol
+        if nt >= 4:
+            its_fit += rg[2]
+            its_plain += rp[2]
+    print(f"GMRES iterations over steps 4-11: fitted {its_fit}, plain {its_plain}", flush=True)
+    assert its_fit < 0.7 * its_plain
+
+
+@pytest.mark.parametrize("name", ["step_962", "step_9122"])
+def test_body_inverse_preconditioner_keeps_the_trajectory(gpu, name):
+    """The exact body-frame preconditioner changes how fast GMRES converges,
+    not what it converges to: trajectories follow the plain run within the
+    Broyden tolerance (the oracle on the small swimmer), and GMRES takes a
+    few iterations per solve."""
+    cfg = geometry.sim_workload(name, seed=1)
+    plain, inv = capi.Sim(cfg), capi.Sim(cfg)
+    for s in (plain, inv):
+        s.make_body_reference_matrices()
+    inv.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+    o = None
+    if name == "step_962":
+        o = orc.Sim(cfg)
+        o.make_body_reference_matrices()
+    DL = cfg.swimmers[0].DL
+    for nt in range(6):
+        rp, ri = plain.step(nt), inv.step(nt)
+        dx = np.abs(inv.state()["x"] - plain.state()["x"]).max()
+        print(f"{name} step {nt}: plain {rp} body-inverse {ri} |dx| {dx:.2e}", flush=True)
+        assert rp[0] == 0 and ri[0] == 0
+        assert abs(rp[1] - ri[1]) <= 1
+        assert dx <= DL * cfg.tol
+        assert ri[2] <= 4 * (ri[1] + 1)  # a few GMRES iterations per solve
+        if o is not None:
+            ro = o.step(nt)
+            assert ro[0] == 0
+            assert np.abs(inv.state()["x"] - o.state()["x"]).max() <= DL * cfg.tol
+
+
 def test_mixed_precision_trajectory(gpu):
     cfg = geometry.sim_workload("step_962", seed=1)
     g64 = capi.Sim(cfg)
