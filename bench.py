@@ -138,6 +138,9 @@ def bench_step(args):
              ("mixed", capi.GMI_FP32_KAHAN, False, False, False),
              ("inverse_fp64", capi.GMI_FP64, False, False, True),
              ("learned_inverse_mixed", capi.GMI_FP32_KAHAN, True, False, True))
+    if args.step_modes:
+        keep = args.step_modes.split(",")
+        modes = tuple(m for m in modes if m[0] in keep)
     stride = 5
     sims = {}
     setup = {}
@@ -164,7 +167,8 @@ def bench_step(args):
             rc, bi, gi = sim.step(nt)
             assert rc == 0, (name, nt, rc, sim.last_error())
     warm_s = time.perf_counter() - t0
-    rows = sims["learned_mixed"].warmstart_dataset_size()
+    rows = next((s.warmstart_dataset_size() for (n, _, lr, _, _), s in
+                 zip(modes, sims.values()) if lr), 0)
     print(f"# {args.step_warmup} warm-up steps x {len(sims)} runs in {warm_s:.1f} s, "
           f"{rows} training rows", file=sys.stderr, flush=True)
     train = {}
@@ -187,27 +191,24 @@ def bench_step(args):
             a1, f1 = sim.warmstart_stats()
             r["learned_steps"], r["fallbacks"] = a1 - a0, f1 - f0
         res[name] = r
-    ref = sims["fp64"].state()["x"]
     for name in setup:
         res[name]["preconditioner_setup_s"] = setup[name]
-    for name in ("learned_mixed", "learned_fp64", "fitted_fp64", "mixed", "inverse_fp64",
-                 "learned_inverse_mixed"):
-        res[name]["max_position_diff_vs_fp64"] = float(np.abs(sims[name].state()["x"] - ref).max())
+    if "fp64" in sims:
+        ref = sims["fp64"].state()["x"]
+        for name in sims:
+            if name != "fp64":
+                res[name]["max_position_diff_vs_fp64"] = float(
+                    np.abs(sims[name].state()["x"] - ref).max())
     for sim in sims.values():
         sim.close()
-    fp64 = res["fp64"]["steps_per_s"]
-    return {"workload": args.step_workload, "warmup_steps": args.step_warmup,
-            "steps_per_window": args.step_steps, "window": f"{nt0}-{nt0 + args.step_steps - 1}",
-            "blob_blob_launch_shape": list(shape),
-            **res,
-            "speedup_learned_mixed_vs_fp64": res["learned_mixed"]["steps_per_s"] / fp64,
-            "speedup_mixed_vs_fp64": res["mixed"]["steps_per_s"] / fp64,
-            "speedup_learned_fp64_vs_fp64": res["learned_fp64"]["steps_per_s"] / fp64,
-            "speedup_fitted_fp64_vs_fp64": res["fitted_fp64"]["steps_per_s"] / fp64,
-            "speedup_inverse_fp64_vs_fp64": res["inverse_fp64"]["steps_per_s"] / fp64,
-            "speedup_learned_inverse_mixed_vs_fp64":
-                res["learned_inverse_mixed"]["steps_per_s"] / fp64,
-            "warm_start": train}
+    out = {"workload": args.step_workload, "warmup_steps": args.step_warmup,
+           "steps_per_window": args.step_steps, "window": f"{nt0}-{nt0 + args.step_steps - 1}",
+           "blob_blob_launch_shape": list(shape), **res, "warm_start": train}
+    if "fp64" in res:
+        for name in res:
+            if name != "fp64":
+                out[f"speedup_{name}_vs_fp64"] = res[name]["steps_per_s"] / res["fp64"]["steps_per_s"]
+    return out
 
 
 def _timed_steps(sim, nt0, k, name):
@@ -242,21 +243,26 @@ def bench_ensemble(args):
     """64 independent swimmers of uneven size (17..474 filaments x 20
     segments, 642..7292 blobs; 306,518 particles) stepped as one batch:
     every pair launch covers all swimmers, Arnoldi chunks replay as
-    captured graphs.  Two FP64 steps untimed, then FP64 and mixed-precision
-    windows from the same state (two contexts)."""
+    captured graphs.  Two steps untimed, then FP64, mixed-precision and
+    mixed-precision with the exact body-frame preconditioner (36.8 GB of
+    per-swimmer inverses in HBM) windows from the same state (three
+    contexts)."""
     cfg = geometry.ensemble_workload("ensemble_64", seed=4)
     out = {"workload": "ensemble_64", "swimmers": len(cfg.swimmers),
            "filaments": int(sum(g.nfil for g in cfg.swimmers)),
            "segments": int(sum(g.ns for g in cfg.swimmers)),
            "blobs": int(sum(g.nblob for g in cfg.swimmers))}
-    for name, prec, k in (("fp64", capi.GMI_FP64, args.ens_steps // 2),
-                          ("mixed", capi.GMI_FP32_KAHAN, args.ens_steps)):
+    for name, prec, k, inverse in (("fp64", capi.GMI_FP64, args.ens_steps // 2, False),
+                                   ("mixed", capi.GMI_FP32_KAHAN, args.ens_steps, False),
+                                   ("inverse_mixed", capi.GMI_FP32_KAHAN, args.ens_steps, True)):
         sim = capi.Sim(cfg, precision=prec)
         sim.set_sim_precision(prec)
         sim.set_graphs(not args.no_graphs)
         t0 = time.perf_counter()
         sim.make_body_reference_matrices()
         sim.autotune_blocks(capi.GMI_BLOCK_BB, reps=5)
+        if inverse:
+            sim.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
         setup_s = time.perf_counter() - t0
         for nt in range(2):
             assert sim.step(nt)[0] == 0
@@ -264,6 +270,8 @@ def bench_ensemble(args):
         out[name]["setup_s"] = setup_s
         sim.close()
     out["speedup_mixed_vs_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["speedup_inverse_mixed_vs_fp64"] = (out["inverse_mixed"]["steps_per_s"] /
+                                            out["fp64"]["steps_per_s"])
     return out
 
 
@@ -272,32 +280,48 @@ def bench_rollout(args):
     642-blob swimmer.  Two identical runs take one driving period (300
     steps) of plain FP64 steps; one records every 5th step (18,880 rows of
     (features, converged - guess)) and trains the network; then both run
-    `rollout` more steps, one plain, one with the network."""
+    `rollout` more steps, one plain, one with the network.  The same pair
+    then repeats with the exact body-frame preconditioner in both runs, so
+    the network's gain is measured with and without GMRES dominating."""
     cfg = geometry.sim_workload("step_962", seed=1)
     plain, learned = capi.Sim(cfg), capi.Sim(cfg)
-    for s in (plain, learned):
+    plain_inv, learned_inv = capi.Sim(cfg), capi.Sim(cfg)
+    for s in (plain, learned, plain_inv, learned_inv):
         s.set_graphs(not args.no_graphs)
-    learned.warmstart_init(hidden=128, seed=7)
-    learned.warmstart_configure(apply=False, collect=20000, stride=5)
-    for s in (plain, learned):
+    for s in (learned, learned_inv):
+        s.warmstart_init(hidden=128, seed=7)
+    for s in (learned, learned_inv):
+        s.warmstart_configure(apply=False, collect=20000, stride=5)
+    for s in (plain, learned, plain_inv, learned_inv):
         s.make_body_reference_matrices()
         s.autotune_blocks(capi.GMI_BLOCK_BB, reps=10)
-        for nt in range(300):
+    for s in (plain_inv, learned_inv):
+        s.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+    for nt in range(300):
+        for s in (plain, learned, plain_inv, learned_inv):
             assert s.step(nt)[0] == 0
     rows = learned.warmstart_dataset_size()
     t0 = time.perf_counter()
     loss = learned.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
     train_s = time.perf_counter() - t0
     learned.warmstart_configure(apply=True, collect=0)
+    # the preconditioned learned run trains on its own rows, same schedule
+    learned_inv.warmstart_train(iters=3000, batch=1024, lr=1e-3, seed=1)
+    learned_inv.warmstart_configure(apply=True, collect=0)
     k = args.rollout
     rp = _timed_steps(plain, 300, k, "rollout plain")
     rl = _timed_steps(learned, 300, k, "rollout learned")
+    rpi = _timed_steps(plain_inv, 300, k, "rollout plain + body inverse")
+    rli = _timed_steps(learned_inv, 300, k, "rollout learned + body inverse")
     dx = float(np.abs(plain.state()["x"] - learned.state()["x"]).max())
-    plain.close()
-    learned.close()
+    for s in (plain, learned, plain_inv, learned_inv):
+        s.close()
     return {"workload": "step_962", "warmup_steps": 300, "rollout_steps": k,
             "plain": rp, "learned": rl,
             "speedup_learned_vs_plain": rl["steps_per_s"] / rp["steps_per_s"],
+            "plain_inverse": rpi, "learned_inverse": rli,
+            "speedup_learned_inverse_vs_plain_inverse": rli["steps_per_s"] / rpi["steps_per_s"],
+            "speedup_learned_inverse_vs_plain": rli["steps_per_s"] / rp["steps_per_s"],
             "final_max_position_diff": dx,
             "warm_start": {"training_rows": rows, "train_iters": 3000, "train_s": train_s,
                            "final_loss_standardised": loss}}
@@ -420,6 +444,8 @@ def main():
     ap.add_argument("--step-steps", type=int, default=40)
     ap.add_argument("--step-warmup", type=int, default=300)
     ap.add_argument("--no-step", action="store_true")
+    ap.add_argument("--step-modes", default=None,
+                    help="comma list of step-leg modes (default: all seven)")
     ap.add_argument("--ens-steps", type=int, default=40)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
@@ -534,8 +560,24 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xs, xb, fs, fb, args.cpu_seconds)
-    if not args.no_step:
-        line["sim_steps"] = bench_step(args)
+    if not args.no_step and args.step_modes:
+        line["sim_steps"] = bench_step(args)  # a subset of modes (diagnostics)
+    elif not args.no_step:
+        st = line["sim_steps"] = bench_step(args)
+        # the second half of the metric: learned + mixed vs plain FP64 steps/s
+        fp = st["fp64"]["steps_per_s"]
+        line["steps_learned_mixed_vs_fp64"] = {
+            "workload": st["workload"],
+            "fp64_steps_per_s": fp,
+            "learned_mixed_steps_per_s": st["learned_inverse_mixed"]["steps_per_s"],
+            "speedup": st["learned_inverse_mixed"]["steps_per_s"] / fp,
+            "path": "network Broyden warm start + FP32-Kahan pairs in relaxed GMRES with FP64 "
+                    "refinement + exact body-frame GMRES preconditioner",
+            "attribution_speedups_vs_fp64": {
+                k: st[k]["steps_per_s"] / fp for k in
+                ("mixed", "fitted_fp64", "learned_fp64", "learned_mixed", "inverse_fp64",
+                 "learned_inverse_mixed")},
+        }
         if not args.no_cpu:
             line["sim_steps"]["cpu_baseline"] = cpu_step_baseline(args)
     if not args.no_ensemble:
