@@ -33,7 +33,8 @@ sys.path.insert(0, ROOT)
 
 from generic_mobility_interface_class_amd import capi, geometry  # noqa: E402
 
-FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40}      # include/gmi_flops.h
+FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
+         "ss_wall_tt": 137}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
@@ -395,6 +396,78 @@ def bench_cilia(args):
     return out
 
 
+def bench_carpet(args):
+    """Prescribed-cilia carpets above a no-slip wall (the reference's
+    INFINITE_PLANE_WALL build with its defaults: coral-larva beat, dynamic
+    phases, rectangular seeding): 64 and 1,024 cilia of 20 segments.  Per
+    workload: the reference generalised-force table of one filament from the
+    device (one period), 3 untimed steps, an FP64 window, then the wall pair
+    kernel alone (M_ss of the last solution on the final geometry, 20
+    launches timed with HIP events on the context stream)."""
+    out = {}
+    names = [args.carpet_workload] if args.carpet_workload else list(geometry.CARPET_WORKLOADS)
+    for name in names:
+        k = args.cilia_steps
+        base = geometry.carpet_workload(name)
+        t0 = time.perf_counter()
+        table = _cilia_table(base, args.cilia_table_steps, not args.no_graphs)
+        table_s = time.perf_counter() - t0
+        cfg = dataclasses.replace(base, dynamic_phase=True, gen_force_refs=table)
+        r = {"segments": cfg.ns, "filaments": cfg.nf, "system_size": cfg.system_size(),
+             "table_s": table_s}
+        g = capi.Cilia(cfg)
+        g.set_graphs(not args.no_graphs)
+        for nt in range(3):
+            assert g.step(nt)[0] == 0
+        r["fp64"] = _cilia_window(g, 3, k, f"{name} fp64")
+        g.segment_velocities()
+        g.call("gmi_profile_begin")
+        for _ in range(20):
+            g.segment_velocities()
+        pair_ms = capi.C.c_double(0.0)
+        n = capi.C.c_int(0)
+        g.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
+        g.close()
+        ms = pair_ms.value / max(1, n.value)
+        pairs = cfg.ns * cfg.ns
+        tflops = FLOPS["ss_wall_tt"] * pairs / (ms * 1e-3) / 1e12
+        r["pair_kernel"] = {"kernel": "k_pair_wall<true>", "ms": ms, "launches": n.value,
+                            "pairs_per_s": pairs / (ms * 1e-3), "achieved_tflops": tflops,
+                            "frac_fp64_peak": tflops / PEAK_FP64_TFLOPS}
+        print(f"# {name} wall pairs: {ms:.3f} ms, {tflops:.1f} TFLOP/s", file=sys.stderr,
+              flush=True)
+        out[name] = r
+    if not args.no_cpu and "carpet_1280" in out:
+        out["carpet_1280"]["cpu_baseline"] = cpu_carpet_baseline()
+    return out
+
+
+def cpu_carpet_baseline():
+    """Oracle (CPU FP64) steps of the 64-cilium carpet (bounded sample: ten
+    steps after two, table from the oracle itself)."""
+    from oracle import binding as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    base = geometry.carpet_workload("carpet_1280")
+    one = dataclasses.replace(base, nfil=np.array([1]),
+                              fil_refs=np.asarray(base.fil_refs).reshape(-1, 3)[:1],
+                              phase=np.zeros(1))
+    table = orc.cilia_reference_gen_forces(one, 60, nthreads=threads)
+    o = orc.Cilia(dataclasses.replace(base, dynamic_phase=True, gen_force_refs=table),
+                  nthreads=threads)
+    for nt in range(2):
+        o.step(nt)
+    t0 = time.perf_counter()
+    its = 0
+    for nt in range(2, 12):
+        rc, it = o.step(nt)
+        assert rc == 0
+        its += it
+    dt = time.perf_counter() - t0
+    return {"value": 10.0 / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"steps 2-11 of carpet_1280 ({its / 10:.1f} GMRES it/step, {dt:.2f} s; "
+                      f"pair sums on {threads} threads, the rest single-threaded; 60-entry table)"}
+
+
 def cpu_cilia_baseline():
     """Oracle (CPU FP64) prescribed-cilia step of the 1,282-particle swimmer
     (bounded sample: one step after two, table from the oracle itself)."""
@@ -456,6 +529,11 @@ def main():
     ap.add_argument("--cilia-table-steps", type=int, default=300,
                     help="entries of the reference generalised-force table (steps per period)")
     ap.add_argument("--no-cilia", action="store_true")
+    ap.add_argument("--carpet-workload", default=None,
+                    help="one of geometry.CARPET_WORKLOADS (default: both)")
+    ap.add_argument("--no-carpet", action="store_true")
+    ap.add_argument("--carpet-only", action="store_true",
+                    help="only the carpets above the wall (diagnostic line)")
     ap.add_argument("--cilia-only", action="store_true",
                     help="only the prescribed-cilia steps (diagnostic line, e.g. for profiling)")
     ap.add_argument("--apply-only", action="store_true",
@@ -474,8 +552,12 @@ def main():
     if args.cilia_only:
         print(json.dumps(finite_json({"cilia": bench_cilia(args)}), allow_nan=False), flush=True)
         return
+    if args.carpet_only:
+        print(json.dumps(finite_json({"carpet": bench_carpet(args)}), allow_nan=False), flush=True)
+        return
     if args.apply_only:
         args.no_cpu = args.no_step = args.no_ensemble = args.no_rollout = args.no_cilia = True
+        args.no_carpet = True
     if args.copy_calibration:
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -586,6 +668,8 @@ def main():
         line["rollout"] = bench_rollout(args)
     if not args.no_cilia:
         line["cilia"] = bench_cilia(args)
+    if not args.no_carpet:
+        line["carpet"] = bench_carpet(args)
     print(json.dumps(finite_json(line), allow_nan=False), flush=True)
 
 

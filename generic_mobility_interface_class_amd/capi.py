@@ -470,6 +470,8 @@ class Cilia(Context):
             cfg.zero_velocity_avoidance, int(bool(cfg.dynamic_phase)), int(bool(cfg.fixed_body)),
             _optp(a["body_velocity"]), _optp(a["gen_force_refs"]), ng, cfg.gmres_tol,
             cfg.gmres_max_iter)
+        if getattr(cfg, "wall", False):
+            self.set_wall(True)
         self.call("gmi_cilia_create", C.byref(self.cp))
         self.cfg = cfg
         n = C.c_int(0)

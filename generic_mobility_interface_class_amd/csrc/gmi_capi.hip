@@ -131,6 +131,25 @@ int rebuild_lists(gmi_ctx *c) {
   return GMI_OK;
 }
 
+int prof_start(gmi_ctx *c, hipEvent_t *ev_stop) {
+  *ev_stop = nullptr;
+  if (!c->profiling) return GMI_OK;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  int rc;
+  if ((rc = hip_check(c, hipStreamIsCapturing(c->stream, &st), "capture status"))) return rc;
+  if (st != hipStreamCaptureStatusNone) return GMI_OK;
+  while (c->prof_events.size() < c->prof_used + 2) {
+    hipEvent_t e;
+    if ((rc = hip_check(c, hipEventCreate(&e), "event"))) return rc;
+    c->prof_events.push_back(e);
+  }
+  if ((rc = hip_check(c, hipEventRecord(c->prof_events[c->prof_used], c->stream), "event")))
+    return rc;
+  *ev_stop = c->prof_events[c->prof_used + 1];
+  c->prof_used += 2;
+  return GMI_OK;
+}
+
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
               const double *Fs, const double *Fb, double *Vs, double *Vb,
               const double *Xs, const double *Xb) {
@@ -147,17 +166,9 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
   a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
   a.tt_only = c->tt_only ? 1 : 0;
   hipEvent_t ev_mid = nullptr;
-  if (c->profiling && wl.nwork > 0) {
+  if (wl.nwork > 0) {
     int rc;
-    while (c->prof_events.size() < c->prof_used + 2) {
-      hipEvent_t e;
-      if ((rc = hip_check(c, hipEventCreate(&e), "event"))) return rc;
-      c->prof_events.push_back(e);
-    }
-    if ((rc = hip_check(c, hipEventRecord(c->prof_events[c->prof_used], c->stream), "event")))
-      return rc;
-    ev_mid = c->prof_events[c->prof_used + 1];
-    c->prof_used += 2;
+    if ((rc = prof_start(c, &ev_mid))) return rc;
   }
   if (c->wall)
     return hip_check(c, launch_pair_wall(a, wl.nwork, wl.nsplit_tiles ? wl.ntiles : 0, c->stream,
@@ -395,7 +406,8 @@ int gmi_set_wall(gmi_ctx *c, int wall) {
   if (!c) return GMI_ERR_ARG;
   if (wall && c->Nb > 0)
     return fail(c, GMI_ERR_STATE, "the wall-bounded mobility has segments only (no blobs)");
-  if (wall && c->extra) return fail(c, GMI_ERR_STATE, "no wall in a time-stepping context");
+  if (c->extra && (wall != 0) != c->wall)
+    return fail(c, GMI_ERR_STATE, "the wall setting is fixed while a time stepper exists");
   c->wall = wall != 0;
   return GMI_OK;
 }

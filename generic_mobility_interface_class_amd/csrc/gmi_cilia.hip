@@ -13,7 +13,9 @@
 // that replaces M by its self-mobility diagonal (:427-650), then moves bodies
 // and phases explicitly (cilia_sim_main.cpp:218-232, filament.cpp:1020-1031).
 // Free bodies and dynamic phases are each optional (the U and phi rows
-// vanish when the body velocity or the phase speed is prescribed).
+// vanish when the body velocity or the phase speed is prescribed).  Above a
+// no-slip wall (the context's wall flag) the carpet has no blobs and no body
+// rows, and M_ss is the translation part of the wall kernel.
 //
 // Device layout of a system vector: segment forces SoA [3][lds] | blob forces
 // SoA [3][ldb] | U [6 Nsw] | phi [Nf], zero padding -- the pair kernels read
@@ -929,7 +931,12 @@ struct CiliaOp {
     pa.tt_only = 1;
     const int nred = wl.nsplit_tiles ? wl.ntiles : 0;
     int rc;
-    if (mode == 2) {
+    if (c->wall) {
+      // FP64 only (the stepper refuses the mixed modes above the wall)
+      pa.skip = done;
+      if ((rc = hip_check(c, launch_pair_wall(pa, wl.nwork, nred, c->stream), "cilia wall pairs")))
+        return rc;
+    } else if (mode == 2) {
       pa.skip = s->flags + F_SKIP64;
       if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, false, c->stream), "cilia pairs64")))
         return rc;
@@ -1043,7 +1050,14 @@ extern "C" {
 int gmi_cilia_create(gmi_ctx *c, const gmi_cilia_params *p) {
   if (!c || !p || !p->nfil || !p->fil_refs) return GMI_ERR_ARG;
   if (c->extra) return fail(c, GMI_ERR_STATE, "context already has a time stepper");
-  if (c->wall) return fail(c, GMI_ERR_STATE, "no prescribed-cilia stepping above a wall");
+  const bool wall = c->wall;
+  if (wall && (c->nsw != 1 || !p->fixed_body || p->body_velocity ||
+               (p->body_x && (p->body_x[0] != 0.0 || p->body_x[1] != 0.0 || p->body_x[2] != 0.0))))
+    return fail(c, GMI_ERR_ARG,
+                "a carpet above the wall is one surface at rest at the origin: one group, "
+                "fixed_body, no body velocity");
+  if (wall && c->precision != GMI_FP64)
+    return fail(c, GMI_ERR_ARG, "the wall-bounded mobility is FP64 only");
   const int S = c->nseg_fil;
   if (S < 2 || S > kMaxSegPerFil) return fail(c, GMI_ERR_ARG, "NSEG must be in [2, 256]");
   int Nf = 0;
@@ -1064,7 +1078,8 @@ int gmi_cilia_create(gmi_ctx *c, const gmi_cilia_params *p) {
     return fail(c, GMI_ERR_ARG, "bad build-a-beat parameters");
   if (p->dynamic_phase && (!p->gen_force_refs || p->n_gen_force_refs < 2))
     return fail(c, GMI_ERR_ARG, "dynamic phases need at least two reference generalised forces");
-  if (!p->phase && !(p->r_over_l > 0)) return fail(c, GMI_ERR_ARG, "R_OVER_L must be positive");
+  if (!wall && !p->phase && !(p->r_over_l > 0))
+    return fail(c, GMI_ERR_ARG, "R_OVER_L must be positive");
   int rc;
   if ((rc = hip_check(c, hipSetDevice(c->device), "set device"))) return rc;
   CiliaHost *s = new CiliaHost();
@@ -1128,13 +1143,19 @@ int gmi_cilia_create(gmi_ctx *c, const gmi_cilia_params *p) {
       const double *r = p->fil_refs + 3 * size_t(fo);
       for (int k = 0; k < 3; k++) fref[3 * size_t(fo) + k] = r[k];
       const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-      if (!(rn > 0)) {
+      if (!wall && !(rn > 0)) {
         delete s;
         return fail(c, GMI_ERR_ARG, "filament base at the body centre");
       }
-      // beat frame: x to the surface normal, the second frame vector to the
-      // polar direction (filament.cpp:245-282)
-      const double dir[3] = {r[0] / rn, r[1] / rn, r[2] / rn};
+      if (wall && !(r[2] > 0)) {
+        delete s;
+        return fail(c, GMI_ERR_ARG, "filament base not above the wall");
+      }
+      // beat frame: x to the surface normal -- the base direction on a body,
+      // +z above the wall (swimmer.cpp:91) -- and the second frame vector to
+      // the polar direction (filament.cpp:245-282)
+      const double dir[3] = {wall ? 0.0 : r[0] / rn, wall ? 0.0 : r[1] / rn,
+                             wall ? 1.0 : r[2] / rn};
       const HQ qt = hq_sqrt(HQ{dir[0], 0.0, -dir[2], dir[1]});
       const double bn[3] = {2.0 * (qt.x * qt.y - qt.s * qt.z), 1.0 - 2.0 * (qt.x * qt.x + qt.z * qt.z),
                             2.0 * (qt.y * qt.z + qt.s * qt.x)};
@@ -1153,6 +1174,9 @@ int gmi_cilia_create(gmi_ctx *c, const gmi_cilia_params *p) {
       fq[4 * size_t(fo) + 2] = q.y; fq[4 * size_t(fo) + 3] = q.z;
       if (p->phase) {
         phase[fo] = p->phase[fo];
+      } else if (wall) {
+        // filament.cpp:222: a wave of one filament length along x
+        phase[fo] = 2.0 * kPiC * r[0] / (d.DL * (S - 1)) + 2.0 * kPiC * 116.0 / 300.0;
       } else {
         const double phi = std::atan2(b[CB_X + 1] + r[1], b[CB_X] + r[0]);
         double ph = phi * (2.0 * kPiC * p->r_over_l);
@@ -1410,6 +1434,8 @@ int gmi_cilia_set_precision(gmi_ctx *c, int precision) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no prescribed-cilia stepper") : GMI_ERR_ARG;
   if (precision != GMI_FP64 && precision != GMI_FP32_KAHAN)
     return fail(c, GMI_ERR_ARG, "unknown precision");
+  if (c->wall && precision != GMI_FP64)
+    return fail(c, GMI_ERR_ARG, "the wall-bounded mobility is FP64 only");
   s->precision = precision;
   return GMI_OK;
 }

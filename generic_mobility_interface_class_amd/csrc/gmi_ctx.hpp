@@ -74,6 +74,10 @@ namespace gmi {
 int fail(gmi_ctx *c, int code, const std::string &msg);
 int hip_check(gmi_ctx *c, hipError_t e, const char *what);
 int rebuild_lists(gmi_ctx *c);
+// Pair-kernel profiling (gmi_profile_begin): records the start event on the
+// context stream and returns in *ev_stop the event to record right after the
+// pair kernel; nullptr when not profiling or while the stream is captured.
+int prof_start(gmi_ctx *c, hipEvent_t *ev_stop);
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32, const double *Fs,
               const double *Fb, double *Vs, double *Vb, const double *Xs,
               const double *Xb);

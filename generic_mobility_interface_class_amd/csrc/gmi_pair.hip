@@ -588,8 +588,74 @@ __device__ __forceinline__ void wall_pair(double dx, double dy, double dz, doubl
   }
 }
 
+// Translations only (segments carry forces, no torques): the velocity rows
+// of wall_self / wall_pair from the force columns, in the same order.
+__device__ __forceinline__ void wall_self_tt(double zi, const double *f, double *vt) {
+  constexpr double FF = 1.0 / (6.0 * kPi);
+  const double hm1 = 1.0 / zi, hm3 = hm1 * hm1 * hm1, hm4 = hm3 * hm1, hm5 = hm4 * hm1;
+  const double par = FF - (9.0 * hm1 - 2.0 * hm3 + hm5) * (FF / 16.0);
+  const double perp = FF - (9.0 * hm1 - 4.0 * hm3 + hm5) * (FF / 8.0);
+  vt[0] += par * f[0]; vt[1] += par * f[1]; vt[2] += perp * f[2];
+}
+
+// 1/sqrt(x) for normal x > 0: the hardware estimate and two Newton steps
+// (each squares the relative error), within a few ulp of the rounded value.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, r, y);
+  r = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, r, y);
+}
+
+// The same velocity rows with the reference's expressions rewritten
+// without divisions: with sz = zi + zj, h = zj / sz and ez = sz / R,
+//   h (1 - h) ez^2 = zi zj / R^2,   h ez / R = zj / R^2,
+//   (1 - h) ez^2 = zi sz / R^2,     h^2 ez^2 / R = zj^2 / R^3,
+// and the unbounded term as (d . f) d / r^2.  Algebraically identical;
+// rounding differs from the literal form by a few ulp per pair.
+__device__ __forceinline__ void wall_pair_tt(double dx, double dy, double dz, double zi,
+                                             double zj, const double *f, double *vt) {
+  constexpr double FF = 1.0 / (6.0 * kPi), TF = 1.0 / (8.0 * kPi);
+  const double q = dx * dx + dy * dy;
+  // unbounded RPY, far form: a0 (b1 f + b2 (d.f) d / r^2)
+  const double r1 = rsqrt_nr(fma(dz, dz, q));
+  const double r2 = r1 * r1;
+  const double a0 = TF * r1;
+  const double b1 = a0 * fma(2.0 / 3.0, r2, 1.0);
+  const double cd = a0 * fma(-2.0, r2, 1.0) * r2 * (dx * f[0] + dy * f[1] + dz * f[2]);
+  vt[0] += b1 * f[0] + cd * dx;
+  vt[1] += b1 * f[1] + cd * dy;
+  vt[2] += b1 * f[2] + cd * dz;
+  // image corrections
+  const double sz = zi + zj;
+  const double rm1 = rsqrt_nr(fma(sz, sz, q));
+  const double rm2 = rm1 * rm1, rm3 = rm2 * rm1, rm5 = rm3 * rm2;
+  const double ex = dx * rm1, ey = dy * rm1, ez = sz * rm1, ez2 = ez * ez;
+  const double ef = ex * f[0] + ey * f[1] + ez * f[2];
+  const double P = zi * zj * rm2;              // h (1 - h) ez^2
+  const double g3 = rm3 * (1.0 - 3.0 * ez2), g5 = rm5 * (1.0 - 5.0 * ez2);
+  const double w5 = 1.0 - 5.0 * ez2;
+  const double c0 = -(0.75 * rm1 * (1.0 + 2.0 * P) + 0.5 * (g3 - g5));
+  const double c1 = -(0.75 * rm1 * (1.0 - 6.0 * P) - 1.5 * rm3 * w5 +
+                      2.5 * rm5 * (1.0 - 7.0 * ez2)) * ef;
+  const double zr = zj * rm2;                  // h ez / R
+  const double c2 = f[2] * (1.5 * zr * (1.0 - 6.0 * zi * sz * rm2) +
+                            ez * (-3.0 * rm3 * w5 + 5.0 * rm5 * (2.0 - 7.0 * ez2)));
+  const double c3ez = (1.5 * zr - 5.0 * rm5 * ez) * ef;
+  const double c4 = -(3.0 * zj * zj * rm3 + 3.0 * rm3 * ez2 + (2.0 - 15.0 * ez2) * rm5) * f[2];
+  const double c12 = c1 + c2;
+  vt[0] += FF * (c0 * f[0] + c12 * ex);
+  vt[1] += FF * (c0 * f[1] + c12 * ey);
+  vt[2] += FF * (c0 * f[2] + c12 * ez + c3ez + c4);
+}
+
+// One lane per target segment, sources staged in LDS tiles.  TT: 3 force
+// rows in, 3 velocity rows out (PairArgs::tt_only); else 6 and 6.
+template <bool TT>
 __global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
-  __shared__ double sx[kTile], sy[kTile], sz[kTile], sf[6][kTile];
+  constexpr int NC = TT ? 3 : 6;
+  __shared__ double sx[kTile], sy[kTile], sz[kTile], sf[NC][kTile];
   if (a.skip && *a.skip) return;
   const WorkItem w = a.work[blockIdx.x];
   const TileDesc T = a.tiles[w.tile];
@@ -608,16 +674,23 @@ __global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
     if (tid < n) {
       const int j = base + tid;
       sx[tid] = a.Xs[j] * inv_a; sy[tid] = a.Xs[a.lds + j] * inv_a; sz[tid] = a.Xs[2 * a.lds + j] * inv_a;
-      for (int c = 0; c < 6; c++) sf[c][tid] = a.Fs[c * a.lds + j];
+      for (int c = 0; c < NC; c++) sf[c][tid] = a.Fs[c * a.lds + j];
     }
     __syncthreads();
     for (int jj = 0; jj < n; jj++) {
       const double f[3] = {sf[0][jj], sf[1][jj], sf[2][jj]};
-      const double t[3] = {sf[3][jj], sf[4][jj], sf[5][jj]};
-      if (base + jj == ti)
-        wall_self(zi, f, t, A);
-      else
-        wall_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], zi, sz[jj], f, t, A);
+      if constexpr (TT) {
+        if (base + jj == ti)
+          wall_self_tt(zi, f, A.vt);
+        else
+          wall_pair_tt(xi - sx[jj], yi - sy[jj], zi - sz[jj], zi, sz[jj], f, A.vt);
+      } else {
+        const double t[3] = {sf[3][jj], sf[4][jj], sf[5][jj]};
+        if (base + jj == ti)
+          wall_self(zi, f, t, A);
+        else
+          wall_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], zi, sz[jj], f, t, A);
+      }
     }
   }
   if (tid >= T.nt) return;
@@ -628,18 +701,16 @@ __global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
     o[3 + k] = s2 * A.wt[k] + s3 * A.wr[k];
   }
   if (T.nsplit == 1) {
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < NC; ++c) {
       double *v = a.Vs + c * a.lds + T.t0 + tid;
       *v = a.accumulate ? *v + o[c] : o[c];
     }
   } else {
     double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
-    for (int c = 0; c < 6; ++c) P[c * kBlock] = o[c];
+    for (int c = 0; c < NC; ++c) P[c * kBlock] = o[c];
   }
 }
 
-// Deterministic reduction of split partials (split order), one workgroup per
-// tile.  Several splits' loads are in flight at once.
 __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
   if (a.skip && *a.skip) return;
   const TileDesc T = a.tiles[blockIdx.x];
@@ -801,7 +872,10 @@ hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp3
 hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,
                             hipEvent_t ev_mid) {
   if (nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pair_wall, dim3(nwork), dim3(kBlock), 0, s, a);
+  if (a.tt_only)
+    hipLaunchKernelGGL(k_pair_wall<true>, dim3(nwork), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_pair_wall<false>, dim3(nwork), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;

@@ -436,6 +436,7 @@ class CiliaConfig:
     gmres_tol: float = 1e-10
     gmres_max_iter: int = 150
     interact: bool = True
+    wall: bool = False        # carpet above a no-slip wall at z = 0
 
     @property
     def n_swimmers(self):
@@ -501,3 +502,65 @@ CILIA_WORKLOADS = {
 
 def cilia_workload(name, **kw):
     return cilia_config(**CILIA_WORKLOADS[name], **kw)
+
+
+def carpet_seed(nfil, nseg, a_seg=1.0, seeding="rectangular"):
+    """Filament bases of a carpet above the wall z = 0 (swimmer.cpp:68-165):
+    a rectangular grid of spacing 1.4 L, or hexagonal rows 0.5 (1.0122 L +
+    DL) apart, centred on the origin, bases 0.5 DL above the plane, with
+    L = DL (NSEG - 1); filament ids run as the reference numbers them."""
+    DL = 2.2 * a_seg
+    L = DL * (nseg - 1)
+    out = []
+    if seeding == "rectangular":
+        nx = int(math.sqrt(float(nfil)))
+        ny = max(1, int(math.ceil(nfil / float(nx))))
+        step_x = step_y = 1.4 * L
+        im, jm = 0.5 * (nx - 1.0), 0.5 * (ny - 1.0)
+        for i in range(nx):
+            for j in range(ny):
+                if j + i * ny < nfil:
+                    out.append(((i - im) * step_x, (j - jm) * step_y, 0.5 * DL))
+    elif seeding == "hexagonal":
+        r3 = math.sqrt(3.0)
+        nx = int(math.floor(0.25 * (3.0 - r3 + math.sqrt(4.0 - 2.0 * r3 + 8.0 * r3 * nfil)) + 0.5))
+        if nx < 2:
+            raise ValueError("hexagonal seeding needs rows of at least two filaments")
+        ny = max(1, int(math.ceil(nfil / float(nx - 1))))
+        step_y = 0.5 * (1.0122 * L + DL)
+        step_x = 2.0 * step_y / r3
+        im, jm = 0.5 * (nx - 1.0), 0.5 * (ny - 1.0)
+        for j in range(ny):
+            odd = j % 2
+            for i in range(nx - odd):
+                if len(out) < nfil:
+                    out.append(((0.5 * odd + i - im) * step_x, (j - jm) * step_y, 0.5 * DL))
+    else:
+        raise ValueError(f"unknown seeding {seeding!r}")
+    if len(out) != nfil:
+        raise ValueError(f"the {seeding} grid holds {len(out)} of {nfil} filaments")
+    return np.array(out, dtype=np.float64)
+
+
+def carpet_config(nfil, nseg, a_seg=1.0, seeding="rectangular", **kw):
+    """A carpet of prescribed cilia above a no-slip wall: the reference's
+    INFINITE_PLANE_WALL build (config.hpp:276-289) -- one surface at rest, no
+    blobs, RSEG = MU = 1 by default, phases 2 pi x / L + 2 pi 116/300 unless
+    given (filament.cpp:222).  Phases are prescribed unless dynamic_phase and
+    a generalised-force table are passed."""
+    kw.setdefault("dynamic_phase", False)
+    return CiliaConfig(nseg=nseg, nfil=np.array([nfil]), nblob=np.array([0]),
+                       blob_refs=np.zeros((0, 3)), fil_refs=carpet_seed(nfil, nseg, a_seg, seeding),
+                       body_x=np.zeros((1, 3)), a_seg=a_seg, fixed_body=True, wall=True, **kw)
+
+
+CARPET_WORKLOADS = {
+    # 64 cilia x 20 segments on a rectangular grid (CPU-runnable)
+    "carpet_1280": dict(nfil=64, nseg=20),
+    # 1024 cilia x 20 segments on a rectangular grid
+    "carpet_20480": dict(nfil=1024, nseg=20),
+}
+
+
+def carpet_workload(name, **kw):
+    return carpet_config(**CARPET_WORKLOADS[name], **kw)

@@ -124,8 +124,10 @@ int gmi_profile_end(gmi_ctx *ctx, double *pair_ms, int *launches);
  * 8-242).  wall != 0 switches gmi_evaluate_segment_segment_mobility (and the
  * grand apply, which then has no other blocks) to it; FP64 whatever the
  * context precision.  The reference's wall build has no bodies: contexts
- * with blobs, or with a time-stepping sim, are refused.  Segments must lie
- * above the wall (z > 0). */
+ * with blobs are refused.  Set it before gmi_cilia_create to step a carpet
+ * of prescribed cilia above the wall (gmi_cilia.h); the setting is fixed
+ * while a time stepper exists, and gmi_sim_create refuses a wall.  Segments
+ * must lie above the wall (z > 0). */
 int gmi_set_wall(gmi_ctx *ctx, int wall);
 
 /* ---- launch shaping (no reference counterpart; the reference launches

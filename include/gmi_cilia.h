@@ -14,7 +14,17 @@
  * reference compiles the rotation terms out, cuda_functions.cu:345,646,808).
  * Vectors of the system in the reference layout (mobility_solver.cpp:
  * 62-110): segment forces [3 Ns] | blob forces [3 Nb] | body velocities
- * [6 Nsw] (free bodies) | phase speeds [Nf] (dynamic phases). */
+ * [6 Nsw] (free bodies) | phase speeds [Nf] (dynamic phases).
+ *
+ * Carpet above a no-slip wall (the reference's INFINITE_PLANE_WALL build,
+ * config.hpp:276-289): call gmi_set_wall(ctx, 1) on a context with one group
+ * of segments and no blobs before gmi_cilia_create.  The surface is at rest
+ * (fixed_body = 1, body_velocity = NULL, body_x NULL or the origin),
+ * fil_refs are the base positions above the plane (z > 0), every beat plane
+ * holds +z, the default
+ * phase is 2 pi x / (DL (NSEG - 1)) + 2 pi 116/300 (filament.cpp:222), the
+ * mobility is the translation part of the Swan-Brady wall kernel
+ * (cuda_functions.cu:8-242 with PRESCRIBED_CILIA), FP64 only. */
 #ifndef GMI_CILIA_H
 #define GMI_CILIA_H
 
@@ -41,7 +51,8 @@ typedef struct gmi_cilia_params {
   const double *body_x;         /* initial body centres [3 Nsw], NULL: origin  */
   const double *phase;          /* initial phases [Nf]; NULL: the reference's
                                    metachronal start, phase = 2 pi R_OVER_L
-                                   atan2(y, x) of the base (filament.cpp:225) */
+                                   atan2(y, x) of the base (filament.cpp:226;
+                                   above a wall: filament.cpp:222)           */
   const double *omega0;         /* natural frequencies [Nf]; NULL: 2 pi        */
   double r_over_l;              /* R_OVER_L (default phases only)              */
   double dt;                    /* DT                                          */

@@ -13,5 +13,9 @@
 #define GMI_FLOPS_SB 56 /* cuda_functions.cu:593-667, seg (v,w) <- blob f */
 #define GMI_FLOPS_BS 56 /* cuda_functions.cu:755-829, blob v <- seg f,tau */
 #define GMI_FLOPS_BB 40 /* cuda_functions.cu:478-517, 3x3 blob<-blob     */
+/* Above a no-slip wall, translations only (the PRESCRIBED_CILIA rows):
+ * unbounded far form 40 + image corrections 97, common subexpressions
+ * counted once */
+#define GMI_FLOPS_SS_WALL_TT 137 /* cuda_functions.cu:114-177 */
 
 #endif

@@ -373,7 +373,7 @@ class OrcCiliaParams(C.Structure):
         ("effective_stroke", C.c_double), ("zero_velocity_avoidance", C.c_double),
         ("dynamic_phase", C.c_int), ("fixed_body", C.c_int), ("body_velocity", _dp),
         ("gen_force_refs", _dp), ("n_gen_force_refs", C.c_int), ("gmres_tol", C.c_double),
-        ("gmres_max_iter", C.c_int),
+        ("gmres_max_iter", C.c_int), ("wall", C.c_int),
     ]
 
 
@@ -428,7 +428,7 @@ class Cilia:
             cfg.amplitude, cfg.recovery_window, cfg.effective_stroke,
             cfg.zero_velocity_avoidance, int(bool(cfg.dynamic_phase)), int(bool(cfg.fixed_body)),
             _opt(a["body_velocity"]), _opt(a["gen_force_refs"]), ng, cfg.gmres_tol,
-            cfg.gmres_max_iter)
+            cfg.gmres_max_iter, int(bool(cfg.wall)))
         self.h = self.lib.orc_cilia_create(C.byref(self.p), nthreads or nthreads_default())
         if not self.h:
             raise ValueError("orc_cilia_create rejected the configuration")

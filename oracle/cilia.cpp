@@ -389,6 +389,12 @@ struct orc_cilia {
     auto group = [&](int s0, int ns, int b0, int nb) {
       const double *X = &xs[3 * size_t(s0)];
       const double *Xb = &xb[3 * size_t(b0)];
+      if (p.wall) {
+        // cuda_functions.cu:8-242 (zero torques: the PRESCRIBED_CILIA rows)
+        orc_Mss_wall(&V6[6 * size_t(s0)], &F6[6 * size_t(s0)], X, ns, 0, ns, p.a_seg, p.mu,
+                     nthreads);
+        return;
+      }
       orc_Mss(&V6[6 * size_t(s0)], &F6[6 * size_t(s0)], X, ns, 0, ns, p.a_seg, p.mu, nthreads);
       if (nb > 0) {
         orc_Msb(&V6[6 * size_t(s0)], fb3 + 3 * size_t(b0), X, ns, Xb, nb, 0, ns, p.a_seg,
@@ -740,6 +746,11 @@ extern "C" {
 
 orc_cilia *orc_cilia_create(const orc_cilia_params *p, int nthreads) {
   if (!p || p->n_swimmers < 1 || p->nseg < 2) return nullptr;
+  // config.hpp:276-289: the wall build has one swimmer, no blobs and
+  // prescribed (zero) body velocities
+  if (p->wall && (p->n_swimmers != 1 || p->nblob[0] != 0 || !p->fixed_body || p->body_velocity ||
+                  (p->body_x && (p->body_x[0] != 0.0 || p->body_x[1] != 0.0 || p->body_x[2] != 0.0))))
+    return nullptr;
   std::unique_ptr<orc_cilia> s(new orc_cilia());
   s->p = *p;
   s->nthreads = std::max(1, nthreads);
@@ -776,7 +787,9 @@ orc_cilia *orc_cilia_create(const orc_cilia_params *p, int nthreads) {
       // filament.cpp:197-282: frame mapping x to the surface normal and the
       // second frame vector to the polar direction; metachronal phase
       const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-      const double dir[3] = {r[0] / rn, r[1] / rn, r[2] / rn};
+      // swimmer.cpp:91: above the wall every filament stands along +z
+      const double dir[3] = {p->wall ? 0.0 : r[0] / rn, p->wall ? 0.0 : r[1] / rn,
+                             p->wall ? 1.0 : r[2] / rn};
       Quat qt;
       qt.s = dir[0]; qt.v[0] = 0.0; qt.v[1] = -dir[2]; qt.v[2] = dir[1];
       qt = qsqrt(qt);
@@ -799,6 +812,9 @@ orc_cilia *orc_cilia_create(const orc_cilia_params *p, int nthreads) {
       c.q = c.qm1;
       if (p->phase) {
         c.phase = p->phase[fo];
+      } else if (p->wall) {
+        // filament.cpp:222
+        c.phase = 2.0 * kPi * r[0] / (s->DL * (s->S - 1)) + 2.0 * kPi * 116.0 / 300.0;
       } else {
         const double base[2] = {B.x[0] + r[0], B.x[1] + r[1]};
         const double phi = std::atan2(base[1], base[0]);

@@ -34,6 +34,10 @@ typedef struct orc_cilia_params {
   int n_gen_force_refs;
   double gmres_tol;           /* LINEAR_SYSTEM_TOL                               */
   int gmres_max_iter;         /* MAX_LINEAR_SYSTEM_ITER                          */
+  int wall;                   /* 1: carpet above a no-slip wall at z = 0 (the
+                                 INFINITE_PLANE_WALL build: one swimmer, no
+                                 blobs, fixed body at rest, beat frames about
+                                 +z, wall phases, wall mobility)               */
 } orc_cilia_params;
 
 typedef struct orc_cilia orc_cilia;
