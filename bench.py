@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 from generic_mobility_interface_class_amd import capi, geometry  # noqa: E402
 
 FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
-         "ss_wall_tt": 137}
+         "ss_wall_tt": 137, "ss_wall": 324}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
@@ -403,7 +403,8 @@ def bench_carpet(args):
     workload: the reference generalised-force table of one filament from the
     device (one period), 3 untimed steps, an FP64 window, then the wall pair
     kernel alone (M_ss of the last solution on the final geometry, 20
-    launches timed with HIP events on the context stream)."""
+    launches timed with HIP events on the context stream), and the full 6x6
+    wall apply (forces and torques, N(0,1), seed 5) on the same positions."""
     out = {}
     names = [args.carpet_workload] if args.carpet_workload else list(geometry.CARPET_WORKLOADS)
     for name in names:
@@ -427,6 +428,7 @@ def bench_carpet(args):
         pair_ms = capi.C.c_double(0.0)
         n = capi.C.c_int(0)
         g.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
+        xs_final = g.state()["x"]
         g.close()
         ms = pair_ms.value / max(1, n.value)
         pairs = cfg.ns * cfg.ns
@@ -435,6 +437,18 @@ def bench_carpet(args):
                             "pairs_per_s": pairs / (ms * 1e-3), "achieved_tflops": tflops,
                             "frac_fp64_peak": tflops / PEAK_FP64_TFLOPS}
         print(f"# {name} wall pairs: {ms:.3f} ms, {tflops:.1f} TFLOP/s", file=sys.stderr,
+              flush=True)
+        # the full 6x6 wall mobility (forces and torques) on the same positions
+        ctx = capi.Context([cfg.ns], [0], cfg.nseg)
+        ctx.set_wall(True)
+        ctx.upload(xs_final, np.random.default_rng(5).standard_normal((cfg.ns, 6)))
+        total_ms, kern_ms = time_apply(ctx, 10, 2)
+        ctx.close()
+        tf6 = FLOPS["ss_wall"] * pairs / (kern_ms * 1e-3) / 1e12
+        r["wall_apply_6x6"] = {"kernel": "k_pair_wall<false>", "ms": kern_ms,
+                               "ms_per_apply": total_ms / 10, "pairs_per_s": pairs / (kern_ms * 1e-3),
+                               "achieved_tflops": tf6, "frac_fp64_peak": tf6 / PEAK_FP64_TFLOPS}
+        print(f"# {name} 6x6 wall apply: {kern_ms:.3f} ms, {tf6:.1f} TFLOP/s", file=sys.stderr,
               flush=True)
         out[name] = r
     if not args.no_cpu and "carpet_1280" in out:

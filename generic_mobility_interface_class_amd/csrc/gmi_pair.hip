@@ -492,6 +492,28 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
 // Positions are divided by a and the four blocks scaled by 1/(mu a),
 // 1/(mu a^2), 1/(mu a^2), 1/(mu a^3) -- the reference's expression when
 // RSEG = MU = 1.  Pairs j != i use the far-field RPY form at any separation.
+// Reciprocals and reciprocal square roots come from the hardware estimates
+// refined by Newton steps (no IEEE division sequences in the pair loop).
+
+// 1/sqrt(x) for normal x > 0: the hardware estimate and two Newton steps
+// (each squares the relative error), within a few ulp of the rounded value.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double r = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, r, y);
+  r = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, r, y);
+}
+
+// 1/x for normal x: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
 struct WallAcc {
   double vt[3], vr[3], wt[3], wr[3];  // v from forces / torques, w from forces / torques
 };
@@ -514,7 +536,7 @@ __device__ __forceinline__ void wall_pair(double dx, double dy, double dz, doubl
                                           const double *f, const double *t, WallAcc &A) {
   constexpr double FF = 1.0 / (6.0 * kPi), TF = 1.0 / (8.0 * kPi);
   // unbounded RPY, far-field form
-  double rm1 = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  double rm1 = rsqrt_nr(dx * dx + dy * dy + dz * dz);
   const double rx = dx * rm1, ry = dy * rm1, rz = dz * rm1;
   const double rf = rx * f[0] + ry * f[1] + rz * f[2];
   const double rt = rx * t[0] + ry * t[1] + rz * t[2];
@@ -536,11 +558,11 @@ __device__ __forceinline__ void wall_pair(double dx, double dy, double dz, doubl
   A.wt[2] += a0 * (f[0] * ry - f[1] * rx);
   // wall corrections: the image of source j at (xj, yj, -zj)
   const double sz = zi + zj;
-  rm1 = 1.0 / sqrt(dx * dx + dy * dy + sz * sz);
+  rm1 = rsqrt_nr(dx * dx + dy * dy + sz * sz);
   const double rm2 = rm1 * rm1, rm3 = rm2 * rm1, rm4 = rm3 * rm1, rm5 = rm4 * rm1;
   double ex = dx * rm1, ey = dy * rm1;
   const double ez = sz * rm1, ez2 = ez * ez;
-  double h = zj / sz;
+  double h = zj * rcp_nr(sz);
   const double ef = ex * f[0] + ey * f[1] + ez * f[2];
   const double et = ex * t[0] + ey * t[1] + ez * t[2];
   {
@@ -596,16 +618,6 @@ __device__ __forceinline__ void wall_self_tt(double zi, const double *f, double 
   const double par = FF - (9.0 * hm1 - 2.0 * hm3 + hm5) * (FF / 16.0);
   const double perp = FF - (9.0 * hm1 - 4.0 * hm3 + hm5) * (FF / 8.0);
   vt[0] += par * f[0]; vt[1] += par * f[1]; vt[2] += perp * f[2];
-}
-
-// 1/sqrt(x) for normal x > 0: the hardware estimate and two Newton steps
-// (each squares the relative error), within a few ulp of the rounded value.
-__device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double r = fma(-x * y, y, 1.0);
-  y = fma(0.5 * y, r, y);
-  r = fma(-x * y, y, 1.0);
-  return fma(0.5 * y, r, y);
 }
 
 // The same velocity rows with the reference's expressions rewritten

@@ -17,5 +17,8 @@
  * unbounded far form 40 + image corrections 97, common subexpressions
  * counted once */
 #define GMI_FLOPS_SS_WALL_TT 137 /* cuda_functions.cu:114-177 */
+/* Above a no-slip wall, 6x6: unbounded 92 (as ss) + image tt 97 + rr 38 +
+ * rt 48 + tr 49 */
+#define GMI_FLOPS_SS_WALL 324    /* cuda_functions.cu:114-213 */
 
 #endif
