@@ -34,6 +34,10 @@
 
 namespace gmi {
 
+constexpr int kSym = 64;  // tile width of the symmetric product
+struct SymTile { int sw, I, J, pad; };
+struct SymRow { int sw, K, nbl, tile0; };  // block row K of a swimmer with nbl block rows
+
 struct DensePrec {
   rocblas_handle blas = nullptr;
   std::vector<size_t> moff_h;   // per swimmer: first element of its matrix
@@ -47,6 +51,11 @@ struct DensePrec {
   double *part = nullptr;       // per blob tile: shares of K^T w
   size_t *moff = nullptr;
   int *voff = nullptr;
+  // symmetric product over the lower triangle in kSym x kSym tiles
+  SymTile *sym_tiles = nullptr;  // (swimmer, block row I >= block column J)
+  SymRow *sym_rows = nullptr;    // (swimmer, block row) of the sums
+  int nsym_tiles = 0, nsym_rows = 0;
+  double *sym_part = nullptr;    // per tile: row partial [kSym] | column partial [kSym]
   std::vector<void *> owned;
 };
 
@@ -222,33 +231,76 @@ __global__ __launch_bounds__(256) void k_dense_in(SimDev a, const BTile *tiles, 
   }
 }
 
-// w = M^-1 t per swimmer (blockIdx.y): one wave per row, reading the
-// contiguous column of the symmetric inverse.
-__global__ __launch_bounds__(256) void k_dense_gemv(const double *__restrict__ Minv,
-                                                    const size_t *__restrict__ moff,
-                                                    const int *__restrict__ voff,
-                                                    const int *__restrict__ sw_nblob,
-                                                    const double *__restrict__ t,
-                                                    double *__restrict__ w, const int *flags) {
+// w = M^-1 t reading only the lower triangle of each swimmer's symmetric
+// inverse: one workgroup per kSym x kSym tile (I >= J) stages the tile in
+// LDS and writes the row partial A_IJ t_J and, below the diagonal, the
+// column partial A_IJ^T t_I -- every stored element is read from HBM once
+// for both uses.  k_dense_symv_sum adds the partials of a block row in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_dense_symv_tiles(const double *__restrict__ Minv,
+                                                          const size_t *__restrict__ moff,
+                                                          const int *__restrict__ voff,
+                                                          const int *__restrict__ sw_nblob,
+                                                          const SymTile *__restrict__ tl,
+                                                          const double *__restrict__ t,
+                                                          double *__restrict__ part,
+                                                          const int *flags) {
   if (flags && flags[F_DONE]) return;
-  const int n = blockIdx.y;
-  const int rows = 3 * sw_nblob[n];
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (row >= rows) return;
-  const double *col = Minv + moff[n] + size_t(row) * rows;
+  constexpr int LD = kSym + 1;  // padded column stride: conflict-free row and column reads
+  __shared__ double L[kSym * LD];
+  __shared__ double xi[kSym], xj[kSym];
+  __shared__ double red[4][kSym];
+  const SymTile T = tl[blockIdx.x];
+  const int n = T.sw, rows = 3 * sw_nblob[n];
+  const int r0 = T.I * kSym, c0 = T.J * kSym;
+  const int nr = min(kSym, rows - r0), nc = min(kSym, rows - c0);
+  const double *A = Minv + moff[n];
   const double *tv = t + voff[n];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int c = lane;
-  for (; c + 192 < rows; c += 256) {
-    s0 += col[c] * tv[c];
-    s1 += col[c + 64] * tv[c + 64];
-    s2 += col[c + 128] * tv[c + 128];
-    s3 += col[c + 192] * tv[c + 192];
+  const int th = threadIdx.x, r = th & (kSym - 1), g = th >> 6;
+  // column c0 + c holds rows r0 .. r0 + nr - 1 contiguously
+  for (int c = g; c < kSym; c += 4)
+    L[c * LD + r] = (r < nr && c < nc) ? A[size_t(c0 + c) * rows + r0 + r] : 0.0;
+  if (th < kSym) xi[th] = th < nr ? tv[r0 + th] : 0.0;
+  else if (th < 2 * kSym) xj[th - kSym] = th - kSym < nc ? tv[c0 + th - kSym] : 0.0;
+  __syncthreads();
+  double s = 0.0;
+  if (g < 2) {                 // row partial, columns g*32 .. g*32+31
+    for (int c = g * 32; c < g * 32 + 32; c++) s += L[c * LD + r] * xj[c];
+  } else if (T.I != T.J) {     // column partial, rows (g-2)*32 .. +31
+    const int h = (g - 2) * 32;
+    for (int q = h; q < h + 32; q++) s += L[r * LD + q] * xi[q];
   }
-  for (; c < rows; c += 64) s0 += col[c] * tv[c];
-  double s = (s0 + s1) + (s2 + s3);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) w[voff[n] + row] = s;
+  red[g][r] = s;
+  __syncthreads();
+  double *P = part + size_t(2 * kSym) * blockIdx.x;
+  if (th < kSym) P[th] = red[0][th] + red[1][th];
+  else if (th < 2 * kSym) P[th] = red[2][th - kSym] + red[3][th - kSym];
+}
+
+// w rows of block row K: sum over J <= K of the row partials of tiles (K, J)
+// and over I > K of the column partials of tiles (I, K); four lane groups
+// take every fourth term, combined in group order.
+__global__ __launch_bounds__(256) void k_dense_symv_sum(const SymRow *__restrict__ rl,
+                                                        const int *__restrict__ voff,
+                                                        const int *__restrict__ sw_nblob,
+                                                        const double *__restrict__ part,
+                                                        double *__restrict__ w, const int *flags) {
+  if (flags && flags[F_DONE]) return;
+  __shared__ double red[4][kSym];
+  const SymRow R = rl[blockIdx.x];
+  const int th = threadIdx.x, k = th & (kSym - 1), g = th >> 6;
+  double s = 0.0;
+  for (int q = g; q < R.nbl; q += 4) {
+    // term q: tile (K, q) row partial for q <= K, tile (q, K) column partial above
+    const size_t tile = q <= R.K ? size_t(R.tile0) + size_t(R.K) * (R.K + 1) / 2 + q
+                                 : size_t(R.tile0) + size_t(q) * (q + 1) / 2 + R.K;
+    s += part[size_t(2 * kSym) * tile + (q <= R.K ? 0 : kSym) + k];
+  }
+  red[g][k] = s;
+  __syncthreads();
+  const int row = R.K * kSym + th;
+  if (th < kSym && row < 3 * sw_nblob[R.sw])
+    w[voff[R.sw] + row] = ((red[0][th] + red[1][th]) + red[2][th]) + red[3][th];
 }
 
 // Per blob tile: shares of K^T w = (sum w_j, sum ref_j x w_j).
@@ -334,6 +386,7 @@ void dense_release(SimHost *s) {
   if (D->blas) (void)rocblas_destroy_handle(D->blas);
   delete D;
   s->dense = nullptr;
+  s->Zk = nullptr;  // owned by D
 }
 
 static int dmalloc(gmi_ctx *c, DensePrec *D, void **p, size_t bytes) {
@@ -381,7 +434,12 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       (rc = dmalloc(c, D, (void **)&D->tc, sizeof(double) * 6 * d.Nsw)) ||
       (rc = dmalloc(c, D, (void **)&D->part, sizeof(double) * 6 * std::max(1, s->nbtiles))) ||
       (rc = dmalloc(c, D, (void **)&D->moff, sizeof(size_t) * d.Nsw)) ||
-      (rc = dmalloc(c, D, (void **)&D->voff, sizeof(int) * d.Nsw)))
+      (rc = dmalloc(c, D, (void **)&D->voff, sizeof(int) * d.Nsw)) ||
+      // the preconditioned Arnoldi directions: each solve applies M^-1 once
+      // per step and not again for the update
+      (rc = dmalloc(c, D, (void **)&s->Zk, sizeof(double) * size_t(d.npad) * (d.maxit + 1))) ||
+      (rc = hip_check(c, hipMemsetAsync(s->Zk, 0, sizeof(double) * size_t(d.npad) * (d.maxit + 1),
+                                        c->stream), "zero directions")))
     return bail(rc);
   // Gauss-Jordan workspaces of the largest swimmer
   double *P = nullptr, *T = nullptr, *Dd = nullptr, *K = nullptr, *S = nullptr;
@@ -390,6 +448,27 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       (rc = dmalloc(c, D, (void **)&Dd, sizeof(double) * kDiag * kDiag)) ||
       (rc = dmalloc(c, D, (void **)&K, sizeof(double) * size_t(maxn) * 6)) ||
       (rc = dmalloc(c, D, (void **)&S, sizeof(double) * 36)))
+    return bail(rc);
+  // tiles of the symmetric product, per swimmer in (I, J <= I) order
+  std::vector<SymTile> tl;
+  std::vector<SymRow> rl;
+  for (int n = 0; n < d.Nsw; n++) {
+    const int nbl = nb_of(3 * c->nblob[n], kSym), t0 = int(tl.size());
+    for (int I = 0; I < nbl; I++) {
+      rl.push_back({n, I, nbl, t0});
+      for (int J = 0; J <= I; J++) tl.push_back({n, I, J, 0});
+    }
+  }
+  D->nsym_tiles = int(tl.size());
+  D->nsym_rows = int(rl.size());
+  if ((rc = dmalloc(c, D, (void **)&D->sym_tiles, sizeof(SymTile) * tl.size())) ||
+      (rc = dmalloc(c, D, (void **)&D->sym_rows, sizeof(SymRow) * rl.size())) ||
+      (rc = dmalloc(c, D, (void **)&D->sym_part, sizeof(double) * 2 * kSym * tl.size())) ||
+      (rc = hip_check(c, hipMemcpyAsync(D->sym_tiles, tl.data(), sizeof(SymTile) * tl.size(),
+                                        hipMemcpyHostToDevice, c->stream), "upload")) ||
+      (rc = hip_check(c, hipMemcpyAsync(D->sym_rows, rl.data(), sizeof(SymRow) * rl.size(),
+                                        hipMemcpyHostToDevice, c->stream), "upload")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))  // tl, rl are locals
     return bail(rc);
   if ((rc = hip_check(c, hipMemcpyAsync(D->moff, D->moff_h.data(), sizeof(size_t) * d.Nsw,
                                         hipMemcpyHostToDevice, c->stream), "upload")) ||
@@ -466,8 +545,10 @@ int dense_apply(gmi_ctx *c, SimHost *s, double *in, double *out, const int *skip
   DensePrec *D = s->dense;
   hipLaunchKernelGGL(k_dense_in, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
                      s->sw_tile0, D->voff, in, s->scal, norm_slot, D->t, D->tc, skip);
-  hipLaunchKernelGGL(k_dense_gemv, dim3(nb_of(D->maxn, 4), d.Nsw), dim3(256), 0, c->stream,
-                     D->Minv, D->moff, D->voff, d.sw_nblob, D->t, D->w, skip);
+  hipLaunchKernelGGL(k_dense_symv_tiles, dim3(D->nsym_tiles), dim3(256), 0, c->stream, D->Minv,
+                     D->moff, D->voff, d.sw_nblob, D->sym_tiles, D->t, D->sym_part, skip);
+  hipLaunchKernelGGL(k_dense_symv_sum, dim3(D->nsym_rows), dim3(256), 0, c->stream, D->sym_rows,
+                     D->voff, d.sw_nblob, D->sym_part, D->w, skip);
   hipLaunchKernelGGL(k_dense_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, D->voff,
                      D->w, D->part, skip);
   hipLaunchKernelGGL(k_dense_out, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,

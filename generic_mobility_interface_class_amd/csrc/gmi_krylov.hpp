@@ -40,6 +40,11 @@ struct KrylovState {
          *sn = nullptr, *beta = nullptr, *h1 = nullptr, *h2 = nullptr, *y = nullptr,
          *z = nullptr, *tmp = nullptr, *scal = nullptr, *part = nullptr, *pp = nullptr;
   size_t pp_half = 0;     // column dot partials: pp[0, pp_half) and pp[pp_half, 2 pp_half)
+  // Optional: the preconditioned directions z_k = P^-1 q_k, one per Arnoldi
+  // step (same layout as Qk).  When set, the solution update is x += Z y and
+  // the preconditioner is not applied again after the last step (the update
+  // of flexible GMRES; identical in exact arithmetic for a fixed P).
+  double *Zk = nullptr;
   int *flags = nullptr;   // device flags (F_*)
   int *h_flags = nullptr; // pinned copies
   double *h_scal = nullptr;
@@ -459,10 +464,11 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
     for (int it = it0; it <= it1; it++) {
       double *qprev = s->Qk + ldq * (it - 1);
       double *w = s->Qk + ldq * it;
+      double *zk = s->Zk ? s->Zk + ldq * (it - 1) : s->z;
       // the previous vector is normalised by the preconditioner's first
       // kernel (the first one was scaled by beta0 already)
-      if ((r = op.precond(qprev, s->z, s->flags, it >= 2 ? 3 : -1))) return r;
-      if ((r = op.mult(s->z, w, mixed ? 2 : 0, s->flags, true))) return r;
+      if ((r = op.precond(qprev, zk, s->flags, it >= 2 ? 3 : -1))) return r;
+      if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
       // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
       hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
                          ds.R, pp1, s->flags);
@@ -523,10 +529,18 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   const int nits = s->h_flags[F_ITS];
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->cols + 1), c->stream,
                      s->H, ldh, s->beta, s->flags, s->y);
-  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, s->flags, s->y,
-                     s->tmp, n);
-  if ((rc = launch_ok(c, "gmres solution"))) return rc;
-  if ((rc = op.precond(s->tmp, s->z, nullptr, -1))) return rc;
+  if (s->Zk) {
+    // x += Z y
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Zk, ldq, s->flags, s->y,
+                       s->z, n);
+    if ((rc = launch_ok(c, "gmres solution"))) return rc;
+  } else {
+    // x += P^-1 (Q y)
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, s->flags, s->y,
+                       s->tmp, n);
+    if ((rc = launch_ok(c, "gmres solution"))) return rc;
+    if ((rc = op.precond(s->tmp, s->z, nullptr, -1))) return rc;
+  }
   hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
   if ((rc = launch_ok(c, "gmres update"))) return rc;
   *its = nits;

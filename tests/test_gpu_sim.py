@@ -116,14 +116,17 @@ def test_fitted_gmres_guess_keeps_the_trajectory(gpu):
     assert its_fit < 0.7 * its_plain
 
 
-@pytest.mark.parametrize("name", ["step_962", "step_9122"])
-def test_body_inverse_preconditioner_keeps_the_trajectory(gpu, name):
-    """The exact body-frame preconditioner changes how fast GMRES converges,
-    not what it converges to: trajectories follow the plain run within the
-    Broyden tolerance (the oracle on the small swimmer), and GMRES takes a
-    few iterations per solve."""
+@pytest.mark.parametrize("name,precision", [("step_962", capi.GMI_FP64),
+                                            ("step_9122", capi.GMI_FP64),
+                                            ("step_962", capi.GMI_FP32_KAHAN)])
+def test_body_inverse_preconditioner_keeps_the_trajectory(gpu, name, precision):
+    """The exact body-frame preconditioner (with the preconditioned
+    directions kept for the update) changes how fast GMRES converges, not
+    what it converges to: trajectories follow the plain FP64 run within the
+    Broyden tolerance (the oracle on the small swimmer), also with mixed
+    precision, and GMRES takes a few iterations per solve."""
     cfg = geometry.sim_workload(name, seed=1)
-    plain, inv = capi.Sim(cfg), capi.Sim(cfg)
+    plain, inv = capi.Sim(cfg), capi.Sim(cfg, precision=precision)
     for s in (plain, inv):
         s.make_body_reference_matrices()
     inv.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
@@ -135,7 +138,8 @@ def test_body_inverse_preconditioner_keeps_the_trajectory(gpu, name):
     for nt in range(6):
         rp, ri = plain.step(nt), inv.step(nt)
         dx = np.abs(inv.state()["x"] - plain.state()["x"]).max()
-        print(f"{name} step {nt}: plain {rp} body-inverse {ri} |dx| {dx:.2e}", flush=True)
+        print(f"{name} precision {precision} step {nt}: plain {rp} body-inverse {ri} "
+              f"|dx| {dx:.2e}", flush=True)
         assert rp[0] == 0 and ri[0] == 0
         assert abs(rp[1] - ri[1]) <= 1
         assert dx <= DL * cfg.tol
