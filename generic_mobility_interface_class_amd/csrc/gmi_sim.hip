@@ -261,15 +261,34 @@ __global__ void k_fil_forces(SimDev a, double t) {
   o[0] = sf.x; o[1] = sf.y; o[2] = sf.z; o[3] = stq.x; o[4] = stq.y; o[5] = stq.z;
 }
 
-__global__ void k_body_forces(SimDev a) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.Nsw) return;
+// Sums v[0..NV) over the 256 threads of a workgroup into out[] of thread 0
+// (waves first, then the four wave sums in wave order).  sh: 4 NV doubles.
+template <int NV>
+__device__ void wg_sums(const double *v, double *out, double *sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = 0; k < NV; k++) {
+    const double t = wave_sum(v[k]);
+    if (lane == 0) sh[4 * k + w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) out[k] = ((sh[4 * k] + sh[4 * k + 1]) + sh[4 * k + 2]) + sh[4 * k + 3];
+  __syncthreads();
+}
+
+// One workgroup of 256 per swimmer.
+__global__ __launch_bounds__(256) void k_body_forces(SimDev a) {
+  __shared__ double sh[4 * 6];
+  const int n = blockIdx.x;
   double *b = a.body + size_t(n) * BREC;
-  double s[6] = {0, 0, 0, 0, 0, 0};
+  double v[6] = {0, 0, 0, 0, 0, 0};
   const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
-  for (int f = f0; f < f1; f++)
-    for (int k = 0; k < 6; k++) s[k] -= a.fil_bf[6 * f + k];
-  for (int k = 0; k < 6; k++) b[BF + k] = s[k];
+  for (int f = f0 + threadIdx.x; f < f1; f += 256)
+    for (int k = 0; k < 6; k++) v[k] -= a.fil_bf[6 * f + k];
+  double s[6];
+  wg_sums<6>(v, s, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) b[BF + k] = s[k];
 }
 
 // -------------------------------------------------------------------------
@@ -1146,14 +1165,37 @@ __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
 }
 
 // Schur complement of each swimmer and its inverse (swimmer.cpp:469-684)
-__global__ void k_schur_body(SimDev a, int nt) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.Nsw) return;
+// One workgroup of 256 per swimmer: the per-filament terms (the tether and
+// clamp couplings and the filaments' B A blocks) are summed across the
+// workgroup, then one lane assembles and inverts the 6x6.
+__global__ __launch_bounds__(256) void k_schur_body(SimDev a, int nt) {
+  __shared__ double sh[4 * 45];
+  const int n = blockIdx.x;
   double *b = a.body + size_t(n) * BREC;
   const bool euler = nt < a.neuler;
   const double dtf = euler ? -a.dt : -2.0 * a.dt / 3.0;
   const d3 ub = ga3(b + BU);
   const m3 blc = mscale(-1.0, rcross(ub));
+  m3 Qi;
+  for (int q = 0; q < 9; q++) Qi.a[q] = b[BQI + q];
+  // v[0..8]: sum over filaments of -rcross(teth) rcross(d) + 0.5 rcross(cl)
+  //          - (rcross(u x cl) + blc rcross(cl)) / 12;  v[9..44]: sum of B A
+  double v[45];
+  for (int k = 0; k < 45; k++) v[k] = 0.0;
+  const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
+  for (int f = f0 + threadIdx.x; f < f1; f += 256) {
+    const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
+    const d3 teth = rowvec(a.teth + 9 * f), cl = rowvec(a.clamp + 9 * f);
+    const d3 ucl = cross3(ub, cl);
+    const m3 t = madd(mscale(-1.0, mm(rcross(teth), rcross(d))),
+                      msub(mscale(0.5, rcross(cl)),
+                           mscale(1.0 / 12.0, madd(rcross(ucl), mm(blc, rcross(cl))))));
+    for (int k = 0; k < 9; k++) v[k] += t.a[k];
+    for (int k = 0; k < 36; k++) v[9 + k] += a.fil_BA[36 * f + k];
+  }
+  double r[45];
+  wg_sums<45>(v, r, sh);
+  if (threadIdx.x != 0) return;
   const m3 cbo = mscale(-1.0 / dtf, blc);
   double Sm[36];
   for (int k = 0; k < 36; k++) Sm[k] = (k % 7 == 0) ? 1.0 : 0.0;
@@ -1162,18 +1204,8 @@ __global__ void k_schur_body(SimDev a, int nt) {
       for (int j = 0; j < 3; j++) Sm[6 * (3 + i) + 3 + j] += M.a[3 * i + j];
   };
   add33(mscale(dtf, msub(mscale(1.0 / 12.0, mm(blc, cbo)), mscale(0.5, cbo))));
-  m3 Qi;
-  for (int q = 0; q < 9; q++) Qi.a[q] = b[BQI + q];
-  m3 old = mzero();
-  const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
-  for (int f = f0; f < f1; f++) {
-    const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
-    const d3 teth = rowvec(a.teth + 9 * f), cl = rowvec(a.clamp + 9 * f);
-    old = msub(old, mm(rcross(teth), rcross(d)));
-    const d3 ucl = cross3(ub, cl);
-    old = madd(old, msub(mscale(0.5, rcross(cl)),
-                         mscale(1.0 / 12.0, madd(rcross(ucl), mm(blc, rcross(cl))))));
-  }
+  m3 old;
+  for (int k = 0; k < 9; k++) old.a[k] = r[k];
   double bm[36];
   rotate66(b + BN, Qi, bm);
   m3 bm33;
@@ -1182,8 +1214,7 @@ __global__ void k_schur_body(SimDev a, int nt) {
   old = mm(bm33, old);
   add33(mscale(dtf, madd(msub(old, mscale(0.5, mm(blc, old))),
                          mscale(1.0 / 12.0, mm(mm(blc, blc), old)))));
-  for (int f = f0; f < f1; f++)
-    for (int k = 0; k < 36; k++) Sm[k] -= a.fil_BA[36 * f + k];
+  for (int k = 0; k < 36; k++) Sm[k] -= r[9 + k];
   if (!gj_small(Sm, 6)) atomicOr(&a.flags[F_SING], 1);
   for (int k = 0; k < 36; k++) b[BS + k] = Sm[k];
 }
@@ -1212,15 +1243,21 @@ __global__ __launch_bounds__(64) void k_jim1(SimDev a, const double *in) {
   }
 }
 
-__global__ void k_jim_body(SimDev a, const double *in, double *out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.Nsw) return;
+// One workgroup of 256 per swimmer: the filaments' B_i in_i are summed
+// across the workgroup, one lane applies S^-1.
+__global__ __launch_bounds__(256) void k_jim_body(SimDev a, const double *in, double *out) {
+  __shared__ double sh[4 * 6];
+  const int n = blockIdx.x;
   const int o = body_off(a, n);
-  double b2[6];
-  for (int k = 0; k < 6; k++) b2[k] = in[o + k];
+  double v[6] = {0, 0, 0, 0, 0, 0};
   const int f0 = a.sw_fil0[n], f1 = f0 + a.sw_nfil[n];
-  for (int f = f0; f < f1; f++)
-    for (int k = 0; k < 6; k++) b2[k] -= a.fil_b2[6 * f + k];
+  for (int f = f0 + threadIdx.x; f < f1; f += 256)
+    for (int k = 0; k < 6; k++) v[k] += a.fil_b2[6 * f + k];
+  double sb[6];
+  wg_sums<6>(v, sb, sh);
+  if (threadIdx.x != 0) return;
+  double b2[6];
+  for (int k = 0; k < 6; k++) b2[k] = in[o + k] - sb[k];
   const double *Si = a.body + size_t(n) * BREC + BS;
   for (int r = 0; r < 6; r++) {
     double s = 0;
@@ -1517,7 +1554,7 @@ static int compute_errors(gmi_ctx *c, SimHost *s, int nt, double *dst, int *big,
 static int jacobian_inv_mult(gmi_ctx *c, SimHost *s, const double *in, double *out) {
   SimDev &d = s->d;
   hipLaunchKernelGGL(k_jim1, dim3(d.Nf), dim3(64), 0, c->stream, d, in);
-  hipLaunchKernelGGL(k_jim_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, in, out);
+  hipLaunchKernelGGL(k_jim_body, dim3(d.Nsw), dim3(256), 0, c->stream, d, in, out);
   hipLaunchKernelGGL(k_jim2, dim3(d.Nf), dim3(256), 0, c->stream, d, in, out);
   return launch_ok(c, "jacobian_inv_mult");
 }
@@ -1785,6 +1822,9 @@ int gmi_sim_set_body_preconditioner(gmi_ctx *c, int kind) {
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   s->drop_graphs();  // captured chunks hold the old preconditioner's kernels
   dense_release(s);
+  // the exact inverse converges in about one Arnoldi step: check after every
+  // step rather than launch (flag-skipped) steps that are not needed
+  s->chunk = kind == GMI_PRECOND_BODY_INVERSE ? 1 : 6;
   if (kind == GMI_PRECOND_BODY_INVERSE) return dense_setup(c, s);
   return GMI_OK;
 }
@@ -1904,7 +1944,7 @@ int gmi_forces_and_torques(gmi_ctx *c, int nt) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   const double t = (nt + 1) * s->p.dt;
   hipLaunchKernelGGL(k_fil_forces, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, t);
-  hipLaunchKernelGGL(k_body_forces, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d);
+  hipLaunchKernelGGL(k_body_forces, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d);
   return launch_ok(c, "forces");
 }
 
@@ -1937,7 +1977,7 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   else
     hipLaunchKernelGGL(k_gj_invert, dim3(d.Nf), dim3(256), 0, c->stream, d);
   hipLaunchKernelGGL(k_schur_fil, dim3(d.Nf), dim3(256), 0, c->stream, d, nt);
-  hipLaunchKernelGGL(k_schur_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, nt);
+  hipLaunchKernelGGL(k_schur_body, dim3(d.Nsw), dim3(256), 0, c->stream, d, nt);
   int rc;
   if ((rc = launch_ok(c, "prepare_jacobian_inv"))) return rc;
   if ((rc = read_flags(c, s))) return rc;

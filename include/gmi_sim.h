@@ -118,8 +118,12 @@ int gmi_sim_set_gmres_history(gmi_ctx *ctx, int m);
                                        in HBM), M^-1 K and (K^T M^-1 K)^-1,
                                        computed once by block Gauss-Jordan */
 /* Selects the preconditioner (set-up cost and memory for BODY_INVERSE:
- * 2 (3 nb)^3 flops and 8 (3 nb)^2 bytes per swimmer).  GMRES keeps its
- * operator and tolerance.  No reference counterpart beyond the default. */
+ * 2 (3 nb)^3 flops and 8 (3 nb)^2 bytes per swimmer, plus the preconditioned
+ * Arnoldi directions, 8 (maxit + 1) rows bytes).  GMRES keeps its operator
+ * and tolerance.  BODY_INVERSE applies M^-1 by a symmetric product over its
+ * lower triangle and checks convergence after every Arnoldi step (chunk 1;
+ * SELF_MOBILITY restores chunk 6; gmi_sim_set_graphs can change it after).
+ * No reference counterpart beyond the default. */
 int gmi_sim_set_body_preconditioner(gmi_ctx *ctx, int kind);
 
 /* ---- seeding points on the unit sphere (seeding.cu:13-389) ------------- */
