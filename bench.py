@@ -667,8 +667,9 @@ def main():
             "fp64_steps_per_s": fp,
             "learned_mixed_steps_per_s": st["learned_inverse_mixed"]["steps_per_s"],
             "speedup": st["learned_inverse_mixed"]["steps_per_s"] / fp,
-            "path": "network Broyden warm start + FP32-Kahan pairs in relaxed GMRES with FP64 "
-                    "refinement + exact body-frame GMRES preconditioner",
+            "path": "network Broyden warm start + FP32-Kahan segment pair blocks + exact "
+                    "body-frame GMRES preconditioner (FP64 Krylov products, one Arnoldi step "
+                    "per solve)",
             "attribution_speedups_vs_fp64": {
                 k: st[k]["steps_per_s"] / fp for k in
                 ("mixed", "fitted_fp64", "learned_fp64", "learned_mixed", "inverse_fp64",

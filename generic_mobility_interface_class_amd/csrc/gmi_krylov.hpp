@@ -300,11 +300,32 @@ __global__ __launch_bounds__(256) void k_givens(const double *norm_part, int npa
   }
 }
 
-__global__ void k_gmres_reset(int *flags) {
-  flags[F_DONE] = 0;
-  flags[F_ITS] = 0;
-  flags[F_SKIP64] = 0;
-  flags[F_SKIP32] = 1;
+// GMRES start on the device, after k_norms left scal[0] = ||rhs|| and
+// scal[1] = ||r0||: a zero right-hand side gives x = 0; a residual within
+// the tolerance needs no Arnoldi step (F_DONE); a warm start worse than zero
+// or not finite is replaced by zero, r0 = rhs (mobility_solver.cpp:1060-1156
+// keeps the previous solution otherwise).  r0 is normalised, its norm goes
+// to beta[0] and scal[2], and the flags are reset.
+__global__ void k_gmres_start(double *x, double *r, const double *rhs, int n, double *scal,
+                              double *beta, int *flags, double tol) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const double nrhs = scal[0], b0 = scal[1];
+  const bool zero = nrhs == 0.0;
+  const bool conv = zero || fabs(b0) <= tol * nrhs;
+  const bool reset = !conv && (!isfinite(b0) || fabs(b0) > nrhs);
+  const double bb = reset ? nrhs : b0;
+  if (e < n) {
+    if (zero || reset) x[e] = 0.0;
+    if (!conv) r[e] = (reset ? rhs[e] : r[e]) / bb;
+  }
+  if (e == 0) {
+    scal[2] = bb;
+    beta[0] = bb;
+    flags[F_DONE] = conv ? 1 : 0;
+    flags[F_ITS] = 0;
+    flags[F_SKIP64] = 0;
+    flags[F_SKIP32] = 1;
+  }
 }
 
 // back substitution for y (mobility_solver.cpp:1236-1257) by one wave:
@@ -415,41 +436,14 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   const int nb = nblk(n, 256);
   int rc;
   *its = 0;
-  // r = rhs - A x ; norms
+  // r = rhs - A x ; norms ; the start decisions on the device (no host
+  // round trip: a solve that needs no step runs one flag-skipped chunk)
   double *r = s->Qk;
   if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
   hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
-  if ((rc = launch_ok(c, "gmres init"))) return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal, s->scal, 2 * sizeof(double),
-                                        hipMemcpyDeviceToHost, c->stream), "scal D2H")))
-    return rc;
-  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
-  const double nrhs = s->h_scal[0];
-  double b0 = s->h_scal[1];
-  if (nrhs == 0.0) {
-    return hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x");
-  }
-  if (std::fabs(b0) <= tol * nrhs) return GMI_OK;
-  // a warm start worse than zero (mobility_solver.cpp:1060-1156 keeps the
-  // previous solution) -- or a non-finite one -- is replaced by zero
-  if (!std::isfinite(b0) || std::fabs(b0) > nrhs) {
-    if ((rc = hip_check(c, hipMemsetAsync(s->x, 0, sizeof(double) * n, c->stream), "zero x")))
-      return rc;
-    if ((rc = hip_check(c, hipMemcpyAsync(r, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                          c->stream), "copy rhs")))
-      return rc;
-    b0 = nrhs;
-  }
-  s->h_scal[2] = b0;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->scal + 2, s->h_scal + 2, sizeof(double),
-                                        hipMemcpyHostToDevice, c->stream), "beta0")))
-    return rc;
-  if ((rc = hip_check(c, hipMemcpyAsync(s->beta, s->h_scal + 2, sizeof(double),
-                                        hipMemcpyHostToDevice, c->stream), "beta0")))
-    return rc;
-  hipLaunchKernelGGL(k_scale_by, dim3(nb), dim3(256), 0, c->stream, r, n, s->scal, 2);
-  hipLaunchKernelGGL(k_gmres_reset, dim3(1), dim3(1), 0, c->stream, s->flags);
+  hipLaunchKernelGGL(k_gmres_start, dim3(nb), dim3(256), 0, c->stream, s->x, r, s->rhs, n, s->scal,
+                     s->beta, s->flags, tol);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
   // scal[0] = ||rhs|| is read by k_givens
@@ -553,7 +547,11 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
 template <class Op>
 int krylov_solve(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, int precision, double tol,
                  int *its) {
-  if (precision == GMI_FP64) return krylov_gmres(c, s, op, n, maxit, false, tol, its);
+  // an operator whose preconditioner is (nearly) exact keeps its Krylov
+  // products in FP64: GMRES then takes one or two steps, and the refinement
+  // pass would cost more than the FP32 products save
+  if (precision == GMI_FP64 || op.fp64_krylov())
+    return krylov_gmres(c, s, op, n, maxit, false, tol, its);
   const int nb = nblk(n, 256);
   int rc, total = 0;
   for (int cycle = 0; cycle < 4; cycle++) {

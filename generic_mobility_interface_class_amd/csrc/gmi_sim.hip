@@ -1460,6 +1460,8 @@ struct SimOp {
     // the body-frame preconditioner leaves no K^T shares behind
     return sys_mult(c, s, z, out, mode, done, zpart_ready && !s->dense);
   }
+  // the exact body-frame preconditioner: FP64 Krylov products
+  bool fp64_krylov() const { return s->dense != nullptr; }
   int precond(double *in, double *out, const int *skip, int norm_slot) {
     if (s->dense) return dense_apply(c, s, in, out, skip, norm_slot);
     return gmi::precond(c, s, in, out, skip, norm_slot);
