@@ -257,11 +257,26 @@ __global__ __launch_bounds__(256) void k_dense_symv_tiles(const double *__restri
   const double *A = Minv + moff[n];
   const double *tv = t + voff[n];
   const int th = threadIdx.x, r = th & (kSym - 1), g = th >> 6;
-  // column c0 + c holds rows r0 .. r0 + nr - 1 contiguously
-  for (int c = g; c < kSym; c += 4)
-    L[c * LD + r] = (r < nr && c < nc) ? A[size_t(c0 + c) * rows + r0 + r] : 0.0;
-  if (th < kSym) xi[th] = th < nr ? tv[r0 + th] : 0.0;
-  else if (th < 2 * kSym) xj[th - kSym] = th - kSym < nc ? tv[c0 + th - kSym] : 0.0;
+  // column c0 + c holds rows r0 .. r0 + nr - 1 contiguously.  All loads of
+  // a thread are issued before its LDS stores (an index outside the tile is
+  // clamped to the tile's first element and its value dropped), so each
+  // lane keeps kSym / 4 loads in flight.
+  constexpr int NL = kSym / 4;
+  double v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int c = g + 4 * k;
+    const bool in = r < nr && c < nc;
+    v[k] = A[in ? size_t(c0 + c) * rows + r0 + r : size_t(c0) * rows + r0];
+    v[k] = in ? v[k] : 0.0;
+  }
+  const int xk = th & (kSym - 1);
+  const bool xin = th < kSym ? xk < nr : xk < nc;
+  const double xv = tv[xin ? (th < kSym ? r0 : c0) + xk : r0];
+#pragma unroll
+  for (int k = 0; k < NL; k++) L[(g + 4 * k) * LD + r] = v[k];
+  if (th < kSym) xi[xk] = xin ? xv : 0.0;
+  else if (th < 2 * kSym) xj[xk] = xin ? xv : 0.0;
   __syncthreads();
   double s = 0.0;
   if (g < 2) {                 // row partial, columns g*32 .. g*32+31
@@ -289,14 +304,23 @@ __global__ __launch_bounds__(256) void k_dense_symv_sum(const SymRow *__restrict
   __shared__ double red[4][kSym];
   const SymRow R = rl[blockIdx.x];
   const int th = threadIdx.x, k = th & (kSym - 1), g = th >> 6;
-  double s = 0.0;
-  for (int q = g; q < R.nbl; q += 4) {
-    // term q: tile (K, q) row partial for q <= K, tile (q, K) column partial above
+  // term q: tile (K, q) row partial for q <= K, tile (q, K) column partial
+  // above; four independent running sums per lane keep four loads in flight
+  auto term = [&](int q) -> double {
     const size_t tile = q <= R.K ? size_t(R.tile0) + size_t(R.K) * (R.K + 1) / 2 + q
                                  : size_t(R.tile0) + size_t(q) * (q + 1) / 2 + R.K;
-    s += part[size_t(2 * kSym) * tile + (q <= R.K ? 0 : kSym) + k];
+    return part[size_t(2 * kSym) * tile + (q <= R.K ? 0 : kSym) + k];
+  };
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = g;
+  for (; q + 12 < R.nbl; q += 16) {
+    s0 += term(q);
+    s1 += term(q + 4);
+    s2 += term(q + 8);
+    s3 += term(q + 12);
   }
-  red[g][k] = s;
+  for (; q < R.nbl; q += 4) s0 += term(q);
+  red[g][k] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   const int row = R.K * kSym + th;
   if (th < kSym && row < 3 * sw_nblob[R.sw])
