@@ -13,6 +13,8 @@ wall = 1, pinned in test_oracle_carpet.py), through the C ABI.
   configuration recovers phase speed 2 pi; the 20,480-segment carpet steps
   with a true residual within the GMRES tolerance (1e-8 at the iteration
   cap, whose iterate the reference accepts).
+* A zero right-hand side (zero frequency) gives the zero solution without
+  an Arnoldi step.
 * Refusals: mixed precision, a free body, and switching the wall off while
   a carpet is stepping.
 """
@@ -136,6 +138,25 @@ def test_dynamic_phase_and_full_carpet(gpu, name):
         gf.integrate_bodies()
         gf.end_of_step(nt)
     assert np.all(np.isfinite(gf.state()["x"]))
+
+
+def test_zero_right_hand_side_gives_zero_solution(gpu):
+    """Prescribed phases at zero frequency: the right-hand side omega k is
+    zero, and the GMRES start (decided on the device) returns x = 0 with no
+    Arnoldi step; the same carpet at the usual frequency still matches the
+    oracle."""
+    base = variant_cfg("rectangular", geometry.BEAT_BUILD, False, nfil=4, nseg=8)
+    g = capi.Cilia(dataclasses.replace(base, omega0=np.zeros(base.nf)))
+    g.initial_guess(0)
+    rc, its = g.compute_velocities(0)
+    assert rc == 0 and its == 0
+    assert np.abs(g.rhs()).max() == 0.0
+    assert np.abs(g.solution()["f_segs"]).max() == 0.0
+    o, h = orc.Cilia(base), capi.Cilia(base)
+    for c in (o, h):
+        c.initial_guess(0)
+        assert c.compute_velocities(0)[0] == 0
+    assert rel(h.solution()["f_segs"], o.solution()["f_segs"]) <= 1e-6
 
 
 def test_carpet_refusals(gpu):
