@@ -723,25 +723,33 @@ __global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
   }
 }
 
+// Split partials of one (tile, component) per workgroup (blockIdx.y is the
+// component): each lane keeps four running sums over the splits, so it has
+// four loads in flight, and the sums are combined in a fixed order.
 __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
   if (a.skip && *a.skip) return;
   const TileDesc T = a.tiles[blockIdx.x];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, c = blockIdx.y;
   if (T.nsplit == 1 || tid >= T.nt) return;
   const bool seg_t = (T.species == 0);
   const int nc = (seg_t && !a.tt_only) ? 6 : 3;
+  if (c >= nc) return;
   double *V = seg_t ? a.Vs : a.Vb;
   const int ld = seg_t ? a.lds : a.ldb;
-  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const double *P = a.part + size_t(T.part0) * 6 * kBlock + tid;
-#pragma unroll 4
-  for (int q = 0; q < T.nsplit; ++q)
-    for (int c = 0; c < 6; ++c)
-      if (c < nc) s[c] += P[size_t(q) * 6 * kBlock + c * kBlock];
-  for (int c = 0; c < nc; ++c) {
-    double *v = V + c * ld + T.t0 + tid;
-    *v = a.accumulate ? *v + s[c] : s[c];
+  const double *P = a.part + size_t(T.part0) * 6 * kBlock + c * kBlock + tid;
+  const size_t stride = size_t(6) * kBlock;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = 0;
+  for (; q + 3 < T.nsplit; q += 4) {
+    s0 += P[stride * q];
+    s1 += P[stride * (q + 1)];
+    s2 += P[stride * (q + 2)];
+    s3 += P[stride * (q + 3)];
   }
+  for (; q < T.nsplit; ++q) s0 += P[stride * q];
+  const double s = (s0 + s1) + (s2 + s3);
+  double *v = V + c * ld + T.t0 + tid;
+  *v = a.accumulate ? *v + s : s;
 }
 
 // Steric barrier (cuda_functions.cu:961-1157).  One lane per target; the
@@ -875,7 +883,7 @@ hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp3
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
   if (ntiles_reduce > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
     e = hipGetLastError();
   }
   return e;
@@ -892,7 +900,7 @@ hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hip
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
   if (ntiles_reduce > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
     e = hipGetLastError();
   }
   return e;
