@@ -210,52 +210,69 @@ __device__ d3 strain_twist(const SimDev &a, int f, double t) {
   return {0.0, 0.0, a.camp * sin(2.0 * kPi * t / a.cper + a.phase[f])};
 }
 
-__global__ void k_fil_forces(SimDev a, double t) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= a.Nf) return;
-  const int S = a.S, g0 = f * S;
-  const d3 st = strain_twist(a, f, t);
+// One wave per filament (NSEG <= 40 < 64): lane j owns segment j and the
+// link (j, j+1).  The link moments run in parallel; each segment's force and
+// torque is assembled from its two links in the order of the sequential loop
+// over links (the same operations, so the same rounding), and lane 0 sums
+// the filament's share of the body balance in segment order.
+constexpr int kFilLanes = 64;
+__global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t) {
+  __shared__ double sl[3 * kFilLanes], sM[3 * kFilLanes], sF[6 * kFilLanes];
+  const int f = blockIdx.x, j = threadIdx.x;
+  const int S = a.S, g0 = f * S, ld = a.lds;
   const double *L = a.lam + size_t(f) * 9 * S;
   double *F = a.Fint;
-  const int ld = a.lds;
-  for (int c = 0; c < 6; c++) F[c * ld + g0] = 0.0;
-  dq q1 = seg_q(a, g0);
-  d3 t1 = qtangent(q1);
-  for (int i = 0; i < S - 1; i++) {
-    const int gi = g0 + i, gn = gi + 1;
-    const d3 lam = rowvec(L + 3 * i);
-    const dq q2 = seg_q(a, gn);
-    const d3 t2 = qtangent(q2);
-    d3 Fi = {F[gi], F[ld + gi], F[2 * ld + gi]};
-    d3 Ti = {F[3 * ld + gi], F[4 * ld + gi], F[5 * ld + gi]};
-    Fi = Fi - lam;
-    const d3 Fn = lam;
-    Ti = Ti - (0.5 * a.DL) * cross3(t1, lam);
-    d3 Tn = (-0.5 * a.DL) * cross3(t2, lam);
-    const d3 M = mv(rotm(qmid(q1, q2)), bfm(a, q1, q2, st));
-    Ti = Ti + M;
-    Tn = Tn - M;
-    F[gi] = Fi.x; F[ld + gi] = Fi.y; F[2 * ld + gi] = Fi.z;
-    F[3 * ld + gi] = Ti.x; F[4 * ld + gi] = Ti.y; F[5 * ld + gi] = Ti.z;
-    F[gn] = Fn.x; F[ld + gn] = Fn.y; F[2 * ld + gn] = Fn.z;
-    F[3 * ld + gn] = Tn.x; F[4 * ld + gn] = Tn.y; F[5 * ld + gn] = Tn.z;
-    q1 = q2;
-    t1 = t2;
+  dq qj = {1.0, 0.0, 0.0, 0.0};
+  d3 tj = {0, 0, 0}, lam = {0, 0, 0}, M = {0, 0, 0};
+  if (j < S) {
+    qj = seg_q(a, g0 + j);
+    tj = qtangent(qj);
   }
-  const d3 teth = rowvec(a.teth + 9 * f);
-  const d3 ct = dexpinv_T(ld3(a.U, ld, g0), rowvec(a.clamp + 9 * f));
-  F[g0] += teth.x; F[ld + g0] += teth.y; F[2 * ld + g0] += teth.z;
-  F[3 * ld + g0] += ct.x; F[4 * ld + g0] += ct.y; F[5 * ld + g0] += ct.z;
+  if (j < S - 1) {
+    const d3 st = strain_twist(a, f, t);
+    lam = rowvec(L + 3 * j);
+    const dq qn = seg_q(a, g0 + j + 1);
+    M = mv(rotm(qmid(qj, qn)), bfm(a, qj, qn, st));
+    sa3(sl + 3 * j, lam);
+    sa3(sM + 3 * j, M);
+  }
+  __syncthreads();
+  if (j < S) {
+    // link j-1 sets segment j, link j then adds its terms
+    d3 Fj = {0, 0, 0}, Tj = {0, 0, 0};
+    if (j > 0) {
+      const d3 lp = ga3(sl + 3 * (j - 1));
+      Fj = lp;
+      Tj = (-0.5 * a.DL) * cross3(tj, lp);
+      Tj = Tj - ga3(sM + 3 * (j - 1));
+    }
+    if (j < S - 1) {
+      Fj = Fj - lam;
+      Tj = Tj - (0.5 * a.DL) * cross3(tj, lam);
+      Tj = Tj + M;
+    }
+    if (j == 0) {
+      const d3 teth = rowvec(a.teth + 9 * f);
+      const d3 ct = dexpinv_T(ld3(a.U, ld, g0), rowvec(a.clamp + 9 * f));
+      Fj = Fj + teth;
+      Tj = Tj + ct;
+    }
+    const int g = g0 + j;
+    F[g] = Fj.x; F[ld + g] = Fj.y; F[2 * ld + g] = Fj.z;
+    F[3 * ld + g] = Tj.x; F[4 * ld + g] = Tj.y; F[5 * ld + g] = Tj.z;
+    sa3(sF + 6 * j, Fj);
+    sa3(sF + 6 * j + 3, Tj);
+  }
+  __syncthreads();
+  if (j != 0) return;
   // this filament's share of the free-swimming balance
   const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
   const d3 bx = ga3(b + BX);
   d3 sf = {0, 0, 0}, stq = {0, 0, 0};
-  for (int j = 0; j < S; j++) {
-    const int g = g0 + j;
-    const d3 fs = {F[g], F[ld + g], F[2 * ld + g]};
-    const d3 ts = {F[3 * ld + g], F[4 * ld + g], F[5 * ld + g]};
+  for (int k = 0; k < S; k++) {
+    const d3 fs = ga3(sF + 6 * k), ts = ga3(sF + 6 * k + 3);
     sf = sf + fs;
-    stq = stq + ts + cross3(ld3(a.X, ld, g) - bx, fs);
+    stq = stq + ts + cross3(ld3(a.X, ld, g0 + k) - bx, fs);
   }
   double *o = a.fil_bf + 6 * f;
   o[0] = sf.x; o[1] = sf.y; o[2] = sf.z; o[3] = stq.x; o[4] = stq.y; o[5] = stq.z;
@@ -1339,23 +1356,41 @@ __global__ void k_body_update(SimDev a, const double *upd) {
   sa4(b + BQ, qmul(lie_exp(ga3(b + BU)), ga4(b + BQM1)));
 }
 
-__global__ void k_fil_update(SimDev a, const double *upd) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= a.Nf) return;
+// One wave per filament: lane 0 takes the base from the body
+// (accept_state, filament.cpp:438-463) and the tether and clamp multipliers,
+// lane j > 0 its multiplier and segment update; the tangents then go through
+// LDS to lane 0, which lays the robot arm out in segment order.  (The robot
+// arm of accept_state is not repeated: the one after the updates rewrites
+// every position.)
+__global__ __launch_bounds__(kFilLanes) void k_fil_update(SimDev a, const double *upd) {
+  __shared__ double stn[3 * kFilLanes];
+  const int f = blockIdx.x, j = threadIdx.x;
   const int S = a.S, g0 = f * S;
-  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
-  accept_state(a, f, base_position(a, f, b), ga3(b + BU));
   const double *u = upd + fil_off(a, f);
-  double *T = a.teth + 9 * f;
-  for (int k = 0; k < 3; k++) T[k] += u[k];
-  double *L = a.lam + size_t(f) * 9 * S;
-  for (int i = 1; i < S; i++) {
-    for (int k = 0; k < 3; k++) L[3 * (i - 1) + k] += u[3 * i + k];
-    seg_update(a, g0 + i, rowvec(u + 3 * (S + i)));
+  if (j == 0) {
+    const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+    st3(a.X, a.lds, g0, base_position(a, f, b));
+    seg_update(a, g0, ga3(b + BU) - ld3(a.U, a.lds, g0));
+    double *T = a.teth + 9 * f;
+    for (int k = 0; k < 3; k++) T[k] += u[k];
+    double *Cl = a.clamp + 9 * f;
+    for (int k = 0; k < 3; k++) Cl[k] += u[3 * S + k];
+  } else if (j < S) {
+    double *L = a.lam + size_t(f) * 9 * S;
+    for (int k = 0; k < 3; k++) L[3 * (j - 1) + k] += u[3 * j + k];
+    seg_update(a, g0 + j, rowvec(u + 3 * (S + j)));
   }
-  double *Cl = a.clamp + 9 * f;
-  for (int k = 0; k < 3; k++) Cl[k] += u[3 * S + k];
-  robot_arm(a, f);
+  if (j < S) sa3(stn + 3 * j, seg_t(a, g0 + j));
+  __syncthreads();
+  if (j != 0) return;
+  d3 x = ld3(a.X, a.lds, g0);
+  d3 t1 = ga3(stn);
+  for (int k = 1; k < S; k++) {
+    const d3 t2 = ga3(stn + 3 * k);
+    x = x + (0.5 * a.DL) * (t1 + t2);
+    st3(a.X, a.lds, g0 + k, x);
+    t1 = t2;
+  }
 }
 
 }  // namespace
@@ -1945,7 +1980,7 @@ int gmi_forces_and_torques(gmi_ctx *c, int nt) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   const double t = (nt + 1) * s->p.dt;
-  hipLaunchKernelGGL(k_fil_forces, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, t);
+  hipLaunchKernelGGL(k_fil_forces, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, t);
   hipLaunchKernelGGL(k_body_forces, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d);
   return launch_ok(c, "forces");
 }
@@ -2015,7 +2050,7 @@ int gmi_apply_update(gmi_ctx *c) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, s->upd);
-  hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, s->upd);
+  hipLaunchKernelGGL(k_fil_update, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, s->upd);
   return launch_ok(c, "update");
 }
 
@@ -2089,7 +2124,7 @@ static int broyden_solve(gmi_ctx *c, SimHost *s, int nt, int max_iter, bool chec
 
 static void apply_update_vector(gmi_ctx *c, SimHost *s, const double *v) {
   hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, v);
-  hipLaunchKernelGGL(k_fil_update, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d, v);
+  hipLaunchKernelGGL(k_fil_update, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, v);
 }
 
 int gmi_step(gmi_ctx *c, int nt, int *broyden_iters, int *gmres_iters) {
