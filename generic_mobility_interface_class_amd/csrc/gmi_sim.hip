@@ -1893,7 +1893,15 @@ int gmi_sim_set_precision(gmi_ctx *c, int precision) {
   if (!c || !sim_of(c)) return GMI_ERR_STATE;
   if (precision != GMI_FP64 && precision != GMI_FP32_KAHAN)
     return fail(c, GMI_ERR_ARG, "unknown precision");
-  sim_of(c)->precision = precision;
+  SimHost *s = sim_of(c);
+  if (s->precision != precision) {
+    // captured chunks hold the old precision's kernels (the body-frame
+    // preconditioner picks its matrix copy by precision at capture)
+    int rc;
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+    s->drop_graphs();
+  }
+  s->precision = precision;
   return GMI_OK;
 }
 
