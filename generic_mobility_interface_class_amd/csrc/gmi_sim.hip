@@ -1895,8 +1895,7 @@ int gmi_sim_set_precision(gmi_ctx *c, int precision) {
     return fail(c, GMI_ERR_ARG, "unknown precision");
   SimHost *s = sim_of(c);
   if (s->precision != precision) {
-    // captured chunks hold the old precision's kernels (the body-frame
-    // preconditioner picks its matrix copy by precision at capture)
+    // captured Arnoldi chunks hold the kernels chosen for the old precision
     int rc;
     if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
     s->drop_graphs();
