@@ -88,13 +88,13 @@ __device__ bool gj_small(double *A, int n) {
 __device__ dq seg_q(const SimDev &a, int g) { return ld4(a.Q, a.lds, g); }
 __device__ d3 seg_t(const SimDev &a, int g) { return qtangent(seg_q(a, g)); }
 
-// filament.cpp:416-436
-__device__ void robot_arm(const SimDev &a, int f) {
-  const int g0 = f * a.S;
+// filament.cpp:416-436: positions from the base along the segment
+// tangents (tangents of the filament's segments in tn[3 S], e.g. in LDS)
+__device__ void robot_arm(const SimDev &a, int g0, const double *tn) {
   d3 x = ld3(a.X, a.lds, g0);
-  d3 t1 = seg_t(a, g0);
+  d3 t1 = ga3(tn);
   for (int j = 1; j < a.S; j++) {
-    const d3 t2 = seg_t(a, g0 + j);
+    const d3 t2 = ga3(tn + 3 * j);
     x = x + (0.5 * a.DL) * (t1 + t2);
     st3(a.X, a.lds, g0 + j, x);
     t1 = t2;
@@ -113,12 +113,12 @@ __device__ d3 base_position(const SimDev &a, int f, const double *b) {
   return ga3(b + BX) + mv(R, rowvec(a.fil_ref + 3 * f));
 }
 
-// filament.cpp:438-463
-__device__ void accept_state(const SimDev &a, int f, d3 xb, d3 ub) {
-  const int g0 = f * a.S;
-  st3(a.X, a.lds, g0, xb);
-  seg_update(a, g0, ub - ld3(a.U, a.lds, g0));
-  robot_arm(a, f);
+// filament.cpp:438-463 (accept_state_from_rigid_body) without its robot arm:
+// the base at the body, segment 0 turned with the body
+__device__ void accept_base(const SimDev &a, int f, int g0) {
+  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
+  st3(a.X, a.lds, g0, base_position(a, f, b));
+  seg_update(a, g0, ga3(b + BU) - ld3(a.U, a.lds, g0));
 }
 
 // -------------------------------------------------------------------------
@@ -148,26 +148,32 @@ __global__ void k_body_initial_guess(SimDev a, int nt) {
   for (int k = 0; k < 9; k++) b[BQI + k] = R.a[k];
 }
 
-__device__ void extrap(double *v, int n, int nt) {
-  // v[0..n) value, v[n..2n) m1, v[2n..3n) m2
-  for (int k = 0; k < n; k++) {
-    double &x = v[k], &m1 = v[n + k], &m2 = v[2 * n + k];
-    if (nt == 1) {
-      m1 = x;
-    } else if (nt == 2) {
-      m2 = m1; m1 = x; x = 2.0 * m1 - m2;
-    } else if (nt > 2) {
-      const double t = 3.0 * (x - m1) + m2;
-      m2 = m1; m1 = x; x = t;
-    }
+// element k of a [value | m1 | m2] record of n-vectors: the linear (second
+// step) or quadratic extrapolation of the value, history shifted
+__device__ void extrap_one(double *v, int n, int k, int nt) {
+  double &x = v[k], &m1 = v[n + k], &m2 = v[2 * n + k];
+  if (nt == 1) {
+    m1 = x;
+  } else if (nt == 2) {
+    m2 = m1; m1 = x; x = 2.0 * m1 - m2;
+  } else if (nt > 2) {
+    const double t = 3.0 * (x - m1) + m2;
+    m2 = m1; m1 = x; x = t;
   }
 }
 
-__global__ void k_fil_initial_guess(SimDev a, int nt) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= a.Nf) return;
-  const int g0 = f * a.S;
-  for (int j = 0; j < a.S; j++) {
+constexpr int kFilLanes = 64;  // one wave per filament (NSEG <= 40)
+
+// One wave per filament: lane j shifts and extrapolates segment j's state
+// and updates its frame; lane 0 then takes the base from the body
+// (accept_base) and lays the robot arm out in segment order from the
+// tangents in LDS; the multiplier extrapolations are elementwise across the
+// lanes.
+__global__ __launch_bounds__(kFilLanes) void k_fil_initial_guess(SimDev a, int nt) {
+  __shared__ double stn[3 * kFilLanes];
+  const int f = blockIdx.x, j = threadIdx.x;
+  const int S = a.S, g0 = f * S;
+  if (j < S) {
     const int g = g0 + j;
     const d3 x = ld3(a.X, a.lds, g);
     st3(a.Xm2, a.lds, g, ld3(a.Xm1, a.lds, g));
@@ -185,13 +191,16 @@ __global__ void k_fil_initial_guess(SimDev a, int nt) {
     st4(a.Qm1, a.lds, g, q);
     st4(a.Q, a.lds, g, qmul(lie_exp(u), q));
   }
-  const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
-  accept_state(a, f, base_position(a, f, b), ga3(b + BU));
+  if (j == 0) accept_base(a, f, g0);
+  if (j < S) sa3(stn + 3 * j, seg_t(a, g0 + j));
+  __syncthreads();
+  if (j == 0) robot_arm(a, g0, stn);
   // lambda: [value | m1 | m2] blocks of 3(S-1) inside a 9S record
-  double *L = a.lam + size_t(f) * 9 * a.S;
-  extrap(L, 3 * (a.S - 1), nt);
-  extrap(a.teth + 9 * f, 3, nt);
-  extrap(a.clamp + 9 * f, 3, nt);
+  const int nl = 3 * (S - 1);
+  double *L = a.lam + size_t(f) * 9 * S;
+  for (int k = j; k < nl; k += kFilLanes) extrap_one(L, nl, k, nt);
+  if (j < 3) extrap_one(a.teth + 9 * f, 3, j, nt);
+  else if (j < 6) extrap_one(a.clamp + 9 * f, 3, j - 3, nt);
 }
 
 // -------------------------------------------------------------------------
@@ -215,7 +224,6 @@ __device__ d3 strain_twist(const SimDev &a, int f, double t) {
 // torque is assembled from its two links in the order of the sequential loop
 // over links (the same operations, so the same rounding), and lane 0 sums
 // the filament's share of the body balance in segment order.
-constexpr int kFilLanes = 64;
 __global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t) {
   __shared__ double sl[3 * kFilLanes], sM[3 * kFilLanes], sF[6 * kFilLanes];
   const int f = blockIdx.x, j = threadIdx.x;
@@ -1356,21 +1364,19 @@ __global__ void k_body_update(SimDev a, const double *upd) {
   sa4(b + BQ, qmul(lie_exp(ga3(b + BU)), ga4(b + BQM1)));
 }
 
-// One wave per filament: lane 0 takes the base from the body
-// (accept_state, filament.cpp:438-463) and the tether and clamp multipliers,
-// lane j > 0 its multiplier and segment update; the tangents then go through
-// LDS to lane 0, which lays the robot arm out in segment order.  (The robot
-// arm of accept_state is not repeated: the one after the updates rewrites
-// every position.)
+// One wave per filament: lane 0 takes the base from the body (accept_base)
+// and the tether and clamp multipliers, lane j > 0 its multiplier and segment
+// update; the tangents then go through LDS to lane 0, which lays the robot
+// arm out in segment order.  (The reference's accept_state_from_rigid_body
+// also lays the arm out before the updates; the layout after them rewrites
+// every position, so it is done once.)
 __global__ __launch_bounds__(kFilLanes) void k_fil_update(SimDev a, const double *upd) {
   __shared__ double stn[3 * kFilLanes];
   const int f = blockIdx.x, j = threadIdx.x;
   const int S = a.S, g0 = f * S;
   const double *u = upd + fil_off(a, f);
   if (j == 0) {
-    const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
-    st3(a.X, a.lds, g0, base_position(a, f, b));
-    seg_update(a, g0, ga3(b + BU) - ld3(a.U, a.lds, g0));
+    accept_base(a, f, g0);
     double *T = a.teth + 9 * f;
     for (int k = 0; k < 3; k++) T[k] += u[k];
     double *Cl = a.clamp + 9 * f;
@@ -1382,15 +1388,7 @@ __global__ __launch_bounds__(kFilLanes) void k_fil_update(SimDev a, const double
   }
   if (j < S) sa3(stn + 3 * j, seg_t(a, g0 + j));
   __syncthreads();
-  if (j != 0) return;
-  d3 x = ld3(a.X, a.lds, g0);
-  d3 t1 = ga3(stn);
-  for (int k = 1; k < S; k++) {
-    const d3 t2 = ga3(stn + 3 * k);
-    x = x + (0.5 * a.DL) * (t1 + t2);
-    st3(a.X, a.lds, g0 + k, x);
-    t1 = t2;
-  }
+  if (j == 0) robot_arm(a, g0, stn);
 }
 
 }  // namespace
@@ -1978,7 +1976,7 @@ int gmi_initial_guess(gmi_ctx *c, int nt) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   hipLaunchKernelGGL(k_body_initial_guess, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d,
                      nt);
-  hipLaunchKernelGGL(k_fil_initial_guess, dim3(nblk(s->d.Nf, 64)), dim3(64), 0, c->stream, s->d,
+  hipLaunchKernelGGL(k_fil_initial_guess, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d,
                      nt);
   return launch_ok(c, "initial guess");
 }
