@@ -989,7 +989,7 @@ constexpr int kGjThreads = 1024;
 __global__ __launch_bounds__(kGjThreads) void k_gj_invert_lds(SimDev a) {
   extern __shared__ double lds_gj[];
   __shared__ int sp;
-  __shared__ double spv;
+  __shared__ double spv, spk, skk;  // |pivot|, pivot A(p,k), A(k,k) before the swap
   const int f = blockIdx.x;
   const int n = 6 * a.S;
   const int t = threadIdx.x;
@@ -1015,7 +1015,14 @@ __global__ __launch_bounds__(kGjThreads) void k_gj_invert_lds(SimDev a) {
         const int i2 = __shfl_xor(bi, o, 64);
         if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
       }
-      if (t == 0) { sp = bi; spv = best; }
+      if (t == 0) {
+        sp = bi;
+        spv = best;
+        if (bi < n) {
+          spk = A[size_t(bi) * n + k];
+          skk = A[size_t(k) * n + k];
+        }
+      }
     }
     __syncthreads();
     const int p = sp;
@@ -1023,20 +1030,19 @@ __global__ __launch_bounds__(kGjThreads) void k_gj_invert_lds(SimDev a) {
       if (t == 0) atomicOr(&a.flags[F_SING], 1);
       return;
     }
-    if (p != k) {
-      for (int j = t; j < n; j += kGjThreads) {
-        const double x = A[size_t(k) * n + j];
-        A[size_t(k) * n + j] = A[size_t(p) * n + j];
-        A[size_t(p) * n + j] = x;
-      }
-      if (t == 0) { const int x = piv[k]; piv[k] = piv[p]; piv[p] = x; }
+    // swap rows k and p, normalise the new row k and keep column k (as it is
+    // after the swap) in one pass: only rows k and p change here, and their
+    // column-k entries come from the pivot search
+    const double inv = 1.0 / spk;
+    if (t < n) {
+      const double akj = A[size_t(k) * n + t], apj = A[size_t(p) * n + t];
+      A[size_t(k) * n + t] = (t == k ? 1.0 : apj) * inv;
+      if (p != k) A[size_t(p) * n + t] = akj;
+    } else if (t >= 512 && t - 512 < n) {
+      const int i = t - 512;
+      colk[i] = i == k ? spk : (i == p ? skk : A[size_t(i) * n + k]);
     }
-    __syncthreads();
-    const double inv = 1.0 / A[size_t(k) * n + k];
-    for (int i = t; i < n; i += kGjThreads) colk[i] = A[size_t(i) * n + k];
-    __syncthreads();
-    for (int j = t; j < n; j += kGjThreads)
-      A[size_t(k) * n + j] = (j == k ? 1.0 : A[size_t(k) * n + j]) * inv;
+    if (t == 0 && p != k) { const int x = piv[k]; piv[k] = piv[p]; piv[p] = x; }
     __syncthreads();
     if (jc < n) {
       const double akj = A[size_t(k) * n + jc];
