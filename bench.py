@@ -35,6 +35,12 @@ from generic_mobility_interface_class_amd import capi, geometry  # noqa: E402
 
 FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
          "ss_wall_tt": 137, "ss_wall": 324}
+# Launch shape (target workgroups, minimum source chunk) of the time-stepping
+# legs.  The shape sets the order of the pair kernels' partial sums, so a
+# timed pick would change trajectories, and the networks trained on them, from
+# run to run.  This is the blob-blob autotuner's usual pick on these
+# workloads.
+STEP_LAUNCH_SHAPE = (4096, 128)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
@@ -159,8 +165,9 @@ def bench_step(args):
             sim.warmstart_init(hidden=128, seed=7)
             sim.warmstart_configure(apply=False, collect=(args.step_warmup // stride) * sim.Ns,
                                     stride=stride)
+        sim.call("gmi_set_launch_shape", *STEP_LAUNCH_SHAPE)
+        shape = STEP_LAUNCH_SHAPE
         sim.make_body_reference_matrices()
-        shape = sim.autotune_blocks(capi.GMI_BLOCK_BB, reps=10)
         sims[name] = sim
     t0 = time.perf_counter()
     for nt in range(args.step_warmup):
@@ -204,7 +211,7 @@ def bench_step(args):
         sim.close()
     out = {"workload": args.step_workload, "warmup_steps": args.step_warmup,
            "steps_per_window": args.step_steps, "window": f"{nt0}-{nt0 + args.step_steps - 1}",
-           "blob_blob_launch_shape": list(shape), **res, "warm_start": train}
+           "launch_shape": list(shape), **res, "warm_start": train}
     if "fp64" in res:
         for name in res:
             if name != "fp64":
@@ -294,8 +301,8 @@ def bench_rollout(args):
     for s in (learned, learned_inv):
         s.warmstart_configure(apply=False, collect=20000, stride=5)
     for s in (plain, learned, plain_inv, learned_inv):
+        s.call("gmi_set_launch_shape", *STEP_LAUNCH_SHAPE)
         s.make_body_reference_matrices()
-        s.autotune_blocks(capi.GMI_BLOCK_BB, reps=10)
     for s in (plain_inv, learned_inv):
         s.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
     for nt in range(300):
