@@ -953,6 +953,7 @@ struct CiliaOp {
     return launch_ok(c, "cilia K products");
   }
   bool fp64_krylov() const { return false; }
+  bool zero_start() const { return false; }
   int precond(double *in, double *out, const int *skip, int norm_slot) {
     CilDev &d = s->d;
     hipLaunchKernelGGL(k_cp1, dim3(s->ntiles), dim3(256), 0, c->stream, d, s->tiles, s->sw_tile0,

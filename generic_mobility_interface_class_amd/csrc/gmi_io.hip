@@ -47,11 +47,16 @@ template <typename T>
 int d2h_vec(gmi_ctx *c, std::vector<T> &v, const T *src, size_t n) {
   v.resize(n);
   if (n == 0) return GMI_OK;
+  // null-stream copies do not wait for the context's (non-blocking) stream
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   return hip_check(c, hipMemcpy(v.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost), "state D2H");
 }
 template <typename T>
 int h2d_vec(gmi_ctx *c, T *dst, const std::vector<T> &v) {
   if (v.empty()) return GMI_OK;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   return hip_check(c, hipMemcpy(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice),
                    "state H2D");
 }

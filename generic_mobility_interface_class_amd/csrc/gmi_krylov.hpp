@@ -439,8 +439,18 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   // r = rhs - A x ; norms ; the start decisions on the device (no host
   // round trip: a solve that needs no step runs one flag-skipped chunk)
   double *r = s->Qk;
-  if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
-  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp, n);
+  if (op.zero_start()) {
+    // an exact preconditioner solves from x0 = 0 in one step: no warm start,
+    // so no operator apply for the initial residual (r0 = rhs)
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 0.0, s->rhs, 0.0,
+                       nullptr, n);
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, 0.0, nullptr,
+                       n);
+  } else {
+    if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp,
+                       n);
+  }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
   hipLaunchKernelGGL(k_gmres_start, dim3(nb), dim3(256), 0, c->stream, s->x, r, s->rhs, n, s->scal,
                      s->beta, s->flags, tol);

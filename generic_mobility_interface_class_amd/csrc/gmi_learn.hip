@@ -524,7 +524,9 @@ int gmi_warmstart_init(gmi_ctx *c, int hidden, unsigned long long seed) {
   std::vector<float> stats(2 * (ni + no));
   for (int k = 0; k < ni; k++) { stats[k] = 0.f; stats[ni + k] = 1.f; }
   for (int k = 0; k < no; k++) { stats[2 * ni + k] = 0.f; stats[2 * ni + no + k] = 1.f; }
-  if ((rc = hip_check(c, hipMemcpy(L->p, P.data(), sizeof(float) * L->np, hipMemcpyHostToDevice), "params")))
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")) ||
+      (rc = hip_check(c, hipMemcpy(L->p, P.data(), sizeof(float) * L->np, hipMemcpyHostToDevice),
+                      "params")))
     return rc;
   if ((rc = hip_check(c, hipMemcpy(L->xs, stats.data(), sizeof(float) * stats.size(),
                                    hipMemcpyHostToDevice), "stats")))
@@ -545,7 +547,10 @@ int gmi_warmstart_params(gmi_ctx *c, float *get, const float *set) {
   if (!L) return c ? fail(c, GMI_ERR_STATE, "no warm start") : GMI_ERR_ARG;
   int rc;
   if (set) {
-    if ((rc = hip_check(c, hipMemcpy(L->p, set, sizeof(float) * L->np, hipMemcpyHostToDevice), "set")))
+    // null-stream copies do not wait for the context's (non-blocking) stream
+    if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")) ||
+        (rc = hip_check(c, hipMemcpy(L->p, set, sizeof(float) * L->np, hipMemcpyHostToDevice),
+                        "set")))
       return rc;
     hipLaunchKernelGGL(k_quantize, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->pq, L->np);
     if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;

@@ -179,8 +179,10 @@ int gmi_seed_sphere(gmi_ctx *c, int n, int kind, double r_over_l, int max_iterat
     for (int b = 0; b < nb; b++) biggest = std::fmax(biggest, hmax[b]);
     it++;
   }
-  rc = hip_check(c, hipMemcpy(unit_xyz, X, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
-                 "seed D2H");
+  rc = hip_check(c, hipStreamSynchronize(c->stream), "seed sync");
+  if (!rc)
+    rc = hip_check(c, hipMemcpy(unit_xyz, X, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+                   "seed D2H");
   cleanup();
   if (rc) return rc;
   if (iterations) *iterations = it;

@@ -1499,8 +1499,10 @@ struct SimOp {
     // the body-frame preconditioner leaves no K^T shares behind
     return sys_mult(c, s, z, out, mode, done, zpart_ready && !s->dense);
   }
-  // the exact body-frame preconditioner: FP64 Krylov products
+  // the exact body-frame preconditioner: FP64 Krylov products, and a start
+  // from zero (the inverse is exact, so a warm start saves no step)
   bool fp64_krylov() const { return s->dense != nullptr; }
+  bool zero_start() const { return s->dense != nullptr; }
   int precond(double *in, double *out, const int *skip, int norm_slot) {
     if (s->dense) return dense_apply(c, s, in, out, skip, norm_slot);
     return gmi::precond(c, s, in, out, skip, norm_slot);
@@ -1915,7 +1917,8 @@ int gmi_make_body_reference_matrices(gmi_ctx *c) {
   int rc;
   // (K^T K)^-1 per swimmer from its body-frame blob references (host, 6x6)
   std::vector<double> body(size_t(d.Nsw) * BREC);
-  if ((rc = hip_check(c, hipMemcpy(body.data(), d.body, sizeof(double) * body.size(),
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")) ||
+      (rc = hip_check(c, hipMemcpy(body.data(), d.body, sizeof(double) * body.size(),
                                    hipMemcpyDeviceToHost), "body D2H")))
     return rc;
   for (int n = 0; n < d.Nsw; n++) {
@@ -2269,6 +2272,9 @@ int gmi_get_broyden(gmi_ctx *c, int which, double *out) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   if (!out || which < 0 || which > 2) return fail(c, GMI_ERR_ARG, "bad arguments");
   const double *src = which == 0 ? s->err : (which == 1 ? s->nerr : s->upd);
+  // a null-stream copy does not wait for the context's (non-blocking) stream
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
   return hip_check(c, hipMemcpy(out, src, sizeof(double) * s->d.nbroy, hipMemcpyDeviceToHost),
                    "D2H");
 }
