@@ -193,6 +193,23 @@ def test_uneven_independent_swimmers(gpu):
         np.testing.assert_allclose(sg["body"], so["body"], rtol=0, atol=DL * cfg.tol)
 
 
+def test_independent_contexts_step_identically(gpu):
+    """With the same inputs and launch shape, two contexts step to the same
+    bits (no floating-point atomics; fixed reduction orders), here through
+    the mixed-precision path with the body-frame preconditioner."""
+    cfg = geometry.sim_workload("step_962", seed=1)
+    sims = [capi.Sim(cfg, precision=capi.GMI_FP32_KAHAN) for _ in range(2)]
+    for s in sims:
+        s.call("gmi_set_launch_shape", 4096, 128)
+        s.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+        s.make_body_reference_matrices()
+    for nt in range(4):
+        r = [s.step(nt) for s in sims]
+        assert r[0] == r[1] and r[0][0] == 0
+        st = [s.state() for s in sims]
+        assert np.array_equal(st[0]["x"], st[1]["x"]) and np.array_equal(st[0]["u"], st[1]["u"])
+
+
 def test_graph_replay_matches_eager(gpu):
     """The captured Arnoldi chunks are replayed across many solves with host
     work in between; every replay must equal the eager launches bit for bit."""
