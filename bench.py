@@ -124,17 +124,22 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
 def bench_step(args):
     """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer.
 
-    Five identical runs take one driving period (300 steps) of FP64 steps;
-    the two learned ones record every 5th step's (features, converged -
-    guess) rows and train the same network.  Then each run steps over the
+    Identical runs take one driving period (300 steps) of FP64 steps; the
+    learned ones record every 5th step's (features, converged - guess) rows
+    and each trains a network on its own rows.  Then each run steps over the
     same `step_steps` steps in its own mode:
-      fp64            plain FP64 (the reference method: extrapolated Broyden
-                      iterate, previous solution as the GMRES guess)
-      learned_mixed   network Broyden warm start + GMRES guess fitted on the
-                      last 16 right-hand sides + mixed precision
-      learned_fp64    network + fitted GMRES guess, FP64
-      fitted_fp64     fitted GMRES guess only, FP64
-      mixed           mixed precision only
+      fp64                  plain FP64 (the reference method: extrapolated
+                            Broyden iterate, previous solution as the GMRES
+                            guess)
+      learned_mixed         network Broyden warm start + GMRES guess fitted on
+                            the last 16 right-hand sides + mixed precision
+      learned_fp64          network + fitted GMRES guess, FP64
+      fitted_fp64           fitted GMRES guess only, FP64
+      mixed                 mixed precision only
+      inverse_fp64          exact body-frame GMRES preconditioner, FP64
+      inverse_mixed         the preconditioner + mixed precision
+      learned_inverse_fp64  network + the preconditioner, FP64
+      learned_inverse_mixed network + the preconditioner + mixed precision
     The fitted guess is on from the first step in the runs that use it.
     """
     cfg = geometry.sim_workload(args.step_workload, seed=1)
@@ -144,6 +149,8 @@ def bench_step(args):
              ("fitted_fp64", capi.GMI_FP64, False, True, False),
              ("mixed", capi.GMI_FP32_KAHAN, False, False, False),
              ("inverse_fp64", capi.GMI_FP64, False, False, True),
+             ("inverse_mixed", capi.GMI_FP32_KAHAN, False, False, True),
+             ("learned_inverse_fp64", capi.GMI_FP64, True, False, True),
              ("learned_inverse_mixed", capi.GMI_FP32_KAHAN, True, False, True))
     if args.step_modes:
         keep = args.step_modes.split(",")
@@ -539,7 +546,7 @@ def main():
     ap.add_argument("--step-warmup", type=int, default=300)
     ap.add_argument("--no-step", action="store_true")
     ap.add_argument("--step-modes", default=None,
-                    help="comma list of step-leg modes (default: all seven)")
+                    help="comma list of step-leg modes (default: all nine)")
     ap.add_argument("--ens-steps", type=int, default=40)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
@@ -680,7 +687,7 @@ def main():
             "attribution_speedups_vs_fp64": {
                 k: st[k]["steps_per_s"] / fp for k in
                 ("mixed", "fitted_fp64", "learned_fp64", "learned_mixed", "inverse_fp64",
-                 "learned_inverse_mixed")},
+                 "inverse_mixed", "learned_inverse_fp64", "learned_inverse_mixed")},
         }
         if not args.no_cpu:
             line["sim_steps"]["cpu_baseline"] = cpu_step_baseline(args)
