@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the RPY grand-mobility hot path on one MI355X.
 
-Second part (sim_steps): implicit time steps of the 256-filament x 20-segment
-+ 4002-blob swimmer after one driving period: plain FP64 against the learned
-warm start with mixed precision (and the two ingredients alone), steps/s with
-Broyden and GMRES iteration counts, and the CPU oracle's step rate.
-
-A step is one full grand-mobility apply (all four reference blocks:
-seg<-seg 6x6, seg<-blob, blob<-seg, blob<-blob) over the 46,722-particle
-swimmer (1024 filaments x 30 segments + 16,002 blobs on a sphere of radius
-40a), forces and torques i.i.d. N(0,1), inputs resident in HBM.  The line
-reports pair-interactions/s of the FP32-Kahan kernel (`value`), the FP64
+Headline line (`value`): one full grand-mobility apply (all four reference
+blocks: seg<-seg 6x6, seg<-blob, blob<-seg, blob<-blob) over the
+46,722-particle swimmer (1024 filaments x 30 segments + 16,002 blobs on a
+sphere of radius 40a), forces and torques i.i.d. N(0,1), inputs resident in
+HBM.  It reports pair-interactions/s of the FP32-Kahan kernel, the FP64
 kernel beside it, the FP32 error against FP64, the pair kernel's achieved
-FLOP/s against the vector roofline, and the CPU FP64 oracle timed on the
-host cores on a bounded sample of the same workload.
+FLOP/s against the vector roofline, and the CPU FP64 oracle timed on the host
+cores on a bounded sample of the same workload.
+
+Further legs in the same JSON line:
+  sim_steps   implicit time steps of the 256-filament x 20-segment + 4002-blob
+              swimmer after two driving periods: plain FP64 against the
+              learned warm start, mixed precision, the fitted GMRES guess and
+              the exact body-frame preconditioner, alone and combined, with
+              Broyden and GMRES iteration counts and the CPU oracle's rate;
+  ensemble    64 uneven swimmers stepped as one batch;
+  rollout     1000 steps of the 962-particle swimmer with and without the
+              learned warm start;
+  cilia       prescribed-cilia saddle-point steps;
+  carpet      a carpet of cilia above a no-slip wall (Swan-Brady kernel).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -124,9 +131,9 @@ def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
 def bench_step(args):
     """Implicit time steps of the 256 x 20-segment + 4002-blob swimmer.
 
-    Identical runs take one driving period (300 steps) of FP64 steps; the
-    learned ones record every 5th step's (features, converged - guess) rows
-    and each trains a network on its own rows.  Then each run steps over the
+    Identical runs take `step_warmup` FP64 steps (default two driving periods,
+    600 steps); the learned ones record every 5th step's (features, converged
+    - guess) rows and each trains a network on its own rows.  Then each run steps over the
     same `step_steps` steps in its own mode:
       fp64                  plain FP64 (the reference method: extrapolated
                             Broyden iterate, previous solution as the GMRES
@@ -543,7 +550,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--step-workload", default="step_9122")
     ap.add_argument("--step-steps", type=int, default=40)
-    ap.add_argument("--step-warmup", type=int, default=300)
+    ap.add_argument("--step-warmup", type=int, default=600,
+                    help="FP64 steps before the timed window (two driving periods: the "
+                         "learned runs train on every 5th step's rows)")
     ap.add_argument("--no-step", action="store_true")
     ap.add_argument("--step-modes", default=None,
                     help="comma list of step-leg modes (default: all nine)")
