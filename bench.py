@@ -50,6 +50,7 @@ FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
 STEP_LAUNCH_SHAPE = (4096, 128)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
 
 
 def measured_traffic():
@@ -221,6 +222,8 @@ def bench_step(args):
             if name != "fp64":
                 res[name]["max_position_diff_vs_fp64"] = float(
                     np.abs(sims[name].state()["x"] - ref).max())
+        res["fp64"]["gram_schmidt"] = _gram_schmidt(sims["fp64"], nt0 + args.step_steps, 2,
+                                                    "step fp64")
     for sim in sims.values():
         sim.close()
     out = {"workload": args.step_workload, "warmup_steps": args.step_warmup,
@@ -261,6 +264,32 @@ def _timed_steps(sim, nt0, k, name):
     return r
 
 
+def _gram_schmidt(sim, nt0, k, name):
+    """The Arnoldi Gram-Schmidt kernels (k_pdots, k_psub_dots, k_psub_norm,
+    with k_colsum_wide at large n) over k steps from nt0, launched eagerly so
+    that every Arnoldi step brackets them with HIP events.  Algorithmic bytes
+    per Arnoldi step with j previous columns of length n: two CGS passes of
+    (j + 2) n doubles (the columns and w read, w written), against 8 TB/s."""
+    sim.set_graphs(False)
+    sim.call("gmi_profile_begin")
+    for nt in range(nt0, nt0 + k):
+        assert sim.step(nt)[0] == 0
+    ms, nbytes, steps = capi.C.c_double(0.0), capi.C.c_double(0.0), capi.C.c_int(0)
+    sim.call("gmi_profile_krylov", capi.C.byref(ms), capi.C.byref(nbytes), capi.C.byref(steps))
+    pair_ms, n = capi.C.c_double(0.0), capi.C.c_int(0)
+    sim.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
+    gbs = nbytes.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
+    r = {"kernels": "k_pdots + k_psub_dots + k_psub_norm (+ k_colsum_wide)",
+         "steps": f"{nt0}-{nt0 + k - 1}", "arnoldi_steps": steps.value, "ms": ms.value,
+         "us_per_arnoldi_step": 1e3 * ms.value / max(1, steps.value),
+         "algorithmic_bytes": nbytes.value, "achieved_gbs": gbs,
+         "frac_hbm_peak": gbs / PEAK_HBM_GBS}
+    print(f"# {name} Gram-Schmidt: {steps.value} Arnoldi steps, "
+          f"{r['us_per_arnoldi_step']:.1f} us each, {gbs:.0f} GB/s algorithmic",
+          file=sys.stderr, flush=True)
+    return r
+
+
 def bench_ensemble(args):
     """64 independent swimmers of uneven size (17..474 filaments x 20
     segments, 642..7292 blobs; 306,518 particles) stepped as one batch:
@@ -268,7 +297,8 @@ def bench_ensemble(args):
     captured graphs.  Two steps untimed, then FP64, mixed-precision and
     mixed-precision with the exact body-frame preconditioner (36.8 GB of
     per-swimmer inverses in HBM) windows from the same state (three
-    contexts)."""
+    contexts); after the FP64 window one more step measures the
+    Gram-Schmidt kernels at the batch's GMRES length."""
     cfg = geometry.ensemble_workload("ensemble_64", seed=4)
     out = {"workload": "ensemble_64", "swimmers": len(cfg.swimmers),
            "filaments": int(sum(g.nfil for g in cfg.swimmers)),
@@ -290,6 +320,8 @@ def bench_ensemble(args):
             assert sim.step(nt)[0] == 0
         out[name] = _timed_steps(sim, 2, k, f"ensemble {name}")
         out[name]["setup_s"] = setup_s
+        if name == "fp64":
+            out[name]["gram_schmidt"] = _gram_schmidt(sim, 2 + k, 1, f"ensemble {name}")
         sim.close()
     out["speedup_mixed_vs_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
     out["speedup_inverse_mixed_vs_fp64"] = (out["inverse_mixed"]["steps_per_s"] /

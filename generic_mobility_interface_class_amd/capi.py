@@ -73,6 +73,7 @@ SIGNATURES = [
     ("gmi_timer_stop", C.c_int, [C.c_void_p, _dp]),
     ("gmi_profile_begin", C.c_int, [C.c_void_p]),
     ("gmi_profile_end", C.c_int, [C.c_void_p, _dp, _ip]),
+    ("gmi_profile_krylov", C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     ("gmi_set_launch_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gmi_autotune", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     ("gmi_autotune_blocks", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),

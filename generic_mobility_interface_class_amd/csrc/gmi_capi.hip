@@ -158,6 +158,26 @@ int prof_start(gmi_ctx *c, hipEvent_t *ev_stop) {
   return GMI_OK;
 }
 
+int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop) {
+  *ev_stop = nullptr;
+  if (!c->profiling) return GMI_OK;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  int rc;
+  if ((rc = hip_check(c, hipStreamIsCapturing(c->stream, &st), "capture status"))) return rc;
+  if (st != hipStreamCaptureStatusNone) return GMI_OK;
+  while (c->gs_events.size() < c->gs_used + 2) {
+    hipEvent_t e;
+    if ((rc = hip_check(c, hipEventCreate(&e), "event"))) return rc;
+    c->gs_events.push_back(e);
+  }
+  if ((rc = hip_check(c, hipEventRecord(c->gs_events[c->gs_used], c->stream), "event")))
+    return rc;
+  c->gs_recs.push_back({c->gs_used, k, n});
+  *ev_stop = c->gs_events[c->gs_used + 1];
+  c->gs_used += 2;
+  return GMI_OK;
+}
+
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
               const double *Fs, const double *Fb, double *Vs, double *Vb,
               const double *Xs, const double *Xb) {
@@ -346,6 +366,7 @@ void gmi_destroy(gmi_ctx *c) {
     if (st.done) (void)hipEventDestroy(st.done);
   }
   for (hipEvent_t e : c->prof_events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->gs_events) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -466,6 +487,30 @@ int gmi_profile_begin(gmi_ctx *c) {
   if (!c) return GMI_ERR_ARG;
   c->profiling = true;
   c->prof_used = 0;
+  c->gs_used = 0;
+  c->gs_recs.clear();
+  return GMI_OK;
+}
+
+int gmi_profile_krylov(gmi_ctx *c, double *gs_ms, double *gs_bytes, int *steps) {
+  if (!c || !gs_ms || !gs_bytes || !steps) return GMI_ERR_ARG;
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "stream sync"))) return rc;
+  double tot = 0.0, bytes = 0.0;
+  for (const gmi_ctx::GsRec &g : c->gs_recs) {
+    float t = 0.f;
+    if ((rc = hip_check(c, hipEventElapsedTime(&t, c->gs_events[g.ev], c->gs_events[g.ev + 1]),
+                        "elapsed")))
+      return rc;
+    tot += t;
+    // two Gram-Schmidt passes, each reading k columns and w and writing w
+    bytes += 2.0 * 8.0 * double(g.n) * double(g.k + 2);
+  }
+  *gs_ms = tot;
+  *gs_bytes = bytes;
+  *steps = int(c->gs_recs.size());
+  c->gs_recs.clear();
+  c->gs_used = 0;
   return GMI_OK;
 }
 

@@ -57,6 +57,16 @@ struct gmi_ctx {
   bool profiling = false;
   std::vector<hipEvent_t> prof_events;  // pairs (start, stop)
   size_t prof_used = 0;
+  // Gram-Schmidt brackets of eager Arnoldi steps while profiling: events
+  // gs_events[2i], gs_events[2i+1] around the step's Gram-Schmidt kernels,
+  // the step's column count k and vector length n
+  struct GsRec {
+    size_t ev;
+    int k, n;
+  };
+  std::vector<hipEvent_t> gs_events;
+  size_t gs_used = 0;
+  std::vector<GsRec> gs_recs;
 
   // hook for subsystems that own further device memory
   void (*extra_release)(gmi_ctx *) = nullptr;
@@ -78,6 +88,10 @@ int rebuild_lists(gmi_ctx *c);
 // context stream and returns in *ev_stop the event to record right after the
 // pair kernel; nullptr when not profiling or while the stream is captured.
 int prof_start(gmi_ctx *c, hipEvent_t *ev_stop);
+// Gram-Schmidt profiling of one Arnoldi step with k previous columns of
+// length n: records the start event and returns the stop event in *ev_stop
+// (nullptr when not profiling or while the stream is captured).
+int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop);
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32, const double *Fs,
               const double *Fb, double *Vs, double *Vb, const double *Xs,
               const double *Xb);

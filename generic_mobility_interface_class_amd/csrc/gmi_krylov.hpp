@@ -474,6 +474,8 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
       if ((r = op.precond(qprev, zk, s->flags, it >= 2 ? 3 : -1))) return r;
       if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
       // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
+      hipEvent_t gs_stop = nullptr;
+      if ((r = prof_gs_start(c, it, n, &gs_stop))) return r;
       hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
                          ds.R, pp1, s->flags);
       if (ds.wide)
@@ -486,6 +488,7 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
                            s->flags);
       hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
                          s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
+      if (gs_stop && (r = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return r;
       hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
                          s->h2, s->H, ldh, s->cs, s->sn, s->beta, s->flags, s->scal, tol, maxit,
                          switch_rel);
@@ -497,6 +500,9 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
     s->drop_graphs();
     s->graph_gen = c->list_generation;
   }
+  // Gram-Schmidt records of this solve start here; steps launched past
+  // convergence (flag-skipped) are dropped once the count is known
+  const size_t gs_mark = c->gs_recs.size();
   int it = 1;
   int done = 0;
   while (!done) {
@@ -531,6 +537,9 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
     if (!done && it > maxit) return fail(c, GMI_ERR_STATE, "GMRES ran past its iteration cap");
   }
   const int nits = s->h_flags[F_ITS];
+  c->gs_recs.erase(std::remove_if(c->gs_recs.begin() + gs_mark, c->gs_recs.end(),
+                                  [nits](const gmi_ctx::GsRec &g) { return g.k > nits; }),
+                   c->gs_recs.end());
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->cols + 1), c->stream,
                      s->H, ldh, s->beta, s->flags, s->y);
   if (s->Zk) {

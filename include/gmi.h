@@ -118,6 +118,16 @@ int gmi_profile_begin(gmi_ctx *ctx);
 /* Waits for the stream, returns the summed pair-kernel time in ms and the
  * number of launches since gmi_profile_begin(), and stops recording. */
 int gmi_profile_end(gmi_ctx *ctx, double *pair_ms, int *launches);
+/* While profiling, every eagerly launched Arnoldi step (graphs off) also
+ * records HIP events around its Gram-Schmidt kernels (both passes: column
+ * dots, subtraction, norm).  Waits for the stream and returns, for the steps
+ * recorded since gmi_profile_begin() or the previous call, the summed time
+ * in ms, the algorithmic bytes 2 x 8 (k + 2) n of each step (k previous
+ * columns of length n: per pass the columns and w read, w written), and the
+ * step count; then clears them.  Steps launched past convergence are not
+ * counted.  The reference times nothing; its Gram-Schmidt is
+ * mobility_solver.cpp:1175-1183. */
+int gmi_profile_krylov(gmi_ctx *ctx, double *gs_ms, double *gs_bytes, int *steps);
 
 /* Segment-segment mobility above a no-slip plane wall at z = 0 (the
  * reference's INFINITE_PLANE_WALL build: Swan & Brady, cuda_functions.cu:

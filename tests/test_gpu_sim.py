@@ -227,3 +227,34 @@ def test_graph_replay_matches_eager(gpu):
         assert np.array_equal(se["x"], sg["x"]) and np.array_equal(se["u"], sg["u"])
     me, mg = ge.mobility(), gg.mobility()
     assert np.array_equal(me["f_blobs"], mg["f_blobs"])
+
+
+def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
+    """gmi_profile_krylov: eagerly launched Arnoldi steps are bracketed one
+    each; steps launched past convergence inside a chunk (flag-skipped) are
+    not counted, so the count is the GMRES iteration total of the step, and
+    the bytes are 16 n (k + 2) summed over steps k = 1..its of each solve
+    (n the padded GMRES length).
+    Captured graphs record nothing."""
+    cfg = geometry.sim_workload("step_962", seed=1)
+    g = capi.Sim(cfg)
+    g.make_body_reference_matrices()
+    n = 3 * cfg.swimmers[0].nblob + 6
+    for use_graphs in (False, True):
+        g.set_graphs(use_graphs, chunk=6)
+        g.call("gmi_profile_begin")
+        rc, _, gits = g.step(int(use_graphs))
+        assert rc == 0 and gits > 0
+        ms, nbytes, steps = capi.C.c_double(0.0), capi.C.c_double(0.0), capi.C.c_int(0)
+        g.call("gmi_profile_krylov", capi.C.byref(ms), capi.C.byref(nbytes), capi.C.byref(steps))
+        pm, pn = capi.C.c_double(0.0), capi.C.c_int(0)
+        g.call("gmi_profile_end", capi.C.byref(pm), capi.C.byref(pn))
+        print(f"graphs {use_graphs}: GMRES {gits}, recorded {steps.value} steps, "
+              f"{ms.value:.3f} ms, {nbytes.value:.3e} B", flush=True)
+        if use_graphs:
+            assert steps.value == 0 and nbytes.value == 0.0
+            continue
+        assert steps.value == gits
+        assert ms.value > 0
+        # 16 n' (k + 2) per step with n' >= n (the padded length) and k >= 1
+        assert nbytes.value >= 16.0 * n * 3 * gits
