@@ -99,33 +99,59 @@ def time_apply(ctx, steps, warmup):
     return ms.value, pair_ms.value / max(1, n.value)
 
 
-def cpu_baseline(xs, xb, fs, fb, seconds=12.0):
+def host_cpu():
+    """CPU model and counts of the host the CPU legs run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu": model, "nproc": os.cpu_count() or 1, "usable": usable}
+
+
+def cpu_baseline(xs, xb, fs, fb, seconds=12.0, seconds_one_core=6.0):
     """Oracle (CPU FP64 restatement) on a bounded sample of target rows
-    against all sources; returns pairs/s."""
+    against all sources, on up to 16 host threads (`value`) and on one
+    (`one_core`); pairs/s."""
     from oracle import binding as orc
     threads = max(1, min(16, os.cpu_count() or 1))
     ns, nb = len(xs), len(xb)
 
-    def run(rs, rb):
+    def run(rs, rb, nt):
         t0 = time.perf_counter()
-        v = orc.Mss(xs, fs, 1.0, rows=(0, rs), nthreads=threads)
-        orc.Msb(xs, xb, fb, 1.0, 1.0, V=v, rows=(0, rs), nthreads=threads)
-        orc.Mbs(xb, xs, fs, 1.0, 1.0, rows=(0, rb), nthreads=threads)
-        orc.Mbb(xb, fb, 1.0, rows=(0, rb), nthreads=threads)
+        v = orc.Mss(xs, fs, 1.0, rows=(0, rs), nthreads=nt)
+        orc.Msb(xs, xb, fb, 1.0, 1.0, V=v, rows=(0, rs), nthreads=nt)
+        orc.Mbs(xb, xs, fs, 1.0, 1.0, rows=(0, rb), nthreads=nt)
+        orc.Mbb(xb, fb, 1.0, rows=(0, rb), nthreads=nt)
         return time.perf_counter() - t0, (rs + rb) * (ns + nb)
 
-    dt, pairs = run(64, 32)
-    scale = max(1.0, seconds / max(dt, 1e-6))
-    rs = min(ns, int(64 * scale))
-    rb = min(nb, int(32 * scale))
-    dt, pairs = run(rs, rb)
+    def sized(nt, target):
+        dt, _ = run(64, 32, nt)
+        scale = max(1.0, target / max(dt, 1e-6))
+        rs = min(ns, int(64 * scale))
+        rb = min(nb, int(32 * scale))
+        dt, pairs = run(rs, rb, nt)
+        return pairs / dt, (f"{rs} segment + {rb} blob target rows x all {ns + nb} sources "
+                            f"(all four blocks, FP64, {dt:.1f} s)")
+
+    value, sample = sized(threads, seconds)
+    one, one_sample = sized(1, seconds_one_core)
     return {
-        "value": pairs / dt,
+        "value": value,
         "unit": "pair-interactions/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{rs} segment + {rb} blob target rows x all {ns + nb} sources "
-                  f"(all four blocks, FP64, {dt:.1f} s)",
+        "sample": sample,
+        "one_core": {"value": one, "sample": one_sample},
+        "host": host_cpu(),
     }
 
 
