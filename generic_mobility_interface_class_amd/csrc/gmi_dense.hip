@@ -43,7 +43,8 @@ struct DensePrec {
   std::vector<size_t> moff_h;   // per swimmer: first element of its matrix
   std::vector<int> voff_h;      // per swimmer: first row of its body-frame vector
   int nrows = 0, maxn = 0;      // total rows (sum 3 nb), largest 3 nb
-  double *Minv = nullptr;       // per swimmer column-major 3nb x 3nb
+  double *Minv = nullptr;       // per swimmer column-major 3nb x 3nb (set-up only)
+  double *Mpk = nullptr;        // lower-triangle tiles of M^-1, kSym x kSym each, contiguous
   double *G = nullptr;          // per swimmer column-major 3nb x 6
   double *Sinv = nullptr;       // [Nsw][36] row-major
   double *t = nullptr, *w = nullptr;  // body-frame blob part of the input, M^-1 t
@@ -231,14 +232,32 @@ __global__ __launch_bounds__(256) void k_dense_in(SimDev a, const BTile *tiles, 
   }
 }
 
+// Lower-triangle tile (I, J) of a swimmer's M^-1 copied to contiguous
+// storage: element (r, c) of tile b at Mpk[b kSym^2 + c kSym + r], zero
+// outside the matrix (one workgroup per tile; set-up only).
+__global__ __launch_bounds__(256) void k_dense_pack(const double *__restrict__ Minv,
+                                                    const size_t *__restrict__ moff,
+                                                    const int *__restrict__ sw_nblob,
+                                                    const SymTile *__restrict__ tl,
+                                                    double *__restrict__ Mpk) {
+  const SymTile T = tl[blockIdx.x];
+  const int n = T.sw, rows = 3 * sw_nblob[n];
+  const int r0 = T.I * kSym, c0 = T.J * kSym;
+  const int nr = min(kSym, rows - r0), nc = min(kSym, rows - c0);
+  const double *A = Minv + moff[n];
+  double *P = Mpk + size_t(kSym * kSym) * blockIdx.x;
+  const int r = threadIdx.x & (kSym - 1), g = threadIdx.x >> 6;
+  for (int c = g; c < kSym; c += 4)
+    P[c * kSym + r] = (r < nr && c < nc) ? A[size_t(c0 + c) * rows + r0 + r] : 0.0;
+}
+
 // w = M^-1 t reading only the lower triangle of each swimmer's symmetric
-// inverse: one workgroup per kSym x kSym tile (I >= J) stages the tile in
-// LDS and writes the row partial A_IJ t_J and, below the diagonal, the
-// column partial A_IJ^T t_I -- every stored element is read from HBM once
-// for both uses.  k_dense_symv_sum adds the partials of a block row in a
-// fixed order.
-__global__ __launch_bounds__(256) void k_dense_symv_tiles(const double *__restrict__ Minv,
-                                                          const size_t *__restrict__ moff,
+// inverse, packed tile by tile (k_dense_pack): one workgroup per kSym x kSym
+// tile (I >= J) streams its 32 KB, stages the tile in LDS and writes the
+// row partial A_IJ t_J and, below the diagonal, the column partial
+// A_IJ^T t_I -- every stored element is read from HBM once for both uses.
+// k_dense_symv_sum adds the partials of a block row in a fixed order.
+__global__ __launch_bounds__(256) void k_dense_symv_tiles(const double *__restrict__ Mpk,
                                                           const int *__restrict__ voff,
                                                           const int *__restrict__ sw_nblob,
                                                           const SymTile *__restrict__ tl,
@@ -254,22 +273,16 @@ __global__ __launch_bounds__(256) void k_dense_symv_tiles(const double *__restri
   const int n = T.sw, rows = 3 * sw_nblob[n];
   const int r0 = T.I * kSym, c0 = T.J * kSym;
   const int nr = min(kSym, rows - r0), nc = min(kSym, rows - c0);
-  const double *A = Minv + moff[n];
+  const double *A = Mpk + size_t(kSym * kSym) * blockIdx.x;
   const double *tv = t + voff[n];
   const int th = threadIdx.x, r = th & (kSym - 1), g = th >> 6;
-  // column c0 + c holds rows r0 .. r0 + nr - 1 contiguously.  All loads of
-  // a thread are issued before its LDS stores (an index outside the tile is
-  // clamped to the tile's first element and its value dropped), so each
-  // lane keeps kSym / 4 loads in flight.
+  // load k of the workgroup covers elements [256 k, 256 k + 256) of the
+  // tile; all loads of a thread are issued before its LDS stores, so each
+  // lane keeps kSym / 4 loads in flight
   constexpr int NL = kSym / 4;
   double v[NL];
 #pragma unroll
-  for (int k = 0; k < NL; k++) {
-    const int c = g + 4 * k;
-    const bool in = r < nr && c < nc;
-    v[k] = A[in ? size_t(c0 + c) * rows + r0 + r : size_t(c0) * rows + r0];
-    v[k] = in ? v[k] : 0.0;
-  }
+  for (int k = 0; k < NL; k++) v[k] = A[(g + 4 * k) * kSym + r];
   const int xk = th & (kSym - 1);
   const bool xin = th < kSym ? xk < nr : xk < nc;
   const double xv = tv[xin ? (th < kSym ? r0 : c0) + xk : r0];
@@ -561,6 +574,19 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
     return bail(rc);
   if (sing) return bail(fail(c, GMI_ERR_SINGULAR, "blob mobility not positive definite"));
+  // the product reads the lower triangle packed tile by tile; the full
+  // matrices are not needed after this (half the HBM stays resident)
+  if ((rc = dmalloc(c, D, (void **)&D->Mpk,
+                    sizeof(double) * size_t(kSym * kSym) * size_t(D->nsym_tiles))))
+    return bail(rc);
+  hipLaunchKernelGGL(k_dense_pack, dim3(D->nsym_tiles), dim3(256), 0, c->stream, D->Minv, D->moff,
+                     d.sw_nblob, D->sym_tiles, D->Mpk);
+  if ((rc = launch_ok(c, "pack")) ||
+      (rc = hip_check(c, hipStreamSynchronize(c->stream), "sync")))
+    return bail(rc);
+  D->owned.erase(std::find(D->owned.begin(), D->owned.end(), static_cast<void *>(D->Minv)));
+  if ((rc = hip_check(c, hipFree(D->Minv), "hipFree full inverse"))) return bail(rc);
+  D->Minv = nullptr;
   return GMI_OK;
 }
 
@@ -569,8 +595,8 @@ int dense_apply(gmi_ctx *c, SimHost *s, double *in, double *out, const int *skip
   DensePrec *D = s->dense;
   hipLaunchKernelGGL(k_dense_in, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
                      s->sw_tile0, D->voff, in, s->scal, norm_slot, D->t, D->tc, skip);
-  hipLaunchKernelGGL(k_dense_symv_tiles, dim3(D->nsym_tiles), dim3(256), 0, c->stream, D->Minv,
-                     D->moff, D->voff, d.sw_nblob, D->sym_tiles, D->t, D->sym_part, skip);
+  hipLaunchKernelGGL(k_dense_symv_tiles, dim3(D->nsym_tiles), dim3(256), 0, c->stream, D->Mpk,
+                     D->voff, d.sw_nblob, D->sym_tiles, D->t, D->sym_part, skip);
   hipLaunchKernelGGL(k_dense_symv_sum, dim3(D->nsym_rows), dim3(256), 0, c->stream, D->sym_rows,
                      D->voff, d.sw_nblob, D->sym_part, D->w, skip);
   hipLaunchKernelGGL(k_dense_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles, D->voff,
