@@ -321,8 +321,8 @@ def bench_ensemble(args):
     segments, 642..7292 blobs; 306,518 particles) stepped as one batch:
     every pair launch covers all swimmers, Arnoldi chunks replay as
     captured graphs.  Two steps untimed, then FP64, mixed-precision and
-    mixed-precision with the exact body-frame preconditioner (36.8 GB of
-    per-swimmer inverses in HBM) windows from the same state (three
+    mixed-precision with the exact body-frame preconditioner (18.4 GB of
+    packed per-swimmer inverses in HBM) windows from the same state (three
     contexts); after the FP64 window one more step measures the
     Gram-Schmidt kernels at the batch's GMRES length."""
     cfg = geometry.ensemble_workload("ensemble_64", seed=4)

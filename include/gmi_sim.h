@@ -118,7 +118,8 @@ int gmi_sim_set_gmres_history(gmi_ctx *ctx, int m);
                                        in HBM), M^-1 K and (K^T M^-1 K)^-1,
                                        computed once by block Gauss-Jordan */
 /* Selects the preconditioner (set-up cost and memory for BODY_INVERSE:
- * 2 (3 nb)^3 flops and 8 (3 nb)^2 bytes per swimmer, plus the preconditioned
+ * 2 (3 nb)^3 flops and 8 (3 nb)^2 bytes per swimmer during set-up, about half
+ * of that resident after it (the packed lower triangle), plus the preconditioned
  * Arnoldi directions, 8 (maxit + 1) rows bytes).  GMRES keeps its operator
  * and tolerance.  BODY_INVERSE applies M^-1 by a symmetric product over its
  * lower triangle and checks convergence after every Arnoldi step (chunk 1;
