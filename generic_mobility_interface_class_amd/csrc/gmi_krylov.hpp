@@ -411,7 +411,8 @@ static int krylov_read_flags(gmi_ctx *c, KrylovState *s) {
 // Row split of the column dot products: nb workgroups of R rows.
 // Up to kMaxDotBlocks workgroups every consumer workgroup re-sums a column's
 // partials itself; beyond that (wide) a k_colsum_wide launch sums them once
-// and up to ~1024 workgroups share the rows.
+// and up to ~kWideDotBlocks workgroups share the rows.
+constexpr int kWideDotBlocks = 4096;
 struct DotShape {
   int nb, R;
   bool wide;
@@ -419,7 +420,8 @@ struct DotShape {
 static DotShape dot_shape(int n) {
   const int chunks = nblk(n, 256);
   if (chunks <= kMaxDotBlocks) return {chunks, 256, false};
-  const int per = std::min((chunks + 1023) / 1024, kMaxRows / 256);  // 256-row chunks per workgroup
+  // 256-row chunks per workgroup
+  const int per = std::min((chunks + kWideDotBlocks - 1) / kWideDotBlocks, kMaxRows / 256);
   const int R = per * 256;
   return {nblk(n, R), R, true};
 }
