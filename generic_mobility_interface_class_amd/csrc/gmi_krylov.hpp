@@ -34,6 +34,47 @@ namespace gmi {
 enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
        F_N = 8 };
 
+// A captured stream segment: the graph is kept as long as its executable
+// (both are destroyed together).
+struct CapturedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void destroy() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+// Captures what `enqueue()` launches on the context stream into *out.  On
+// any failure the capture is ended, everything it created is destroyed and
+// *out is left empty.
+template <class F>
+int capture_graph(gmi_ctx *c, F &&enqueue, CapturedGraph *out) {
+  int rc;
+  if ((rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                      "begin capture")))
+    return rc;
+  const int erc = enqueue();
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  if (erc || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return erc ? erc : hip_check(c, ec, "end capture");
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_check(c, ei, "instantiate");
+  }
+  out->graph = graph;
+  out->exec = exec;
+  return GMI_OK;
+}
+
 // Device buffers and host state of one GMRES solver.
 struct KrylovState {
   double *x = nullptr, *rhs = nullptr, *Qk = nullptr, *H = nullptr, *cs = nullptr,
@@ -53,9 +94,9 @@ struct KrylovState {
   bool use_graphs = true;
   int graph_gen = -1;
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  std::map<std::pair<int, int>, CapturedGraph> graphs;
   void drop_graphs() {
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : graphs) kv.second.destroy();
     graphs.clear();
   }
 };
@@ -515,21 +556,12 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
       const std::pair<int, int> key{mixed ? 1 : 0, it};
       auto g = s->graphs.find(key);
       if (g == s->graphs.end()) {
-        hipGraph_t graph;
-        if ((rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
-                            "begin capture")))
-          return rc;
-        const int erc = enqueue(it, last);
-        const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (erc) return erc;
-        if ((rc = hip_check(c, ec, "end capture"))) return rc;
-        hipGraphExec_t exec;
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if ((rc = hip_check(c, ei, "instantiate"))) return rc;
-        g = s->graphs.emplace(key, exec).first;
+        CapturedGraph cg;
+        if ((rc = capture_graph(c, [&]() { return enqueue(it, last); }, &cg))) return rc;
+        g = s->graphs.emplace(key, cg).first;
       }
-      if ((rc = hip_check(c, hipGraphLaunch(g->second, c->stream), "graph launch"))) return rc;
+      if ((rc = hip_check(c, hipGraphLaunch(g->second.exec, c->stream), "graph launch")))
+        return rc;
     } else {
       if ((rc = enqueue(it, last))) return rc;
     }

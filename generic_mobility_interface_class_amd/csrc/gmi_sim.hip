@@ -1044,12 +1044,13 @@ __global__ __launch_bounds__(kGjThreads) void k_gj_invert_lds(SimDev a) {
     }
     if (t == 0 && p != k) { const int x = piv[k]; piv[k] = piv[p]; piv[p] = x; }
     __syncthreads();
-    if (jc < n) {
-      const double akj = A[size_t(k) * n + jc];
+    // columns jc, jc + 128, ... (n exceeds 128 from NSEG = 22 on)
+    for (int j = jc; j < n; j += 128) {
+      const double akj = A[size_t(k) * n + j];
       for (int i = rg; i < n; i += kGjThreads / 128) {
         if (i == k) continue;
-        const double base = (jc == k) ? 0.0 : A[size_t(i) * n + jc];
-        A[size_t(i) * n + jc] = base - colk[i] * akj;
+        const double base = (j == k) ? 0.0 : A[size_t(i) * n + j];
+        A[size_t(i) * n + j] = base - colk[i] * akj;
       }
     }
     __syncthreads();
