@@ -43,16 +43,20 @@ def test_small_ensemble_matches_oracle(gpu):
         np.testing.assert_allclose(sg["body"], so["body"], rtol=0, atol=DL * cfg.tol)
 
 
-def test_large_ensemble_swimmers_follow_solo_trajectories(gpu):
+@pytest.mark.parametrize("precision", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
+def test_large_ensemble_swimmers_follow_solo_trajectories(gpu, precision):
+    """FP64, and mixed precision (FP32-Kahan pair blocks, relaxed GMRES with
+    FP64 refinement) in both the batch and the solo runs."""
     cfg = geometry.ensemble_workload("ensemble_64", seed=4)
     nfil = np.array([g.nfil for g in cfg.swimmers])
     picks = [int(nfil.argmin()), int(nfil.argmax()), int(np.argsort(nfil)[len(nfil) // 2])]
-    print(f"ensemble_64 seed 4: {nfil.sum()} filaments; solo swimmers {picks} "
+    print(f"ensemble_64 seed 4 precision {precision}: {nfil.sum()} filaments; solo swimmers {picks} "
           f"with {nfil[picks].tolist()} filaments", flush=True)
-    batch = capi.Sim(cfg)
-    batch.make_body_reference_matrices()
-    solo = {k: capi.Sim(alone(cfg, k)) for k in picks}
-    for s in solo.values():
+    batch = capi.Sim(cfg, precision=precision)
+    solo = {k: capi.Sim(alone(cfg, k), precision=precision) for k in picks}
+    for s in [batch, *solo.values()]:
+        if precision == capi.GMI_FP32_KAHAN:
+            s.set_sim_precision(capi.GMI_FP32_KAHAN)
         s.make_body_reference_matrices()
     seg0 = np.cumsum([0] + [g.nfil * g.nseg for g in cfg.swimmers])
     DL = cfg.swimmers[0].DL

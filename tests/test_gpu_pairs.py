@@ -156,10 +156,22 @@ def test_barrier_parity(gpu):
     assert relerr(vs, orc.Mss(xs, f_ref, 1.0)) <= 1e-12
 
 
+_REF_46722 = {}
+
+
+def _oracle_46722(g, xs, xb, fs, fb):
+    """The oracle's whole 46,722-particle grand apply (all rows; about a
+    second on 16 host threads), computed once per session."""
+    if not _REF_46722:
+        _REF_46722["v"] = orc.grand(xs, xb, fs, fb, 1.0, 1.0)
+    return _REF_46722["v"]
+
+
 @pytest.mark.parametrize("prec", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
-def test_apply_46722_slices(gpu, prec):
-    """Full 46,722-particle apply on the GPU, checked on row slices against the
-    oracle (FP64: 1e-12; FP32-Kahan: 1e-6)."""
+def test_apply_46722_all_rows(gpu, prec):
+    """Full 46,722-particle apply on the GPU against the oracle on every row
+    (FP64: 1e-12; FP32-Kahan: 1e-6), relative to the largest entry of each
+    species' velocities."""
     g = geometry.workload_geometry("apply_46722")
     xs, xb = g.positions()
     fs, fb = geometry.random_forces(g.ns, g.nblob, seed=3)
@@ -168,11 +180,9 @@ def test_apply_46722_slices(gpu, prec):
     ctx.upload(xs, fs, xb, fb)
     ctx.call("gmi_evaluate_grand_mobility")
     vs, vb = ctx.velocities()
+    ctx.close()
+    rs, rb = _oracle_46722(g, xs, xb, fs, fb)
     tol = 1e-12 if prec == capi.GMI_FP64 else 1e-6
-    for r0, r1 in [(0, 300), (g.ns - 300, g.ns)]:
-        ref = orc.Mss(xs, fs, 1.0, rows=(r0, r1))
-        ref = orc.Msb(xs, xb, fb, 1.0, 1.0, V=ref, rows=(r0, r1))
-        assert relerr(vs[r0:r1], ref[r0:r1]) <= tol
-    for r0, r1 in [(0, 300), (g.nblob - 300, g.nblob)]:
-        ref = orc.Mbs(xb, xs, fs, 1.0, 1.0, rows=(r0, r1)) + orc.Mbb(xb, fb, 1.0, rows=(r0, r1))
-        assert relerr(vb[r0:r1], ref[r0:r1]) <= tol
+    es, eb = relerr(vs, rs), relerr(vb, rb)
+    print(f"  all rows: segments {es:.2e}, blobs {eb:.2e}", flush=True)
+    assert es <= tol and eb <= tol

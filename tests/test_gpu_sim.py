@@ -258,3 +258,88 @@ def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
         assert ms.value > 0
         # 16 n' (k + 2) per step with n' >= n (the padded length) and k >= 1
         assert nbytes.value >= 16.0 * n * 3 * gits
+
+
+@pytest.mark.parametrize("nseg", [23, 30])
+def test_long_filaments_match_oracle(gpu, nseg):
+    """Filaments longer than 21 segments: NSEG = 23 (6 NSEG = 138 columns,
+    the Gauss-Jordan inverse held in LDS) and NSEG = 30 (180 columns, the
+    matrix in global memory).  J^-1 e at the first step and four steps
+    against the oracle."""
+    cfg = geometry.sim_config(nfil=6, nseg=nseg, nblob=162, R=6.0, seed=2)
+    o, g = orc.Sim(cfg), capi.Sim(cfg)
+    print(f"long filaments nseg {nseg}", flush=True)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    o.initial_guess(0); o.forces(); o.compute_velocities()
+    g.call("gmi_initial_guess", 0)
+    g.call("gmi_forces_and_torques", 0)
+    its = capi.C.c_int(0)
+    g.call("gmi_compute_velocities", 0, capi.C.byref(its))
+    big, bad = capi.C.c_int(0), capi.C.c_int(0)
+    g.call("gmi_compute_errors", 0, 0, capi.C.byref(big), capi.C.byref(bad))
+    eg = g.broyden(0)
+    o.prepare_jacobian(0)
+    g.call("gmi_prepare_jacobian_inv", 0)
+    g.call("gmi_broyden_find_update")
+    ug = g.broyden(2)
+    uo = -cfg.jacobian_confidence * o.jacobian_inv_mult(eg)
+    print(f"  J^-1 e rel err {rel(ug, uo):.2e}", flush=True)
+    assert rel(ug, uo) < 1e-10
+    cfg2 = geometry.sim_config(nfil=6, nseg=nseg, nblob=162, R=6.0, seed=2)
+    o, g = orc.Sim(cfg2), capi.Sim(cfg2)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    DL = cfg2.swimmers[0].DL
+    for nt in range(4):
+        ro, rg = o.step(nt), g.step(nt)
+        so, sg = o.state(), g.state()
+        dx = np.abs(sg["x"] - so["x"]).max()
+        du = np.abs(sg["u"] - so["u"]).max()
+        print(f"  step {nt}: oracle {ro} gpu {rg} |dx| {dx:.2e} |du| {du:.2e}", flush=True)
+        assert ro[0] == 0 and rg[0] == 0
+        assert abs(ro[1] - rg[1]) <= 1
+        assert dx <= DL * cfg2.tol and du <= cfg2.tol
+
+
+_ORACLE_9122 = {}
+
+
+def _oracle_9122_steps(cfg, nsteps):
+    """States of the oracle's first `nsteps` steps of the 9,122-particle
+    swimmer (a few seconds per step on 16 host threads), once per session."""
+    if not _ORACLE_9122:
+        o = orc.Sim(cfg)
+        o.make_body_reference_matrices()
+        out = []
+        for nt in range(nsteps):
+            r = o.step(nt)
+            print(f"  oracle step_9122 step {nt}: {r}", flush=True)
+            out.append((r, o.state()))
+        _ORACLE_9122["steps"] = out
+    return _ORACLE_9122["steps"]
+
+
+@pytest.mark.parametrize("precision", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
+def test_step_9122_matches_oracle(gpu, precision):
+    """The 5,120-segment swimmer (256 x 20 segments + 4,002 blobs), three
+    steps on the GPU against the CPU oracle, FP64 and mixed precision:
+    Broyden counts within one, positions within DL*TOL, Lie elements within
+    TOL."""
+    cfg = geometry.sim_workload("step_9122", seed=1)
+    print(f"step_9122 precision {precision}", flush=True)
+    ref = _oracle_9122_steps(cfg, 3)
+    g = capi.Sim(cfg, precision=precision)
+    if precision == capi.GMI_FP32_KAHAN:
+        g.set_sim_precision(capi.GMI_FP32_KAHAN)
+    g.make_body_reference_matrices()
+    DL = cfg.swimmers[0].DL
+    for nt, (ro, so) in enumerate(ref):
+        rg = g.step(nt)
+        sg = g.state()
+        dx = np.abs(sg["x"] - so["x"]).max()
+        du = np.abs(sg["u"] - so["u"]).max()
+        print(f"  step {nt}: oracle {ro} gpu {rg} |dx| {dx:.2e} |du| {du:.2e}", flush=True)
+        assert ro[0] == 0 and rg[0] == 0
+        assert abs(ro[1] - rg[1]) <= 1
+        assert dx <= DL * cfg.tol and du <= cfg.tol
