@@ -24,6 +24,7 @@ HIP_SOURCES = [
     "gmi_seed.hip",
     "gmi_cilia.hip",
     "gmi_dense.hip",
+    "gmi_noalloc.hip",
 ]
 # rocBLAS supplies the DGEMMs of the one-time body-frame inversion (gmi_dense.hip)
 LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

@@ -19,6 +19,8 @@ GMI_ERR_HIP = -2
 GMI_ERR_STATE = -3
 GMI_ERR_NOT_FINITE = -4
 GMI_ERR_SINGULAR = -5
+GMI_ERR_TOO_LARGE = -7
+GMI_NOALLOC_MAX_TILES = 4096
 
 GMI_FP64 = 0
 GMI_BLOCK_SS, GMI_BLOCK_SB, GMI_BLOCK_BS, GMI_BLOCK_BB = 1, 2, 4, 8
@@ -78,6 +80,9 @@ SIGNATURES = [
     ("gmi_autotune", C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     ("gmi_autotune_blocks", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip]),
     ("gmi_set_wall", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_evaluate_grand_mobility_noalloc", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+      C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p]),
     # gmi_sim.h
     ("gmi_sim_create", C.c_int, [C.c_void_p, C.c_void_p]),
     ("gmi_make_body_reference_matrices", C.c_int, [C.c_void_p]),

@@ -158,6 +158,42 @@ int gmi_autotune(gmi_ctx *ctx, int reps, int *best_target_blocks, int *best_min_
 int gmi_autotune_blocks(gmi_ctx *ctx, int mask, int reps, int *best_target_blocks,
                         int *best_min_chunk);
 
+/* ---- grand apply that allocates nothing ------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  It is gmi_evaluate_grand_mobility (same kernels, same
+ * source splits as a context with the default launch shape) on arrays the
+ * caller owns, with no context:
+ *   - every pointer is DEVICE memory of the current device, SoA: component c
+ *     of particle i at p[c*ld + i] (x_*: 3 rows, f_segs: 6 rows (f, tau),
+ *     f_blobs: 3 rows, v_segs: 6 rows (v, omega), v_blobs: 3 rows), ld >= n;
+ *   - v_segs = M_ss f_segs + M_sb f_blobs, v_blobs = M_bs f_segs + M_bb f_blobs
+ *     over all particles (one interacting group), precision GMI_FP64 or
+ *     GMI_FP32_KAHAN;
+ *   - work is queued on `stream` (a hipStream_t; NULL: the null stream) and
+ *     is complete when that stream is;
+ *   - no hipMalloc/hipFree and no allocating library call: the tile table,
+ *     work list and split partials live in a fixed device array of
+ *     GMI_NOALLOC_SCRATCH_BYTES per device, taken on the first call on a
+ *     device (make that first call outside graph capture) and held until the
+ *     process ends;
+ *   - limit: ceil(n_segs/256) + ceil(n_blobs/256) <= GMI_NOALLOC_MAX_TILES
+ *     (at least 1,048,064 particles); larger inputs return GMI_ERR_TOO_LARGE
+ *     before any device work; devices numbered >= GMI_NOALLOC_MAX_DEVICES
+ *     return GMI_ERR_ARG;
+ *   - all calls on one device share its array: calls that overlap on one
+ *     device (several streams or host threads) race; calls queued on one
+ *     stream are ordered. */
+#define GMI_ERR_TOO_LARGE (-7)
+#define GMI_NOALLOC_MAX_TILES 4096
+#define GMI_NOALLOC_MAX_DEVICES 64
+#define GMI_NOALLOC_SCRATCH_BYTES 31637504
+int gmi_evaluate_grand_mobility_noalloc(
+    const double *x_segs, const double *f_segs, int n_segs, int ld_segs,
+    const double *x_blobs, const double *f_blobs, int n_blobs, int ld_blobs,
+    double *v_segs, double *v_blobs, double mu, double a_seg, double a_blob, int precision,
+    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

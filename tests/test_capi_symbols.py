@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_covers_every_declared_symbol():
     bound = {n for n, _, _ in capi.SIGNATURES}
-    assert declared_symbols() <= bound | {"gmi_noalloc_placeholder"}
+    assert declared_symbols() <= bound
 
 
 def test_create_rejects_bad_parameters_without_device():
