@@ -240,6 +240,12 @@ class SimConfig:
     barrier_factor: float = 0.1       # K = barrier_factor * kb
     curvature_period: float = 1.0
     curvature_amp: float = None       # default 0.5 / L
+    # the reference's default CILIA_TYPE 0 physics: a follower force of this
+    # magnitude at every filament tip (END_FORCE_MAGNITUDE) and, at nt = 0,
+    # per filament the two perturbation forces on segments 4 and 12
+    # ((n_filaments, 6) array, or None)
+    end_force: float = 0.0
+    perturbation: np.ndarray = None
 
     def __post_init__(self):
         g0 = self.swimmers[0]
@@ -266,9 +272,14 @@ class SimConfig:
                                    for g in self.swimmers])
         body_x = np.concatenate([np.asarray(g.centre, float) for g in self.swimmers])
         radius = np.array([g.R for g in self.swimmers], dtype=np.float64)
+        pert = None
+        if self.perturbation is not None:
+            pert = np.ascontiguousarray(self.perturbation, dtype=np.float64).reshape(-1)
+            assert pert.size == 6 * int(nfil.sum())
         return dict(nfil=nfil, nblob=nblob, blob_refs=np.ascontiguousarray(blob_refs),
                     fil_refs=np.ascontiguousarray(fil_refs), body_x=body_x,
-                    phases=np.ascontiguousarray(self.phases, dtype=np.float64), radius=radius)
+                    phases=np.ascontiguousarray(self.phases, dtype=np.float64), radius=radius,
+                    perturbation=pert)
 
 
 def sim_config(nfil, nseg, nblob, R, seed=0, **kw):

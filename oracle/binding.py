@@ -160,6 +160,7 @@ class OrcSimParams(C.Structure):
         ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
         ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
         ("sphere_radius", C.c_double), ("sphere_radii", _dp),
+        ("end_force", C.c_double), ("perturbation", _dp),
     ]
 
 
@@ -198,6 +199,15 @@ def _sim_lib():
         lib.orc_sim_get_backup.restype = None
         lib.orc_sim_set_backup.argtypes = [vp, _dp]
         lib.orc_sim_set_backup.restype = None
+        lib.orc_test_filament_forces.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                                 _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                                 _dp, _dp]
+        lib.orc_test_filament_forces.restype = None
+        lib.orc_sim_test_set_kinematics.argtypes = [vp, _dp, _dp, _dp, _dp]
+        lib.orc_sim_test_set_kinematics.restype = None
+        lib.orc_sim_test_broyden_begin.argtypes = [vp, C.c_int]
+        lib.orc_sim_test_broyden_iterate.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+        lib.orc_sim_test_broyden_columns.argtypes = [vp, _dp, _dp]
         lib._sim_ready = True
     return lib
 
@@ -217,8 +227,10 @@ class Sim:
             g0.mu, g0.a_seg, g0.a_blob, cfg.kb, cfg.kt, cfg.dt, cfg.tol,
             cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
             cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
-            float(a["radius"][0]), _p(a["radius"]))
+            float(a["radius"][0]), _p(a["radius"]), cfg.end_force,
+            _p(a["perturbation"]) if a["perturbation"] is not None else None)
         self.h = self.lib.orc_sim_create(C.byref(self.p), nthreads or nthreads_default())
+        assert self.h, "oracle refused the configuration"
         self.Ns = self.lib.orc_sim_num_segments(self.h)
         self.Nb = self.lib.orc_sim_num_blobs(self.h)
         self.nbroy = self.lib.orc_sim_nbroy(self.h)
@@ -303,6 +315,30 @@ class Sim:
         self.lib.orc_sim_test_jacobian_inv_mult(self.h, _p(v), _p(out))
         return out
 
+    def set_kinematics(self, seg, v_segs, body, v_bodies):
+        """seg: (Ns, 15) = x, xm1, xm2, u, um1; body: (nsw, 15); v_segs (Ns, 6);
+        v_bodies (nsw, 6)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                for a in (seg, v_segs, body, v_bodies)]
+        assert arrs[0].size == 15 * self.Ns and arrs[1].size == 6 * self.Ns
+        assert arrs[2].size == 15 * self.nsw and arrs[3].size == 6 * self.nsw
+        self.lib.orc_sim_test_set_kinematics(self.h, *[_p(a) for a in arrs])
+
+    def broyden_begin(self, nt):
+        rc = self.lib.orc_sim_test_broyden_begin(self.h, nt)
+        assert rc >= 0, rc
+        return bool(rc)
+
+    def broyden_iterate(self, nt):
+        dx, eo, en = (np.zeros(self.nbroy) for _ in range(3))
+        big = self.lib.orc_sim_test_broyden_iterate(self.h, nt, _p(dx), _p(eo), _p(en))
+        return dx, eo, en, bool(big)
+
+    def broyden_columns(self, max_iter):
+        Cm = np.zeros(self.nbroy * max_iter); Dm = np.zeros(self.nbroy * max_iter)
+        it = self.lib.orc_sim_test_broyden_columns(self.h, _p(Cm), _p(Dm))
+        return Cm.reshape(self.nbroy, max_iter), Dm.reshape(self.nbroy, max_iter), it
+
     def filament_jacobian(self, fil, nt):
         n = 6 * self.nseg
         J = np.zeros(n * n); Ji = np.zeros(n * n)
@@ -351,6 +387,19 @@ def dexpinv(u, v):
 
 def dexpinv_transpose(u, v):
     return _uv("orc_lie_dexpinv_transpose", u, v)
+
+
+def filament_forces(q, lam, teth, clamp, u0, st, kb, kt, DL, end_force=0.0, pert=None):
+    """Internal forces/torques (S, 6) of one filament in a given state."""
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    S = q.size // 4
+    f = np.zeros(6 * S)
+    args = [_v(lam, 3 * (S - 1)), _v(teth, 3), _v(clamp, 3), _v(u0, 3), _v(st, 3)]
+    lam_, teth_, clamp_, u0_, st_ = args
+    pp = _p(_v(pert, 6)) if pert is not None else None
+    load().orc_test_filament_forces(S, DL, kb, kt, _p(q), _p(u0_), _p(lam_), _p(teth_),
+                                    _p(clamp_), _p(st_), end_force, pp, _p(f))
+    return f.reshape(S, 6)
 
 
 def invert(A):

@@ -29,6 +29,12 @@ typedef struct orc_sim_params {
   double curvature_period;
   double sphere_radius;     /* body radius in the Schur RPY correction      */
   const double *sphere_radii; /* per swimmer, or NULL: sphere_radius for all */
+  double end_force;         /* END_FORCE_MAGNITUDE of the follower force at
+                               each filament tip (CILIA_TYPE 0); 0: none   */
+  const double *perturbation; /* per filament 6 values (perturbation1,
+                               perturbation2 of filament.cpp:336-360) added
+                               to the forces on segments 4 and 12 at nt = 0;
+                               NULL: none (needs NSEG >= 13)               */
 } orc_sim_params;
 
 typedef struct orc_sim orc_sim;
@@ -62,6 +68,29 @@ void orc_lie_dexpinv(const double *u, const double *v, double *out);
 void orc_lie_dexpinv_transpose(const double *u, const double *v, double *out);
 int orc_invert(double *A, int n);
 void orc_sim_test_initial_guess(orc_sim *s, int nt);
+/* Internal forces and torques of one filament in a given state
+ * (filament.cpp:1112-1220): q[4S] (s, v0, v1, v2), u0[3] (segment 0's Lie
+ * element, for the clamp torque), lam[3(S-1)], teth[3], clamp[3], strain
+ * twist st[3]; end_force and pert (6 values or NULL, applied as at nt = 0).
+ * Writes f[6S] (force, torque per segment). */
+void orc_test_filament_forces(int S, double DL, double kb, double kt, const double *q,
+                              const double *u0, const double *lam, const double *teth,
+                              const double *clamp, const double *st, double end_force,
+                              const double *pert, double *f);
+/* Sets the kinematic state the BDF2 residual reads: per segment 15 values
+ * (x, xm1, xm2, u, um1), v_segs[6Ns], per swimmer 15 values (x, xm1, xm2, u,
+ * um1) and v_bodies[6 per swimmer]. */
+void orc_sim_test_set_kinematics(orc_sim *s, const double *seg, const double *v_segs,
+                                 const double *body, const double *v_bodies);
+/* Broyden pieces: begin = the step prologue up to the approximate Jacobian
+ * (returns 1 if the residual is above tolerance); iterate = find_update,
+ * update, forces, velocities, residual, end_of_iter (forced to store the
+ * rank-one pair) with dx = the update applied, e_old/e_new the residuals
+ * before and after (returns 1 if above tolerance); columns copies C, D
+ * (nbroy x max_broyden_iter, row-major) and returns the iteration count. */
+int orc_sim_test_broyden_begin(orc_sim *s, int nt);
+int orc_sim_test_broyden_iterate(orc_sim *s, int nt, double *dx, double *e_old, double *e_new);
+int orc_sim_test_broyden_columns(const orc_sim *s, double *C, double *D);
 
 #ifdef __cplusplus
 }

@@ -9,11 +9,18 @@
 //   rigid body             rigid_body.cpp:247-381
 //   mobility solver        mobility_solver.cpp:132-192, 361-421, 651-727,
 //                          905-964, 970-1354, 1358-1402, 1404-1689, 1693-1771
-// Build-specific physics (kept identical on the device):
+// Physics (kept identical on the device):
+//  * the reference's default instability-driven cilia (CILIA_TYPE 0): a
+//    follower force -END_FORCE_MAGNITUDE t at each filament tip and, at
+//    nt = 0, the perturbations on segments 4 and 12 (filament.cpp:1177-1202),
+//    with their approximate-Jacobian and Schur-complement terms
+//    (filament.cpp:1466-1470; swimmer.cpp:497-503,557-571,630-633).  The
+//    perturbation is an input (the reference draws it from
+//    std::random_device, filament.cpp:336-360); end force 0 switches all of
+//    this off;
 //  * driving: preferred curvature kappa0(t) = amp sin(2 pi t / period + phi_f)
 //    in the binormal component of strain_twist (filament.cpp:1104-1106),
-//    evaluated at the end-of-step time t = (nt+1) dt; no end force and no
-//    random perturbation (filament.cpp:1177-1202 dropped);
+//    evaluated at the end-of-step time t = (nt+1) dt (amp 0: none);
 //  * swimmers are hydrodynamically independent; each swimmer has its own
 //    blob/filament references and its own body-mobility reference matrices;
 //  * the blob part of the preconditioner is scaled once per blob (the
@@ -44,6 +51,7 @@ struct Filament {
   double teth[3] = {0, 0, 0}, tethm1[3] = {0, 0, 0}, tethm2[3] = {0, 0, 0};
   double clamp[3] = {0, 0, 0}, clampm1[3] = {0, 0, 0}, clampm2[3] = {0, 0, 0};
   double st[3] = {0, 0, 0};               // strain_twist
+  double pert[6] = {0, 0, 0, 0, 0, 0};    // perturbation1, perturbation2
   double phase = 0;
   Mat Jinv;                               // 6S x 6S
   M3 ecb1, ecb2;                          // elastic clamping blocks
@@ -87,6 +95,7 @@ struct orc_sim {
   int last_gmres = 0;
   long long total_gmres = 0;
   int nthreads = 1;
+  int nt_cur = 0;           // time step of the forces (set_driving)
   double DL() const { return 2.2 * p.a_seg; }
 };
 
@@ -208,8 +217,10 @@ M3 bfm_lie_derivative(const Quat &q1, const Quat &q2, bool upper, double DL, dou
   return out;
 }
 
-// filament.cpp:1112-1220 (instability branch without end force / perturbation)
-void internal_forces(Filament &F, double DL, double KB, double KT) {
+// filament.cpp:1112-1220 (instability branch: end force at the tip, the
+// perturbations at nt = 0 when `perturb`)
+void internal_forces(Filament &F, double DL, double KB, double KT, double end_force,
+                     bool perturb) {
   const int S = int(F.seg.size());
   double *f = F.f;
   std::fill(f, f + 6 * S, 0.0);
@@ -230,6 +241,15 @@ void internal_forces(Filament &F, double DL, double KB, double KT) {
     for (int k = 0; k < 3; k++) { f[6 * i + 3 + k] += M[k]; f[6 * (i + 1) + 3 + k] -= M[k]; }
   }
   for (int k = 0; k < 3; k++) f[k] += F.teth[k];
+  if (end_force != 0.0) {
+    tangent(F.seg[S - 1].q, t);
+    for (int k = 0; k < 3; k++) f[6 * (S - 1) + k] -= end_force * t[k];
+    if (perturb)
+      for (int k = 0; k < 3; k++) {
+        f[6 * 4 + k] += F.pert[k];
+        f[6 * 12 + k] += F.pert[3 + k];
+      }
+  }
   double ct[3];
   dexpinv_transpose(ct, F.seg[0].u, F.clamp);
   for (int k = 0; k < 3; k++) f[3 + k] += ct[k];
@@ -335,6 +355,10 @@ bool invert_approx_jacobian(Filament &F, int nt, const orc_sim_params &p, double
     J.set3(3 * j, jpos, 0.5 * tm);
     for (int i = j + 1; i < S; i++) J.set3(3 * i, jpos, tm);
   }
+  if (p.end_force != 0.0) {  // filament.cpp:1466-1470
+    tangent(F.seg[S - 1].q, t);
+    J.add3(3 * (S - 1), 3 * (2 * S - 1), -(fac * p.end_force) * rcross(t));
+  }
   fac = (nt < p.num_euler_steps) ? DL * p.dt / (16.0 * kPi * mu * a * a * a)
                                  : DL * p.dt / (24.0 * kPi * mu * a * a * a);
   for (int j = 1; j < S; j++) {
@@ -425,10 +449,10 @@ void sw_initial_guess(Swimmer &B, int nt, int neuler, double DL) {
 }
 
 // swimmer.cpp:276-325
-void sw_forces(Swimmer &B, const orc_sim_params &p, double DL) {
+void sw_forces(Swimmer &B, const orc_sim_params &p, double DL, int nt) {
   std::fill(B.f, B.f + 6, 0.0);
   for (Filament &F : B.fil) {
-    internal_forces(F, DL, p.kb, p.kt);
+    internal_forces(F, DL, p.kb, p.kt, p.end_force, nt == 0 && p.perturbation);
     const int S = int(F.seg.size());
     for (int m = 0; m < S; m++) {
       const double *fs = &F.f[6 * m];
@@ -486,6 +510,13 @@ bool sw_prepare_jacobian_inv(Swimmer &B, int nt, const orc_sim_params &p, double
     double ucl[3];
     cross(B.u, F.clamp, ucl);
     old = old + 0.5 * rcross(F.clamp) - (1.0 / 12.0) * (rcross(ucl) + blc * rcross(F.clamp));
+    if (p.end_force != 0.0) {  // swimmer.cpp:497-503
+      double t[3], v[3];
+      tangent(F.seg[S - 1].q, t);
+      const double dn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      for (int k = 0; k < 3; k++) { t[k] *= p.end_force; v[k] = (0.5 * DL + dn) * d[k] / dn; }
+      old = old + rcross(t) * rcross(v);
+    }
   }
   const Mat bm = rotate66(B.Nref, B.Qinit);
   old = bm.get3(3, 3) * old;
@@ -503,6 +534,23 @@ bool sw_prepare_jacobian_inv(Swimmer &B, int nt, const orc_sim_params &p, double
     mv3(B.Qinit, &B.fil_ref[3 * i], d);
     Bm.set3(3, 0, rcross(d));
     Bm.set3(3, 3 * S, -1.0 * DinvT);
+    if (p.end_force != 0.0) {  // swimmer.cpp:557-571
+      double t[3], ft[3], arm[3];
+      tangent(F.seg[S - 1].q, t);
+      for (int k = 0; k < 3; k++) {
+        ft[k] = p.end_force * t[k];
+        arm[k] = 0.5 * DL * t[k] - (F.seg[S - 1].x[k] - B.x[k]);
+      }
+      const M3 Ef = rcross(ft);
+      Bm.set3(0, 6 * S - 3, Ef);
+      Bm.set3(3, 6 * S - 3, rcross(arm) * Ef);
+      for (int m = 1; m < S - 1; m++) {
+        double tmv[3];
+        tangent(F.seg[m].q, tmv);
+        for (int k = 0; k < 3; k++) tmv[k] *= DL;
+        Bm.set3(3, 3 * (S + m), Ef * rcross(tmv));
+      }
+    }
     Bm = matmul(bm, Bm);
     for (int n = 0; n < S; n++) {
       double rh[3] = {F.seg[n].x[0] - B.x[0], F.seg[n].x[1] - B.x[1], F.seg[n].x[2] - B.x[2]};
@@ -526,6 +574,10 @@ bool sw_prepare_jacobian_inv(Swimmer &B, int nt, const orc_sim_params &p, double
         Bm.add3(0, 3, 0.5 * DL * (Mtr * tm) - Mtt);
         Bm.add3(3, 3, 0.5 * DL * (Mrr * tm) - Mtr);
       } else if (n == S - 1) {
+        if (p.end_force != 0.0) {  // swimmer.cpp:630-633
+          Bm.add3(0, 6 * S - 3, -p.end_force * (Mtt * tm));
+          Bm.add3(3, 6 * S - 3, -p.end_force * (Mtr * tm));
+        }
         Bm.add3(0, 3 * S - 3, 0.5 * DL * (Mtr * tm) + Mtt);
         Bm.add3(3, 3 * S - 3, 0.5 * DL * (Mrr * tm) + Mtr);
       } else {
@@ -914,6 +966,7 @@ void end_of_iter(orc_sim *s, bool too_large) {
 }
 
 void set_driving(orc_sim *s, int nt) {
+  s->nt_cur = nt;
   const double t = (nt + 1) * s->p.dt;
   for (Swimmer &B : s->sw)
     for (Filament &F : B.fil)
@@ -921,7 +974,7 @@ void set_driving(orc_sim *s, int nt) {
 }
 
 void all_forces(orc_sim *s) {
-  for (Swimmer &B : s->sw) sw_forces(B, s->p, s->DL());
+  for (Swimmer &B : s->sw) sw_forces(B, s->p, s->DL(), s->nt_cur);
 }
 
 }  // namespace
@@ -931,6 +984,7 @@ void all_forces(orc_sim *s) {
 extern "C" {
 
 orc_sim *orc_sim_create(const orc_sim_params *p, int nthreads) {
+  if (p->perturbation && p->nseg < 13) return nullptr;  // segments 4 and 12
   auto s = std::make_unique<orc_sim>();
   s->p = *p;
   s->S = p->nseg;
@@ -975,6 +1029,8 @@ orc_sim *orc_sim_create(const orc_sim_params *p, int nthreads) {
       F.lam.assign(3 * (p->nseg - 1), 0); F.lamm1 = F.lam; F.lamm2 = F.lam;
       F.f = &s->fs[6 * (B.seg0 + i * p->nseg)];
       F.phase = p->fil_phase ? p->fil_phase[fg] : 0.0;
+      if (p->perturbation)
+        for (int k = 0; k < 6; k++) F.pert[k] = p->perturbation[6 * fg + k];
       const double *r = &B.fil_ref[3 * i];
       const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
       const double d[3] = {r[0] / rn, r[1] / rn, r[2] / rn};
@@ -1253,6 +1309,88 @@ int orc_sim_test_errors(orc_sim *s, int nt, double *out) {
 void orc_sim_test_initial_guess(orc_sim *s, int nt) {
   set_driving(s, nt);
   for (Swimmer &B : s->sw) sw_initial_guess(B, nt, s->p.num_euler_steps, s->DL());
+}
+
+void orc_test_filament_forces(int S, double DL, double kb, double kt, const double *q,
+                              const double *u0, const double *lam, const double *teth,
+                              const double *clamp, const double *st, double end_force,
+                              const double *pert, double *f) {
+  Filament F;
+  F.seg.resize(S);
+  for (int j = 0; j < S; j++) {
+    F.seg[j].q.s = q[4 * j];
+    for (int k = 0; k < 3; k++) F.seg[j].q.v[k] = q[4 * j + 1 + k];
+  }
+  for (int k = 0; k < 3; k++) {
+    F.seg[0].u[k] = u0[k];
+    F.teth[k] = teth[k];
+    F.clamp[k] = clamp[k];
+    F.st[k] = st[k];
+  }
+  F.lam.assign(lam, lam + 3 * (S - 1));
+  if (pert)
+    for (int k = 0; k < 6; k++) F.pert[k] = pert[k];
+  F.f = f;
+  internal_forces(F, DL, kb, kt, end_force, pert != nullptr);
+}
+
+void orc_sim_test_set_kinematics(orc_sim *s, const double *seg, const double *v_segs,
+                                 const double *body, const double *v_bodies) {
+  int g = 0;
+  for (size_t n = 0; n < s->sw.size(); n++) {
+    Swimmer &B = s->sw[n];
+    for (Filament &F : B.fil)
+      for (Segment &sg : F.seg) {
+        const double *r = seg + 15 * g;
+        for (int k = 0; k < 3; k++) {
+          sg.x[k] = r[k]; sg.xm1[k] = r[3 + k]; sg.xm2[k] = r[6 + k];
+          sg.u[k] = r[9 + k]; sg.um1[k] = r[12 + k];
+        }
+        g++;
+      }
+    const double *r = body + 15 * n;
+    for (int k = 0; k < 3; k++) {
+      B.x[k] = r[k]; B.xm1[k] = r[3 + k]; B.xm2[k] = r[6 + k];
+      B.u[k] = r[9 + k]; B.um1[k] = r[12 + k];
+    }
+    for (int k = 0; k < 6; k++) s->v_bodies[6 * n + k] = v_bodies[6 * n + k];
+  }
+  std::copy(v_segs, v_segs + 6 * s->Ns, s->vs.begin());
+}
+
+int orc_sim_test_broyden_begin(orc_sim *s, int nt) {
+  set_driving(s, nt);
+  const double DL = s->DL();
+  for (Swimmer &B : s->sw) sw_initial_guess(B, nt, s->p.num_euler_steps, DL);
+  all_forces(s);
+  sync_positions(s);
+  compute_velocities(s);
+  const bool big = compute_errors(s, s->err, nt);
+  for (Swimmer &B : s->sw)
+    if (!sw_prepare_jacobian_inv(B, nt, s->p, DL)) return -5;
+  s->iter = 0;
+  return big ? 1 : 0;
+}
+
+int orc_sim_test_broyden_iterate(orc_sim *s, int nt, double *dx, double *e_old, double *e_new) {
+  const double DL = s->DL();
+  find_update(s);
+  std::copy(s->upd.begin(), s->upd.end(), dx);
+  for (Swimmer &B : s->sw) sw_update(B, &s->upd[B.err0], DL);
+  all_forces(s);
+  sync_positions(s);
+  compute_velocities(s);
+  const bool big = compute_errors(s, s->new_err, nt);
+  std::copy(s->err.begin(), s->err.end(), e_old);
+  std::copy(s->new_err.begin(), s->new_err.end(), e_new);
+  end_of_iter(s, true);
+  return big ? 1 : 0;
+}
+
+int orc_sim_test_broyden_columns(const orc_sim *s, double *C, double *D) {
+  std::copy(s->C.d.begin(), s->C.d.end(), C);
+  std::copy(s->D.d.begin(), s->D.d.end(), D);
+  return s->iter;
 }
 
 }  // extern "C"

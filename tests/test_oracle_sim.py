@@ -15,7 +15,18 @@ against facts independent of its own code:
   differences of the robot-arm recurrence;
 * a straight unforced swimmer stays at rest; a converged step meets the
   BDF2 residual bound; error paths (non-convergence) report as the
-  reference's driver would.
+  reference's driver would;
+* the BDF2 / backward-Euler residual vanishes on paths the schemes
+  integrate exactly (quadratic positions and Lie increments, angular
+  velocities from scipy finite differences);
+* internal forces: net force and torque of a free filament are those of the
+  end force and perturbations alone, forces and torques rotate with the
+  filament, a uniform bend or twist gets the closed-form restoring moment
+  K 4 sin(theta/4)/DL, and the torques are minus the gradient of the
+  discrete elastic energy to second order;
+* the end force's approximate-Jacobian block against finite differences of
+  the force;
+* the Broyden secant condition H_{k+1} (e_{k+1} - e_k) = dx_k.
 """
 import math
 
@@ -225,3 +236,277 @@ def test_non_convergence_is_reported():
     s.make_body_reference_matrices()
     rc, bi, gi = s.step(0)
     assert rc == -6 and bi == 1
+
+
+# ---------------------------------------------------------------------------
+# Step mechanics pinned by exact solutions, balance laws and closed forms
+# (compute_errors, internal forces and moments, the end force, the Broyden
+# compact inverse update)
+
+def _rotmats(qs):
+    return Rotation.from_quat(np.column_stack([qs[:, 1], qs[:, 2], qs[:, 3], qs[:, 0]])).as_matrix()
+
+
+def _quat(R):
+    x, y, z, w = Rotation.from_matrix(R).as_quat()
+    return np.array([w, x, y, z])
+
+
+def _quadratic_path(rng, dt):
+    """x(tau) = a + b tau + c tau^2 sampled at tau = -dt, 0, dt (tau measured
+    from t_{n-1}): returns x, xm1, xm2 and v = x'(dt)."""
+    a, b, c = rng.normal(size=3), rng.normal(size=3), rng.normal(size=3)
+    x = lambda t: a + b * t + c * t * t
+    return x(dt), x(0.0), x(-dt), b + 2 * c * dt
+
+
+def _lie_path(rng, dt, quadratic=True):
+    """Rotation path q(tau) = exp(u(tau)) q_{n-1} with u(tau) = al tau +
+    be tau^2 (be = 0: linear): u = u(dt), um1 = -u(-dt) (the increment from
+    q_{n-2} to q_{n-1}) and the spatial angular velocity at tau = dt from a
+    central finite difference of the rotation matrices (scipy)."""
+    al, be = rng.normal(size=3), (rng.normal(size=3) if quadratic else np.zeros(3))
+    u = lambda t: al * t + be * t * t
+    R = lambda t: Rotation.from_rotvec(u(t)).as_matrix()
+    h = 1e-6
+    W = (R(dt + h) - R(dt - h)) / (2 * h) @ R(dt).T
+    w = np.array([W[2, 1], W[0, 2], W[1, 0]])
+    return u(dt), -u(-dt), w
+
+
+@pytest.mark.parametrize("nt", [0, 5])
+def test_bdf2_residual_vanishes_on_exact_paths(nt):
+    """compute_errors (mobility_solver.cpp:1404-1504,1641-1689) returns zero
+    when every segment and the body follow paths the scheme integrates
+    exactly: quadratic positions and a quadratic Lie-algebra increment for
+    BDF2 (nt >= NUM_EULER_STEPS), linear ones for the backward-Euler step
+    (nt = 0).  Velocities and angular velocities come from the paths
+    themselves (finite differences of scipy rotation matrices), not from the
+    oracle."""
+    cfg = geometry.sim_config(nfil=3, nseg=6, nblob=42, R=4.0, seed=2)
+    s = orc.Sim(cfg)
+    dt = cfg.dt * 7.0          # a larger step makes the check sharper
+    cfg_dt = cfg.dt
+    rng = np.random.default_rng(11 + nt)
+    bdf2 = nt >= cfg.num_euler_steps
+    seg, vseg = np.zeros((s.Ns, 15)), np.zeros((s.Ns, 6))
+    body, vbody = np.zeros((1, 15)), np.zeros((1, 6))
+
+    def sample():
+        x, xm1, xm2, v = _quadratic_path(rng, dt)
+        if not bdf2:   # linear path through xm1 with velocity v
+            x = xm1 + v * dt
+        u, um1, w = _lie_path(rng, dt, quadratic=bdf2)
+        return np.concatenate([x, xm1, xm2, u, um1]), v, w
+
+    for g in range(s.Ns):
+        row, v, w = sample()
+        seg[g], vseg[g] = row, np.concatenate([v, w])
+    row, v, w = sample()
+    body[0], vbody[0] = row, np.concatenate([v, w])
+    # the residual's time step is the configuration's: rescale the paths
+    scale = cfg_dt / dt
+    for arr in (vseg, vbody):
+        arr *= 1.0 / scale
+    s.set_kinematics(seg, vseg, body, vbody)
+    e, big = s.errors(nt)
+    S, nf = cfg.nseg, cfg.swimmers[0].nfil
+    mag = np.abs(seg).max()
+    assert not big
+    assert np.abs(e).max() <= 1e-9 * mag, np.abs(e).max()
+    # a wrong velocity on one segment shows up in that segment's rows only
+    vseg[3, 1] += 1.0
+    s.set_kinematics(seg, vseg, body, vbody)
+    e2, big2 = s.errors(nt)
+    coef = (2.0 / 3.0 if bdf2 else 1.0) * cfg_dt
+    fil, j = divmod(3, S)
+    row = 6 * S * fil + 3 * j + 1
+    assert abs(e2[row] + coef) <= 1e-9 and big2
+
+
+def _straight_then_bent(S, rng, bend=0.2):
+    """Unit quaternions of a randomly bent and twisted filament."""
+    R0 = Rotation.random(random_state=rng.integers(1 << 30)).as_matrix()
+    qs = [_quat(R0)]
+    R = R0
+    for _ in range(S - 1):
+        R = Rotation.from_rotvec(bend * rng.normal(size=3)).as_matrix() @ R
+        q = _quat(R)
+        qs.append(q if q @ qs[-1] >= 0 else -q)   # a continuous sign along the filament
+    return np.array(qs)
+
+
+def _arm(qs, DL, x0=None):
+    t = _rotmats(qs)[:, :, 0]
+    x = np.zeros((len(qs), 3)) if x0 is None else np.tile(x0, (len(qs), 1)).astype(float)
+    for i in range(1, len(qs)):
+        x[i] = x[i - 1] + 0.5 * DL * (t[i - 1] + t[i])
+    return x
+
+
+@pytest.mark.parametrize("end_force", [0.0, 204.5])
+def test_internal_forces_balance_and_rotate_with_the_filament(end_force):
+    """filament.cpp:1112-1220 on a free filament (tether and clamp zero):
+    the constraint forces are internal, so the net force is the end force
+    plus the perturbations alone and the net torque about any point (forces
+    at robot-arm positions) is the end force's moment alone -- whatever the
+    multipliers, bends and twists.  A rigid rotation of the filament and its
+    multipliers rotates every force and torque."""
+    rng = np.random.default_rng(21)
+    S, DL, kb, kt = 20, 2.2, 1800.0, 1300.0
+    qs = _straight_then_bent(S, rng)
+    lam = rng.normal(size=3 * (S - 1)) * 50
+    pert = rng.normal(size=6)
+    z3 = np.zeros(3)
+    st = np.array([0.0, 0.01, -0.02])
+    f = orc.filament_forces(qs, lam, z3, z3, z3, st, kb, kt, DL, end_force, pert)
+    x = _arm(qs, DL, x0=rng.normal(size=3))
+    t_end = _rotmats(qs[-1:])[0, :, 0]
+    if end_force == 0.0:     # the perturbations belong to the end-force build
+        pert = np.zeros(6)
+    F_ext = -end_force * t_end + pert[:3] + pert[3:]
+    np.testing.assert_allclose(f[:, :3].sum(0), F_ext, atol=1e-9 * np.abs(lam).max())
+    for p in (np.zeros(3), rng.normal(size=3) * 10):
+        T = (np.cross(x - p, f[:, :3]) + f[:, 3:]).sum(0)
+        T_ext = (np.cross(x[-1] - p, -end_force * t_end) + np.cross(x[4] - p, pert[:3]) +
+                 np.cross(x[12] - p, pert[3:]))
+        np.testing.assert_allclose(T, T_ext, atol=1e-8 * np.abs(f).max() * S * DL)
+    # covariance under a rigid rotation Q
+    Q = Rotation.random(random_state=7).as_matrix()
+    qQ = _quat(Q)
+    qs2 = np.array([quat_mul(qQ, q) for q in qs])
+    lam2 = (lam.reshape(-1, 3) @ Q.T).reshape(-1)
+    pert2 = (pert.reshape(2, 3) @ Q.T).reshape(-1)
+    f2 = orc.filament_forces(qs2, lam2, z3, z3, z3, st, kb, kt, DL, end_force, pert2)
+    if end_force == 0.0:
+        f3 = orc.filament_forces(qs, lam, z3, z3, z3, st, kb, kt, DL, 0.0, None)
+        np.testing.assert_array_equal(f3, f)
+    np.testing.assert_allclose(f2[:, :3], f[:, :3] @ Q.T, atol=1e-9 * np.abs(f).max())
+    np.testing.assert_allclose(f2[:, 3:], f[:, 3:] @ Q.T, atol=1e-9 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("axis,modulus", [(2, "kb"), (1, "kb"), (0, "kt")])
+def test_uniform_bend_gets_the_restoring_closed_form_moment(axis, modulus):
+    """A filament bent (about its binormal or normal) or twisted (about its
+    tangent) uniformly by theta per link: q_{i+1} = exp(theta a) q_i.  The
+    discrete strain 2 conj(q_mid)(q_{i+1} - q_i)/DL is 4 sin(theta/4)/DL
+    along a, so each link carries the moment M = K 4 sin(theta/4)/DL a
+    (K = KB for bending, KT for twist, filament.cpp:1096-1110): +M on the
+    first segment, -M on the last -- the distal end is turned back against
+    the bend -- and nothing on the interior ones.  A preferred strain equal
+    to it (strain_twist) leaves no moment at all."""
+    S, DL, kb, kt, theta = 12, 2.2, 1800.0, 1300.0, 0.3
+    a = np.eye(3)[axis]                       # frame axes of the identity segment
+    K = kb if modulus == "kb" else kt
+    qs = np.array([_quat(Rotation.from_rotvec(i * theta * a).as_matrix()) for i in range(S)])
+    z3 = np.zeros(3)
+    f = orc.filament_forces(qs, np.zeros(3 * (S - 1)), z3, z3, z3, z3, kb, kt, DL)
+    M = K * 4 * math.sin(theta / 4) / DL * a
+    np.testing.assert_allclose(f[:, :3], 0, atol=1e-12)
+    np.testing.assert_allclose(f[0, 3:], M, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(f[-1, 3:], -M, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(f[1:-1, 3:], 0, atol=1e-9 * K)
+    assert f[-1, 3:] @ a < 0                  # restoring on the distal end
+    st = 4 * math.sin(theta / 4) / DL * a     # body frame of every segment: a
+    f0 = orc.filament_forces(qs, np.zeros(3 * (S - 1)), z3, z3, z3, st, kb, kt, DL)
+    np.testing.assert_allclose(f0, 0, atol=1e-9 * K)
+
+
+def test_elastic_torques_are_the_bending_energy_gradient_to_second_order():
+    """For small bends the link moments are minus the gradient of the
+    discrete elastic energy E = sum_links DL/2 (k - k0)^T diag(KT,KB,KB)
+    (k - k0), with k the body-frame rotation vector of R_i^T R_{i+1} divided
+    by DL (computed here with scipy), under a spatial rotation of one
+    segment: the mismatch falls like theta^2."""
+    S, DL, kb, kt = 8, 2.2, 1800.0, 1300.0
+    Kd = np.array([kt, kb, kb])
+    z3 = np.zeros(3)
+
+    def energy(qs):
+        R = _rotmats(qs)
+        E = 0.0
+        for i in range(S - 1):
+            k = Rotation.from_matrix(R[i].T @ R[i + 1]).as_rotvec() / DL
+            E += 0.5 * DL * k @ (Kd * k)
+        return E
+
+    errs = []
+    for bend in (2e-2, 2e-3):
+        rng = np.random.default_rng(31)
+        qs = _straight_then_bent(S, rng, bend=bend)
+        f = orc.filament_forces(qs, np.zeros(3 * (S - 1)), z3, z3, z3, z3, kb, kt, DL)
+        g = np.zeros((S, 3))
+        h = 1e-4 * bend
+        for j in range(S):
+            for c in range(3):
+                d = np.zeros(3); d[c] = h
+                qp, qm = qs.copy(), qs.copy()
+                Rj = _rotmats(qs[j:j + 1])[0]
+                qp[j] = _quat(Rotation.from_rotvec(d).as_matrix() @ Rj)
+                qm[j] = _quat(Rotation.from_rotvec(-d).as_matrix() @ Rj)
+                g[j, c] = (energy(qp) - energy(qm)) / (2 * h)
+        errs.append(np.abs(f[:, 3:] + g).max() / np.abs(g).max())
+    assert errs[0] < 1e-2 and errs[1] < 1e-4 and errs[1] < errs[0] / 50, errs
+
+
+def test_end_force_jacobian_block_matches_finite_differences():
+    """The end-force entry of the approximate Jacobian (filament.cpp:
+    1466-1470) is the self-mobility factor times the derivative of the tip
+    force -END_FORCE t with respect to the tip segment's Lie algebra element,
+    taken here by finite differences of the oracle's force function."""
+    ef = 204.5
+    cfg = geometry.sim_config(nfil=3, nseg=20, nblob=42, R=4.0, seed=2, end_force=ef)
+    cfg0 = geometry.sim_config(nfil=3, nseg=20, nblob=42, R=4.0, seed=2)
+    mats = []
+    for c in (cfg, cfg0):
+        s = orc.Sim(c)
+        s.initial_guess(3)
+        s.forces()
+        mats.append(s.filament_jacobian(0, 3)[0])
+        q = s.state()["q"][:c.nseg]
+    S, DL = cfg.nseg, cfg.swimmers[0].DL
+    blk = (mats[0] - mats[1])[3 * (S - 1):3 * S, 3 * (2 * S - 1):]
+    z3 = np.zeros(3)
+    fac = -cfg.dt / (9 * math.pi * cfg.swimmers[0].mu * cfg.swimmers[0].a_seg)   # BDF2 row
+    fd = np.zeros((3, 3))
+    e = 1e-6
+    for c in range(3):
+        du = np.zeros(3); du[c] = e
+        qp, qm = q.copy(), q.copy()
+        qp[-1] = quat_mul(orc.lie_exp(du), q[-1])
+        qm[-1] = quat_mul(orc.lie_exp(-du), q[-1])
+        fp = orc.filament_forces(qp, np.zeros(3 * (S - 1)), z3, z3, z3, z3, 1.0, 1.0, DL, ef)
+        fm = orc.filament_forces(qm, np.zeros(3 * (S - 1)), z3, z3, z3, z3, 1.0, 1.0, DL, ef)
+        fd[:, c] = (fp[-1, :3] - fm[-1, :3]) / (2 * e)
+    np.testing.assert_allclose(blk, fac * fd, rtol=1e-6, atol=1e-9 * abs(fac) * ef)
+
+
+def test_broyden_update_satisfies_the_secant_condition():
+    """After end_of_iter the inverse-Jacobian approximation
+    H_{k+1} = gamma J~^-1 + sum_i C_i D_i^T maps the residual change of the
+    iteration onto the update it applied: H_{k+1} (e_{k+1} - e_k) = dx_k
+    (broyden_solver.cpp:70-130), at the first and the second iteration (the
+    second also exercises the earlier rank-one pairs); dx_k itself is
+    -H_k e_k (broyden_solver.cpp:26-68).  H is assembled here from the
+    stored columns and the oracle's J~^-1 products."""
+    cfg = geometry.sim_config(nfil=4, nseg=10, nblob=92, R=4.0, seed=3)
+    s = orc.Sim(cfg)
+    s.make_body_reference_matrices()
+    assert s.broyden_begin(0)
+    g = cfg.jacobian_confidence
+
+    def H(v, it):
+        Cm, Dm, _ = s.broyden_columns(cfg.max_broyden_iter)
+        out = g * s.jacobian_inv_mult(v)
+        for i in range(it):
+            out += Cm[:, i] * (Dm[:, i] @ v)
+        return out
+
+    for k in range(2):
+        Hk_e = None
+        dx, eo, en, big = s.broyden_iterate(0)
+        _, _, it = s.broyden_columns(cfg.max_broyden_iter)
+        assert it == k + 1
+        np.testing.assert_allclose(-H(eo, k), dx, rtol=0, atol=1e-11 * np.abs(dx).max())
+        lhs = H(en - eo, k + 1)
+        np.testing.assert_allclose(lhs, dx, rtol=0, atol=1e-10 * np.abs(dx).max())
