@@ -288,6 +288,7 @@ class GmiSimParams(C.Structure):
         ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
         ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
         ("sphere_radius", C.c_double), ("sphere_radii", _dp),
+        ("end_force", C.c_double), ("perturbation", _dp),
     ]
 
 
@@ -306,7 +307,8 @@ class Sim(Context):
             ptr(a["body_x"]), ptr(a["phases"]), cfg.kb, cfg.kt, cfg.dt, cfg.tol,
             cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
             cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
-            float(a["radius"][0]), ptr(a["radius"]))
+            float(a["radius"][0]), ptr(a["radius"]), float(cfg.end_force),
+            ptr(a["perturbation"]) if a["perturbation"] is not None else None)
         self.call("gmi_sim_create", C.byref(self.sp))
         self.nsw = len(a["nfil"])
         self.nfil_total = int(a["nfil"].sum())

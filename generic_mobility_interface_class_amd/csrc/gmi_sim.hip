@@ -224,7 +224,10 @@ __device__ d3 strain_twist(const SimDev &a, int f, double t) {
 // torque is assembled from its two links in the order of the sequential loop
 // over links (the same operations, so the same rounding), and lane 0 sums
 // the filament's share of the body balance in segment order.
-__global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t) {
+// The instability-driven build adds the follower force -efm t at the tip
+// and, when `perturb` (nt = 0), the two perturbation forces on segments 4 and
+// 12, after the link terms (filament.cpp:1177-1199).
+__global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t, int perturb) {
   __shared__ double sl[3 * kFilLanes], sM[3 * kFilLanes], sF[6 * kFilLanes];
   const int f = blockIdx.x, j = threadIdx.x;
   const int S = a.S, g0 = f * S, ld = a.lds;
@@ -259,12 +262,15 @@ __global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t) {
       Tj = Tj - (0.5 * a.DL) * cross3(tj, lam);
       Tj = Tj + M;
     }
-    if (j == 0) {
-      const d3 teth = rowvec(a.teth + 9 * f);
-      const d3 ct = dexpinv_T(ld3(a.U, ld, g0), rowvec(a.clamp + 9 * f));
-      Fj = Fj + teth;
-      Tj = Tj + ct;
+    if (j == 0) Fj = Fj + rowvec(a.teth + 9 * f);
+    if (a.efm != 0.0) {
+      if (j == S - 1) Fj = Fj - a.efm * tj;
+      if (perturb) {
+        if (j == 4) Fj = Fj + rowvec(a.pert + 6 * f);
+        if (j == 12) Fj = Fj + rowvec(a.pert + 6 * f + 3);
+      }
     }
+    if (j == 0) Tj = Tj + dexpinv_T(ld3(a.U, ld, g0), rowvec(a.clamp + 9 * f));
     const int g = g0 + j;
     F[g] = Fj.x; F[ld + g] = Fj.y; F[2 * ld + g] = Fj.z;
     F[3 * ld + g] = Tj.x; F[4 * ld + g] = Tj.y; F[5 * ld + g] = Tj.z;
@@ -829,6 +835,11 @@ __global__ __launch_bounds__(64) void k_fil_jacobian(SimDev a, int nt, double t)
       const m3 tm = mscale(DL * (j == k ? 0.5 : 1.0), rcross(seg_t(a, g0 + j)));
       put3(J, n, 3 * k, 3 * (S + j), tm);
     }
+    // tip position <- tip Lie element through the end force (filament.cpp:1466-1470)
+    if (a.efm != 0.0 && k == S - 1) {
+      const m3 e = mscale(fac1 * a.efm, rcross(seg_t(a, g)));
+      put3(J, n, 3 * k, 3 * (2 * S - 1), msub(get3(J, n, 3 * k, 3 * (2 * S - 1)), e));
+    }
     // Lie <- lambda (filament.cpp:1463-1486)
     const double fac2 = euler ? DL * dt / (16.0 * kPi * mu * as * as * as)
                               : DL * dt / (24.0 * kPi * mu * as * as * as);
@@ -1107,6 +1118,33 @@ __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
         for (int k = 0; k < 6; k++) s += bm[6 * r + k] * raw[k][c];
         B[r * n + (c < 3 ? c : 3 * S + c - 3)] = s;
       }
+    if (a.efm != 0.0) {
+      // the end force's couplings to the Lie elements (swimmer.cpp:557-571):
+      // tip column [Ef ; rcross(DL/2 t - (x_tip - x)) Ef], columns of
+      // segments 1..S-2 [0 ; Ef rcross(DL t_m)], Ef = rcross(efm t_tip);
+      // then premultiplied by the rotated body mobility like the rest of B
+      const int gt = f * S + S - 1;
+      const d3 tt = seg_t(a, gt);
+      const m3 Ef = rcross(a.efm * tt);
+      const d3 arm = (0.5 * a.DL) * tt - (ld3(a.X, a.lds, gt) - ga3(b + BX));
+      for (int m = 1; m < S; m++) {
+        m3 top = mzero(), bot;
+        if (m == S - 1) {
+          top = Ef;
+          bot = mm(rcross(arm), Ef);
+        } else {
+          bot = mm(Ef, rcross(a.DL * seg_t(a, f * S + m)));
+        }
+        const int c0 = 3 * (S + m);
+        for (int r = 0; r < 6; r++)
+          for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++)
+              s += bm[6 * r + k] * top.a[3 * k + j] + bm[6 * r + 3 + k] * bot.a[3 * k + j];
+            B[r * n + c0 + j] = s;
+          }
+      }
+    }
     // sphere-segment RPY corrections (swimmer.cpp:580-650)
     const double R = a.sw_R[a.fil_sw[f]], mu = a.mu, as = a.as;
     const d3 bx = ga3(b + BX);
@@ -1139,6 +1177,10 @@ __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
         add3(0, 3, msub(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
         add3(3, 3, msub(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
       } else if (s == S - 1) {
+        if (a.efm != 0.0) {  // swimmer.cpp:630-633
+          add3(0, 6 * S - 3, mscale(-a.efm, mm(Mtt, tm)));
+          add3(3, 6 * S - 3, mscale(-a.efm, mm(Mtr, tm)));
+        }
         add3(0, 3 * S - 3, madd(mscale(0.5 * a.DL, mm(Mtr, tm)), Mtt));
         add3(3, 3 * S - 3, madd(mscale(0.5 * a.DL, mm(Mrr, tm)), Mtr));
       } else {
@@ -1219,9 +1261,14 @@ __global__ __launch_bounds__(256) void k_schur_body(SimDev a, int nt) {
     const d3 d = mv(Qi, rowvec(a.fil_ref + 3 * f));
     const d3 teth = rowvec(a.teth + 9 * f), cl = rowvec(a.clamp + 9 * f);
     const d3 ucl = cross3(ub, cl);
-    const m3 t = madd(mscale(-1.0, mm(rcross(teth), rcross(d))),
-                      msub(mscale(0.5, rcross(cl)),
-                           mscale(1.0 / 12.0, madd(rcross(ucl), mm(blc, rcross(cl))))));
+    m3 t = madd(mscale(-1.0, mm(rcross(teth), rcross(d))),
+                msub(mscale(0.5, rcross(cl)),
+                     mscale(1.0 / 12.0, madd(rcross(ucl), mm(blc, rcross(cl))))));
+    if (a.efm != 0.0) {  // swimmer.cpp:497-503
+      const double dn = sqrt(dot3(d, d));
+      const d3 v = ((0.5 * a.DL + dn) / dn) * d;
+      t = madd(t, mm(rcross(a.efm * seg_t(a, f * a.S + a.S - 1)), rcross(v)));
+    }
     for (int k = 0; k < 9; k++) v[k] += t.a[k];
     for (int k = 0; k < 36; k++) v[9 + k] += a.fil_BA[36 * f + k];
   }
@@ -1630,6 +1677,9 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if (p->sphere_radii)
     for (int n = 0; n < c->nsw; n++)
       if (!(p->sphere_radii[n] > 0)) return fail(c, GMI_ERR_ARG, "sphere_radii must be positive");
+  if (!std::isfinite(p->end_force)) return fail(c, GMI_ERR_ARG, "end_force must be finite");
+  if (p->perturbation && S < 13)
+    return fail(c, GMI_ERR_ARG, "the perturbations act on segments 4 and 12: NSEG >= 13");
   SimHost *s = new SimHost();
   s->p = *p;
   // the caller's arrays need not outlive this call: keep copies of what is
@@ -1645,6 +1695,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
     s->p.body_x = nullptr;
     s->p.fil_phase = nullptr;
     s->p.sphere_radii = nullptr;
+    s->p.perturbation = nullptr;
   }
   s->precision = c->precision;
   c->extra = s;
@@ -1660,6 +1711,8 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   d.kb = p->kb; d.kt = p->kt; d.dt = p->dt; d.tol = p->tol; d.gamma = p->jacobian_confidence;
   d.gtol = p->gmres_tol; d.neuler = p->num_euler_steps;
   d.camp = p->curvature_amp; d.cper = p->curvature_period;
+  d.efm = p->end_force;
+  d.pert = nullptr;
   d.X = c->Xs; d.Fs = c->Fs; d.Vs = c->Vs; d.Xb = c->Xb; d.Vb = c->Vb; d.Frep = c->Frep;
   int rc;
   const size_t ls = size_t(d.lds), nn = size_t(6 * S) * (6 * S);
@@ -1789,6 +1842,12 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, d.blob_ref, bref.data(), bref.size()))) return rc;
   if ((rc = upload(c, d.fil_ref, p->fil_refs, 3 * size_t(Nf)))) return rc;
   if ((rc = upload(c, d.phase, phase.data(), phase.size()))) return rc;
+  if (p->perturbation) {
+    double *pp;
+    if ((rc = dalloc(c, s, &pp, 6 * size_t(Nf)))) return rc;
+    if ((rc = upload(c, pp, p->perturbation, 6 * size_t(Nf)))) return rc;
+    d.pert = pp;
+  }
   if ((rc = upload(c, d.X, X.data(), X.size()))) return rc;
   if ((rc = upload(c, d.Xm1, X.data(), X.size()))) return rc;
   if ((rc = upload(c, d.Xm2, X.data(), X.size()))) return rc;
@@ -1995,7 +2054,9 @@ int gmi_forces_and_torques(gmi_ctx *c, int nt) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   const double t = (nt + 1) * s->p.dt;
-  hipLaunchKernelGGL(k_fil_forces, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, t);
+  const int perturb = (nt == 0 && s->d.pert) ? 1 : 0;
+  hipLaunchKernelGGL(k_fil_forces, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, t,
+                     perturb);
   hipLaunchKernelGGL(k_body_forces, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d);
   return launch_ok(c, "forces");
 }

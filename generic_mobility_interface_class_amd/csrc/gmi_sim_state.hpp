@@ -20,6 +20,8 @@ struct SimDev {
   double DL, mu, as, ab, kb, kt, dt, tol, gamma, gtol;
   int neuler;
   double camp, cper;
+  double efm;          // END_FORCE_MAGNITUDE (0: no end force)
+  const double *pert;  // [Nf][6] perturbations at nt = 0, or null
   // index maps
   const int *sw_fil0, *sw_nfil, *sw_blob0, *sw_nblob, *fil_sw, *blob_sw;
   const double *sw_R;  // body radius per swimmer (Schur RPY correction)

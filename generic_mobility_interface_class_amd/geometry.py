@@ -302,6 +302,46 @@ def sim_workload(name, seed=0, **kw):
     return sim_config(seed=seed, **SIM_WORKLOADS[name], **kw)
 
 
+def end_force_magnitude(nseg, kb=1800.0, a_seg=1.0, dimensionless_force=220.0):
+    """END_FORCE_MAGNITUDE = DIMENSIONLESS_FORCE KB / (DL^2 NSEG^2)
+    (config.hpp:108-110, 239)."""
+    DL = 2.2 * a_seg
+    return dimensionless_force * kb / (DL * DL * nseg * nseg)
+
+
+def perturbations(nfil, magnitude, seed):
+    """Per filament the two initial perturbation forces of the reference
+    (filament.cpp:336-360): magnitude 0.01 END_FORCE_MAGNITUDE along
+    directions of normalised N(0,1) triples.  The reference draws them from
+    std::random_device; here they come from the project RNG (draw order:
+    filament by filament, perturbation1 then perturbation2)."""
+    rng = Xoshiro256ss(seed)
+    out = np.zeros((nfil, 6))
+    for f in range(nfil):
+        for k in range(2):
+            d = rng.normal(3)
+            out[f, 3 * k:3 * k + 3] = magnitude * d / np.linalg.norm(d)
+    return out
+
+
+def instability_config(swimmers, seed=0, **kw):
+    """The reference's default physics (CILIA_TYPE 0, config.hpp:13,108-110,
+    239-241): follower end force END_FORCE_MAGNITUDE at every filament tip,
+    barrier K = REPULSIVE_FORCE_FACTOR (2) x END_FORCE_MAGNITUDE, DT =
+    36.3833/STEPS_PER_PERIOD with 300 steps per period, no preferred
+    curvature, GMRES tolerance LINEAR_SYSTEM_TOL = 1e-10, and the nt = 0
+    perturbations (0.01 END_FORCE_MAGNITUDE) from `seed`."""
+    g0 = swimmers[0]
+    kb = kw.pop("kb", 1800.0)
+    ef = end_force_magnitude(g0.nseg, kb, g0.a_seg)
+    nf = sum(g.nfil for g in swimmers)
+    args = dict(dt=36.3833 / 300.0, kb=kb, kt=kb, curvature_amp=0.0, gmres_tol=1e-10,
+                barrier_factor=2.0 * ef / kb, end_force=ef,
+                perturbation=perturbations(nf, 0.01 * ef, seed))
+    args.update(kw)
+    return SimConfig(swimmers=list(swimmers), phases=np.zeros(nf), **args)
+
+
 def ensemble_config(n_swimmers=64, seed=0, nfil_min=16, nfil_max=512, nseg=20,
                     freq_min=8, freq_max=32, spacing=None, **kw):
     """Batch of hydrodynamically independent swimmers of uneven size.

@@ -49,6 +49,19 @@ typedef struct gmi_sim_params {
   const double *sphere_radii; /* per-swimmer body radius [n_swimmers], or
                                  NULL for sphere_radius on every swimmer (the
                                  reference has one R_OVER_L, config.hpp:126)  */
+  double end_force;           /* END_FORCE_MAGNITUDE (config.hpp:239): the
+                                 instability-driven cilia of the default
+                                 CILIA_TYPE 0 build -- a follower force
+                                 -end_force t at each filament tip, with its
+                                 Jacobian and Schur terms (filament.cpp:
+                                 1177-1202,1466-1470; swimmer.cpp:497-503,
+                                 557-571,630-633); 0: none                   */
+  const double *perturbation; /* [6 Nfil] or NULL: per filament the forces
+                                 perturbation1, perturbation2 added to
+                                 segments 4 and 12 at nt = 0 in the end-force
+                                 build (filament.cpp:336-360,1186-1199; the
+                                 reference draws them from std::random_device,
+                                 here they are an input); needs NSEG >= 13   */
 } gmi_sim_params;
 
 /* swimmers[n].initial_setup for all n + mobility.initialise()

@@ -510,3 +510,30 @@ def test_broyden_update_satisfies_the_secant_condition():
         np.testing.assert_allclose(-H(eo, k), dx, rtol=0, atol=1e-11 * np.abs(dx).max())
         lhs = H(en - eo, k + 1)
         np.testing.assert_allclose(lhs, dx, rtol=0, atol=1e-10 * np.abs(dx).max())
+
+
+def test_end_force_swimmer_steps_and_perturbs_only_the_first_step():
+    """The default instability-driven physics on a small swimmer: the
+    perturbation forces enter the first step only (forces of the same state
+    with and without the perturbation input differ by about their magnitude
+    at nt = 0 and agree bit for bit afterwards), and steps converge."""
+    g = geometry.SwimmerGeometry(nfil=4, nseg=20, nblob=162, R=6.0)
+    cfg = geometry.instability_config([g], seed=5)
+    cfg_np = geometry.instability_config([g], seed=5, perturbation=None)
+    o, o2 = orc.Sim(cfg), orc.Sim(cfg_np)
+    for s in (o, o2):
+        s.make_body_reference_matrices()
+    for nt in range(3):
+        for s in (o, o2):
+            s.initial_guess(nt)
+            s.forces()
+        d = np.abs(o.mobility()["f_segs"] - o2.mobility()["f_segs"]).max()
+        if nt == 0:
+            assert 0.5 * 0.01 * cfg.end_force < d <= 2 * 0.01 * cfg.end_force
+        else:
+            assert d == 0.0
+    s = orc.Sim(cfg)
+    s.make_body_reference_matrices()
+    for nt in range(4):
+        rc, bi, _ = s.step(nt)
+        assert rc == 0 and bi >= 1
