@@ -45,9 +45,10 @@ FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
 # Launch shape (target workgroups, minimum source chunk) of the time-stepping
 # legs.  The shape sets the order of the pair kernels' partial sums, so a
 # timed pick would change trajectories, and the networks trained on them, from
-# run to run.  This is the blob-blob autotuner's usual pick on these
-# workloads.
-STEP_LAUNCH_SHAPE = (4096, 128)
+# run to run.  Of the shapes timed on the 9,122-particle swimmer's steps
+# (target workgroups 1024-8192, minimum source chunks 64-256) this one gave
+# the shortest step.
+STEP_LAUNCH_SHAPE = (4096, 64)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0

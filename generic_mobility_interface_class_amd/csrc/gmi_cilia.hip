@@ -954,6 +954,10 @@ struct CiliaOp {
   }
   bool fp64_krylov() const { return false; }
   bool zero_start() const { return false; }
+  // Arnoldi steps go through the engine's generic kernels
+  bool fused() const { return false; }
+  int fused_parts() const { return 0; }
+  int fused_step(KrylovState *, int, bool, bool, const GivensArgs &) { return GMI_ERR_STATE; }
   int precond(double *in, double *out, const int *skip, int norm_slot) {
     CilDev &d = s->d;
     hipLaunchKernelGGL(k_cp1, dim3(s->ntiles), dim3(256), 0, c->stream, d, s->tiles, s->sw_tile0,
@@ -1243,7 +1247,8 @@ int gmi_cilia_create(gmi_ctx *c, const gmi_cilia_params *p) {
       (rc = calloc_dev(c, s, &s->Qk, np * (cols + 1))) ||
       (rc = calloc_dev(c, s, &s->H, size_t(cols + 1) * (cols + 1))) ||
       (rc = calloc_dev(c, s, &s->cs, cols + 1)) || (rc = calloc_dev(c, s, &s->sn, cols + 1)) ||
-      (rc = calloc_dev(c, s, &s->beta, cols + 1)) || (rc = calloc_dev(c, s, &s->h1, cols + 1)) ||
+      (rc = calloc_dev(c, s, &s->beta, cols + 1)) ||
+      (rc = calloc_dev(c, s, &s->bpre, cols + 1)) || (rc = calloc_dev(c, s, &s->h1, cols + 1)) ||
       (rc = calloc_dev(c, s, &s->h2, cols + 1)) || (rc = calloc_dev(c, s, &s->y, cols + 1)) ||
       (rc = calloc_dev(c, s, &s->z, np)) || (rc = calloc_dev(c, s, &s->tmp, np)) ||
       (rc = calloc_dev(c, s, &s->scal, 16)) || (rc = calloc_dev(c, s, &s->part, nblk(d.n, 256))) ||

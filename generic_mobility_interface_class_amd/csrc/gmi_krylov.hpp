@@ -17,6 +17,15 @@
 //   int precond(double *in, double *out, const int *skip, int norm_slot)
 //       out = P^-1 in; norm_slot >= 0: `in` is first divided in place by
 //       scal[norm_slot] (the previous Arnoldi vector's norm).
+//   bool fused(), int fused_parts(),
+//   int fused_step(KrylovState *s, int it, bool chunk_start, bool mixed,
+//                  const GivensArgs &g)
+//       an operator may run a whole Arnoldi step itself (preconditioner,
+//       operator, both Gram-Schmidt passes) on a row partition of its own,
+//       folding the Givens step of iteration it-1 into its first kernel
+//       unless the step starts a chunk (then a k_givens launch ran it); it
+//       leaves the new vector unnormalised, fused_parts() norm shares in
+//       s->part, and h1, h2 in s->h1, s->h2.
 // Every kernel either operator launches must return at once when `done`
 // (the F_DONE flag) is set, so that a chunk needs no host decisions.
 #pragma once
@@ -86,6 +95,12 @@ struct KrylovState {
   // the preconditioner is not applied again after the last step (the update
   // of flexible GMRES; identical in exact arithmetic for a fixed P).
   double *Zk = nullptr;
+  // Optional: the right-hand side of the least-squares problem before each
+  // Givens rotation touches it (bpre[c] = beta[c] as it was when column c's
+  // rotation was computed).  With it the rotation of a column can be
+  // computed redundantly by every workgroup of a kernel while one of them
+  // writes beta (the fused Arnoldi step).
+  double *bpre = nullptr;
   int *flags = nullptr;   // device flags (F_*)
   int *h_flags = nullptr; // pinned copies
   double *h_scal = nullptr;
@@ -171,7 +186,8 @@ __device__ void col_sums(const double *part, int k, int nb, double *hs, double *
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
     const double *p = part + size_t(j) * nb;
     double s = 0.0;
-#pragma unroll 4
+    // all the loads of a column in flight at once; the sum keeps its order
+#pragma unroll 16
     for (int b = 0; b < nb; b++) s += p[b];
     hs[j] = s;
     if (hout && blockIdx.x == 0) hout[j] = s;
@@ -275,30 +291,45 @@ __global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq,
   if (threadIdx.x == 0) norm_part[blockIdx.x] = s2;
 }
 
-// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234).  The
-// new vector's norm is the sum of the workgroup shares; H column = h1 + h2
-// (the two Gram-Schmidt passes).  The division by the norm happens in the
-// next iteration's k_precond_kt.
-__global__ __launch_bounds__(256) void k_givens(const double *norm_part, int nparts, int it,
-                                                const double *h1, const double *h2, double *H,
-                                                int ldh, double *cs, double *sn, double *beta,
-                                                int *flags, double *scal, double tol, int maxit,
-                                                double switch_rel) {
-  if (flags[F_DONE]) return;
-  __shared__ double sh[256];
-  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
+// The norm of the new Arnoldi vector from its workgroup shares (256
+// threads; the same order wherever it is taken).
+__device__ double norm_of_parts(const double *norm_part, int nparts, double *sh) {
   double s = 0.0;
   for (int p = threadIdx.x; p < nparts; p += 256) s += norm_part[p];
-  // the new column and the previous rotations, staged in LDS
+  return sqrt(block_sum<256>(s, sh));
+}
+
+// What the Givens step of Arnoldi iteration `it` reads and writes.
+struct GivensArgs {
+  const double *h1, *h2;  // the two Gram-Schmidt passes' coefficients
+  double *H;
+  int ldh;
+  double *cs, *sn, *beta, *bpre;
+  int *flags;
+  double *scal;
+  double tol;
+  int maxit;
+  double switch_rel;
+};
+
+// Givens step of Arnoldi iteration `it` (mobility_solver.cpp:1183-1234) on a
+// workgroup of 256: H column it-1 = h1 + h2 and the new vector's norm qn,
+// rotated by the previous rotations; the new rotation and the residual
+// estimate |beta[it]|.  Returns whether the solve has converged (or reached
+// maxit).  Only a workgroup with `write` stores the H column, the rotation,
+// beta, scal[3] = qn and the flags; the others read nothing that it writes
+// (the unrotated right-hand side comes from bpre), so the step may be taken
+// redundantly by every workgroup of a kernel.  LDS: hc[kMaxCols + 1],
+// scs[kMaxCols], ssn[kMaxCols], one int.
+__device__ bool givens_column(double qn, int it, const GivensArgs &g, bool write, double *hc,
+                              double *scs, double *ssn, int *sconv) {
   for (int i = threadIdx.x; i < it; i += 256) {
-    hc[i] = h1[i] + h2[i];
-    if (i < it - 1) { scs[i] = cs[i]; ssn[i] = sn[i]; }
+    hc[i] = g.h1[i] + g.h2[i];
+    if (i < it - 1) { scs[i] = g.cs[i]; ssn[i] = g.sn[i]; }
   }
-  s = block_sum<256>(s, sh);
+  __syncthreads();
   const int c = it - 1;
   if (threadIdx.x == 0) {
-    const double qn = sqrt(s);
-    scal[3] = qn;
     hc[it] = qn;
     for (int i = 1; i < it; i++) {
       const double t = scs[i - 1] * hc[i - 1] + ssn[i - 1] * hc[i];
@@ -317,28 +348,50 @@ __global__ __launch_bounds__(256) void k_givens(const double *norm_part, int npa
       csc = (2.0 * double(hc[c] > 0) - 1.0) / sqrt(1.0 + t * t);
       snc = t * csc;
     }
-    cs[c] = csc; sn[c] = snc;
     hc[c] = csc * hc[c] + snc * hc[it];
     hc[it] = 0.0;
-    beta[it] = -snc * beta[c];
-    beta[c] = csc * beta[c];
+    const double bc = g.bpre[c];
+    const double bit = -snc * bc;
+    const double rel = fabs(bit) / g.scal[0];
+    const bool conv = rel <= g.tol || it == g.maxit;
+    *sconv = conv;
+    if (write) {
+      g.cs[c] = csc; g.sn[c] = snc;
+      g.beta[it] = bit;
+      g.bpre[it] = bit;
+      g.beta[c] = csc * bc;
+      g.scal[3] = qn;
+      if (conv) {
+        g.flags[F_DONE] = 1;
+        g.flags[F_ITS] = it;
+        g.flags[F_SKIP64] = 1;
+        g.flags[F_SKIP32] = 1;
+      } else if (rel <= g.switch_rel) {
+        // relaxed Krylov: once the residual has dropped below switch_rel the
+        // operator may carry a relative error up to tol / rel, so the FP32
+        // pair kernel replaces the FP64 one for the remaining Arnoldi steps
+        g.flags[F_SKIP64] = 1;
+        g.flags[F_SKIP32] = 0;
+      }
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i <= it; i += 256) H[size_t(i) * ldh + c] = hc[i];
-  if (threadIdx.x != 0) return;
-  const double rel = fabs(beta[it]) / scal[0];
-  if (rel <= tol || it == maxit) {
-    flags[F_DONE] = 1;
-    flags[F_ITS] = it;
-    flags[F_SKIP64] = 1;
-    flags[F_SKIP32] = 1;
-  } else if (rel <= switch_rel) {
-    // relaxed Krylov: once the residual has dropped below switch_rel the
-    // operator may carry a relative error up to tol / rel, so the FP32 pair
-    // kernel replaces the FP64 one for the remaining Arnoldi steps
-    flags[F_SKIP64] = 1;
-    flags[F_SKIP32] = 0;
-  }
+  if (write)
+    for (int i = threadIdx.x; i <= it; i += 256) g.H[size_t(i) * g.ldh + c] = hc[i];
+  return *sconv != 0;
+}
+
+// Givens step of Arnoldi iteration `it` as a kernel of its own: the new
+// vector's norm is the sum of the workgroup shares.  The division by the
+// norm happens in the next iteration's first kernel.
+__global__ __launch_bounds__(256) void k_givens(const double *norm_part, int nparts, int it,
+                                                GivensArgs g) {
+  if (g.flags[F_DONE]) return;
+  __shared__ double sh[256];
+  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
+  __shared__ int sconv;
+  const double qn = norm_of_parts(norm_part, nparts, sh);
+  (void)givens_column(qn, it, g, true, hc, scs, ssn, &sconv);
 }
 
 // GMRES start on the device, after k_norms left scal[0] = ||rhs|| and
@@ -348,7 +401,7 @@ __global__ __launch_bounds__(256) void k_givens(const double *norm_part, int npa
 // keeps the previous solution otherwise).  r0 is normalised, its norm goes
 // to beta[0] and scal[2], and the flags are reset.
 __global__ void k_gmres_start(double *x, double *r, const double *rhs, int n, double *scal,
-                              double *beta, int *flags, double tol) {
+                              double *beta, double *bpre, int *flags, double tol) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const double nrhs = scal[0], b0 = scal[1];
   const bool zero = nrhs == 0.0;
@@ -362,6 +415,7 @@ __global__ void k_gmres_start(double *x, double *r, const double *rhs, int n, do
   if (e == 0) {
     scal[2] = bb;
     beta[0] = bb;
+    bpre[0] = bb;
     flags[F_DONE] = conv ? 1 : 0;
     flags[F_ITS] = 0;
     flags[F_SKIP64] = 0;
@@ -496,7 +550,7 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
   hipLaunchKernelGGL(k_gmres_start, dim3(nb), dim3(256), 0, c->stream, s->x, r, s->rhs, n, s->scal,
-                     s->beta, s->flags, tol);
+                     s->beta, s->bpre, s->flags, tol);
   if ((rc = launch_ok(c, "gmres start"))) return rc;
   const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
   // scal[0] = ||rhs|| is read by k_givens
@@ -506,8 +560,21 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   const DotShape ds = dot_shape(n);
   const size_t shm = sizeof(double) * ds.R;
   double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
+  const GivensArgs ga{s->h1, s->h2, s->H, ldh, s->cs, s->sn, s->beta, s->bpre,
+                      s->flags, s->scal, tol, maxit, switch_rel};
   auto enqueue = [&](int it0, int it1) -> int {
     int r;
+    if (op.fused()) {
+      // the operator runs each Arnoldi step on its own row partition (the
+      // Givens step of the previous iteration folded into the first kernel)
+      // and ends the chunk with a Givens step of its own, so that the host
+      // can read the flags
+      for (int it = it0; it <= it1; it++)
+        if ((r = op.fused_step(s, it, it == it0, mixed, ga))) return r;
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, op.fused_parts(),
+                         it1, ga);
+      return launch_ok(c, "givens");
+    }
     for (int it = it0; it <= it1; it++) {
       double *qprev = s->Qk + ldq * (it - 1);
       double *w = s->Qk + ldq * it;
@@ -532,9 +599,7 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
       hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
                          s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
       if (gs_stop && (r = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return r;
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, s->h1,
-                         s->h2, s->H, ldh, s->cs, s->sn, s->beta, s->flags, s->scal, tol, maxit,
-                         switch_rel);
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, ga);
       if ((r = launch_ok(c, "arnoldi"))) return r;
     }
     return GMI_OK;

@@ -518,6 +518,297 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
   st3(out, a.ldb, m, v - (V + cross3(W, d)));
 }
 
+// ---- fused Arnoldi step of the rigid-body solve ----------------------------
+// With the reference's preconditioner an Arnoldi step runs on a chunk
+// partition of the GMRES vector: workgroup b owns chunk b's blob rows (three
+// components of up to 64 blobs of one swimmer) and, if it is its swimmer's
+// first chunk, the swimmer's 6 body rows.  Five launches per step instead of
+// eight:
+//   k_arnoldi_precond  Givens step of the previous iteration (every
+//                      workgroup takes it, workgroup 0 stores it), the
+//                      previous vector's normalisation, the preconditioner
+//   k_pair (bb)        the operator's pair sums (unchanged)
+//   k_kprod_dots       K products, then the first Gram-Schmidt pass's dots
+//   k_gs_chunk<1>      h1, w -= Q h1, the second pass's dots
+//   k_gs_chunk<2>      h2, w -= Q h2, |w|^2 shares and the -K^T w shares the
+//                      next preconditioner needs
+// Partials are per chunk and every consumer sums them in chunk order.  Rows
+// past a chunk's blobs (the padding of the SoA layout) are zero in every
+// Krylov vector and take no part.  Thread t < 198 owns slot t of its chunk:
+// slots 0..191 are component t / 64 of blob t % 64, slots 192..197 body rows.
+
+constexpr int kChunk = 64;                 // blobs per chunk
+constexpr int kChunkRows = 3 * kChunk + 6; // slots per chunk
+
+// GMRES row of slot q of chunk T, or -1 when the chunk has no such row.
+__device__ __forceinline__ int chunk_row(const SimDev &a, const BTile &T, bool first, int q) {
+  if (q < 3 * kChunk) {
+    const int i = q % kChunk;
+    return i < T.n ? (q / kChunk) * a.ldb + T.m0 + i : -1;
+  }
+  return first && q < kChunkRows ? 3 * a.ldb + 6 * T.sw + (q - 3 * kChunk) : -1;
+}
+
+// Dots of the chunk's rows (values sv by slot) with columns j < k of Qb:
+// wave w takes the columns j = w (mod 4), kDotBatch at a time, and writes
+// part[j * nparts + chunk].
+__device__ void chunk_col_dots(const SimDev &a, const BTile &T, bool first, const double *sv,
+                               const double *Qb, size_t ldq, int k, double *part, int nparts) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NR = (kChunkRows + 63) / 64;
+  int rows[NR];
+  double vals[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int q = lane + 64 * r;
+    rows[r] = chunk_row(a, T, first, q);
+    vals[r] = rows[r] >= 0 ? sv[q] : 0.0;
+  }
+  for (int j0 = wave; j0 < k; j0 += 4 * kDotBatch) {
+    double acc[kDotBatch];
+#pragma unroll
+    for (int c = 0; c < kDotBatch; c++) acc[c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (rows[r] < 0) continue;
+#pragma unroll
+      for (int c = 0; c < kDotBatch; c++) {
+        // past the last column a batch re-reads column k-1 and stores nothing
+        const int j = min(j0 + 4 * c, k - 1);
+        acc[c] += Qb[ldq * j + rows[r]] * vals[r];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kDotBatch; c++) {
+      const double s = wave_sum(acc[c]);
+      const int j = j0 + 4 * c;
+      if (lane == 0 && j < k) part[size_t(j) * nparts + blockIdx.x] = s;
+    }
+  }
+}
+
+// h[j] = sum over chunks of part[j * nparts + q] into LDS, and from
+// workgroup 0 into hout.  Wave w takes the columns j = w (mod 4), four at a
+// time; lane l sums the chunks q = l (mod 64), then the wave sums the lanes
+// (a fixed order, the same in every workgroup).
+__device__ void chunk_col_sums(const double *part, int k, int nparts, double *hs, double *hout) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j0 = wave; j0 < k; j0 += 16) {
+    double acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int j = min(j0 + 4 * c, k - 1);
+      const double *p = part + size_t(j) * nparts;
+      double s = 0.0;
+      for (int q = lane; q < nparts; q += 64) s += p[q];
+      acc[c] = s;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const double s = wave_sum(acc[c]);
+      const int j = j0 + 4 * c;
+      if (lane == 0 && j < k) {
+        hs[j] = s;
+        if (blockIdx.x == 0) hout[j] = s;
+      }
+    }
+  }
+}
+
+// Sum over the swimmer's chunks [q0, q0 + nq) of part[6 q + c] for c < 6,
+// into out[c]: lanes 32 c' + l of waves 0..2 (c = 2 wave + c') take the
+// chunks q0 + l (mod 32), then each half-wave sums its lanes.  All 256
+// threads call it; out is LDS.
+__device__ void swimmer_share_sums(const double *part, int q0, int nq, double *out) {
+  const int t = threadIdx.x;
+  double s = 0.0;
+  const int c = t >> 5, l = t & 31;
+  if (c < 6)
+    for (int q = l; q < nq; q += 32) s += part[6 * (q0 + q) + c];
+  // sum over the 32 lanes of each half-wave (xor offsets below 32 stay inside it)
+  for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (c < 6 && l == 0) out[c] = s;
+  __syncthreads();
+}
+
+// Preconditioner of Arnoldi step it (mobility_solver.cpp:651-717) on the
+// previous vector w = in.  qn_mode 1: first the Givens step of iteration
+// it-1 from the previous step's shares (|w|^2 in npart, -K^T w in kpart),
+// taken by every workgroup and stored by workgroup 0, then w is divided by
+// its norm; 0: a k_givens launch took that step and left |w| in scal[3];
+// 2: w is normalised already (the first step).  The workgroup writes its
+// blob rows of w / |w| back (the body rows are normalised by the swimmer's
+// first chunk in k_kprod_dots: every chunk of the swimmer reads them here),
+// and out = P^-1 (w / |w|) with the chunk's -K^T out share.
+__global__ __launch_bounds__(256) void k_arnoldi_precond(
+    SimDev a, const BTile *chunks, const int *sw_chunk0, const int *sw_nchunk,
+    const double *kpart, const double *npart, int nparts, const double *rot, double *in,
+    double *out, double *part_out, GivensArgs g, int it_prev, int qn_mode) {
+  if (g.flags[F_DONE]) return;
+  __shared__ double sh[256];
+  __shared__ double hc[kMaxCols + 1], scs[kMaxCols], ssn[kMaxCols];
+  __shared__ int sconv;
+  __shared__ double sb[6], so[6];
+  double qn = 1.0;
+  if (qn_mode == 1) {
+    qn = norm_of_parts(npart, nparts, sh);
+    if (givens_column(qn, it_prev, g, blockIdx.x == 0, hc, scs, ssn, &sconv)) return;
+  } else if (qn_mode == 0) {
+    qn = g.scal[3];
+  }
+  const BTile T = chunks[blockIdx.x];
+  const int n = T.sw, t = threadIdx.x;
+  const int vid = 3 * a.ldb + 6 * n;
+  swimmer_share_sums(kpart, sw_chunk0[n], sw_nchunk[n], sb);
+  if (t < 6) sb[t] /= qn;
+  __syncthreads();
+  if (t < 6) {
+    const double *bm = rot + 72 * size_t(n), *kk = bm + 36;
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < 6; k++) { s1 += kk[6 * t + k] * sb[k]; s2 += bm[6 * t + k] * (in[vid + k] / qn); }
+    so[t] = s1 - s2;
+    if (blockIdx.x == sw_chunk0[n]) out[vid + t] = s1 - s2;
+  }
+  __syncthreads();
+  const double *b = a.body + size_t(n) * BREC;
+  const m3 Qm = rotm(ga4(b + BQ));
+  const bool live = t < T.n;
+  const int m = T.m0 + t;
+  d3 z = {0, 0, 0};
+  if (live) {
+    d3 l = ld3(in, a.ldb, m);
+    if (qn_mode != 2) {
+      l = {l.x / qn, l.y / qn, l.z / qn};
+      st3(in, a.ldb, m, l);
+    }
+    const d3 V = {so[0], so[1], so[2]}, W = {so[3], so[4], so[5]};
+    const double Minv = 6.0 * kPi * a.mu * a.ab;
+    const d3 d = mv(Qm, ld3(a.blob_ref, a.ldb, m));
+    z = Minv * (l + V + cross3(W, d));
+    st3(out, a.ldb, m, z);
+  }
+  tile_kt_vals(Qm, a, m, live, z, part_out + 6 * blockIdx.x, sh);
+}
+
+// k_kprod followed by the first Gram-Schmidt pass's dots of the new vector
+// out with the columns j < k of Qb.  The blob-blob pair partials of chunk b
+// are those of blob tile ch_tile[b].  norm_prev: column k-1's body rows are
+// still unnormalised (k_arnoldi_precond left them); the swimmer's first
+// chunk divides them by scal[3] before they are used.
+__global__ __launch_bounds__(256) void k_kprod_dots(
+    SimDev a, const BTile *chunks, const int *ch_tile, const int *sw_chunk0,
+    const int *sw_nchunk, const TileDesc *ptiles, const double *ppart, const double *z,
+    const double *vb, double *out, const double *zpart, const int *flags, double *Qb,
+    size_t ldq, int k, int norm_prev, const double *scal, double *part, int nparts) {
+  if (flags[F_DONE]) return;
+  __shared__ double sv[kChunkRows];
+  __shared__ double sz[6], spp[4][3][kChunk];
+  const BTile T = chunks[blockIdx.x];
+  const int n = T.sw, t = threadIdx.x;
+  const int vid = 3 * a.ldb + 6 * n;
+  const bool first = blockIdx.x == sw_chunk0[n];
+  if (t < kChunkRows) sv[t] = 0.0;
+  if (first) swimmer_share_sums(zpart, sw_chunk0[n], sw_nchunk[n], sz);
+  if (first && t < 6) {
+    if (norm_prev) Qb[ldq * (k - 1) + vid + t] /= scal[3];
+    out[vid + t] = sz[t];
+    sv[3 * kChunk + t] = sz[t];
+  }
+  // the pair kernel's split partials: thread (blob i, quarter r) sums the
+  // splits q = r (mod 4), then the quarters are added in order
+  const TileDesc P = ptiles[ch_tile[blockIdx.x]];
+  {
+    const int i = t & (kChunk - 1), r = t >> 6;
+    if (P.nsplit > 1 && i < T.n) {
+      const double *pp = ppart + size_t(P.part0) * 6 * kBlock + (T.m0 + i - P.t0);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll 8
+      for (int q = r; q < P.nsplit; q += 4) {
+        const double *pq = pp + size_t(q) * 6 * kBlock;
+        s0 += pq[0]; s1 += pq[kBlock]; s2 += pq[2 * kBlock];
+      }
+      spp[r][0][i] = s0; spp[r][1][i] = s1; spp[r][2][i] = s2;
+    }
+  }
+  __syncthreads();
+  if (t < T.n) {
+    const int m = T.m0 + t;
+    d3 v;
+    if (P.nsplit == 1) {
+      v = ld3(vb, a.ldb, m);
+    } else {
+      double s[3];
+      for (int c = 0; c < 3; c++) s[c] = (spp[0][c][t] + spp[1][c][t]) + (spp[2][c][t] + spp[3][c][t]);
+      v = {s[0], s[1], s[2]};
+    }
+    const double *b = a.body + size_t(n) * BREC;
+    const m3 Qm = rotm(ga4(b + BQ));
+    const d3 V = {z[vid], z[vid + 1], z[vid + 2]}, W = {z[vid + 3], z[vid + 4], z[vid + 5]};
+    const d3 d = mv(Qm, ld3(a.blob_ref, a.ldb, m));
+    const d3 w = v - (V + cross3(W, d));
+    st3(out, a.ldb, m, w);
+    sv[t] = w.x; sv[kChunk + t] = w.y; sv[2 * kChunk + t] = w.z;
+  }
+  __syncthreads();
+  chunk_col_dots(a, T, first, sv, Qb, ldq, k, part, nparts);
+}
+
+// Gram-Schmidt pass PASS on the chunk partition: h = sum of the pass's dot
+// partials (workgroup 0 stores it in hout), w -= Q h on the chunk's rows
+// (thread t its slot t); PASS 1: the dots of the new w with the columns
+// (the next pass's partials); PASS 2: the chunk's share of |w|^2 (npart)
+// and of -K^T w (kpart).
+template <int PASS>
+__global__ __launch_bounds__(256) void k_gs_chunk(SimDev a, const BTile *chunks,
+                                                  const int *sw_chunk0, const double *Qb,
+                                                  size_t ldq, int k, const double *part_in,
+                                                  double *hout, double *w, double *part_out,
+                                                  int nparts, double *npart, double *kpart,
+                                                  const int *flags) {
+  if (flags[F_DONE]) return;
+  __shared__ double hs[kMaxCols];
+  __shared__ double sv[kChunkRows];
+  __shared__ double sh[256];
+  const BTile T = chunks[blockIdx.x];
+  const int t = threadIdx.x;
+  const bool first = blockIdx.x == sw_chunk0[T.sw];
+  chunk_col_sums(part_in, k, nparts, hs, hout);
+  __syncthreads();
+  const int row = chunk_row(a, T, first, t);
+  double v = 0.0;
+  if (row >= 0) {
+    // 16 column loads in flight, four running sums
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 15 < k; j += 16) {
+      double q[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) q[u] = Qb[ldq * (j + u) + row];
+#pragma unroll
+      for (int u = 0; u < 16; u += 4) {
+        s0 += hs[j + u] * q[u];
+        s1 += hs[j + u + 1] * q[u + 1];
+        s2 += hs[j + u + 2] * q[u + 2];
+        s3 += hs[j + u + 3] * q[u + 3];
+      }
+    }
+    for (; j < k; j++) s0 += hs[j] * Qb[ldq * j + row];
+    v = w[row] - ((s0 + s1) + (s2 + s3));
+    w[row] = v;
+  }
+  if (t < kChunkRows) sv[t] = v;
+  __syncthreads();
+  if (PASS == 1) {
+    chunk_col_dots(a, T, first, sv, Qb, ldq, k, part_out, nparts);
+  } else {
+    const double s = block_sum<256>(v * v, sh);
+    if (t == 0) npart[blockIdx.x] = s;
+    const d3 wl = {sv[t % kChunk], sv[kChunk + t % kChunk], sv[2 * kChunk + t % kChunk]};
+    const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
+    tile_kt_vals(Qm, a, T.m0 + t, t < T.n, wl, kpart + 6 * blockIdx.x, sh);
+  }
+}
+
 // -------------------------------------------------------------------------
 // vector kernels (GMRES / Broyden)
 
@@ -1493,12 +1784,9 @@ static int read_flags(gmi_ctx *c, SimHost *s) { return krylov_read_flags(c, s); 
 // device flag (F_SKIP64 / F_SKIP32) so the precision switch needs no sync.
 // The tile shares of -K^T z are in s->zpart when zpart_ready (z came from
 // the preconditioner); otherwise they are computed first.
-static int sys_mult(gmi_ctx *c, SimHost *s, double *z, double *out, int mode, const int *done,
-                    bool zpart_ready) {
+// The blob-blob pair sums of out = A z (split partials left for the K products).
+static int sys_pairs(gmi_ctx *c, SimHost *s, double *z, int mode, const int *done) {
   SimDev &d = s->d;
-  if (!zpart_ready)
-    hipLaunchKernelGGL(k_precond_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                       s->sw_tile0, z, s->scal, -1, s->zpart, done);
   PairArgs pa;
   const WorkList &wl = c->lists[kBB];
   pa.Xs = c->Xs; pa.Fs = c->Fs; pa.lds = c->lds;
@@ -1521,6 +1809,18 @@ static int sys_mult(gmi_ctx *c, SimHost *s, double *z, double *out, int mode, co
     if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, mode == 1, c->stream), "pair bb")))
       return rc;
   }
+  return GMI_OK;
+}
+
+static int sys_mult(gmi_ctx *c, SimHost *s, double *z, double *out, int mode, const int *done,
+                    bool zpart_ready) {
+  SimDev &d = s->d;
+  if (!zpart_ready)
+    hipLaunchKernelGGL(k_precond_kt, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                       s->sw_tile0, z, s->scal, -1, s->zpart, done);
+  int rc;
+  if ((rc = sys_pairs(c, s, z, mode, done))) return rc;
+  const WorkList &wl = c->lists[kBB];
   hipLaunchKernelGGL(k_kprod, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
                      s->sw_tile0, s->sw_ntile, wl.d_tiles, c->part, z, c->Vb, out, s->zpart,
                      done);
@@ -1554,6 +1854,39 @@ struct SimOp {
   int precond(double *in, double *out, const int *skip, int norm_slot) {
     if (s->dense) return dense_apply(c, s, in, out, skip, norm_slot);
     return gmi::precond(c, s, in, out, skip, norm_slot);
+  }
+  // the reference's preconditioner: whole Arnoldi steps on 64-blob chunks
+  bool fused() const { return s->dense == nullptr; }
+  int fused_parts() const { return s->nchunks; }
+  int fused_step(KrylovState *, int it, bool chunk_start, bool mixed, const GivensArgs &g) {
+    SimDev &d = s->d;
+    const size_t ldq = size_t(d.npad);
+    double *qprev = s->Qk + ldq * (it - 1), *w = s->Qk + ldq * it;
+    const int nc = s->nchunks;
+    if (it == 1)  // -K^T q_0 shares (q_0 normalised by the GMRES start)
+      hipLaunchKernelGGL(k_precond_kt, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+                         s->sw_chunk0, qprev, s->scal, -1, s->ch_kpart, s->flags);
+    hipLaunchKernelGGL(k_arnoldi_precond, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+                       s->sw_chunk0, s->sw_nchunk, s->ch_kpart, s->part, nc, s->rot, qprev, s->z,
+                       s->ch_zpart, g, it - 1, it == 1 ? 2 : (chunk_start ? 0 : 1));
+    int rc;
+    if ((rc = launch_ok(c, "arnoldi precond"))) return rc;
+    if ((rc = sys_pairs(c, s, s->z, mixed ? 2 : 0, s->flags))) return rc;
+    hipEvent_t gs_stop = nullptr;
+    if ((rc = prof_gs_start(c, it, d.npad, &gs_stop))) return rc;
+    double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
+    const WorkList &wl = c->lists[kBB];
+    hipLaunchKernelGGL(k_kprod_dots, dim3(nc), dim3(256), 0, c->stream, d, s->chunks, s->ch_tile,
+                       s->sw_chunk0, s->sw_nchunk, wl.d_tiles, c->part, s->z, c->Vb, w,
+                       s->ch_zpart, s->flags, s->Qk, ldq, it, it > 1 ? 1 : 0, s->scal, pp1, nc);
+    hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+                       s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, pp2, nc, nullptr, nullptr,
+                       s->flags);
+    hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+                       s->sw_chunk0, s->Qk, ldq, it, pp2, s->h2, w, nullptr, nc, s->part,
+                       s->ch_kpart, s->flags);
+    if (gs_stop && (rc = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return rc;
+    return launch_ok(c, "arnoldi step");
   }
 };
 
@@ -1758,16 +2091,22 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &s->cs, maxit + 1))) return rc;
   if ((rc = dalloc(c, s, &s->sn, maxit + 1))) return rc;
   if ((rc = dalloc(c, s, &s->beta, maxit + 2))) return rc;
+  if ((rc = dalloc(c, s, &s->bpre, maxit + 2))) return rc;
   if ((rc = dalloc(c, s, &s->h1, maxit + 1))) return rc;
   if ((rc = dalloc(c, s, &s->h2, maxit + 1))) return rc;
   if ((rc = dalloc(c, s, &s->y, maxit + 1))) return rc;
   if ((rc = dalloc(c, s, &s->z, np))) return rc;
   if ((rc = dalloc(c, s, &s->tmp, np))) return rc;
   if ((rc = dalloc(c, s, &s->scal, 16))) return rc;
-  if ((rc = dalloc(c, s, &s->part, nblk(d.npad, 256)))) return rc;
-  // chunk partials of the column dot products (GMRES basis, Broyden D)
-  // column dot partials: two halves (Gram-Schmidt passes 1 and 2)
-  s->pp_half = std::max(size_t(maxit + 1) * dot_shape(d.npad).nb,
+  // blob chunks of the fused Arnoldi steps (built below): up to 64 blobs of
+  // one swimmer each
+  size_t ntiles_blob = 0;
+  for (int n = 0; n < d.Nsw; n++) ntiles_blob += (c->nblob[n] + kChunk - 1) / kChunk;
+  if ((rc = dalloc(c, s, &s->part, std::max<size_t>(nblk(d.npad, 256), ntiles_blob)))) return rc;
+  // column dot partials: two halves (Gram-Schmidt passes 1 and 2), per
+  // row chunk (generic kernels, Broyden D) or per blob tile (fused steps)
+  s->pp_half = std::max(std::max(size_t(maxit + 1) * dot_shape(d.npad).nb,
+                                 size_t(maxit + 1) * ntiles_blob),
                         size_t(p->max_broyden_iter) * dot_shape(d.nbroy).nb);
   if ((rc = dalloc(c, s, &s->pp, 2 * s->pp_half))) return rc;
   if ((rc = dalloc(c, s, &s->rot, 72 * size_t(d.Nsw)))) return rc;
@@ -1875,6 +2214,34 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = upload(c, i1, nt.data(), d.Nsw))) return rc;
   s->sw_tile0 = i0;
   s->sw_ntile = i1;
+  {
+    // 64-blob chunks of the tiles, in tile order
+    std::vector<BTile> ch;
+    std::vector<int> ct, c0(d.Nsw), cn(d.Nsw);
+    for (int n = 0; n < d.Nsw; n++) {
+      c0[n] = int(ch.size());
+      for (int q = t0[n]; q < t0[n] + nt[n]; q++)
+        for (int o = 0; o < bt[q].n; o += kChunk) {
+          ch.push_back({n, bt[q].m0 + o, std::min(kChunk, bt[q].n - o)});
+          ct.push_back(q);
+        }
+      cn[n] = int(ch.size()) - c0[n];
+    }
+    s->nchunks = int(ch.size());
+    BTile *dch;
+    int *dct, *dc0, *dcn;
+    if ((rc = dalloc(c, s, &dch, ch.size()))) return rc;
+    if ((rc = dalloc(c, s, &dct, ct.size()))) return rc;
+    if ((rc = dalloc(c, s, &dc0, d.Nsw))) return rc;
+    if ((rc = dalloc(c, s, &dcn, d.Nsw))) return rc;
+    if ((rc = upload(c, dch, ch.data(), ch.size()))) return rc;
+    if ((rc = upload(c, dct, ct.data(), ct.size()))) return rc;
+    if ((rc = upload(c, dc0, c0.data(), d.Nsw))) return rc;
+    if ((rc = upload(c, dcn, cn.data(), d.Nsw))) return rc;
+    s->chunks = dch; s->ch_tile = dct; s->sw_chunk0 = dc0; s->sw_nchunk = dcn;
+    if ((rc = dalloc(c, s, &s->ch_kpart, 6 * ch.size()))) return rc;
+    if ((rc = dalloc(c, s, &s->ch_zpart, 6 * ch.size()))) return rc;
+  }
   // Gauss-Jordan in LDS when one workgroup can hold the filament matrix
   {
     int lds_max = 0;

@@ -76,6 +76,12 @@ struct SimHost : KrylovState {
   int nbtiles = 0;
   int *sw_tile0 = nullptr, *sw_ntile = nullptr;
   double *tpart = nullptr, *zpart = nullptr;  // tile shares of -K^T v (preconditioner, K products)
+  // chunks of up to 64 blobs of one swimmer (fused Arnoldi steps), the blob
+  // tile each lies in, and their shares of -K^T v
+  BTile *chunks = nullptr;
+  int nchunks = 0;
+  int *ch_tile = nullptr, *sw_chunk0 = nullptr, *sw_nchunk = nullptr;
+  double *ch_kpart = nullptr, *ch_zpart = nullptr;
 };
 
 
