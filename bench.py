@@ -170,6 +170,7 @@ def bench_step(args):
                             the last 16 right-hand sides + mixed precision
       learned_fp64          network + fitted GMRES guess, FP64
       fitted_fp64           fitted GMRES guess only, FP64
+      fitted_mixed          fitted GMRES guess + mixed precision
       mixed                 mixed precision only
       inverse_fp64          exact body-frame GMRES preconditioner, FP64
       inverse_mixed         the preconditioner + mixed precision
@@ -182,6 +183,7 @@ def bench_step(args):
              ("learned_mixed", capi.GMI_FP32_KAHAN, True, True, False),
              ("learned_fp64", capi.GMI_FP64, True, True, False),
              ("fitted_fp64", capi.GMI_FP64, False, True, False),
+             ("fitted_mixed", capi.GMI_FP32_KAHAN, False, True, False),
              ("mixed", capi.GMI_FP32_KAHAN, False, False, False),
              ("inverse_fp64", capi.GMI_FP64, False, False, True),
              ("inverse_mixed", capi.GMI_FP32_KAHAN, False, False, True),
@@ -759,7 +761,8 @@ def main():
                     "preconditioner",
             "attribution_speedups_vs_fp64": {
                 k: st[k]["steps_per_s"] / fp for k in
-                ("mixed", "fitted_fp64", "learned_fp64", "learned_mixed", "inverse_fp64",
+                ("mixed", "fitted_fp64", "fitted_mixed", "learned_fp64", "learned_mixed",
+                 "inverse_fp64",
                  "inverse_mixed", "learned_inverse_fp64", "learned_inverse_mixed")},
         }
         if not args.no_cpu:

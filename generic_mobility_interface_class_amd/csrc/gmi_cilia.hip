@@ -937,11 +937,10 @@ struct CiliaOp {
       if ((rc = hip_check(c, launch_pair_wall(pa, wl.nwork, nred, c->stream), "cilia wall pairs")))
         return rc;
     } else if (mode == 2) {
-      pa.skip = s->flags + F_SKIP64;
-      if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, false, c->stream), "cilia pairs64")))
-        return rc;
-      pa.skip = s->flags + F_SKIP32;
-      if ((rc = hip_check(c, launch_pair(pa, wl.nwork, nred, true, c->stream), "cilia pairs32")))
+      // one launch: FP32 once the relaxed switch cleared F_SKIP32, else FP64
+      pa.skip = s->flags + F_DONE;
+      pa.sel32 = s->flags + F_SKIP32;
+      if ((rc = hip_check(c, launch_pair_dyn(pa, wl.nwork, nred, c->stream), "cilia pairs")))
         return rc;
     } else {
       pa.skip = done;

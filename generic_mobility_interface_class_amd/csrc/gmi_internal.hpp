@@ -46,6 +46,7 @@ struct PairArgs {
   const int *skip = nullptr; // if non-null and *skip != 0 the launch is a no-op
   int tt_only = 0;           // 1: translations only -- segments carry forces
                              // (3 rows of Fs) and get velocities (3 rows of Vs)
+  const int *sel32 = nullptr; // launch_pair_dyn: FP32-Kahan iff *sel32 == 0
 };
 
 // Work list of one pair launch (built once per mask).
@@ -61,6 +62,9 @@ struct WorkList {
 // ev_mid (optional) is recorded between the pair kernel and the reduction.
 hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp32,
                        hipStream_t s, hipEvent_t ev_mid = nullptr);
+// One launch whose precision a device flag picks (a.sel32); split partials
+// summed as by launch_pair.
+hipError_t launch_pair_dyn(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s);
 // Segment-segment mobility above a no-slip wall at z = 0 (FP64; kSS lists).
 hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,
                             hipEvent_t ev_mid = nullptr);

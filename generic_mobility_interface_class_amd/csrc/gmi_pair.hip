@@ -408,9 +408,7 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
 }
 
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
-  __shared__ Lds<R> lds;
-  if (a.skip && *a.skip) return;
+__device__ __forceinline__ void pair_body(const PairArgs &a, Lds<R> &lds) {
   const WorkItem w = a.work[blockIdx.x];
   const TileDesc T = a.tiles[w.tile];
   const int tid = threadIdx.x;
@@ -483,6 +481,26 @@ __global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
     double *P = a.part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
     for (int c = 0; c < nc; ++c) P[c * kBlock] = o[c];
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pair(const PairArgs a) {
+  __shared__ Lds<R> lds;
+  if (a.skip && *a.skip) return;
+  pair_body<R>(a, lds);
+}
+
+// Precision picked on the device: FP32-Kahan when *a.sel32 == 0, FP64
+// otherwise (the relaxed GMRES switches with a device flag, so one launch
+// serves both precisions).  The LDS tile is sized for FP64.
+__global__ __launch_bounds__(kBlock) void k_pair_dyn(const PairArgs a) {
+  __shared__ Lds<double> lds;
+  static_assert(sizeof(Lds<float>) <= sizeof(Lds<double>), "LDS tile");
+  if (a.skip && *a.skip) return;
+  if (*a.sel32 == 0)
+    pair_body<float>(a, *reinterpret_cast<Lds<float> *>(&lds));
+  else
+    pair_body<double>(a, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -887,6 +905,16 @@ hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp3
     e = hipGetLastError();
   }
   return e;
+}
+
+hipError_t launch_pair_dyn(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  if (!a.sel32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pair_dyn, dim3(nwork), dim3(kBlock), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || ntiles_reduce <= 0) return e;
+  hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,

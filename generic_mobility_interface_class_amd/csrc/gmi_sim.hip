@@ -549,9 +549,15 @@ __device__ __forceinline__ int chunk_row(const SimDev &a, const BTile &T, bool f
   return first && q < kChunkRows ? 3 * a.ldb + 6 * T.sw + (q - 3 * kChunk) : -1;
 }
 
+// The fused Gram-Schmidt kernels run 1024 threads (16 waves) per chunk: a
+// step's columns are spread over the waves so that each phase is about one
+// round trip to memory.
+constexpr int kGsThreads = 1024;
+constexpr int kGsWaves = kGsThreads / 64;
+
 // Dots of the chunk's rows (values sv by slot) with columns j < k of Qb:
-// wave w takes the columns j = w (mod 4), kDotBatch at a time, and writes
-// part[j * nparts + chunk].
+// wave w takes the columns j = w (mod kGsWaves), kDotBatch at a time, and
+// writes part[j * nparts + chunk].
 __device__ void chunk_col_dots(const SimDev &a, const BTile &T, bool first, const double *sv,
                                const double *Qb, size_t ldq, int k, double *part, int nparts) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -564,7 +570,7 @@ __device__ void chunk_col_dots(const SimDev &a, const BTile &T, bool first, cons
     rows[r] = chunk_row(a, T, first, q);
     vals[r] = rows[r] >= 0 ? sv[q] : 0.0;
   }
-  for (int j0 = wave; j0 < k; j0 += 4 * kDotBatch) {
+  for (int j0 = wave; j0 < k; j0 += kGsWaves * kDotBatch) {
     double acc[kDotBatch];
 #pragma unroll
     for (int c = 0; c < kDotBatch; c++) acc[c] = 0.0;
@@ -574,30 +580,30 @@ __device__ void chunk_col_dots(const SimDev &a, const BTile &T, bool first, cons
 #pragma unroll
       for (int c = 0; c < kDotBatch; c++) {
         // past the last column a batch re-reads column k-1 and stores nothing
-        const int j = min(j0 + 4 * c, k - 1);
+        const int j = min(j0 + kGsWaves * c, k - 1);
         acc[c] += Qb[ldq * j + rows[r]] * vals[r];
       }
     }
 #pragma unroll
     for (int c = 0; c < kDotBatch; c++) {
       const double s = wave_sum(acc[c]);
-      const int j = j0 + 4 * c;
+      const int j = j0 + kGsWaves * c;
       if (lane == 0 && j < k) part[size_t(j) * nparts + blockIdx.x] = s;
     }
   }
 }
 
 // h[j] = sum over chunks of part[j * nparts + q] into LDS, and from
-// workgroup 0 into hout.  Wave w takes the columns j = w (mod 4), four at a
-// time; lane l sums the chunks q = l (mod 64), then the wave sums the lanes
-// (a fixed order, the same in every workgroup).
+// workgroup 0 into hout.  Wave w takes the columns j = w (mod kGsWaves),
+// four at a time; lane l sums the chunks q = l (mod 64), then the wave sums
+// the lanes (a fixed order, the same in every workgroup).
 __device__ void chunk_col_sums(const double *part, int k, int nparts, double *hs, double *hout) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j0 = wave; j0 < k; j0 += 16) {
+  for (int j0 = wave; j0 < k; j0 += 4 * kGsWaves) {
     double acc[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const int j = min(j0 + 4 * c, k - 1);
+      const int j = min(j0 + kGsWaves * c, k - 1);
       const double *p = part + size_t(j) * nparts;
       double s = 0.0;
       for (int q = lane; q < nparts; q += 64) s += p[q];
@@ -606,7 +612,7 @@ __device__ void chunk_col_sums(const double *part, int k, int nparts, double *hs
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const double s = wave_sum(acc[c]);
-      const int j = j0 + 4 * c;
+      const int j = j0 + kGsWaves * c;
       if (lane == 0 && j < k) {
         hs[j] = s;
         if (blockIdx.x == 0) hout[j] = s;
@@ -617,8 +623,8 @@ __device__ void chunk_col_sums(const double *part, int k, int nparts, double *hs
 
 // Sum over the swimmer's chunks [q0, q0 + nq) of part[6 q + c] for c < 6,
 // into out[c]: lanes 32 c' + l of waves 0..2 (c = 2 wave + c') take the
-// chunks q0 + l (mod 32), then each half-wave sums its lanes.  All 256
-// threads call it; out is LDS.
+// chunks q0 + l (mod 32), then each half-wave sums its lanes.  Every thread
+// of the workgroup calls it; out is LDS.
 __device__ void swimmer_share_sums(const double *part, int q0, int nq, double *out) {
   const int t = threadIdx.x;
   double s = 0.0;
@@ -629,6 +635,24 @@ __device__ void swimmer_share_sums(const double *part, int q0, int nq, double *o
   for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (c < 6 && l == 0) out[c] = s;
   __syncthreads();
+}
+
+// The chunk's share of -K^T w from its blob rows (values l of blob t in
+// wave 0): one wave sum per component.
+__device__ void chunk_kt_vals(const m3 &Qm, const SimDev &a, int m, bool live, d3 l,
+                              double *part) {
+  if (threadIdx.x >= 64) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (live) {
+    const d3 c = cross3(mv(Qm, ld3(a.blob_ref, a.ldb, m)), l);
+    acc[0] = -l.x; acc[1] = -l.y; acc[2] = -l.z;
+    acc[3] = -c.x; acc[4] = -c.y; acc[5] = -c.z;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double r = wave_sum(acc[k]);
+    if (threadIdx.x == 0) part[k] = r;
+  }
 }
 
 // Preconditioner of Arnoldi step it (mobility_solver.cpp:651-717) on the
@@ -695,7 +719,7 @@ __global__ __launch_bounds__(256) void k_arnoldi_precond(
 // are those of blob tile ch_tile[b].  norm_prev: column k-1's body rows are
 // still unnormalised (k_arnoldi_precond left them); the swimmer's first
 // chunk divides them by scal[3] before they are used.
-__global__ __launch_bounds__(256) void k_kprod_dots(
+__global__ __launch_bounds__(kGsThreads) void k_kprod_dots(
     SimDev a, const BTile *chunks, const int *ch_tile, const int *sw_chunk0,
     const int *sw_nchunk, const TileDesc *ptiles, const double *ppart, const double *z,
     const double *vb, double *out, const double *zpart, const int *flags, double *Qb,
@@ -719,7 +743,7 @@ __global__ __launch_bounds__(256) void k_kprod_dots(
   const TileDesc P = ptiles[ch_tile[blockIdx.x]];
   {
     const int i = t & (kChunk - 1), r = t >> 6;
-    if (P.nsplit > 1 && i < T.n) {
+    if (r < 4 && P.nsplit > 1 && i < T.n) {
       const double *pp = ppart + size_t(P.part0) * 6 * kBlock + (T.m0 + i - P.t0);
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll 8
@@ -759,53 +783,63 @@ __global__ __launch_bounds__(256) void k_kprod_dots(
 // (the next pass's partials); PASS 2: the chunk's share of |w|^2 (npart)
 // and of -K^T w (kpart).
 template <int PASS>
-__global__ __launch_bounds__(256) void k_gs_chunk(SimDev a, const BTile *chunks,
-                                                  const int *sw_chunk0, const double *Qb,
-                                                  size_t ldq, int k, const double *part_in,
-                                                  double *hout, double *w, double *part_out,
-                                                  int nparts, double *npart, double *kpart,
-                                                  const int *flags) {
+__global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *chunks,
+                                                         const int *sw_chunk0, const double *Qb,
+                                                         size_t ldq, int k, const double *part_in,
+                                                         double *hout, double *w,
+                                                         double *part_out, int nparts,
+                                                         double *npart, double *kpart,
+                                                         const int *flags) {
   if (flags[F_DONE]) return;
-  __shared__ double hs[kMaxCols];
+  __shared__ double hs[kMaxCols + 16];
   __shared__ double sv[kChunkRows];
-  __shared__ double sh[256];
+  __shared__ double sq[4][256];
+  __shared__ double sh[kGsThreads];
   const BTile T = chunks[blockIdx.x];
   const int t = threadIdx.x;
   const bool first = blockIdx.x == sw_chunk0[T.sw];
   chunk_col_sums(part_in, k, nparts, hs, hout);
+  if (t < 16) hs[k + t] = 0.0;  // the padding of the last group of 16
   __syncthreads();
-  const int row = chunk_row(a, T, first, t);
-  double v = 0.0;
-  if (row >= 0) {
-    // 16 column loads in flight, four running sums
+  // thread (slot t % 256, quarter t / 256): the quarter's share of
+  // sum_j h_j Q_j[row] over the columns j = 16 g + u, g = quarter (mod 4)
+  const int slot = t & 255, quarter = t >> 8;
+  const int row = chunk_row(a, T, first, slot);
+  {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int j = 0;
-    for (; j + 15 < k; j += 16) {
-      double q[16];
+    if (row >= 0)
+      for (int j = 16 * quarter; j < k; j += 64) {
+        double q[16];
 #pragma unroll
-      for (int u = 0; u < 16; u++) q[u] = Qb[ldq * (j + u) + row];
+        for (int u = 0; u < 16; u++) q[u] = Qb[ldq * min(j + u, k - 1) + row];
 #pragma unroll
-      for (int u = 0; u < 16; u += 4) {
-        s0 += hs[j + u] * q[u];
-        s1 += hs[j + u + 1] * q[u + 1];
-        s2 += hs[j + u + 2] * q[u + 2];
-        s3 += hs[j + u + 3] * q[u + 3];
+        for (int u = 0; u < 16; u += 4) {
+          s0 += hs[j + u] * q[u];
+          s1 += hs[j + u + 1] * q[u + 1];
+          s2 += hs[j + u + 2] * q[u + 2];
+          s3 += hs[j + u + 3] * q[u + 3];
+        }
       }
-    }
-    for (; j < k; j++) s0 += hs[j] * Qb[ldq * j + row];
-    v = w[row] - ((s0 + s1) + (s2 + s3));
-    w[row] = v;
+    sq[quarter][slot] = (s0 + s1) + (s2 + s3);
   }
-  if (t < kChunkRows) sv[t] = v;
+  __syncthreads();
+  double v = 0.0;
+  if (t < 256) {
+    if (row >= 0) {
+      v = w[row] - ((sq[0][t] + sq[1][t]) + (sq[2][t] + sq[3][t]));
+      w[row] = v;
+    }
+    if (t < kChunkRows) sv[t] = v;
+  }
   __syncthreads();
   if (PASS == 1) {
     chunk_col_dots(a, T, first, sv, Qb, ldq, k, part_out, nparts);
   } else {
-    const double s = block_sum<256>(v * v, sh);
+    const double s = block_sum<kGsThreads>(v * v, sh);
     if (t == 0) npart[blockIdx.x] = s;
     const d3 wl = {sv[t % kChunk], sv[kChunk + t % kChunk], sv[2 * kChunk + t % kChunk]};
     const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
-    tile_kt_vals(Qm, a, T.m0 + t, t < T.n, wl, kpart + 6 * blockIdx.x, sh);
+    chunk_kt_vals(Qm, a, T.m0 + t, t < T.n, wl, kpart + 6 * blockIdx.x);
   }
 }
 
@@ -1798,12 +1832,10 @@ static int sys_pairs(gmi_ctx *c, SimHost *s, double *z, int mode, const int *don
   int rc;
   // split partials are summed by k_kprod (its tiles are the list's tiles)
   if (mode == 2) {
-    pa.skip = d.flags + F_SKIP64;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, false, c->stream), "pair bb64")))
-      return rc;
-    pa.skip = d.flags + F_SKIP32;
-    if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, true, c->stream), "pair bb32")))
-      return rc;
+    // one launch: FP32 once the relaxed switch cleared F_SKIP32, else FP64
+    pa.skip = d.flags + F_DONE;
+    pa.sel32 = d.flags + F_SKIP32;
+    if ((rc = hip_check(c, launch_pair_dyn(pa, wl.nwork, 0, c->stream), "pair bb"))) return rc;
   } else {
     pa.skip = done;
     if ((rc = hip_check(c, launch_pair(pa, wl.nwork, 0, mode == 1, c->stream), "pair bb")))
@@ -1876,13 +1908,13 @@ struct SimOp {
     if ((rc = prof_gs_start(c, it, d.npad, &gs_stop))) return rc;
     double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
     const WorkList &wl = c->lists[kBB];
-    hipLaunchKernelGGL(k_kprod_dots, dim3(nc), dim3(256), 0, c->stream, d, s->chunks, s->ch_tile,
+    hipLaunchKernelGGL(k_kprod_dots, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks, s->ch_tile,
                        s->sw_chunk0, s->sw_nchunk, wl.d_tiles, c->part, s->z, c->Vb, w,
                        s->ch_zpart, s->flags, s->Qk, ldq, it, it > 1 ? 1 : 0, s->scal, pp1, nc);
-    hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+    hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
                        s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, pp2, nc, nullptr, nullptr,
                        s->flags);
-    hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(256), 0, c->stream, d, s->chunks,
+    hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
                        s->sw_chunk0, s->Qk, ldq, it, pp2, s->h2, w, nullptr, nc, s->part,
                        s->ch_kpart, s->flags);
     if (gs_stop && (rc = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return rc;
