@@ -25,9 +25,9 @@ HIP_SOURCES = [
     "gmi_cilia.hip",
     "gmi_dense.hip",
     "gmi_noalloc.hip",
+    "gmi_gemm.hip",
 ]
-# rocBLAS supplies the DGEMMs of the one-time body-frame inversion (gmi_dense.hip)
-LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+LINK_FLAGS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
 
 # Packed FP32 ops are disabled for every HIP source: the compiler would
 # otherwise form v_pk_* from float code.

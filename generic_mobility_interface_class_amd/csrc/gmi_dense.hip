@@ -26,7 +26,6 @@
 #include "gmi_lie.hpp"
 #include "gmi_sim_state.hpp"
 
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -39,7 +38,6 @@ struct SymTile { int sw, I, J, pad; };
 struct SymRow { int sw, K, nbl, tile0; };  // block row K of a swimmer with nbl block rows
 
 struct DensePrec {
-  rocblas_handle blas = nullptr;
   std::vector<size_t> moff_h;   // per swimmer: first element of its matrix
   std::vector<int> voff_h;      // per swimmer: first row of its body-frame vector
   int nrows = 0, maxn = 0;      // total rows (sum 3 nb), largest 3 nb
@@ -420,7 +418,6 @@ void dense_release(SimHost *s) {
   DensePrec *D = s->dense;
   if (!D) return;
   for (void *p : D->owned) (void)hipFree(p);
-  if (D->blas) (void)rocblas_destroy_handle(D->blas);
   delete D;
   s->dense = nullptr;
   s->Zk = nullptr;  // owned by D
@@ -432,10 +429,6 @@ static int dmalloc(gmi_ctx *c, DensePrec *D, void **p, size_t bytes) {
   return rc;
 }
 
-static int blas_check(gmi_ctx *c, rocblas_status st, const char *what) {
-  if (st == rocblas_status_success) return GMI_OK;
-  return fail(c, GMI_ERR_HIP, std::string(what) + ": rocBLAS status " + std::to_string(int(st)));
-}
 
 int dense_setup(gmi_ctx *c, SimHost *s) {
   const SimDev &d = s->d;
@@ -446,9 +439,6 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
     dense_release(s);
     return code;
   };
-  if ((rc = blas_check(c, rocblas_create_handle(&D->blas), "rocblas_create_handle")) ||
-      (rc = blas_check(c, rocblas_set_stream(D->blas, c->stream), "rocblas_set_stream")))
-    return bail(rc);
   D->moff_h.resize(d.Nsw);
   D->voff_h.resize(d.Nsw);
   size_t mtot = 0;
@@ -518,7 +508,6 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
                       "diagonal-block LDS")))
     return bail(rc);
   hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
-  const double one = 1.0, zero = 0.0, minus = -1.0;
   for (int n = 0; n < d.Nsw; n++) {
     const int nb = c->nblob[n], rows = 3 * nb;
     const size_t ld = size_t(rows);
@@ -531,24 +520,20 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       hipLaunchKernelGGL(k_dense_diag_inv, dim3(1), dim3(256), sizeof(double) * kb * kb,
                          c->stream, A, ld, k0, kb, Dd, d.flags);
       // T = D A[K, :]  (kb x rows)
-      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
-                                            rocblas_operation_none, kb, rows, kb, &one, Dd, kb,
-                                            A + k0, rows, &zero, T, kb), "dgemm T")))
+      if ((rc = hip_check(c, launch_dgemm(false, kb, rows, kb, 1.0, Dd, kb, A + k0, ld, 0.0, T,
+                                          kb, c->stream), "dgemm T")))
         return bail(rc);
       // P = A[:, K]
       if ((rc = hip_check(c, hipMemcpyAsync(P, A + size_t(k0) * ld, sizeof(double) * ld * kb,
                                             hipMemcpyDeviceToDevice, c->stream), "copy P")))
         return bail(rc);
       // A -= P T
-      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
-                                            rocblas_operation_none, rows, rows, kb, &minus, P,
-                                            rows, T, kb, &one, A, rows), "dgemm update")))
+      if ((rc = hip_check(c, launch_dgemm(false, rows, rows, kb, -1.0, P, ld, T, kb, 1.0, A, ld,
+                                          c->stream), "dgemm update")))
         return bail(rc);
       // A[:, K] = -P D, then A[K, :] = T and A[K, K] = D
-      if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none,
-                                            rocblas_operation_none, rows, kb, kb, &minus, P,
-                                            rows, Dd, kb, &zero, A + size_t(k0) * ld, rows),
-                           "dgemm column")))
+      if ((rc = hip_check(c, launch_dgemm(false, rows, kb, kb, -1.0, P, ld, Dd, kb, 0.0,
+                                          A + size_t(k0) * ld, ld, c->stream), "dgemm column")))
         return bail(rc);
       hipLaunchKernelGGL(k_dense_fix_rows, dim3(nb_of(kb * rows, 256)), dim3(256), 0, c->stream,
                          A, ld, k0, kb, T, Dd);
@@ -558,12 +543,10 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
     hipLaunchKernelGGL(k_dense_k, dim3(nb_of(nb, 256)), dim3(256), 0, c->stream, d.blob_ref,
                        d.ldb, c->blob_off[n], nb, K);
     double *Gn = D->G + size_t(6) * D->voff_h[n];
-    if ((rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_none, rocblas_operation_none,
-                                          rows, 6, rows, &one, A, rows, K, rows, &zero, Gn, rows),
-                         "dgemm G")) ||
-        (rc = blas_check(c, rocblas_dgemm(D->blas, rocblas_operation_transpose,
-                                          rocblas_operation_none, 6, 6, rows, &one, K, rows, Gn,
-                                          rows, &zero, S, 6), "dgemm S")))
+    if ((rc = hip_check(c, launch_dgemm(false, rows, 6, rows, 1.0, A, ld, K, ld, 0.0, Gn, ld,
+                                        c->stream), "dgemm G")) ||
+        (rc = hip_check(c, launch_dgemm(true, 6, 6, rows, 1.0, K, ld, Gn, ld, 0.0, S, 6,
+                                        c->stream), "dgemm S")))
       return bail(rc);
     hipLaunchKernelGGL(k_dense_sinv, dim3(1), dim3(1), 0, c->stream, S, D->Sinv + 36 * n, d.flags);
     if ((rc = launch_ok(c, "schur"))) return bail(rc);

@@ -74,6 +74,12 @@ hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb,
                           double *part, int nseg_per_fil, double a_s, double a_b, double K,
                           hipStream_t s);
 
+// C = alpha op(A) B + beta C, column-major FP64 on the matrix cores
+// (gmi_gemm.hip); op(A) = A (m x k) or, with transA, A^T (A is k x m).
+hipError_t launch_dgemm(bool transA, int m, int n, int k, double alpha, const double *A,
+                        size_t lda, const double *B, size_t ldb, double beta, double *C,
+                        size_t ldc, hipStream_t s);
+
 // AoS <-> SoA transposes (gmi_pair.hip).
 hipError_t launch_aos_to_soa(double *dst, int ld, const double *src, int n,
                              int ncomp, hipStream_t s);

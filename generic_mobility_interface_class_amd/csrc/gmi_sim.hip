@@ -538,6 +538,7 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
 // slots 0..191 are component t / 64 of blob t % 64, slots 192..197 body rows.
 
 constexpr int kChunk = 64;                 // blobs per chunk
+constexpr int kWideChunks = 256;           // above: column partials summed once per column
 constexpr int kChunkRows = 3 * kChunk + 6; // slots per chunk
 
 // GMRES row of slot q of chunk T, or -1 when the chunk has no such row.
@@ -789,7 +790,7 @@ __global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *
                                                          double *hout, double *w,
                                                          double *part_out, int nparts,
                                                          double *npart, double *kpart,
-                                                         const int *flags) {
+                                                         const int *flags, int h_ready) {
   if (flags[F_DONE]) return;
   __shared__ double hs[kMaxCols + 16];
   __shared__ double sv[kChunkRows];
@@ -798,7 +799,10 @@ __global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *
   const BTile T = chunks[blockIdx.x];
   const int t = threadIdx.x;
   const bool first = blockIdx.x == sw_chunk0[T.sw];
-  chunk_col_sums(part_in, k, nparts, hs, hout);
+  if (h_ready)  // many chunks: a k_colsum_wide launch summed the partials once
+    for (int j = t; j < k; j += kGsThreads) hs[j] = hout[j];
+  else
+    chunk_col_sums(part_in, k, nparts, hs, hout);
   if (t < 16) hs[k + t] = 0.0;  // the padding of the last group of 16
   __syncthreads();
   // thread (slot t % 256, quarter t / 256): the quarter's share of
@@ -1911,12 +1915,21 @@ struct SimOp {
     hipLaunchKernelGGL(k_kprod_dots, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks, s->ch_tile,
                        s->sw_chunk0, s->sw_nchunk, wl.d_tiles, c->part, s->z, c->Vb, w,
                        s->ch_zpart, s->flags, s->Qk, ldq, it, it > 1 ? 1 : 0, s->scal, pp1, nc);
+    // beyond kWideChunks chunks every workgroup re-summing every column's
+    // partials would cost more than one launch that sums each column once
+    const bool wide = nc > kWideChunks;
+    if (wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, nc, s->h1,
+                         s->flags);
     hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
                        s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, pp2, nc, nullptr, nullptr,
-                       s->flags);
+                       s->flags, int(wide));
+    if (wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, nc, s->h2,
+                         s->flags);
     hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
                        s->sw_chunk0, s->Qk, ldq, it, pp2, s->h2, w, nullptr, nc, s->part,
-                       s->ch_kpart, s->flags);
+                       s->ch_kpart, s->flags, int(wide));
     if (gs_stop && (rc = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return rc;
     return launch_ok(c, "arnoldi step");
   }
