@@ -94,43 +94,65 @@ __global__ __launch_bounds__(64) void k_dense_assemble(const double *__restrict_
       M[size_t(c) * nb + i + (size_t(e) * nb + j) * n] = (c == e ? A : 0.0) + B * d[c] * d[e];
 }
 
-// In-LDS Gauss-Jordan inverse (no pivoting: the diagonal blocks of the
-// Schur complements of an SPD matrix are SPD) of the kb x kb diagonal block
-// of A (column-major, ld n) at (k0, k0) into D (column-major, ld kb).
-__global__ __launch_bounds__(256) void k_dense_diag_inv(const double *__restrict__ A, size_t n,
-                                                        int k0, int kb, double *__restrict__ D,
-                                                        int *flags) {
-  extern __shared__ double L[];  // kb x kb column-major
-  __shared__ double colk[kDiag];
-  const int t = threadIdx.x;
-  for (int e = t; e < kb * kb; e += 256) {
-    const int r = e % kb, c = e / kb;
-    L[e] = A[size_t(k0 + r) + size_t(k0 + c) * n];
+// Gauss-Jordan inverse (no pivoting: the diagonal blocks of the Schur
+// complements of an SPD matrix are SPD) of the kb x kb diagonal block of A
+// (column-major, ld n) at (k0, k0) into D (column-major, ld kb).  1024
+// threads hold the block in registers: thread t owns row t / 8, columns
+// 16 (t % 8) .. + 15 (rows and columns past kb are the identity's).  Each
+// pivot step publishes row k and column k in LDS (double-buffered, so one
+// barrier per step) and every thread updates its 16 entries.
+__global__ __launch_bounds__(1024) void k_dense_diag_inv(const double *__restrict__ A, size_t n,
+                                                         int k0, int kb, double *__restrict__ D,
+                                                         int *flags) {
+  static_assert(kDiag == 128, "thread layout: 128 rows x 8 groups of 16 columns");
+  __shared__ double srow[2][kDiag], scol[2][kDiag];
+  const int t = threadIdx.x, i = t >> 3, g = t & 7;
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int c = 16 * g + u;
+    v[u] = (i < kb && c < kb) ? A[size_t(k0 + i) + size_t(k0 + c) * n] : (i == c ? 1.0 : 0.0);
   }
-  __syncthreads();
   for (int k = 0; k < kb; k++) {
-    const double p = L[k + k * kb];
+    const int b = k & 1;
+    if (i == k)
+#pragma unroll
+      for (int u = 0; u < 16; u++) srow[b][16 * g + u] = v[u];
+    if (g == (k >> 4))
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        if (u == (k & 15)) scol[b][i] = v[u];
+    __syncthreads();
+    const double p = srow[b][k];
     if (!(p > 0.0)) {
       if (t == 0) flags[F_SING] = 1;
       return;
     }
     const double inv = 1.0 / p;
-    // column k of the other rows before the update
-    for (int r = t; r < kb; r += 256) colk[r] = L[r + k * kb];
-    __syncthreads();
-    // row k scaled: L(k, j) = L(k, j) / p, L(k, k) = 1 / p
-    for (int j = t; j < kb; j += 256) L[k + j * kb] = (j == k ? 1.0 : L[k + j * kb]) * inv;
-    __syncthreads();
-    // other rows: L(i, j) -= colk(i) L(k, j); column k: L(i, k) = -colk(i) / p
-    for (int e = t; e < kb * kb; e += 256) {
-      const int r = e % kb, c = e / kb;
-      if (r == k) continue;
-      const double base = (c == k) ? 0.0 : L[e];
-      L[e] = base - colk[r] * L[k + c * kb];
+    if (i == k) {
+      // row k scaled: L(k, j) = L(k, j) / p, L(k, k) = 1 / p
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int c = 16 * g + u;
+        v[u] = (c == k ? 1.0 : v[u]) * inv;
+      }
+    } else {
+      // other rows: L(i, j) -= L(i, k) L(k, j) / p; column k: L(i, k) = -L(i, k) / p
+      const double ci = scol[b][i];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int c = 16 * g + u;
+        const double rk = (c == k ? 1.0 : srow[b][c]) * inv;
+        v[u] = (c == k ? 0.0 : v[u]) - ci * rk;
+      }
     }
-    __syncthreads();
   }
-  for (int e = t; e < kb * kb; e += 256) D[e] = L[e];
+  if (i < kb)
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int c = 16 * g + u;
+      if (c < kb) D[i + size_t(c) * kb] = v[u];
+    }
 }
 
 // Gauss-Jordan fix-ups of block column k0: A[K, :] <- T (kb x n, ld kb) and
@@ -502,11 +524,6 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       (rc = hip_check(c, hipMemcpyAsync(D->voff, D->voff_h.data(), sizeof(int) * d.Nsw,
                                         hipMemcpyHostToDevice, c->stream), "upload")))
     return bail(rc);
-  if ((rc = hip_check(c, hipFuncSetAttribute(reinterpret_cast<const void *>(k_dense_diag_inv),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             int(sizeof(double) * kDiag * kDiag)),
-                      "diagonal-block LDS")))
-    return bail(rc);
   hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
   for (int n = 0; n < d.Nsw; n++) {
     const int nb = c->nblob[n], rows = 3 * nb;
@@ -517,8 +534,8 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
     if ((rc = launch_ok(c, "assemble"))) return bail(rc);
     for (int k0 = 0; k0 < rows; k0 += kDiag) {
       const int kb = std::min(kDiag, rows - k0);
-      hipLaunchKernelGGL(k_dense_diag_inv, dim3(1), dim3(256), sizeof(double) * kb * kb,
-                         c->stream, A, ld, k0, kb, Dd, d.flags);
+      hipLaunchKernelGGL(k_dense_diag_inv, dim3(1), dim3(1024), 0, c->stream, A, ld, k0, kb, Dd,
+                         d.flags);
       // T = D A[K, :]  (kb x rows)
       if ((rc = hip_check(c, launch_dgemm(false, kb, rows, kb, 1.0, Dd, kb, A + k0, ld, 0.0, T,
                                           kb, c->stream), "dgemm T")))
