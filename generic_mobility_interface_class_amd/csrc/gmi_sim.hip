@@ -23,8 +23,10 @@
 #include "gmi_krylov.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 namespace gmi {
@@ -162,7 +164,8 @@ __device__ void extrap_one(double *v, int n, int k, int nt) {
   }
 }
 
-constexpr int kFilLanes = 64;  // one wave per filament (NSEG <= 40)
+constexpr int kMaxSeg = 64;    // segments per filament of the time step
+constexpr int kFilLanes = 64;  // one wave per filament (NSEG <= kMaxSeg)
 
 // One wave per filament: lane j shifts and extrapolates segment j's state
 // and updates its frame; lane 0 then takes the base from the body
@@ -219,7 +222,7 @@ __device__ d3 strain_twist(const SimDev &a, int f, double t) {
   return {0.0, 0.0, a.camp * sin(2.0 * kPi * t / a.cper + a.phase[f])};
 }
 
-// One wave per filament (NSEG <= 40 < 64): lane j owns segment j and the
+// One wave per filament (NSEG <= 64): lane j owns segment j and the
 // link (j, j+1).  The link moments run in parallel; each segment's force and
 // torque is assembled from its two links in the order of the sequential loop
 // over links (the same operations, so the same rounding), and lane 0 sums
@@ -632,8 +635,7 @@ __device__ void swimmer_share_sums(const double *part, int q0, int nq, double *o
   const int c = t >> 5, l = t & 31;
   if (c < 6)
     for (int q = l; q < nq; q += 32) s += part[6 * (q0 + q) + c];
-  // sum over the 32 lanes of each half-wave (xor offsets below 32 stay inside it)
-  for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = half_wave_sum(s);
   if (c < 6 && l == 0) out[c] = s;
   __syncthreads();
 }
@@ -1258,8 +1260,8 @@ __global__ __launch_bounds__(256) void k_gj_invert(SimDev a) {
   const int n = 6 * a.S;
   double *A = a.Jw + size_t(f) * n * n;
   double *Out = a.Jinv + size_t(f) * n * n;
-  __shared__ int piv[256];
-  __shared__ double colk[256];
+  __shared__ int piv[6 * kMaxSeg];
+  __shared__ double colk[6 * kMaxSeg];
   __shared__ double rv[256];
   __shared__ int ri[256];
   const int t = threadIdx.x;
@@ -1407,12 +1409,215 @@ inline size_t gj_lds_bytes(int S) {
   return n * n * sizeof(double) + n * sizeof(double) + n * sizeof(int);
 }
 
+// The same Gauss-Jordan inverse with the matrix in registers, for n <= 128
+// (NSEG <= 21).  256 threads, one wave per SIMD: thread t holds rows
+// t / 8 + 32 rho (rho < 4) and columns t % 8 + 8u (u < 16), so each step's
+// pivot row, read once from LDS, serves four rows, and the per-step
+// bookkeeping runs on four waves only.  Rows are not swapped: step k takes
+// as pivot an unused row of largest |A(i,k)|, so row p_k of the in-place
+// result is row k of the inverse of the row-permuted matrix, and
+// Out(k, p_j) = A(p_k, j).
+//
+// The pivot is the largest key over the candidate rows: |A(i,k)| (clamped
+// to the largest finite double) with the low 7 mantissa bits replaced by
+// 127 - i, compared as a double (a non-negative double orders as its bit
+// pattern), so the lowest row wins among magnitudes that agree to 2^-45;
+// each wave takes its maximum by DPP and publishes it with the inverse of
+// its candidate's value, computed before the barrier.
+//
+// One step, k: the elimination of step k - 1 is applied first to the slot
+// u = k / 8 that holds column k (the outer loop is unrolled, so that slot is
+// a register), column k and the wave keys are published (barrier), the
+// rest of step k - 1's elimination runs while the keys are read, the pivot
+// row normalises and publishes itself (barrier).  Every entry sees the same
+// operations as in the in-LDS kernel; two buffers let step k + 1 publish
+// while step k is still being read.  (This takes about 1.3 us per step at
+// n = 120: the chain of dependent LDS, DPP and FP64 operations between the
+// barriers, not the arithmetic, sets the time.)
+constexpr int kGjRegMax = 128;
+constexpr int kGjRegCols = kGjRegMax / 8;
+constexpr int kGjRegRows = 4;
+
+struct GjRegShared {
+  double scol[2][kGjRegMax], srow[2][kGjRegMax];
+  double skey[2][4], sinv[2][4];  // per wave: best key, inverse of its pivot
+  int prow[kGjRegMax];
+};
+
+struct GjRegState {
+  double v[kGjRegRows][kGjRegCols];  // rows t / 8 + 32 rho, columns t % 8 + 8u
+  bool used[kGjRegRows];
+  int mystep[kGjRegRows];
+  int plast;  // pivot row of the last step, whose elimination is pending
+};
+
+// The eliminations of step kp (pivot row st.plast, its normalised row and
+// column kp in buffer kp & 1) on slots u in [U0, U1), skipping slot SKIP;
+// column kp lies in a slot of [C0, C1), the only ones that need a select.
+template <int U0, int U1, int SKIP, int C0, int C1>
+__device__ __forceinline__ void gj_reg_eliminate(GjRegState &st, const double (&pr)[kGjRegCols],
+                                                 const double (&fi)[kGjRegRows], int kp) {
+  const int g = threadIdx.x & 7;
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const bool piv = (threadIdx.x >> 3) + 32 * q == st.plast;
+#pragma unroll
+    for (int u = U0; u < U1; u++) {
+      if (u == SKIP) continue;
+      if (u >= C0 && u < C1) {
+        const bool colk = g + 8 * u == kp && !piv;
+        st.v[q][u] = (colk ? 0.0 : st.v[q][u]) - fi[q] * pr[u];
+      } else {
+        st.v[q][u] = st.v[q][u] - fi[q] * pr[u];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void gj_reg_pending(const GjRegShared &sh, const GjRegState &st, int kp,
+                                               double (&pr)[kGjRegCols], double (&fi)[kGjRegRows]) {
+  const int t = threadIdx.x, g = t & 7, b = kp & 1;
+#pragma unroll
+  for (int u = 0; u < kGjRegCols; u++) pr[u] = sh.srow[b][g + 8 * u];
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const int r = (t >> 3) + 32 * q;
+    fi[q] = r == st.plast ? 0.0 : sh.scol[b][r];
+  }
+}
+
+// Steps k = 8 UK + gk, gk < 8, k < n; false if a pivot is zero or not finite.
+template <int UK>
+__device__ __forceinline__ bool gj_reg_block(GjRegShared &sh, GjRegState &st, int n) {
+  const int t = threadIdx.x, g = t & 7, lane = t & 63, wave = t >> 6;
+  for (int gk = 0; gk < 8; gk++) {
+    const int k = 8 * UK + gk;
+    if (k >= n) break;
+    const int b = k & 1;
+    double pr[kGjRegCols], fi[kGjRegRows];
+    if (k > 0) {
+      gj_reg_pending(sh, st, k - 1, pr, fi);
+      gj_reg_eliminate<UK, UK + 1, -1, UK, UK + 1>(st, pr, fi, k - 1);
+    }
+    // owner lanes (g = gk): the best of their four rows as a key and the
+    // inverse of its value (used only if that row wins)
+    double key = 0.0, kinv = 0.0;
+    if (g == gk) {
+      double xb = 0.0;
+#pragma unroll
+      for (int q = 0; q < kGjRegRows; q++) {
+        const int r = (t >> 3) + 32 * q;
+        const double x = st.v[q][UK];
+        sh.scol[b][r] = x;
+        const double ax = fabs(x);
+        const unsigned long long bits = __double_as_longlong(ax <= DBL_MAX ? ax : DBL_MAX);
+        const double kq = __longlong_as_double((long long)((bits & ~0x7full) |
+                                                           (unsigned long long)(127 - r)));
+        if (!st.used[q] && kq > key) { key = kq; xb = x; }
+      }
+      kinv = 1.0 / xb;
+    }
+    // the wave's best: owner lanes gk + 8m pair up by a rotation of 8 within
+    // each row of 16 lanes, then the four rows
+    key = fmax(key, dpp_d<0x128>(key));  // row_ror:8
+    key = fmax(fmax(lane_d(key, gk), lane_d(key, gk + 16)),
+               fmax(lane_d(key, gk + 32), lane_d(key, gk + 48)));
+    {
+      const int rw = 127 - int(__double_as_longlong(key) & 0x7f);
+      const double winv = lane_d(kinv, 8 * (rw & 7) + gk);
+      if (lane == 0) { sh.skey[b][wave] = key; sh.sinv[b][wave] = winv; }
+    }
+    __syncthreads();
+    double kw[4], iw[4];
+#pragma unroll
+    for (int w = 0; w < 4; w++) { kw[w] = sh.skey[b][w]; iw[w] = sh.sinv[b][w]; }
+    // column k - 1 is in slot UK, or UK - 1 when gk = 0
+    if (k > 0) gj_reg_eliminate<0, kGjRegCols, UK, UK - 1, UK>(st, pr, fi, k - 1);
+    double inv;
+    {
+      const bool a01 = kw[0] > kw[1], a23 = kw[2] > kw[3];
+      const double k01 = a01 ? kw[0] : kw[1], k23 = a23 ? kw[2] : kw[3];
+      const double i01 = a01 ? iw[0] : iw[1], i23 = a23 ? iw[2] : iw[3];
+      key = k01 > k23 ? k01 : k23;
+      inv = k01 > k23 ? i01 : i23;
+    }
+    // a zero, infinite or NaN pivot: singular
+    if (!(key > 0.0) || !(fabs(inv) <= DBL_MAX) || inv == 0.0) return false;
+    const int p = 127 - int(__double_as_longlong(key) & 0x7f);
+#pragma unroll
+    for (int q = 0; q < kGjRegRows; q++) {
+      if ((t >> 3) + 32 * q == p) {
+#pragma unroll
+        for (int u = 0; u < kGjRegCols; u++) {
+          st.v[q][u] = ((u == UK && g == gk) ? 1.0 : st.v[q][u]) * inv;
+          sh.srow[b][g + 8 * u] = st.v[q][u];
+        }
+        st.used[q] = true;
+        st.mystep[q] = k;
+        if (g == 0) sh.prow[k] = p;
+      }
+    }
+    st.plast = p;
+    __syncthreads();
+  }
+  return true;
+}
+
+template <int... UK>
+__device__ __forceinline__ bool gj_reg_all(GjRegShared &sh, GjRegState &st, int n,
+                                           std::integer_sequence<int, UK...>) {
+  // blocks in order; a block past n does nothing
+  return ((8 * UK >= n || gj_reg_block<UK>(sh, st, n)) && ...);
+}
+
+__global__ __launch_bounds__(256) void k_gj_invert_reg(SimDev a) {
+  __shared__ GjRegShared sh;
+  const int f = blockIdx.x;
+  const int n = 6 * a.S;
+  const int t = threadIdx.x, g = t & 7;
+  const double *G = a.Jw + size_t(f) * n * n;
+  GjRegState st;
+  st.plast = -1;
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const int r = (t >> 3) + 32 * q;
+    st.used[q] = r >= n;
+    st.mystep[q] = 0;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) {
+      const int c = g + 8 * u;
+      st.v[q][u] = (r < n && c < n) ? G[size_t(r) * n + c] : 0.0;
+    }
+  }
+  if (!gj_reg_all(sh, st, n, std::make_integer_sequence<int, kGjRegCols>{})) {
+    if (t == 0) atomicOr(&a.flags[F_SING], 1);
+    return;
+  }
+  {
+    // the last step's elimination
+    double pr[kGjRegCols], fi[kGjRegRows];
+    gj_reg_pending(sh, st, n - 1, pr, fi);
+    gj_reg_eliminate<0, kGjRegCols, -1, 0, kGjRegCols>(st, pr, fi, n - 1);
+  }
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const int r = (t >> 3) + 32 * q;
+    if (r >= n) continue;
+    double *Out = a.Jinv + size_t(f) * n * n + size_t(st.mystep[q]) * n;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) {
+      const int c = g + 8 * u;
+      if (c < n) Out[sh.prow[c]] = st.v[q][u];
+    }
+  }
+}
+
 // Schur-complement blocks of one filament (swimmer.cpp:526-680):
 // Bblk = B J^-1 (6 x 6S) and BA = Bblk A (6 x 6).
 __global__ __launch_bounds__(256) void k_schur_fil(SimDev a, int nt) {
   const int f = blockIdx.x;
   const int S = a.S, n = 6 * S;
-  __shared__ double B[6 * 240];
+  __shared__ double B[6 * 6 * kMaxSeg];
   const int t = threadIdx.x;
   const double *b = a.body + size_t(a.fil_sw[f]) * BREC;
   const bool euler = nt < a.neuler;
@@ -1675,7 +1880,7 @@ __global__ __launch_bounds__(256) void k_jim_body(SimDev a, const double *in, do
 }
 
 __global__ __launch_bounds__(256) void k_jim2(SimDev a, const double *in, double *out) {
-  __shared__ double tmp[240];
+  __shared__ double tmp[6 * kMaxSeg];
   const int f = blockIdx.x;
   const int S = a.S, n = 6 * S;
   const int sw = a.fil_sw[f];
@@ -2041,7 +2246,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if (c->interact && c->nsw > 1)
     return fail(c, GMI_ERR_ARG, "a sim needs hydrodynamically independent swimmers");
   const int S = c->nseg_fil;
-  if (S < 2 || 6 * S > 240) return fail(c, GMI_ERR_ARG, "NSEG must be in [2, 40]");
+  if (S < 2 || S > kMaxSeg) return fail(c, GMI_ERR_ARG, "NSEG must be in [2, 64]");
   int Nf = 0;
   for (int n = 0; n < c->nsw; n++) {
     if (p->nfil[n] < 1 || p->nfil[n] * S != c->nseg[n])
@@ -2496,7 +2701,9 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   const double t = (nt + 1) * s->p.dt;
   hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
   hipLaunchKernelGGL(k_fil_jacobian, dim3(d.Nf), dim3(64), 0, c->stream, d, nt, t);
-  if (s->gj_lds)
+  if (6 * d.S <= kGjRegMax)
+    hipLaunchKernelGGL(k_gj_invert_reg, dim3(d.Nf), dim3(256), 0, c->stream, d);
+  else if (s->gj_lds)
     hipLaunchKernelGGL(k_gj_invert_lds, dim3(d.Nf), dim3(kGjThreads), gj_lds_bytes(d.S), c->stream,
                        d);
   else

@@ -260,12 +260,12 @@ def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
         assert nbytes.value >= 16.0 * n * 3 * gits
 
 
-@pytest.mark.parametrize("nseg", [23, 30])
+@pytest.mark.parametrize("nseg", [23, 30, 48, 64])
 def test_long_filaments_match_oracle(gpu, nseg):
     """Filaments longer than 21 segments: NSEG = 23 (6 NSEG = 138 columns,
-    the Gauss-Jordan inverse held in LDS) and NSEG = 30 (180 columns, the
-    matrix in global memory).  J^-1 e at the first step and four steps
-    against the oracle."""
+    the Gauss-Jordan inverse held in LDS), NSEG = 30, 48 and 64 (up to 384
+    columns, the matrix in global memory; 64 is the largest NSEG a time step
+    takes).  J^-1 e at the first step and four steps against the oracle."""
     cfg = geometry.sim_config(nfil=6, nseg=nseg, nblob=162, R=6.0, seed=2)
     o, g = orc.Sim(cfg), capi.Sim(cfg)
     print(f"long filaments nseg {nseg}", flush=True)
