@@ -49,6 +49,11 @@ FLOPS = {"ss": 92, "sb": 56, "bs": 56, "bb": 40,      # include/gmi_flops.h
 # (target workgroups 1024-8192, minimum source chunks 64-256) this one gave
 # the shortest step.
 STEP_LAUNCH_SHAPE = (4096, 64)
+# The 962-particle swimmer's tiles are too few for that shape to fill the
+# chip (16 target tiles; 64-source chunks leave each lane a long serial loop):
+# of (2048-8192, 16-64) timed over its body-inverse rollout, 16-source chunks
+# gave the shortest step (1.49 against 1.71 ms).
+ROLLOUT_LAUNCH_SHAPE = (4096, 16)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -265,10 +270,19 @@ def bench_step(args):
     return out
 
 
+# Name of the one timed window whose bounds --profile-window prints, for
+# --summarize-trace to cut a whole-run kernel trace to it (None: no window).
+PROFILE_WINDOW = None
+
+
 def _timed_steps(sim, nt0, k, name):
     """k steps from nt0 timed with the context's HIP events.  A step that
     fails (non-convergence, divergence) ends the window and is reported."""
+    window = name == PROFILE_WINDOW
     sim.call("gmi_wait_for_device")
+    if window:
+        print(f"# {name}: profiled window opens at {time.monotonic_ns()} ns (CLOCK_MONOTONIC)",
+              file=sys.stderr, flush=True)
     sim.call("gmi_timer_start")
     tb = tg = 0
     failed = None
@@ -283,6 +297,10 @@ def _timed_steps(sim, nt0, k, name):
             break
     ms = capi.C.c_double(0.0)
     sim.call("gmi_timer_stop", capi.C.byref(ms))
+    if window:
+        sim.call("gmi_wait_for_device")
+        print(f"# {name}: profiled window closes at {time.monotonic_ns()} ns (CLOCK_MONOTONIC)",
+              file=sys.stderr, flush=True)
     if failed:
         return {"failed": failed, "steps": f"{nt0}-{nt0 + k - 1}", "steps_per_s": 0.0}
     r = dict(steps=f"{nt0}-{nt0 + k - 1}", steps_per_s=k / (ms.value * 1e-3),
@@ -376,7 +394,7 @@ def bench_rollout(args):
     for s in (learned, learned_inv):
         s.warmstart_configure(apply=False, collect=20000, stride=5)
     for s in (plain, learned, plain_inv, learned_inv):
-        s.call("gmi_set_launch_shape", *STEP_LAUNCH_SHAPE)
+        s.call("gmi_set_launch_shape", *ROLLOUT_LAUNCH_SHAPE)
         s.make_body_reference_matrices()
     for s in (plain_inv, learned_inv):
         s.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
@@ -400,6 +418,7 @@ def bench_rollout(args):
     for s in (plain, learned, plain_inv, learned_inv):
         s.close()
     return {"workload": "step_962", "warmup_steps": 300, "rollout_steps": k,
+            "launch_shape": list(ROLLOUT_LAUNCH_SHAPE),
             "plain": rp, "learned": rl,
             "speedup_learned_vs_plain": rl["steps_per_s"] / rp["steps_per_s"],
             "plain_inverse": rpi, "learned_inverse": rli,
@@ -601,6 +620,44 @@ def cpu_step_baseline(args):
                       f"{dt:.1f} s; pair sums on {threads} threads, the rest single-threaded)"}
 
 
+def summarize_trace(trace_csv, bench_log, steps):
+    """Markdown table of the kernels of a rocprofv3 kernel trace (CSV) that
+    ran inside the window a --profile-window run printed (CLOCK_MONOTONIC ns,
+    the trace's clock): calls, time and share per kernel, per step."""
+    import csv
+    import re
+    steps = int(steps)
+    bounds = {}
+    for line in open(bench_log):
+        m = re.search(r"profiled window (opens|closes) at (\d+) ns", line)
+        if m:
+            bounds[m.group(1)] = int(m.group(2))
+    if "opens" not in bounds or "closes" not in bounds:
+        sys.exit(f"{bench_log}: no profiled window (run bench.py --profile-window MODE)")
+    t0, t1 = bounds["opens"], bounds["closes"]
+    per = {}
+    busy = 0
+    for r in csv.DictReader(open(trace_csv)):
+        s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if s < t0 or e > t1:
+            continue
+        name = r["Kernel_Name"].replace("void ", "").replace("gmi::(anonymous namespace)::", "")
+        name = re.sub(r"\(.*", "", name)
+        c = per.setdefault(name, [0, 0])
+        c[0] += 1
+        c[1] += e - s
+        busy += e - s
+    wall = t1 - t0
+    print(f"window {wall / 1e6:.2f} ms over {steps} steps: {wall / 1e6 / steps:.3f} ms/step wall, "
+          f"{busy / 1e6 / steps:.3f} ms/step in kernels ({100.0 * busy / wall:.0f}% busy), "
+          f"{sum(c for c, _ in per.values()) / steps:.1f} kernels/step\n")
+    print("| kernel | calls/step | us/step | avg us | share of kernel time |")
+    print("|---|---:|---:|---:|---:|")
+    for name, (c, d) in sorted(per.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{name}` | {c / steps:.2f} | {d / 1e3 / steps:.1f} | {d / 1e3 / c:.2f} | "
+              f"{100.0 * d / busy:.1f}% |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -641,7 +698,19 @@ def main():
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the Arnoldi kernels eagerly (for kernel-trace profiling)")
+    ap.add_argument("--summarize-trace", nargs=3, default=None,
+                    metavar=("KERNEL_TRACE_CSV", "BENCH_LOG", "STEPS"),
+                    help="no GPU work: summarise a rocprofv3 kernel trace over the window "
+                         "that a --profile-window run printed to BENCH_LOG (per step over "
+                         "STEPS steps), as a markdown table")
+    ap.add_argument("--profile-window", default=None, metavar="MODE",
+                    help="run only the time-step leg in MODE (e.g. learned_inverse_mixed), "
+                         "eagerly, and print the bounds of its timed window, so that "
+                         "--summarize-trace can cut a kernel trace of the run to it")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(*args.summarize_trace)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one MI355X; "
               "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
@@ -656,6 +725,12 @@ def main():
     if args.apply_only:
         args.no_cpu = args.no_step = args.no_ensemble = args.no_rollout = args.no_cilia = True
         args.no_carpet = True
+    if args.profile_window:
+        global PROFILE_WINDOW
+        PROFILE_WINDOW = f"step {args.profile_window}"
+        args.step_modes = args.profile_window
+        args.no_cpu = args.no_ensemble = args.no_rollout = args.no_cilia = True
+        args.no_carpet = args.no_graphs = True
     if args.copy_calibration:
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")

@@ -38,7 +38,7 @@ int hip_check(gmi_ctx *c, hipError_t e, const char *what) {
 int pad64(int n) { return ((n + 63) / 64) * 64; }
 
 // Build the work list of one block mask.
-constexpr int kBarrierTargetBlocks = 4096, kBarrierMinChunk = 256;
+constexpr int kBarrierTargetBlocks = 1024, kBarrierMinChunk = 64;
 
 static int build_list(gmi_ctx *c, int mask, WorkList &wl, int target_blocks, int min_chunk) {
   wl.h_tiles.clear();
@@ -121,9 +121,9 @@ int rebuild_lists(gmi_ctx *c) {
   }
   // the barrier visits every source of the target's group (the all-blocks
   // list) with a shape of its own: a distance test per pair is far lighter
-  // than a mobility pair, so it wants more, shorter source splits than the
-  // pair kernels' tuned shape gives it (measured: 63 us at 9,122 particles
-  // against up to 143 us under the pair shapes)
+  // than a mobility pair, so it wants short source splits on small systems
+  // (64-source chunks: 17.5 against 39.5 us at 962 particles with 256) but
+  // few partials to reduce on large ones (about 1,024 workgroups)
   if ((rc = build_list(c, kSS | kSB | kBS | kBB, c->barrier_tiles, kBarrierTargetBlocks,
                        kBarrierMinChunk)))
     return rc;

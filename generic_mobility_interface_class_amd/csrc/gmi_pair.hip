@@ -24,6 +24,7 @@
 // compensated sums across tiles; partial sums across source splits are
 // reduced in FP64.
 #include "gmi_internal.hpp"
+#include "gmi_wave.hpp"
 
 namespace gmi {
 namespace {
@@ -788,21 +789,31 @@ __device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, do
 }
 
 // Work item = (target tile, source split) of the all-sources work list;
-// split partials are summed by k_barrier_reduce.
+// split partials are summed by k_barrier_reduce.  A pair contributes only
+// inside 1.1 (a_i + a_j), so each wave skips the 64-source slices of a
+// staged tile whose bounding box lies farther than that from the bounding
+// box of its own 64 targets (wave-uniform tests; a skipped pair would have
+// added exactly nothing, so the sums are those of the full loop).
 __global__ __launch_bounds__(kBlock) void k_barrier(
     double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
     const double *__restrict__ Xs, const double *__restrict__ Xb,
     const TileDesc *__restrict__ tiles, const WorkItem *__restrict__ work,
     double *__restrict__ part, int nseg_fil, double as, double ab, double K) {
+  static_assert(kTile == kBlock && kBlock % 64 == 0, "one 64-source slice per wave");
+  constexpr int kSlices = kTile / 64;
   __shared__ double sx[kTile], sy[kTile], sz[kTile];
+  __shared__ double sbox[kSlices][6];  // per slice: min x y z, max x y z
   const WorkItem w = work[blockIdx.x];
   const TileDesc T = tiles[w.tile];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const bool seg_t = (T.species == 0);
   const int i = T.t0 + (tid < T.nt ? tid : 0);
   const double *X = seg_t ? Xs : Xb;
   const int ld = seg_t ? lds : ldb;
   const double xi = X[i], yi = X[ld + i], zi = X[2 * ld + i];
+  // this wave's targets' box (lanes past the tile repeat its first target)
+  const double tlo[3] = {wave_min(xi), wave_min(yi), wave_min(zi)};
+  const double thi[3] = {wave_max(xi), wave_max(yi), wave_max(zi)};
   // excluded sources: a segment itself and its neighbours in the same
   // filament (:1040) -- the index range [lo, hi]; a blob itself (:1120)
   int lo = i, hi = i;
@@ -824,17 +835,39 @@ __global__ __launch_bounds__(kBlock) void k_barrier(
     const int j1 = (src_seg ? T.sseg0 : T.sblob0) + r1;
     const bool same = (src_seg == seg_t);
     const double asum = src_seg ? (seg_t ? 2.0 * as : as + ab) : (seg_t ? as + ab : 2.0 * ab);
+    // boxes farther apart than the contact distance (with a relative margin
+    // far above rounding) hold no contributing pair
+    const double cut = 1.1 * asum * (1.0 + 1e-9), cut2 = cut * cut;
     for (int base = j0; base < j1; base += kTile) {
       const int n = min(kTile, j1 - base);
       __syncthreads();
-      if (tid < n) {
-        sx[tid] = S[base + tid]; sy[tid] = S[sld + base + tid]; sz[tid] = S[2 * sld + base + tid];
+      {
+        // lanes past the tile repeat its last source
+        const int j = base + min(tid, n - 1);
+        const double px = S[j], py = S[sld + j], pz = S[2 * sld + j];
+        if (tid < n) { sx[tid] = px; sy[tid] = py; sz[tid] = pz; }
+        const double b0 = wave_min(px), b1 = wave_min(py), b2 = wave_min(pz);
+        const double b3 = wave_max(px), b4 = wave_max(py), b5 = wave_max(pz);
+        if ((tid & 63) == 0) {
+          sbox[wave][0] = b0; sbox[wave][1] = b1; sbox[wave][2] = b2;
+          sbox[wave][3] = b3; sbox[wave][4] = b4; sbox[wave][5] = b5;
+        }
       }
       __syncthreads();
-      for (int jj = 0; jj < n; ++jj) {
-        const int j = base + jj;
-        if (same && j >= lo && j <= hi) continue;
-        barrier_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], asum, K, fx, fy, fz);
+      for (int sl = 0; sl < kSlices && 64 * sl < n; ++sl) {
+        double gap2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double g = fmax(0.0, fmax(sbox[sl][c] - thi[c], tlo[c] - sbox[sl][3 + c]));
+          gap2 += g * g;
+        }
+        if (gap2 > cut2) continue;
+        const int jj1 = min(n, 64 * sl + 64);
+        for (int jj = 64 * sl; jj < jj1; ++jj) {
+          const int j = base + jj;
+          if (same && j >= lo && j <= hi) continue;
+          barrier_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], asum, K, fx, fy, fz);
+        }
       }
     }
   }
