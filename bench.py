@@ -312,11 +312,13 @@ def _timed_steps(sim, nt0, k, name):
 
 
 def _gram_schmidt(sim, nt0, k, name):
-    """The Arnoldi Gram-Schmidt kernels (k_pdots, k_psub_dots, k_psub_norm,
-    with k_colsum_wide at large n) over k steps from nt0, launched eagerly so
-    that every Arnoldi step brackets them with HIP events.  Algorithmic bytes
-    per Arnoldi step with j previous columns of length n: two CGS passes of
-    (j + 2) n doubles (the columns and w read, w written), against 8 TB/s."""
+    """The Gram-Schmidt kernels of the fused Arnoldi step (k_kprod_dots,
+    k_gs_chunk<1>, k_gs_chunk<2>, with k_colsum_wide beyond 256 chunks) over k
+    steps from nt0, launched eagerly so that every Arnoldi step brackets them
+    with HIP events.  Algorithmic bytes per Arnoldi step with j previous
+    columns of length n: two CGS passes of (j + 2) n doubles (the columns and
+    w read, w written), against 8 TB/s; the K products and split sums that
+    k_kprod_dots also does are not counted."""
     sim.set_graphs(False)
     sim.call("gmi_profile_begin")
     for nt in range(nt0, nt0 + k):
@@ -326,7 +328,8 @@ def _gram_schmidt(sim, nt0, k, name):
     pair_ms, n = capi.C.c_double(0.0), capi.C.c_int(0)
     sim.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
     gbs = nbytes.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
-    r = {"kernels": "k_pdots + k_psub_dots + k_psub_norm (+ k_colsum_wide)",
+    r = {"kernels": "k_kprod_dots + k_gs_chunk<1> + k_gs_chunk<2> (+ k_colsum_wide): "
+                    "K products and both CGS passes of the fused Arnoldi step",
          "steps": f"{nt0}-{nt0 + k - 1}", "arnoldi_steps": steps.value, "ms": ms.value,
          "us_per_arnoldi_step": 1e3 * ms.value / max(1, steps.value),
          "algorithmic_bytes": nbytes.value, "achieved_gbs": gbs,
