@@ -300,7 +300,7 @@ class Sim(Context):
         g0 = cfg.swimmers[0]
         super().__init__(a["nfil"] * cfg.nseg, a["nblob"], cfg.nseg, mu=g0.mu,
                          a_seg=g0.a_seg, a_blob=g0.a_blob, precision=precision,
-                         interact=False, device=device)
+                         interact=bool(cfg.interact), device=device)
         self._a = a
         self.sp = GmiSimParams(
             a["nfil"].ctypes.data_as(_ip), ptr(a["blob_refs"]), ptr(a["fil_refs"]),

@@ -45,19 +45,26 @@ static int build_list(gmi_ctx *c, int mask, WorkList &wl, int target_blocks, int
   std::vector<WorkItem> work;
   const bool seg_targets = mask & (kSS | kSB);
   const bool blob_targets = mask & (kBS | kBB);
-  struct G { int s0, ns, b0, nb; };
+  // a group: target ranges (ts0, tns segments; tb0, tnb blobs) and the
+  // source ranges they see (s0, ns; b0, nb)
+  struct G { int ts0, tns, tb0, tnb, s0, ns, b0, nb; };
   std::vector<G> groups;
-  if (c->interact || c->nsw == 1) {
-    groups.push_back({0, c->Ns, 0, c->Nb});
+  if (c->nsw == 1) {
+    groups.push_back({0, c->Ns, 0, c->Nb, 0, c->Ns, 0, c->Nb});
   } else {
-    for (int g = 0; g < c->nsw; ++g)
-      groups.push_back({c->seg_off[g], c->nseg[g], c->blob_off[g], c->nblob[g]});
+    // tiles never straddle two swimmers; coupled swimmers see every source
+    for (int g = 0; g < c->nsw; ++g) {
+      G q{c->seg_off[g], c->nseg[g], c->blob_off[g], c->nblob[g],
+          c->seg_off[g], c->nseg[g], c->blob_off[g], c->nblob[g]};
+      if (c->interact) { q.s0 = 0; q.ns = c->Ns; q.b0 = 0; q.nb = c->Nb; }
+      groups.push_back(q);
+    }
   }
   // total pair work, to spread splits in proportion
   long long tot_tiles = 0;
   for (const G &g : groups) {
-    if (seg_targets) tot_tiles += (g.ns + kBlock - 1) / kBlock;
-    if (blob_targets) tot_tiles += (g.nb + kBlock - 1) / kBlock;
+    if (seg_targets) tot_tiles += (g.tns + kBlock - 1) / kBlock;
+    if (blob_targets) tot_tiles += (g.tnb + kBlock - 1) / kBlock;
   }
   const long long target = std::max<long long>(1, target_blocks);
   const int base_split =
@@ -67,8 +74,8 @@ static int build_list(gmi_ctx *c, int mask, WorkList &wl, int target_blocks, int
     for (int sp = 0; sp < 2; ++sp) {
       const bool segt = (sp == 0);
       if (segt ? !seg_targets : !blob_targets) continue;
-      const int nt_all = segt ? g.ns : g.nb;
-      const int t_off = segt ? g.s0 : g.b0;
+      const int nt_all = segt ? g.tns : g.tnb;
+      const int t_off = segt ? g.ts0 : g.tb0;
       const bool use_s = segt ? (mask & kSS) : (mask & kBS);
       const bool use_b = segt ? (mask & kSB) : (mask & kBB);
       const int L = (use_s ? g.ns : 0) + (use_b ? g.nb : 0);

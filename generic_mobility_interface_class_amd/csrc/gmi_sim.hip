@@ -2243,8 +2243,6 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if (!c || !p || !p->nfil || !p->blob_refs || !p->fil_refs) return GMI_ERR_ARG;
   if (c->extra) return fail(c, GMI_ERR_STATE, "context already has a sim");
   if (c->wall) return fail(c, GMI_ERR_STATE, "no time stepping above a wall");
-  if (c->interact && c->nsw > 1)
-    return fail(c, GMI_ERR_ARG, "a sim needs hydrodynamically independent swimmers");
   const int S = c->nseg_fil;
   if (S < 2 || S > kMaxSeg) return fail(c, GMI_ERR_ARG, "NSEG must be in [2, 64]");
   int Nf = 0;

@@ -216,7 +216,8 @@ def workload_geometry(name: str) -> SwimmerGeometry:
 
 @dataclass
 class SimConfig:
-    """Inputs of an implicit-timestep run (one or more independent swimmers).
+    """Inputs of an implicit-timestep run (one or more swimmers, coupled or
+    independent).
 
     Physics follows the reference defaults (config.hpp:100-154) where the
     workload does not fix them: KB = KT = 1800, Broyden TOL 1e-4 with
@@ -246,6 +247,9 @@ class SimConfig:
     # ((n_filaments, 6) array, or None)
     end_force: float = 0.0
     perturbation: np.ndarray = None
+    # swimmers couple hydrodynamically (every particle with every other, the
+    # reference's NSWIM > 1 run) or are independent (a batch)
+    interact: bool = False
 
     def __post_init__(self):
         g0 = self.swimmers[0]

@@ -8,10 +8,14 @@
  * device; the host only reads convergence flags.  Every entry point below
  * names the reference member it replaces.
  *
- * Swimmers are hydrodynamically independent (a block-diagonal mobility) and
- * may differ in filament and blob counts; they share NSEG and the radii.
- * A sim is attached to a gmi_ctx created with the same n_swimmers and
- * per-swimmer sizes (nseg_per_swimmer = nfil*NSEG) and swimmers_interact = 0.
+ * Swimmers may differ in filament and blob counts; they share NSEG and the
+ * radii.  A sim is attached to a gmi_ctx created with the same n_swimmers and
+ * per-swimmer sizes (nseg_per_swimmer = nfil*NSEG).  With swimmers_interact
+ * = 1 every particle couples to every other, as in the reference's NSWIM > 1
+ * runs (cuda_functions.cu:278,457); with 0 the mobility is block-diagonal
+ * (independent swimmers stepped as one batch).  The preconditioners and the
+ * fitted GMRES guess act per swimmer either way; between coupled swimmers
+ * they are no longer exact, so GMRES takes more iterations.
  */
 #ifndef GMI_SIM_H
 #define GMI_SIM_H

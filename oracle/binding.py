@@ -160,7 +160,7 @@ class OrcSimParams(C.Structure):
         ("num_euler_steps", C.c_int), ("barrier_K", C.c_double),
         ("curvature_amp", C.c_double), ("curvature_period", C.c_double),
         ("sphere_radius", C.c_double), ("sphere_radii", _dp),
-        ("end_force", C.c_double), ("perturbation", _dp),
+        ("end_force", C.c_double), ("perturbation", _dp), ("interact", C.c_int),
     ]
 
 
@@ -228,7 +228,8 @@ class Sim:
             cfg.jacobian_confidence, cfg.gmres_tol, cfg.max_broyden_iter, cfg.gmres_max_iter,
             cfg.num_euler_steps, cfg.barrier_K, cfg.curvature_amp, cfg.curvature_period,
             float(a["radius"][0]), _p(a["radius"]), cfg.end_force,
-            _p(a["perturbation"]) if a["perturbation"] is not None else None)
+            _p(a["perturbation"]) if a["perturbation"] is not None else None,
+            int(bool(cfg.interact)))
         self.h = self.lib.orc_sim_create(C.byref(self.p), nthreads or nthreads_default())
         assert self.h, "oracle refused the configuration"
         self.Ns = self.lib.orc_sim_num_segments(self.h)

@@ -35,6 +35,10 @@ typedef struct orc_sim_params {
                                perturbation2 of filament.cpp:336-360) added
                                to the forces on segments 4 and 12 at nt = 0;
                                NULL: none (needs NSEG >= 13)               */
+  int interact;             /* 1: every particle couples to every other in
+                               the pair sums and the barrier (the reference's
+                               NSWIM > 1 run, cuda_functions.cu:278,457);
+                               0: independent swimmers                     */
 } orc_sim_params;
 
 typedef struct orc_sim orc_sim;

@@ -21,8 +21,10 @@
 //  * driving: preferred curvature kappa0(t) = amp sin(2 pi t / period + phi_f)
 //    in the binormal component of strain_twist (filament.cpp:1104-1106),
 //    evaluated at the end-of-step time t = (nt+1) dt (amp 0: none);
-//  * swimmers are hydrodynamically independent; each swimmer has its own
-//    blob/filament references and its own body-mobility reference matrices;
+//  * swimmers couple through the pair sums and the barrier (interact = 1, the
+//    reference's NSWIM > 1 run) or are hydrodynamically independent
+//    (interact = 0); either way each swimmer has its own blob/filament
+//    references and its own body-mobility reference matrices;
 //  * the blob part of the preconditioner is scaled once per blob (the
 //    reference rescales all blobs once per swimmer, mobility_solver.cpp:719,
 //    which only matters for NSWIM > 1 and never changes the solution).
@@ -657,29 +659,40 @@ void sw_update(Swimmer &B, const double *up, double DL) {
 
 // ------------------------------------------------------- mobility solver
 
-void pair_apply_segs_blobs(orc_sim *s, const double *Fs, const double *Fb, bool ss, bool sb,
-                           bool bs, bool bb, double *Vs, double *Vb) {
-  // independent swimmers: apply per swimmer
-  for (const Swimmer &B : s->sw) {
-    const int ns = B.nfil * s->S, nb = B.nblob;
-    const double *xs = &s->xs[3 * B.seg0], *xb = &s->xb[3 * B.blob0];
-    double *vs = Vs ? &Vs[6 * B.seg0] : nullptr;
-    double *vb = Vb ? &Vb[3 * B.blob0] : nullptr;
-    if (ss) orc_Mss(vs, &Fs[6 * B.seg0], xs, ns, 0, ns, s->p.a_seg, s->p.mu, s->nthreads);
-    if (sb) orc_Msb(vs, &Fb[3 * B.blob0], xs, ns, xb, nb, 0, ns, s->p.a_seg, s->p.a_blob, s->p.mu,
-                    s->nthreads);
-    if (bs) orc_Mbs(vb, &Fs[6 * B.seg0], xb, nb, xs, ns, 0, nb, s->p.a_seg, s->p.a_blob, s->p.mu,
-                    s->nthreads);
-    if (bb) {
-      if (bs) {
-        std::vector<double> t(3 * nb);
-        orc_Mbb(t.data(), &Fb[3 * B.blob0], xb, nb, 0, nb, s->p.a_blob, s->p.mu, s->nthreads);
-        for (int k = 0; k < 3 * nb; k++) vb[k] += t[k];
-      } else {
-        orc_Mbb(vb, &Fb[3 * B.blob0], xb, nb, 0, nb, s->p.a_blob, s->p.mu, s->nthreads);
-      }
+// The pair sums over one group of particles: segments [seg0, seg0 + ns),
+// blobs [blob0, blob0 + nb).
+void pair_apply_group(orc_sim *s, int seg0, int ns, int blob0, int nb, const double *Fs,
+                      const double *Fb, bool ss, bool sb, bool bs, bool bb, double *Vs,
+                      double *Vb) {
+  const double *xs = &s->xs[3 * seg0], *xb = &s->xb[3 * blob0];
+  double *vs = Vs ? &Vs[6 * seg0] : nullptr;
+  double *vb = Vb ? &Vb[3 * blob0] : nullptr;
+  if (ss) orc_Mss(vs, &Fs[6 * seg0], xs, ns, 0, ns, s->p.a_seg, s->p.mu, s->nthreads);
+  if (sb) orc_Msb(vs, &Fb[3 * blob0], xs, ns, xb, nb, 0, ns, s->p.a_seg, s->p.a_blob, s->p.mu,
+                  s->nthreads);
+  if (bs) orc_Mbs(vb, &Fs[6 * seg0], xb, nb, xs, ns, 0, nb, s->p.a_seg, s->p.a_blob, s->p.mu,
+                  s->nthreads);
+  if (bb) {
+    if (bs) {
+      std::vector<double> t(3 * nb);
+      orc_Mbb(t.data(), &Fb[3 * blob0], xb, nb, 0, nb, s->p.a_blob, s->p.mu, s->nthreads);
+      for (int k = 0; k < 3 * nb; k++) vb[k] += t[k];
+    } else {
+      orc_Mbb(vb, &Fb[3 * blob0], xb, nb, 0, nb, s->p.a_blob, s->p.mu, s->nthreads);
     }
   }
+}
+
+// cuda_functions.cu:246-863 over all particles (coupled swimmers) or per
+// swimmer (independent swimmers)
+void pair_apply_segs_blobs(orc_sim *s, const double *Fs, const double *Fb, bool ss, bool sb,
+                           bool bs, bool bb, double *Vs, double *Vb) {
+  if (s->p.interact) {
+    pair_apply_group(s, 0, s->Ns, 0, s->Nb, Fs, Fb, ss, sb, bs, bb, Vs, Vb);
+    return;
+  }
+  for (const Swimmer &B : s->sw)
+    pair_apply_group(s, B.seg0, B.nfil * s->S, B.blob0, B.nblob, Fs, Fb, ss, sb, bs, bb, Vs, Vb);
 }
 
 // mobility_solver.cpp:132-192
@@ -687,11 +700,16 @@ void read_positions_and_forces(orc_sim *s) {
   for (const Swimmer &B : s->sw) body_blob_positions(B, &s->xb[3 * B.blob0]);
   s->fs_dev = s->fs;
   std::fill(s->frep.begin(), s->frep.end(), 0.0);
-  for (const Swimmer &B : s->sw) {
-    const int ns = B.nfil * s->S;
-    orc_barrier(&s->fs_dev[6 * B.seg0], &s->frep[3 * B.blob0], &s->xs[3 * B.seg0], ns,
-                &s->xb[3 * B.blob0], B.nblob, s->S, s->p.a_seg, s->p.a_blob, s->p.barrier_K,
-                s->nthreads);
+  if (s->p.interact) {
+    orc_barrier(s->fs_dev.data(), s->frep.data(), s->xs.data(), s->Ns, s->xb.data(), s->Nb, s->S,
+                s->p.a_seg, s->p.a_blob, s->p.barrier_K, s->nthreads);
+  } else {
+    for (const Swimmer &B : s->sw) {
+      const int ns = B.nfil * s->S;
+      orc_barrier(&s->fs_dev[6 * B.seg0], &s->frep[3 * B.blob0], &s->xs[3 * B.seg0], ns,
+                  &s->xb[3 * B.blob0], B.nblob, s->S, s->p.a_seg, s->p.a_blob, s->p.barrier_K,
+                  s->nthreads);
+    }
   }
   for (Swimmer &B : s->sw) {
     const M3 R = rot_mat(B.q);
