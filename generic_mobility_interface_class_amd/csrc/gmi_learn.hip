@@ -66,6 +66,7 @@ struct Learn {
   int bmax = 0;
   float *Xb = nullptr, *Tb = nullptr, *H1 = nullptr, *H2 = nullptr, *Y = nullptr;
   float *dY = nullptr, *dH1 = nullptr, *dH2 = nullptr, *loss = nullptr;
+  float *wpart = nullptr;  // weight-gradient partials [batch splits][K + 1][N]
   int *idx = nullptr;
   int adam_t = 0;
   std::vector<void *> owned;
@@ -90,47 +91,149 @@ __device__ __forceinline__ float bf2f(uint16_t h) {
 }
 
 // ---- dense layers --------------------------------------------------------
-// H[i, n] = act(sum_k A[i, k] W[k, n] + b[n]),  W row-major K x N in BF16.
-__global__ void k_dense_fwd(const float *A, int B, int K, const uint16_t *W, const float *b, int N,
-                            int act, float *H) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= B * N) return;
-  const int i = e / N, n = e % N;
-  float s = b[n];
-  const float *a = A + size_t(i) * K;
-  for (int k = 0; k < K; k++) s = fmaf(a[k], bf2f(W[size_t(k) * N + n]), s);
-  H[e] = act ? tanhf(s) : s;
-}
+// One tiled product serves the three layer operations:
+//   forward      H[i, n]  = act(sum_k A[i, k] W[k, n] + b[n])
+//   input grad   dA[i, k] = (sum_n dZ[i, n] W[k, n]) (1 - Hp[i, k]^2)
+//   weight grad  dW[k, n] = sum_i A[i, k] dZ[i, n],  db[n] = sum_i dZ[i, n]
+// (W row-major K x N in BF16, everything else FP32).  A workgroup of 256
+// threads owns a 64 x 64 tile of the result, each thread 4 x 4 of it; the
+// reduction runs in steps of 32 staged in LDS, in index order, with the next
+// step's operands loaded into registers while the current one is consumed.
+// The weight gradient reduces over the batch: its workgroups also split the
+// batch (blockIdx.z), write partials, and k_mlp_wsum adds them in split
+// order (deterministic sums), and its left operand gets one more row of
+// ones, whose products are the bias gradient.
+constexpr int kMT = 64, kMK = 32, kMSplitRows = 64;
+constexpr int kMLoads = kMT * kMK / 256;  // staged elements per thread and operand
 
-// dA[i, k] = (sum_n dZ[i, n] W[k, n]) * (1 - Hp[i, k]^2)
-__global__ void k_dense_bwd_data(const float *dZ, int B, int N, const uint16_t *W, int K,
-                                 const float *Hp, float *dA) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= B * K) return;
-  const int i = e / K, k = e % K;
-  float s = 0.f;
-  const float *dz = dZ + size_t(i) * N;
-  const uint16_t *w = W + size_t(k) * N;
-  for (int n = 0; n < N; n++) s = fmaf(dz[n], bf2f(w[n]), s);
-  const float h = Hp[e];
-  dA[e] = s * (1.f - h * h);
-}
+enum MlpOp { kFwdTanh = 0, kFwdLinear = 1, kBwdData = 2, kBwdWeight = 3 };
 
-// dW[k, n] = sum_i A[i, k] dZ[i, n];  db[n] = sum_i dZ[i, n]  (fixed order)
-__global__ void k_dense_bwd_weight(const float *A, const float *dZ, int B, int K, int N, float *dW,
-                                   float *db) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < K * N) {
-    const int k = e / N, n = e % N;
-    float s = 0.f;
-    for (int i = 0; i < B; i++) s = fmaf(A[size_t(i) * K + k], dZ[size_t(i) * N + n], s);
-    dW[e] = s;
-  } else if (e < K * N + N) {
-    const int n = e - K * N;
-    float s = 0.f;
-    for (int i = 0; i < B; i++) s += dZ[size_t(i) * N + n];
-    db[n] = s;
+// operand elements of one reduction step: thread t's e-th element of the
+// left (a) and right (b) tiles, with their LDS coordinates
+template <int OP>
+__device__ __forceinline__ void mlp_load(int M, int N, int Kd, int k0, int kend, int m0, int n0,
+                                         const float *__restrict__ A, int lda,
+                                         const void *__restrict__ Bm, int ldb, float (&va)[kMLoads],
+                                         float (&vb)[kMLoads]) {
+  const uint16_t *W = static_cast<const uint16_t *>(Bm);
+  const float *Fz = static_cast<const float *>(Bm);
+#pragma unroll
+  for (int q = 0; q < kMLoads; q++) {
+    const int e = threadIdx.x + 256 * q;
+    if (OP == kBwdWeight) {
+      // opA(m, k) = A[k, m] plus a row of ones at m = M - 1; opB(k, n) = dZ[k, n]
+      const int mm = e & (kMT - 1), kk = e >> 6;
+      const int gm = m0 + mm, gk = k0 + kk, gn = n0 + mm;
+      va[q] = (gk < kend && gm < M) ? (gm == M - 1 ? 1.f : A[size_t(gk) * lda + gm]) : 0.f;
+      vb[q] = (gk < kend && gn < N) ? Fz[size_t(gk) * ldb + gn] : 0.f;
+    } else {
+      // opA(m, k) = A[m, k]: k along memory
+      const int kk = e & (kMK - 1), mm = e >> 5;
+      const int gm = m0 + mm, gk = k0 + kk;
+      va[q] = (gm < M && gk < Kd) ? A[size_t(gm) * lda + gk] : 0.f;
+      if (OP == kBwdData) {
+        // opB(k, n) = W[n, k] (the reduction runs over W's columns)
+        const int gn = n0 + mm;
+        vb[q] = (gn < N && gk < Kd) ? bf2f(W[size_t(gn) * ldb + gk]) : 0.f;
+      } else {
+        // opB(k, n) = W[k, n]
+        const int nn = e & (kMT - 1), k2 = e >> 6, gk2 = k0 + k2, gn = n0 + nn;
+        vb[q] = (gk2 < Kd && gn < N) ? bf2f(W[size_t(gk2) * ldb + gn]) : 0.f;
+      }
+    }
   }
+}
+
+template <int OP>
+__device__ __forceinline__ void mlp_stage(float (&sA)[kMK][kMT + 4], float (&sB)[kMK][kMT + 4],
+                                          const float (&va)[kMLoads], const float (&vb)[kMLoads]) {
+#pragma unroll
+  for (int q = 0; q < kMLoads; q++) {
+    const int e = threadIdx.x + 256 * q;
+    if (OP == kBwdWeight) {
+      sA[e >> 6][e & (kMT - 1)] = va[q];
+      sB[e >> 6][e & (kMT - 1)] = vb[q];
+    } else {
+      sA[e & (kMK - 1)][e >> 5] = va[q];
+      if (OP == kBwdData) sB[e & (kMK - 1)][e >> 5] = vb[q];
+      else sB[e >> 6][e & (kMT - 1)] = vb[q];
+    }
+  }
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int Kd, const float *__restrict__ A,
+                                                  int lda, const void *__restrict__ Bm, int ldb,
+                                                  float *__restrict__ C, int ldc,
+                                                  const float *__restrict__ aux) {
+  __shared__ float sA[kMK][kMT + 4];  // opA(m0 + m, k0 + k) at [k][m]
+  __shared__ float sB[kMK][kMT + 4];  // opB(k0 + k, n0 + n) at [k][n]
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int m0 = blockIdx.x * kMT, n0 = blockIdx.y * kMT;
+  // the weight gradient's workgroup z reduces batch rows [kbeg, kend)
+  const int kbeg = OP == kBwdWeight ? blockIdx.z * kMSplitRows : 0;
+  const int kend = OP == kBwdWeight ? min(Kd, kbeg + kMSplitRows) : Kd;
+  float acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[r][c] = 0.f;
+  float va[kMLoads], vb[kMLoads];
+  mlp_load<OP>(M, N, Kd, kbeg, kend, m0, n0, A, lda, Bm, ldb, va, vb);
+  for (int k0 = kbeg; k0 < kend; k0 += kMK) {
+    mlp_stage<OP>(sA, sB, va, vb);
+    __syncthreads();
+    if (k0 + kMK < kend) mlp_load<OP>(M, N, Kd, k0 + kMK, kend, m0, n0, A, lda, Bm, ldb, va, vb);
+#pragma unroll 8
+    for (int k = 0; k < kMK; k++) {
+      float a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) a[r] = sA[k][4 * ty + r];
+#pragma unroll
+      for (int c = 0; c < 4; c++) b[c] = sB[k][4 * tx + c];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = fmaf(a[r], b[c], acc[r][c]);
+    }
+    __syncthreads();
+  }
+  // the weight gradient's partials: [z][M][N]
+  float *out = OP == kBwdWeight ? C + size_t(blockIdx.z) * M * N : C;
+  const int ldo = OP == kBwdWeight ? N : ldc;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int gm = m0 + 4 * ty + r;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int gn = n0 + 4 * tx + c;
+      if (gn >= N) continue;
+      const float v = acc[r][c];
+      if (OP == kFwdTanh) {
+        out[size_t(gm) * ldo + gn] = tanhf(v + aux[gn]);
+      } else if (OP == kFwdLinear) {
+        out[size_t(gm) * ldo + gn] = v + aux[gn];
+      } else if (OP == kBwdData) {
+        const float h = aux[size_t(gm) * ldc + gn];
+        out[size_t(gm) * ldo + gn] = v * (1.f - h * h);
+      } else {
+        out[size_t(gm) * ldo + gn] = v;
+      }
+    }
+  }
+}
+
+// dW (rows 0..M-2) and db (row M-1) from the nsplit partials [z][M][N], in
+// split order
+__global__ void k_mlp_wsum(const float *__restrict__ part, int nsplit, int M, int N,
+                           float *__restrict__ dW, float *__restrict__ db) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M * N) return;
+  float s = 0.f;
+  for (int z = 0; z < nsplit; z++) s += part[size_t(z) * M * N + e];
+  if (e < (M - 1) * N) dW[e] = s;
+  else db[e - (M - 1) * N] = s;
 }
 
 // dY = 2 (Y - T) / (B N); loss = mean (Y - T)^2 (one workgroup)
@@ -350,20 +453,40 @@ static int ensure_batch(gmi_ctx *c, Learn *L, int B) {
   if ((rc = lalloc(c, L, &L->dY, b * L->nout))) return rc;
   if ((rc = lalloc(c, L, &L->dH1, b * L->nh))) return rc;
   if ((rc = lalloc(c, L, &L->dH2, b * L->nh))) return rc;
+  {
+    const size_t kmax = size_t(std::max(L->nin, L->nh)) + 1;
+    if ((rc = lalloc(c, L, &L->wpart, size_t(nb(int(b), kMSplitRows)) * kmax * size_t(L->nh))))
+      return rc;
+  }
   if ((rc = lalloc(c, L, &L->idx, b))) return rc;
   L->bmax = B;
   return GMI_OK;
 }
 
-// forward of B rows of Xin (standardised) -> L->Y; keeps H1, H2
+static dim3 mlp_grid(int M, int N) { return dim3(nb(M, kMT), nb(N, kMT)); }
+
+// weight and bias gradients of one layer: dW (K x N), db (N) from the layer
+// input A (B x K) and the output gradient dZ (B x N), through L->wpart
+static void mlp_weight_grad(gmi_ctx *c, Learn *L, const float *A, int K, const float *dZ, int N,
+                            int B, float *dW, float *db) {
+  const int nsplit = nb(B, kMSplitRows);
+  dim3 grid = mlp_grid(K + 1, N);
+  grid.z = nsplit;
+  hipLaunchKernelGGL(k_mlp_gemm<kBwdWeight>, grid, dim3(256), 0, c->stream, K + 1, N, B, A, K, dZ,
+                     N, L->wpart, N, nullptr);
+  hipLaunchKernelGGL(k_mlp_wsum, dim3(nb((K + 1) * N, 256)), dim3(256), 0, c->stream, L->wpart,
+                     nsplit, K + 1, N, dW, db);
+}
+
+// forward of B rows of Xin (standardised) -> Yout; keeps H1, H2
 static int mlp_forward(gmi_ctx *c, Learn *L, const float *Xin, int B, float *Yout) {
-  const int nh = L->nh;
-  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, Xin, B, L->nin,
-                     L->pq + L->oW1, L->p + L->ob1, nh, 1, L->H1);
-  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->H1, B, nh,
-                     L->pq + L->oW2, L->p + L->ob2, nh, 1, L->H2);
-  hipLaunchKernelGGL(k_dense_fwd, dim3(nb(B * L->nout, 256)), dim3(256), 0, c->stream, L->H2, B,
-                     nh, L->pq + L->oW3, L->p + L->ob3, L->nout, 0, Yout);
+  const int nh = L->nh, ni = L->nin, no = L->nout;
+  hipLaunchKernelGGL(k_mlp_gemm<kFwdTanh>, mlp_grid(B, nh), dim3(256), 0, c->stream, B, nh, ni,
+                     Xin, ni, L->pq + L->oW1, nh, L->H1, nh, L->p + L->ob1);
+  hipLaunchKernelGGL(k_mlp_gemm<kFwdTanh>, mlp_grid(B, nh), dim3(256), 0, c->stream, B, nh, nh,
+                     L->H1, nh, L->pq + L->oW2, nh, L->H2, nh, L->p + L->ob2);
+  hipLaunchKernelGGL(k_mlp_gemm<kFwdLinear>, mlp_grid(B, no), dim3(256), 0, c->stream, B, no, nh,
+                     L->H2, nh, L->pq + L->oW3, no, Yout, no, L->p + L->ob3);
   return hip_check(c, hipGetLastError(), "mlp forward");
 }
 
@@ -373,16 +496,14 @@ static int mlp_backward(gmi_ctx *c, Learn *L, const float *Xin, const float *T, 
   int rc;
   if ((rc = mlp_forward(c, L, Xin, B, L->Y))) return rc;
   hipLaunchKernelGGL(k_mse, dim3(1), dim3(256), 0, c->stream, L->Y, T, B * no, L->dY, L->loss);
-  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(nh * no + no, 256)), dim3(256), 0, c->stream,
-                     L->H2, L->dY, B, nh, no, L->g + L->oW3, L->g + L->ob3);
-  hipLaunchKernelGGL(k_dense_bwd_data, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->dY, B,
-                     no, L->pq + L->oW3, nh, L->H2, L->dH2);
-  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(nh * nh + nh, 256)), dim3(256), 0, c->stream,
-                     L->H1, L->dH2, B, nh, nh, L->g + L->oW2, L->g + L->ob2);
-  hipLaunchKernelGGL(k_dense_bwd_data, dim3(nb(B * nh, 256)), dim3(256), 0, c->stream, L->dH2, B,
-                     nh, L->pq + L->oW2, nh, L->H1, L->dH1);
-  hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nb(ni * nh + nh, 256)), dim3(256), 0, c->stream,
-                     Xin, L->dH1, B, ni, nh, L->g + L->oW1, L->g + L->ob1);
+  // dW3 (nh x no) and db3 from H2 and dY; dH2 from dY and W3 (nh x no)
+  mlp_weight_grad(c, L, L->H2, nh, L->dY, no, B, L->g + L->oW3, L->g + L->ob3);
+  hipLaunchKernelGGL(k_mlp_gemm<kBwdData>, mlp_grid(B, nh), dim3(256), 0, c->stream, B, nh, no,
+                     L->dY, no, L->pq + L->oW3, no, L->dH2, nh, L->H2);
+  mlp_weight_grad(c, L, L->H1, nh, L->dH2, nh, B, L->g + L->oW2, L->g + L->ob2);
+  hipLaunchKernelGGL(k_mlp_gemm<kBwdData>, mlp_grid(B, nh), dim3(256), 0, c->stream, B, nh, nh,
+                     L->dH2, nh, L->pq + L->oW2, nh, L->dH1, nh, L->H1);
+  mlp_weight_grad(c, L, Xin, ni, L->dH1, nh, B, L->g + L->oW1, L->g + L->ob1);
   return hip_check(c, hipGetLastError(), "mlp backward");
 }
 
@@ -704,16 +825,25 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   double lsum = 0.0;
   int lcount = 0;
-  hipEvent_t ev;
-  if ((rc = hip_check(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event"))) return rc;
-  (void)hipEventRecord(ev, c->stream);
+  // two pinned halves of batch indices: iteration it writes half it & 1
+  // once the copy that read it two iterations ago has run (event ev[it & 1]),
+  // so the host draws the next batch while the device trains on this one
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (int k = 0; k < 2; k++) {
+    if ((rc = hip_check(c, hipEventCreateWithFlags(&ev[k], hipEventDisableTiming), "event"))) {
+      if (ev[0]) (void)hipEventDestroy(ev[0]);
+      return rc;
+    }
+    (void)hipEventRecord(ev[k], c->stream);
+  }
   for (int it = 0; it < iters; it++) {
     int *h = hidx + (it & 1) * batch;
-    (void)hipEventSynchronize(ev);  // the pinned half written two iterations ago is consumed
+    (void)hipEventSynchronize(ev[it & 1]);
     for (int i = 0; i < batch; i++) h[i] = int(next() % uint64_t(N));
     if ((rc = hip_check(c, hipMemcpyAsync(L->idx, h, sizeof(int) * batch, hipMemcpyHostToDevice,
                                           c->stream), "batch indices")))
       break;
+    (void)hipEventRecord(ev[it & 1], c->stream);
     hipLaunchKernelGGL(k_gather, dim3(nb(batch * (ni + no), 256)), dim3(256), 0, c->stream, SX, ST,
                        L->idx, batch, ni, no, L->Xb, L->Tb);
     if ((rc = mlp_backward(c, L, L->Xb, L->Tb, batch))) break;
@@ -721,7 +851,6 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
     const float c1 = 1.f - std::pow(b1, float(L->adam_t)), c2 = 1.f - std::pow(b2, float(L->adam_t));
     hipLaunchKernelGGL(k_adam, dim3(nb(L->np, 256)), dim3(256), 0, c->stream, L->p, L->g, L->m,
                        L->v, L->pq, L->np, float(lr), b1, b2, eps, c1, c2);
-    (void)hipEventRecord(ev, c->stream);
     if (it >= iters - 20) {
       float lf = 0.f;
       if ((rc = hip_check(c, hipMemcpyAsync(&lf, L->loss, sizeof(float), hipMemcpyDeviceToHost,
@@ -732,7 +861,9 @@ int gmi_warmstart_train(gmi_ctx *c, int iters, int batch, double lr, unsigned lo
       lcount++;
     }
   }
-  (void)hipEventDestroy(ev);
+  // the copies may still read the pinned indices
+  (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 2; k++) (void)hipEventDestroy(ev[k]);
   (void)hipHostFree(hidx);
   if (rc) return rc;
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
