@@ -101,6 +101,7 @@ SIGNATURES = [
     ("gmi_sim_set_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gmi_sim_set_gmres_history", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_sim_set_jacobian_pivoting", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_body_preconditioner", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_warmstart_init", C.c_int, [C.c_void_p, C.c_int, C.c_ulonglong]),
     ("gmi_warmstart_configure", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -359,6 +360,9 @@ class Sim(Context):
         """GMRES initial guess fitted on the last m right-hand sides (0: the
         reference's previous solution)."""
         self.call("gmi_sim_set_gmres_history", int(m))
+
+    def set_jacobian_pivoting(self, always_pivot=True):
+        self.call("gmi_sim_set_jacobian_pivoting", int(bool(always_pivot)))
 
     # ---- output and checkpoint files (reference formats) --------------
     def write_par(self, name, total_time_steps, plot_frequency):

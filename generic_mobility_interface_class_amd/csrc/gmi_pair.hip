@@ -51,6 +51,9 @@ template <typename R> struct PC {
   R c16x3;      // 3/(16 pi mu)
   // near-field (overlap) constants
   R a;          // equal radius (ss / bb)
+  R inv_a;      // 1/a
+  R c3_32a;     // 3/(32 a)
+  R c3_8a;      // 3/(8 a)
   R c6a;        // 1/(6 pi mu a)
   R c8a3;       // 1/(8 pi mu a^3)
   R c16a3;      // 1/(16 pi mu a^3)
@@ -69,6 +72,9 @@ __host__ __device__ PC<R> make_pc(double ai, double aj, double mu) {
   k.c16 = R(1.0 / (16.0 * kPi * mu));
   k.c16x3 = R(3.0 / (16.0 * kPi * mu));
   k.a = R(ai);
+  k.inv_a = R(1.0 / ai);
+  k.c3_32a = R(3.0 / (32.0 * ai));
+  k.c3_8a = R(3.0 / (8.0 * ai));
   k.c6a = R(1.0 / (6.0 * kPi * mu * ai));
   k.c8a3 = R(1.0 / (8.0 * kPi * mu * ai * ai * ai));
   k.c16a3 = R(1.0 / (16.0 * kPi * mu * ai * ai * ai));
@@ -89,37 +95,46 @@ __host__ __device__ PC<R> make_pc_unequal(double as, double ab, double mu) {
   return k;
 }
 
-template <typename R> __device__ __forceinline__ R dsqrt(R x) { return sqrt(x); }
+// r and 1/r of an overlapping pair from the far field's 1/sqrt(r^2): the
+// reference's 1/(r + 1e-20) is 1/r for every r > 0 at this precision, and
+// stays finite (1e20) for the self pair, r = 0, where ri is not.
+template <typename R>
+__device__ __forceinline__ void near_r(R r2, R ri, R &r, R &rm1) {
+  const bool pos = r2 > R(0);
+  r = pos ? r2 * ri : R(0);
+  rm1 = pos ? ri : R(1e20);
+}
 
 // Equal-radius overlap coefficients (cuda_functions.cu:336-350, 364-386,
-// 400-406) expressed as multipliers of f, d and (x d).
+// 400-406) expressed as multipliers of f, d and (x d).  Divisions by the
+// radius are multiplications by precomputed reciprocals.
 template <typename R>
 __device__ __forceinline__ void near_equal(
-    R r2, R fd, R td, const PC<R> &k, R &At, R &Bt, R &Ar, R &Br, R &g) {
-  const R r = dsqrt(r2);
-  const R rm1 = R(1) / (r + R(1e-20));
-  const R b = R(3) * r / (R(32) * k.a);
+    R r2, R ri, R fd, R td, const PC<R> &k, R &At, R &Bt, R &Ar, R &Br, R &g) {
+  R r, rm1;
+  near_r(r2, ri, r, rm1);
+  const R b = r * k.c3_32a;
   At = (R(1) - R(3) * b) * k.c6a;
   Bt = ((b * k.c6a * fd) * rm1) * rm1;
-  const R t = r / k.a;
+  const R t = r * k.inv_a;
   const R t3 = t * t * t;
   Ar = (R(1) - R(27) * t / R(32) + R(5) * t3 / R(64)) * k.c8a3;
   Br = (((R(9) * t / R(32) - R(3) * t3 / R(64)) * k.c8a3 * td) * rm1) * rm1;
-  g = (r * (R(1) - R(3) * r / (R(8) * k.a)) * k.c16a3) * rm1;
+  g = (r * (R(1) - r * k.c3_8a) * k.c16a3) * rm1;
 }
 
 // Unequal-radius overlap coefficients (cuda_functions.cu:622-646, 656-664).
 template <typename R>
 __device__ __forceinline__ void near_unequal(
-    R r2, R fd, const PC<R> &k, R &At, R &Bt, R &g) {
-  const R r = dsqrt(r2);
-  const R rm1 = R(1) / (r + R(1e-20));
+    R r2, R ri, R fd, const PC<R> &k, R &At, R &Bt, R &g) {
+  R r, rm1;
+  near_r(r2, ri, r, rm1);
   if (r > k.dlt) {
-    const R r3x32 = R(32) * r * r * r;
+    const R inv_r3x32 = rm1 * rm1 * rm1 * R(1.0 / 32.0);  // 1/(32 r^3)
     const R p = k.dlt * k.dlt + R(3) * r * r;
     const R q = k.dlt * k.dlt - r * r;
-    At = (R(0.5) * (k.as + k.ab) - p * p / r3x32) * k.c6ab;
-    Bt = (((R(3) * q * q / r3x32) * k.c6ab * fd) * rm1) * rm1;
+    At = (R(0.5) * (k.as + k.ab) - p * p * inv_r3x32) * k.c6ab;
+    Bt = (((R(3) * q * q * inv_r3x32) * k.c6ab * fd) * rm1) * rm1;
     const R s = k.as - k.ab + r;
     const R h = k.as - r;
     g = (s * s * rm1 * rm1 *
@@ -162,7 +177,7 @@ __device__ __forceinline__ void pair_ss(R dx, R dy, R dz, R fx, R fy, R fz,
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
     R At2, Bt2, Ar2, Br2, g2;
-    near_equal(r2, fd, td, k, At2, Bt2, Ar2, Br2, g2);
+    near_equal(r2, ri, fd, td, k, At2, Bt2, Ar2, Br2, g2);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Ar = nr ? Ar2 : Ar; Br = nr ? Br2 : Br;
     g = nr ? g2 : g;
   }
@@ -187,7 +202,7 @@ __device__ __forceinline__ void pair_sb(R dx, R dy, R dz, R fx, R fy, R fz,
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
     R At2, Bt2, g2;
-    near_unequal(r2, fd, k, At2, Bt2, g2);
+    near_unequal(r2, ri, fd, k, At2, Bt2, g2);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
   }
   o.vx += At * fx + Bt * dx;
@@ -212,7 +227,7 @@ __device__ __forceinline__ void pair_bs(R dx, R dy, R dz, R fx, R fy, R fz,
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
     R At2, Bt2, g2;
-    near_unequal(r2, fd, k, At2, Bt2, g2);
+    near_unequal(r2, ri, fd, k, At2, Bt2, g2);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
   }
   o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
@@ -234,7 +249,7 @@ __device__ __forceinline__ void pair_ut(R dx, R dy, R dz, R fx, R fy, R fz,
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
     R At2, Bt2, g;
-    near_unequal(r2, fd, k, At2, Bt2, g);
+    near_unequal(r2, ri, fd, k, At2, Bt2, g);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
   o.vx += At * fx + Bt * dx;
@@ -255,7 +270,7 @@ __device__ __forceinline__ void pair_bb(R dx, R dy, R dz, R fx, R fy, R fz,
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
     R At2, Bt2, Ar, Br, g;
-    near_equal(r2, fd, R(0), k, At2, Bt2, Ar, Br, g);
+    near_equal(r2, ri, fd, R(0), k, At2, Bt2, Ar, Br, g);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
   o.vx += At * fx + Bt * dx;

@@ -1570,9 +1570,10 @@ __device__ __forceinline__ bool gj_reg_all(GjRegShared &sh, GjRegState &st, int 
   return ((8 * UK >= n || gj_reg_block<UK>(sh, st, n)) && ...);
 }
 
-__global__ __launch_bounds__(256) void k_gj_invert_reg(SimDev a) {
+__global__ __launch_bounds__(256) void k_gj_invert_reg(SimDev a, const int *redo) {
   __shared__ GjRegShared sh;
   const int f = blockIdx.x;
+  if (redo && !redo[f]) return;
   const int n = 6 * a.S;
   const int t = threadIdx.x, g = t & 7;
   const double *G = a.Jw + size_t(f) * n * n;
@@ -1608,6 +1609,163 @@ __global__ __launch_bounds__(256) void k_gj_invert_reg(SimDev a) {
     for (int u = 0; u < kGjRegCols; u++) {
       const int c = g + 8 * u;
       if (c < n) Out[sh.prow[c]] = st.v[q][u];
+    }
+  }
+}
+
+// Gauss-Jordan without pivoting, same register layout as k_gj_invert_reg,
+// one barrier per step.  The filament Jacobians are inverted this way first:
+// on them the pivots stay within four orders of magnitude of their column's
+// largest entry, and the inverse's residual is within a factor of a few of
+// the pivoted one's.  Each inverse is checked on a fixed vector x with
+// entries in [1, 2): y = J x is formed from the loaded matrix, and
+// gj_redo[f] = 1 unless max |J^-1 y - x| <= 2e-6 (a NaN fails); then
+// k_gj_invert_reg (partial pivoting) redoes the filaments that failed.
+// Rounding alone leaves up to about cond(J) x 1e-16 there (8e-10 for the
+// pivoted and 3e-9 for the pivot-free inverses of the 962-particle
+// swimmer's filaments, cond ~3e6), so the check passes rounding and catches
+// an elimination that lost the matrix.
+// Step k finds in LDS the normalised pivot row k and column k of the
+// partially eliminated matrix, published by the owners at the end of step
+// k - 1; every thread eliminates its entries (the pivot row is already its
+// normalised copy), then the owners of row k + 1 normalise it and the owners
+// of column k + 1 publish it for the next step.
+struct GjNpShared {
+  double srow[2][kGjRegMax], scol[2][kGjRegMax];
+  double y[kGjRegMax];  // J x of the check
+  int fail;
+};
+
+__device__ __forceinline__ double gj_check_x(int c) { return 1.0 + double((c * 37) % 64) / 64.0; }
+
+// Sum over the 8 lanes holding one row (lanes 8m .. 8m + 7), in each of them.
+__device__ __forceinline__ double row8_sum(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  return v;
+}
+
+// The owners of column kp (slot u of each held row) and of row kp publish
+// step kp's column and normalised row into buffer kp & 1.
+template <int SLOT>
+__device__ __forceinline__ void gj_np_publish(GjNpShared &sh, double (&v)[kGjRegRows][kGjRegCols],
+                                              int kp) {
+  const int t = threadIdx.x, g = t & 7, b = kp & 1;
+  if (g == (kp & 7)) {
+#pragma unroll
+    for (int q = 0; q < kGjRegRows; q++) sh.scol[b][(t >> 3) + 32 * q] = v[q][SLOT];
+  }
+  if ((t >> 6) == ((kp & 31) >> 3)) {  // the wave holding row kp
+#pragma unroll
+    for (int q = 0; q < kGjRegRows; q++) {
+      if (q != (kp >> 5)) continue;
+      // the pivot from the lane of row kp that holds column kp
+      const double inv = 1.0 / lane_d(v[q][SLOT], 9 * (kp & 7));
+      if ((t >> 3) == (kp & 31)) {
+#pragma unroll
+        for (int u = 0; u < kGjRegCols; u++) {
+          v[q][u] = ((u == SLOT && g == (kp & 7)) ? 1.0 : v[q][u]) * inv;
+          sh.srow[b][g + 8 * u] = v[q][u];
+        }
+      }
+    }
+  }
+}
+
+template <int UK>
+__device__ __forceinline__ void gj_np_block(GjNpShared &sh, double (&v)[kGjRegRows][kGjRegCols],
+                                            int n) {
+  const int t = threadIdx.x, g = t & 7;
+  for (int gk = 0; gk < 8; gk++) {
+    const int k = 8 * UK + gk;
+    if (k >= n) break;
+    const int b = k & 1;
+    double pr[kGjRegCols], fi[kGjRegRows];
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) pr[u] = sh.srow[b][g + 8 * u];
+#pragma unroll
+    for (int q = 0; q < kGjRegRows; q++) {
+      const int r = (t >> 3) + 32 * q;
+      fi[q] = r == k ? 0.0 : sh.scol[b][r];
+    }
+#pragma unroll
+    for (int q = 0; q < kGjRegRows; q++) {
+      const bool piv = (t >> 3) + 32 * q == k;
+#pragma unroll
+      for (int u = 0; u < kGjRegCols; u++) {
+        if (u == UK)
+          v[q][u] = ((g == gk && !piv) ? 0.0 : v[q][u]) - fi[q] * pr[u];
+        else
+          v[q][u] = v[q][u] - fi[q] * pr[u];
+      }
+    }
+    if (k + 1 < n) {
+      if (gk < 7) gj_np_publish<UK>(sh, v, k + 1);
+      else if (UK + 1 < kGjRegCols) gj_np_publish<(UK + 1 < kGjRegCols ? UK + 1 : UK)>(sh, v, k + 1);
+    }
+    __syncthreads();
+  }
+}
+
+template <int... UK>
+__device__ __forceinline__ void gj_np_all(GjNpShared &sh, double (&v)[kGjRegRows][kGjRegCols],
+                                          int n, std::integer_sequence<int, UK...>) {
+  ((8 * UK < n ? gj_np_block<UK>(sh, v, n) : void()), ...);
+}
+
+__global__ __launch_bounds__(256) void k_gj_nopiv_reg(SimDev a) {
+  __shared__ GjNpShared sh;
+  const int f = blockIdx.x;
+  const int n = 6 * a.S;
+  const int t = threadIdx.x, g = t & 7;
+  const double *G = a.Jw + size_t(f) * n * n;
+  double v[kGjRegRows][kGjRegCols];
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const int r = (t >> 3) + 32 * q;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) {
+      const int c = g + 8 * u;
+      v[q][u] = (r < n && c < n) ? G[size_t(r) * n + c] : 0.0;
+    }
+  }
+  // the check's y = J x (rows past n hold zeros)
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    double sq = 0.0;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) sq += v[q][u] * gj_check_x(g + 8 * u);
+    sq = row8_sum(sq);
+    if (g == 0) sh.y[(t >> 3) + 32 * q] = sq;
+  }
+  if (t == 0) sh.fail = 0;
+  gj_np_publish<0>(sh, v, 0);
+  __syncthreads();
+  gj_np_all(sh, v, n, std::make_integer_sequence<int, kGjRegCols>{});
+  // the check: J^-1 y against x, row by row
+  bool fail = false;
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    double sq = 0.0;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) sq += v[q][u] * sh.y[g + 8 * u];
+    sq = row8_sum(sq);
+    const int r = (t >> 3) + 32 * q;
+    if (r < n && !(fabs(sq - gj_check_x(r)) <= 2e-6)) fail = true;
+  }
+  if (fail) sh.fail = 1;
+  __syncthreads();
+  if (t == 0) a.gj_redo[f] = sh.fail;
+  double *Out = a.Jinv + size_t(f) * n * n;
+#pragma unroll
+  for (int q = 0; q < kGjRegRows; q++) {
+    const int r = (t >> 3) + 32 * q;
+    if (r >= n) continue;
+#pragma unroll
+    for (int u = 0; u < kGjRegCols; u++) {
+      const int c = g + 8 * u;
+      if (c < n) Out[size_t(r) * n + c] = v[q][u];
     }
   }
 }
@@ -2326,6 +2484,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &d.blob_ref, 3 * size_t(d.ldb)))) return rc;
   if ((rc = dalloc(c, s, &d.Jw, size_t(Nf) * nn))) return rc;
   if ((rc = dalloc(c, s, &d.Jinv, size_t(Nf) * nn))) return rc;
+  if ((rc = dalloc(c, s, &d.gj_redo, size_t(Nf)))) return rc;
   if ((rc = dalloc(c, s, &d.ecb, 18 * size_t(Nf)))) return rc;
   if ((rc = dalloc(c, s, &d.Bblk, size_t(Nf) * 36 * S))) return rc;
   if ((rc = dalloc(c, s, &d.fil_BA, 36 * size_t(Nf)))) return rc;
@@ -2547,6 +2706,14 @@ int gmi_sim_set_body_preconditioner(gmi_ctx *c, int kind) {
   return GMI_OK;
 }
 
+int gmi_sim_set_jacobian_pivoting(gmi_ctx *c, int always_pivot) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (always_pivot != 0 && always_pivot != 1) return fail(c, GMI_ERR_ARG, "always_pivot is 0 or 1");
+  s->gj_pivot_always = always_pivot != 0;
+  return GMI_OK;
+}
+
 int gmi_sim_set_gmres_history(gmi_ctx *c, int m) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
@@ -2699,8 +2866,14 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   const double t = (nt + 1) * s->p.dt;
   hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
   hipLaunchKernelGGL(k_fil_jacobian, dim3(d.Nf), dim3(64), 0, c->stream, d, nt, t);
-  if (6 * d.S <= kGjRegMax)
-    hipLaunchKernelGGL(k_gj_invert_reg, dim3(d.Nf), dim3(256), 0, c->stream, d);
+  if (6 * d.S <= kGjRegMax && s->gj_pivot_always) {
+    hipLaunchKernelGGL(k_gj_invert_reg, dim3(d.Nf), dim3(256), 0, c->stream, d, nullptr);
+  } else if (6 * d.S <= kGjRegMax) {
+    // pivot-free first; partial pivoting for the filaments whose inverse
+    // fails the check
+    hipLaunchKernelGGL(k_gj_nopiv_reg, dim3(d.Nf), dim3(256), 0, c->stream, d);
+    hipLaunchKernelGGL(k_gj_invert_reg, dim3(d.Nf), dim3(256), 0, c->stream, d, d.gj_redo);
+  }
   else if (s->gj_lds)
     hipLaunchKernelGGL(k_gj_invert_lds, dim3(d.Nf), dim3(kGjThreads), gj_lds_bytes(d.S), c->stream,
                        d);

@@ -38,6 +38,7 @@ struct SimDev {
   // Jacobian
   double *Jw, *Jinv, *ecb, *Bblk, *fil_BA, *fil_b2;
   int *flags;
+  int *gj_redo;  // [Nf] 1: the pivot-free inverse failed its check (pivoted rerun)
 };
 
 
@@ -69,6 +70,7 @@ struct SimHost : KrylovState {
   double *hist_b = nullptr, *hist_x = nullptr, *hist_bnew = nullptr, *hist_G = nullptr,
          *hist_Gw = nullptr, *hist_part = nullptr, *hist_coef = nullptr, *hist_gsave = nullptr;
   bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS
+  bool gj_pivot_always = false;  // skip the checked pivot-free inverse
   Learn *learn = nullptr;
   DensePrec *dense = nullptr;  // null: the reference's self-mobility preconditioner
   // blob tiles for the preconditioner and K products

@@ -126,6 +126,14 @@ int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
  * counterpart: a warm start of this build. */
 int gmi_sim_set_gmres_history(gmi_ctx *ctx, int m);
 
+/* How the filaments' approximate Jacobians are inverted (the reference's
+ * LAPACK getrf/getri, matrix.cpp:538-641).  0 (default): Gauss-Jordan
+ * without pivoting, each inverse checked on a fixed vector and redone with
+ * partial pivoting if the check fails (NSEG <= 21; longer filaments always
+ * pivot).  1: partial pivoting for every filament.  No reference
+ * counterpart beyond the choice of algorithm. */
+int gmi_sim_set_jacobian_pivoting(gmi_ctx *ctx, int always_pivot);
+
 /* Preconditioner of the rigid-body GMRES. */
 #define GMI_PRECOND_SELF_MOBILITY 0 /* the reference's: M ~ I/(6 pi mu a)
                                        (mobility_solver.cpp:651-727)         */

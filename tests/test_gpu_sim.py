@@ -372,3 +372,44 @@ def test_coupled_swimmers_match_oracle(gpu, precond):
         assert abs(ro[1] - rg[1]) <= 1
         assert dx <= DL * cfg.tol
         np.testing.assert_allclose(sg["body"], so["body"], rtol=0, atol=DL * cfg.tol)
+
+
+@pytest.mark.parametrize("always_pivot", [False, True])
+def test_jacobian_inverse_modes_match_oracle(gpu, always_pivot):
+    """The filament inverses of the 962-particle swimmer (6 NSEG = 120), by
+    the checked pivot-free Gauss-Jordan (default) and by partial pivoting
+    for every filament: J^-1 e at the first step against the oracle's LU
+    inverse, then five steps with the oracle's Broyden counts."""
+    cfg = geometry.sim_workload("step_962", seed=1)
+    o, g = orc.Sim(cfg), capi.Sim(cfg)
+    g.set_jacobian_pivoting(always_pivot)
+    print(f"jacobian inverse always_pivot {always_pivot}", flush=True)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    o.initial_guess(0); o.forces(); o.compute_velocities()
+    g.call("gmi_initial_guess", 0)
+    g.call("gmi_forces_and_torques", 0)
+    its = capi.C.c_int(0)
+    g.call("gmi_compute_velocities", 0, capi.C.byref(its))
+    big, bad = capi.C.c_int(0), capi.C.c_int(0)
+    g.call("gmi_compute_errors", 0, 0, capi.C.byref(big), capi.C.byref(bad))
+    eg = g.broyden(0)
+    o.prepare_jacobian(0)
+    g.call("gmi_prepare_jacobian_inv", 0)
+    g.call("gmi_broyden_find_update")
+    ug = g.broyden(2)
+    uo = -cfg.jacobian_confidence * o.jacobian_inv_mult(eg)
+    print(f"  J^-1 e rel err {rel(ug, uo):.2e}", flush=True)
+    assert rel(ug, uo) < 1e-10
+    o, g = orc.Sim(cfg), capi.Sim(cfg)
+    g.set_jacobian_pivoting(always_pivot)
+    o.make_body_reference_matrices()
+    g.make_body_reference_matrices()
+    DL = cfg.swimmers[0].DL
+    for nt in range(5):
+        ro, rg = o.step(nt), g.step(nt)
+        dx = np.abs(g.state()["x"] - o.state()["x"]).max()
+        print(f"  step {nt}: oracle {ro} gpu {rg} |dx| {dx:.2e}", flush=True)
+        assert ro[0] == 0 and rg[0] == 0
+        assert abs(ro[1] - rg[1]) <= 1
+        assert dx <= DL * cfg.tol
