@@ -214,8 +214,8 @@ def bench_step(args):
             sim.warmstart_init(hidden=128, seed=7)
             sim.warmstart_configure(apply=False, collect=(args.step_warmup // stride) * sim.Ns,
                                     stride=stride)
-        sim.call("gmi_set_launch_shape", *STEP_LAUNCH_SHAPE)
-        shape = STEP_LAUNCH_SHAPE
+        shape = tuple(args.step_shape) if args.step_shape else STEP_LAUNCH_SHAPE
+        sim.call("gmi_set_launch_shape", *shape)
         sim.make_body_reference_matrices()
         sims[name] = sim
     t0 = time.perf_counter()
@@ -677,6 +677,9 @@ def main():
     ap.add_argument("--no-step", action="store_true")
     ap.add_argument("--step-modes", default=None,
                     help="comma list of step-leg modes (default: all nine)")
+    ap.add_argument("--step-shape", type=int, nargs=2, default=None,
+                    metavar=("TARGET_WORKGROUPS", "MIN_CHUNK"),
+                    help="pair launch shape of the step leg (default: STEP_LAUNCH_SHAPE)")
     ap.add_argument("--ens-steps", type=int, default=40)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--rollout", type=int, default=1000)
