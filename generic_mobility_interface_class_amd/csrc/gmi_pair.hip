@@ -11,11 +11,11 @@
 // reads the same LDS source at the same time (a broadcast read), so LDS
 // bandwidth per pair is one source record per wave.
 //
-// Far pairs (the common case) use the unnormalised separation d = x_i - x_j
-// and powers of 1/|d| from one reciprocal square root:
-//   v += A f + B d + g (tau x d),   w += A' tau + B' d + g (f x d).
-// Near pairs (|d| <= a_i + a_j, including the self pair) take a divergent
-// branch that reproduces the reference's overlap formulas with
+// Far pairs (the common case) use the unnormalised separation d = x_i - x_j,
+// powers of 1/|d| from one reciprocal square root, and sources staged
+// pre-scaled by the mobility constants (the pair forms are written out above
+// pair_ss).  Near pairs (|d| <= a_i + a_j, including the self pair) take a
+// wave-uniform branch that reproduces the reference's overlap formulas with
 // r_hat = d / (|d| + 1e-20).
 //
 // FP64 variant: v_rsq_f64 refined by one third-order Newton step.
@@ -42,56 +42,56 @@ template <> __device__ __forceinline__ double rsq<double>(double x) {
 }
 
 // Constants of one radius pair (a_i target, a_j source), precomputed in FP64.
+// Sources are staged pre-scaled: F' = f/(8 pi mu), T' = tau/(16 pi mu) and
+// T2 = 2 T' (see the pair forms below), so the far-field coefficients carry
+// no mobility constants and the overlap coefficients carry 8 pi mu or
+// 16 pi mu folded into their constants.
 template <typename R> struct PC {
   R far2;       // (a_i + a_j)^2: pairs with r^2 above it use the far forms
-  R c8;         // 1/(8 pi mu)
-  R c8s3;       // c8 (a_i^2 + a_j^2)/3
-  R c8s;        // c8 (a_i^2 + a_j^2)
-  R c16;        // 1/(16 pi mu)
-  R c16x3;      // 3/(16 pi mu)
-  // near-field (overlap) constants
-  R a;          // equal radius (ss / bb)
+  R s3;         // (a_i^2 + a_j^2)/3
+  R s;          // a_i^2 + a_j^2
+  // near-field (overlap) constants, equal radius (ss / bb)
   R inv_a;      // 1/a
   R c3_32a;     // 3/(32 a)
   R c3_8a;      // 3/(8 a)
-  R c6a;        // 1/(6 pi mu a)
-  R c8a3;       // 1/(8 pi mu a^3)
-  R c16a3;      // 1/(16 pi mu a^3)
-  R as, ab, dlt, c6ab, c128, c6max, c8as3;  // unequal radii (sb / bs)
+  R c6a;        // 8 pi mu / (6 pi mu a)       = 4/(3a)
+  R c8a3;       // 16 pi mu / (8 pi mu a^3)    = 2/a^3
+  R c16a3;      // 8 pi mu / (16 pi mu a^3)    = 1/(2 a^3)
+  // unequal radii (sb / bs): as segment, ab blob
+  R as, ab, dlt;
+  R c6ab;       // 8 pi mu / (6 pi mu as ab)
+  R c128;       // 8 pi mu / (128 pi mu ab as^3)
+  R c6max;      // 8 pi mu / (6 pi mu max(as, ab))
+  R c8as3;      // 8 pi mu / (8 pi mu as^3) when as > ab, else 0
 };
 
 template <typename R>
-__host__ __device__ PC<R> make_pc(double ai, double aj, double mu) {
+__host__ __device__ PC<R> make_pc(double ai, double aj) {
   PC<R> k;
-  const double c8 = 1.0 / (8.0 * kPi * mu);
   const double sig = ai * ai + aj * aj;
   k.far2 = R((ai + aj) * (ai + aj));
-  k.c8 = R(c8);
-  k.c8s3 = R(c8 * sig / 3.0);
-  k.c8s = R(c8 * sig);
-  k.c16 = R(1.0 / (16.0 * kPi * mu));
-  k.c16x3 = R(3.0 / (16.0 * kPi * mu));
-  k.a = R(ai);
+  k.s3 = R(sig / 3.0);
+  k.s = R(sig);
   k.inv_a = R(1.0 / ai);
   k.c3_32a = R(3.0 / (32.0 * ai));
   k.c3_8a = R(3.0 / (8.0 * ai));
-  k.c6a = R(1.0 / (6.0 * kPi * mu * ai));
-  k.c8a3 = R(1.0 / (8.0 * kPi * mu * ai * ai * ai));
-  k.c16a3 = R(1.0 / (16.0 * kPi * mu * ai * ai * ai));
+  k.c6a = R(4.0 / (3.0 * ai));
+  k.c8a3 = R(2.0 / (ai * ai * ai));
+  k.c16a3 = R(0.5 / (ai * ai * ai));
   return k;
 }
 
 // Unequal-radius constants: as = segment radius, ab = blob radius.
 template <typename R>
-__host__ __device__ PC<R> make_pc_unequal(double as, double ab, double mu) {
-  PC<R> k = make_pc<R>(as, ab, mu);
+__host__ __device__ PC<R> make_pc_unequal(double as, double ab) {
+  PC<R> k = make_pc<R>(as, ab);
   k.as = R(as);
   k.ab = R(ab);
   k.dlt = R(as > ab ? as - ab : ab - as);
-  k.c6ab = R(1.0 / (6.0 * kPi * mu * as * ab));
-  k.c128 = R(1.0 / (128.0 * kPi * mu * ab * as * as * as));
-  k.c6max = R(1.0 / (6.0 * kPi * mu * (as > ab ? as : ab)));
-  k.c8as3 = R(as > ab ? 1.0 / (8.0 * kPi * mu * as * as * as) : 0.0);
+  k.c6ab = R(4.0 / (3.0 * as * ab));
+  k.c128 = R(1.0 / (16.0 * ab * as * as * as));
+  k.c6max = R(4.0 / (3.0 * (as > ab ? as : ab)));
+  k.c8as3 = R(as > ab ? 1.0 / (as * as * as) : 0.0);
   return k;
 }
 
@@ -106,11 +106,11 @@ __device__ __forceinline__ void near_r(R r2, R ri, R &r, R &rm1) {
 }
 
 // Equal-radius overlap coefficients (cuda_functions.cu:336-350, 364-386,
-// 400-406) expressed as multipliers of f, d and (x d).  Divisions by the
-// radius are multiplications by precomputed reciprocals.
+// 400-406) in the scaled pair form: fd = F'.d and td = T'.d.  E is the
+// difference of the f x d and tau coefficients of w (zero in the far field).
 template <typename R>
 __device__ __forceinline__ void near_equal(
-    R r2, R ri, R fd, R td, const PC<R> &k, R &At, R &Bt, R &Ar, R &Br, R &g) {
+    R r2, R ri, R fd, R td, const PC<R> &k, R &At, R &Bt, R &Br, R &Cg, R &E) {
   R r, rm1;
   near_r(r2, ri, r, rm1);
   const R b = r * k.c3_32a;
@@ -118,15 +118,16 @@ __device__ __forceinline__ void near_equal(
   Bt = ((b * k.c6a * fd) * rm1) * rm1;
   const R t = r * k.inv_a;
   const R t3 = t * t * t;
-  Ar = (R(1) - R(27) * t / R(32) + R(5) * t3 / R(64)) * k.c8a3;
+  const R Ar = (R(1) - R(27) * t / R(32) + R(5) * t3 / R(64)) * k.c8a3;  // w = Ar T'
   Br = (((R(9) * t / R(32) - R(3) * t3 / R(64)) * k.c8a3 * td) * rm1) * rm1;
-  g = (r * (R(1) - r * k.c3_8a) * k.c16a3) * rm1;
+  Cg = (r * (R(1) - r * k.c3_8a) * k.c16a3) * rm1;
+  E = Cg + Ar;
 }
 
 // Unequal-radius overlap coefficients (cuda_functions.cu:622-646, 656-664).
 template <typename R>
 __device__ __forceinline__ void near_unequal(
-    R r2, R ri, R fd, const PC<R> &k, R &At, R &Bt, R &g) {
+    R r2, R ri, R fd, const PC<R> &k, R &At, R &Bt, R &Cg) {
   R r, rm1;
   near_r(r2, ri, r, rm1);
   if (r > k.dlt) {
@@ -137,13 +138,13 @@ __device__ __forceinline__ void near_unequal(
     Bt = (((R(3) * q * q * inv_r3x32) * k.c6ab * fd) * rm1) * rm1;
     const R s = k.as - k.ab + r;
     const R h = k.as - r;
-    g = (s * s * rm1 * rm1 *
-         (k.ab * k.ab + R(2) * k.ab * (r + k.as) - R(3) * h * h) * k.c128) *
-        rm1;
+    Cg = (s * s * rm1 * rm1 *
+          (k.ab * k.ab + R(2) * k.ab * (r + k.as) - R(3) * h * h) * k.c128) *
+         rm1;
   } else {
     At = k.c6max;
     Bt = R(0);
-    g = (r * k.c8as3) * rm1;
+    Cg = (r * k.c8as3) * rm1;
   }
 }
 
@@ -152,130 +153,162 @@ template <typename R> struct Acc {
   __device__ void zero() { vx = vy = vz = wx = wy = wz = R(0); }
 };
 
+// One staged source: position (FP32 keeps a low part too), F', and for
+// segment sources T' and T2.
+template <typename R> struct Lds {
+  R x[kTile], y[kTile], z[kTile];
+  R fx[kTile], fy[kTile], fz[kTile];
+  R tx[kTile], ty[kTile], tz[kTile];
+  R ux[kTile], uy[kTile], uz[kTile];
+  float lx[kTile], ly[kTile], lz[kTile];   // used by FP32 only
+};
+
 // ---- per-pair kernels ------------------------------------------------------
-// Every lane evaluates the far forms; the overlap forms run only when some
-// lane of the wave has an overlapping pair (a wave-uniform branch, so the
-// common case needs no per-pair execution-mask changes), and are selected
-// per lane.  The far values of an overlapping pair may be infinite (r = 0);
-// the select discards them.
+// With d = x_i - x_j, ri = 1/|d| and the pre-scaled sources, the far forms
+// (cuda_functions.cu:302-398, 478-517, 593-667, 755-829) are
+//   v += At F' + Bt d + Cg (T2 x d)
+//   w += Cg (F' x d - T') + Br d
+// with At = ri + s3 ri^3, Bt = (ri^3 - s ri^5) F'.d, Br = 3 ri^5 T'.d and
+// Cg = ri^3: one reciprocal square root, and every product folded into the
+// running sums as a fused multiply-add.  The overlap forms (r <= a_i + a_j,
+// including the self pair) run only when some lane of the wave has an
+// overlapping pair (a wave-uniform branch, so the common case needs no
+// per-pair execution-mask changes) and are selected per lane; there the f x d
+// and tau coefficients of w differ by E, added as E T'.  The far values of an
+// overlapping pair may be infinite (r = 0); the select discards them.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+// v += At F' + Bt d
+template <typename R>
+__device__ __forceinline__ void acc_v(R At, R Bt, R dx, R dy, R dz, const Lds<R> &s, int j,
+                                      Acc<R> &o) {
+  o.vx = fma(At, s.fx[j], o.vx); o.vx = fma(Bt, dx, o.vx);
+  o.vy = fma(At, s.fy[j], o.vy); o.vy = fma(Bt, dy, o.vy);
+  o.vz = fma(At, s.fz[j], o.vz); o.vz = fma(Bt, dz, o.vz);
+}
+
+// v += Cg (T2 x d)
+template <typename R>
+__device__ __forceinline__ void acc_vrot(R Cg, R dx, R dy, R dz, const Lds<R> &s, int j,
+                                         Acc<R> &o) {
+  const R ux = s.ux[j], uy = s.uy[j], uz = s.uz[j];
+  o.vx = fma(Cg, fma(uy, dz, -(uz * dy)), o.vx);
+  o.vy = fma(Cg, fma(uz, dx, -(ux * dz)), o.vy);
+  o.vz = fma(Cg, fma(ux, dy, -(uy * dx)), o.vz);
+}
 
 // ss: 6x6 equal radius.
 template <typename R>
-__device__ __forceinline__ void pair_ss(R dx, R dy, R dz, R fx, R fy, R fz,
-                                        R tx, R ty, R tz, const PC<R> &k,
-                                        Acc<R> &o) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R fd = fx * dx + fy * dy + fz * dz;
-  const R td = tx * dx + ty * dy + tz * dz;
-  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-  R At = ri * (k.c8 + k.c8s3 * ri2);
-  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-  R Ar = -k.c16 * ri3;
-  R Br = k.c16x3 * ri3 * ri2 * td;
-  R g = k.c8 * ri3;
+__device__ __forceinline__ void pair_ss(R dx, R dy, R dz, const Lds<R> &s, int j,
+                                        const PC<R> &k, Acc<R> &o) {
+  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R tx = s.tx[j], ty = s.ty[j], tz = s.tz[j];
+  const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
+  const R td = fma(tz, dz, fma(ty, dy, tx * dx));
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+  R At = fma(k.s3, ri3, ri);
+  R Bt = fma(-k.s, ri5, ri3) * fd;
+  R Br = (R(3) * ri5) * td;
+  R Cg = ri3;
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
-    R At2, Bt2, Ar2, Br2, g2;
-    near_equal(r2, ri, fd, td, k, At2, Bt2, Ar2, Br2, g2);
-    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Ar = nr ? Ar2 : Ar; Br = nr ? Br2 : Br;
-    g = nr ? g2 : g;
+    R At2, Bt2, Br2, Cg2, E2;
+    near_equal(r2, ri, fd, td, k, At2, Bt2, Br2, Cg2, E2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Br = nr ? Br2 : Br; Cg = nr ? Cg2 : Cg;
+    const R E = nr ? E2 : R(0);
+    o.wx = fma(E, tx, o.wx); o.wy = fma(E, ty, o.wy); o.wz = fma(E, tz, o.wz);
   }
-  o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
-  o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
-  o.vz += At * fz + Bt * dz + g * (tx * dy - ty * dx);
-  o.wx += Ar * tx + Br * dx + g * (fy * dz - fz * dy);
-  o.wy += Ar * ty + Br * dy + g * (fz * dx - fx * dz);
-  o.wz += Ar * tz + Br * dz + g * (fx * dy - fy * dx);
+  acc_vrot(Cg, dx, dy, dz, s, j, o);
+  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  o.wx = fma(Cg, fma(-fz, dy, fma(fy, dz, -tx)), o.wx); o.wx = fma(Br, dx, o.wx);
+  o.wy = fma(Cg, fma(-fx, dz, fma(fz, dx, -ty)), o.wy); o.wy = fma(Br, dy, o.wy);
+  o.wz = fma(Cg, fma(-fy, dx, fma(fx, dy, -tz)), o.wz); o.wz = fma(Br, dz, o.wz);
 }
 
 // sb: segment (v, w) <- blob f, unequal radii.
 template <typename R>
-__device__ __forceinline__ void pair_sb(R dx, R dy, R dz, R fx, R fy, R fz,
+__device__ __forceinline__ void pair_sb(R dx, R dy, R dz, const Lds<R> &s, int j,
                                         const PC<R> &k, Acc<R> &o) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R fd = fx * dx + fy * dy + fz * dz;
-  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-  R At = ri * (k.c8 + k.c8s3 * ri2);
-  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-  R g = k.c8 * ri3;
+  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+  R At = fma(k.s3, ri3, ri);
+  R Bt = fma(-k.s, ri5, ri3) * fd;
+  R Cg = ri3;
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
-    R At2, Bt2, g2;
-    near_unequal(r2, ri, fd, k, At2, Bt2, g2);
-    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
+    R At2, Bt2, Cg2;
+    near_unequal(r2, ri, fd, k, At2, Bt2, Cg2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Cg = nr ? Cg2 : Cg;
   }
-  o.vx += At * fx + Bt * dx;
-  o.vy += At * fy + Bt * dy;
-  o.vz += At * fz + Bt * dz;
-  o.wx += g * (fy * dz - fz * dy);
-  o.wy += g * (fz * dx - fx * dz);
-  o.wz += g * (fx * dy - fy * dx);
+  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  o.wx = fma(Cg, fma(fy, dz, -(fz * dy)), o.wx);
+  o.wy = fma(Cg, fma(fz, dx, -(fx * dz)), o.wy);
+  o.wz = fma(Cg, fma(fx, dy, -(fy * dx)), o.wz);
 }
 
 // bs: blob v <- segment (f, tau), unequal radii.
 template <typename R>
-__device__ __forceinline__ void pair_bs(R dx, R dy, R dz, R fx, R fy, R fz,
-                                        R tx, R ty, R tz, const PC<R> &k,
-                                        Acc<R> &o) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R fd = fx * dx + fy * dy + fz * dz;
-  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-  R At = ri * (k.c8 + k.c8s3 * ri2);
-  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
-  R g = k.c8 * ri3;
+__device__ __forceinline__ void pair_bs(R dx, R dy, R dz, const Lds<R> &s, int j,
+                                        const PC<R> &k, Acc<R> &o) {
+  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+  R At = fma(k.s3, ri3, ri);
+  R Bt = fma(-k.s, ri5, ri3) * fd;
+  R Cg = ri3;
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
-    R At2, Bt2, g2;
-    near_unequal(r2, ri, fd, k, At2, Bt2, g2);
-    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; g = nr ? g2 : g;
+    R At2, Bt2, Cg2;
+    near_unequal(r2, ri, fd, k, At2, Bt2, Cg2);
+    At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Cg = nr ? Cg2 : Cg;
   }
-  o.vx += At * fx + Bt * dx + g * (ty * dz - tz * dy);
-  o.vy += At * fy + Bt * dy + g * (tz * dx - tx * dz);
-  o.vz += At * fz + Bt * dz + g * (tx * dy - ty * dx);
+  acc_vrot(Cg, dx, dy, dz, s, j, o);
+  acc_v(At, Bt, dx, dy, dz, s, j, o);
 }
 
 // translation only, unequal radii: segment v <- blob f or blob v <- segment f
 // (the tt parts of sb / bs; the prescribed-cilia operator has no torques and
 // no angular velocities, cuda_functions.cu:646,808)
 template <typename R>
-__device__ __forceinline__ void pair_ut(R dx, R dy, R dz, R fx, R fy, R fz,
+__device__ __forceinline__ void pair_ut(R dx, R dy, R dz, const Lds<R> &s, int j,
                                         const PC<R> &k, Acc<R> &o) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R fd = fx * dx + fy * dy + fz * dz;
-  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-  R At = ri * (k.c8 + k.c8s3 * ri2);
-  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+  R At = fma(k.s3, ri3, ri);
+  R Bt = fma(-k.s, ri5, ri3) * fd;
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
-    R At2, Bt2, g;
-    near_unequal(r2, ri, fd, k, At2, Bt2, g);
+    R At2, Bt2, Cg;
+    near_unequal(r2, ri, fd, k, At2, Bt2, Cg);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
-  o.vx += At * fx + Bt * dx;
-  o.vy += At * fy + Bt * dy;
-  o.vz += At * fz + Bt * dz;
+  acc_v(At, Bt, dx, dy, dz, s, j, o);
 }
 
 // bb: blob v <- blob f, equal radius (also segment v <- segment f when only
 // translations couple).
 template <typename R>
-__device__ __forceinline__ void pair_bb(R dx, R dy, R dz, R fx, R fy, R fz,
+__device__ __forceinline__ void pair_bb(R dx, R dy, R dz, const Lds<R> &s, int j,
                                         const PC<R> &k, Acc<R> &o) {
-  const R r2 = dx * dx + dy * dy + dz * dz;
-  const R fd = fx * dx + fy * dy + fz * dz;
-  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri;
-  R At = ri * (k.c8 + k.c8s3 * ri2);
-  R Bt = ri3 * (k.c8 - k.c8s * ri2) * fd;
+  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
+  const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+  R At = fma(k.s3, ri3, ri);
+  R Bt = fma(-k.s, ri5, ri3) * fd;
   const bool nr = !(r2 > k.far2);
   if (__builtin_expect(wave_any(nr), 0)) {
-    R At2, Bt2, Ar, Br, g;
-    near_equal(r2, ri, fd, R(0), k, At2, Bt2, Ar, Br, g);
+    R At2, Bt2, Br, Cg, E;
+    near_equal(r2, ri, fd, R(0), k, At2, Bt2, Br, Cg, E);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
-  o.vx += At * fx + Bt * dx;
-  o.vy += At * fy + Bt * dy;
-  o.vz += At * fz + Bt * dz;
+  acc_v(At, Bt, dx, dy, dz, s, j, o);
 }
 
 // ---- precision traits ----------------------------------------------------
@@ -293,13 +326,6 @@ template <> struct Pos<float> {
     x = float(px); y = float(py); z = float(pz);
     lx = float(px - double(x)); ly = float(py - double(y)); lz = float(pz - double(z));
   }
-};
-
-template <typename R> struct Lds {
-  R x[kTile], y[kTile], z[kTile];
-  R fx[kTile], fy[kTile], fz[kTile];
-  R tx[kTile], ty[kTile], tz[kTile];
-  float lx[kTile], ly[kTile], lz[kTile];   // used by FP32 only
 };
 
 template <typename R>
@@ -376,11 +402,13 @@ template <> struct Sum<float> {
 enum Kind { SS, SB, BS, BB, UT };
 
 // Walk sources [j0, j1) of one species in LDS tiles and accumulate into sum.
+// c8 = 1/(8 pi mu) scales the staged sources (F' = c8 f, T' = c8 tau / 2,
+// T2 = c8 tau).
 template <typename R, int K>
 __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
                                       const double *__restrict__ X,
                                       const double *__restrict__ F, int ld,
-                                      int j0, int j1, const PC<R> &k,
+                                      int j0, int j1, const PC<R> &k, double c8,
                                       Sum<R> &sum) {
   constexpr bool six = (K == SS || K == BS);  // segment sources carry tau
   const int tid = threadIdx.x;
@@ -390,13 +418,17 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
     if (tid < n) {
       const int j = base + tid;
       stage_pos<R>(lds, tid, X[j], X[ld + j], X[2 * ld + j]);
-      lds.fx[tid] = R(F[j]);
-      lds.fy[tid] = R(F[ld + j]);
-      lds.fz[tid] = R(F[2 * ld + j]);
+      lds.fx[tid] = R(c8 * F[j]);
+      lds.fy[tid] = R(c8 * F[ld + j]);
+      lds.fz[tid] = R(c8 * F[2 * ld + j]);
       if (six) {
-        lds.tx[tid] = R(F[3 * ld + j]);
-        lds.ty[tid] = R(F[4 * ld + j]);
-        lds.tz[tid] = R(F[5 * ld + j]);
+        const double tx = c8 * F[3 * ld + j], ty = c8 * F[4 * ld + j], tz = c8 * F[5 * ld + j];
+        lds.ux[tid] = R(tx);
+        lds.uy[tid] = R(ty);
+        lds.uz[tid] = R(tz);
+        lds.tx[tid] = R(0.5 * tx);
+        lds.ty[tid] = R(0.5 * ty);
+        lds.tz[tid] = R(0.5 * tz);
       }
     }
     __syncthreads();
@@ -407,17 +439,15 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
       R dx, dy, dz;
       sep<R>(p, lds, j, dx, dy, dz);
       if (K == SS)
-        pair_ss<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], lds.tx[j],
-                   lds.ty[j], lds.tz[j], k, acc);
+        pair_ss<R>(dx, dy, dz, lds, j, k, acc);
       else if (K == SB)
-        pair_sb<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
+        pair_sb<R>(dx, dy, dz, lds, j, k, acc);
       else if (K == BS)
-        pair_bs<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], lds.tx[j],
-                   lds.ty[j], lds.tz[j], k, acc);
+        pair_bs<R>(dx, dy, dz, lds, j, k, acc);
       else if (K == UT)
-        pair_ut<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
+        pair_ut<R>(dx, dy, dz, lds, j, k, acc);
       else
-        pair_bb<R>(dx, dy, dz, lds.fx[j], lds.fy[j], lds.fz[j], k, acc);
+        pair_bb<R>(dx, dy, dz, lds, j, k, acc);
     }
     sum.add(acc);
   }
@@ -447,34 +477,35 @@ __device__ __forceinline__ void pair_body(const PairArgs &a, Lds<R> &lds) {
 
   Sum<R> sum;
   sum.zero();
+  const double c8 = 1.0 / (8.0 * kPi * a.mu);
   const int s0 = beg, s1 = min(end, ns);
   const int b0 = max(beg, ns) - ns, b1 = end - ns;
   if (seg_t) {
     if (s1 > s0) {
-      const PC<R> k = make_pc<R>(a.a_s, a.a_s, a.mu);
+      const PC<R> k = make_pc<R>(a.a_s, a.a_s);
       if (a.tt_only)
-        sweep<R, BB>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+        sweep<R, BB>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, c8, sum);
       else
-        sweep<R, SS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+        sweep<R, SS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, c8, sum);
     }
     if (b1 > b0) {
-      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
+      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b);
       if (a.tt_only)
-        sweep<R, UT>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+        sweep<R, UT>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, c8, sum);
       else
-        sweep<R, SB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+        sweep<R, SB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, c8, sum);
     }
   } else {
     if (s1 > s0) {
-      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b, a.mu);
+      const PC<R> k = make_pc_unequal<R>(a.a_s, a.a_b);
       if (a.tt_only)
-        sweep<R, UT>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+        sweep<R, UT>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, c8, sum);
       else
-        sweep<R, BS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, sum);
+        sweep<R, BS>(lds, p, a.Xs, a.Fs, a.lds, T.sseg0 + s0, T.sseg0 + s1, k, c8, sum);
     }
     if (b1 > b0) {
-      const PC<R> k = make_pc<R>(a.a_b, a.a_b, a.mu);
-      sweep<R, BB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, sum);
+      const PC<R> k = make_pc<R>(a.a_b, a.a_b);
+      sweep<R, BB>(lds, p, a.Xb, a.Fb, a.ldb, T.sblob0 + b0, T.sblob0 + b1, k, c8, sum);
     }
   }
   const bool live = tid < T.nt;
