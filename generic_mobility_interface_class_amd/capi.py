@@ -100,6 +100,7 @@ SIGNATURES = [
     ("gmi_get_broyden", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("gmi_sim_set_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gmi_sim_graph_stats", C.c_int, [C.c_void_p, _ip, _ip]),
     ("gmi_sim_set_gmres_history", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_jacobian_pivoting", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_body_preconditioner", C.c_int, [C.c_void_p, C.c_int]),
@@ -350,6 +351,12 @@ class Sim(Context):
 
     def set_graphs(self, use_graphs=True, chunk=6):
         self.call("gmi_sim_set_graphs", int(bool(use_graphs)), int(chunk))
+
+    def graph_stats(self):
+        """(Broyden iterations run from captured graphs, graphs held)."""
+        it, ng = C.c_int(0), C.c_int(0)
+        self.call("gmi_sim_graph_stats", C.byref(it), C.byref(ng))
+        return it.value, ng.value
 
     def set_body_preconditioner(self, kind=GMI_PRECOND_BODY_INVERSE):
         """Rigid-body GMRES preconditioner: the reference's self-mobility

@@ -187,7 +187,7 @@ int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop) {
 
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
               const double *Fs, const double *Fb, double *Vs, double *Vb,
-              const double *Xs, const double *Xb) {
+              const double *Xs, const double *Xb, const int *skip) {
   const WorkList &wl = c->lists[mask];
   PairArgs a;
   a.Xs = Xs ? Xs : c->Xs; a.Fs = Fs ? Fs : c->Fs; a.lds = c->lds;
@@ -200,6 +200,7 @@ int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32,
   a.accumulate = accumulate;
   a.a_s = c->as; a.a_b = c->ab; a.mu = c->mu;
   a.tt_only = c->tt_only ? 1 : 0;
+  a.skip = skip;
   hipEvent_t ev_mid = nullptr;
   if (wl.nwork > 0) {
     int rc;

@@ -987,7 +987,8 @@ static int initial_guess(gmi_ctx *c, CiliaHost *s) {
   int rc;
   if ((rc = launch_ok(c, "cilia shapes"))) return rc;
   if (!d.free_body) return GMI_OK;
-  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, s->flags + F_SING, 1, 0);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, s->flags + F_SING, 1, 0,
+                     nullptr);
   hipLaunchKernelGGL(k_cb_ktmk, dim3(d.Nsw), dim3(256), 0, c->stream, d, s->sw_seg0, s->sw_fil0,
                      s->sw_nfil, s->sw_blob0, s->sw_nblob, s->flags);
   if ((rc = launch_ok(c, "cilia preconditioner matrices"))) return rc;

@@ -92,7 +92,8 @@ int prof_start(gmi_ctx *c, hipEvent_t *ev_stop);
 // length n: records the start event and returns the stop event in *ev_stop
 // (nullptr when not profiling or while the stream is captured).
 int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop);
+// skip: the launch is a no-op when non-null and *skip != 0
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32, const double *Fs,
               const double *Fb, double *Vs, double *Vb, const double *Xs,
-              const double *Xb);
+              const double *Xb, const int *skip = nullptr);
 }  // namespace gmi

@@ -524,7 +524,8 @@ int dense_setup(gmi_ctx *c, SimHost *s) {
       (rc = hip_check(c, hipMemcpyAsync(D->voff, D->voff_h.data(), sizeof(int) * d.Nsw,
                                         hipMemcpyHostToDevice, c->stream), "upload")))
     return bail(rc);
-  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0,
+                     nullptr);
   for (int n = 0; n < d.Nsw; n++) {
     const int nb = c->nblob[n], rows = 3 * nb;
     const size_t ld = size_t(rows);

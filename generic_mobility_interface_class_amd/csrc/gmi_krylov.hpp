@@ -41,8 +41,11 @@
 namespace gmi {
 
 // device flags
+// F_HOLD: set while a captured Broyden iteration's GMRES is unfinished; the
+// kernels after the solve in that graph are no-ops then (the host finishes
+// the iteration with eager launches).
 enum { F_DONE = 0, F_ITS = 1, F_BIG = 2, F_NAN = 3, F_SING = 4, F_SKIP64 = 5, F_SKIP32 = 6,
-       F_N = 8 };
+       F_HOLD = 7, F_N = 8 };
 
 // A captured stream segment: the graph is kept as long as its executable
 // (both are destroyed together).
@@ -111,9 +114,14 @@ struct KrylovState {
   int graph_gen = -1;
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
   std::map<std::pair<int, int>, CapturedGraph> graphs;
+  // captured outer iterations of a solver built on this one (their key is
+  // the owner's)
+  std::map<long long, CapturedGraph> outer_graphs;
   void drop_graphs() {
     for (auto &kv : graphs) kv.second.destroy();
     graphs.clear();
+    for (auto &kv : outer_graphs) kv.second.destroy();
+    outer_graphs.clear();
   }
 };
 
@@ -225,7 +233,9 @@ __global__ __launch_bounds__(256) void k_colsum_wide(const double *part, int nb,
 }
 
 // h[j] = sum of the partials (one workgroup per 256 columns)
-__global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h) {
+__global__ __launch_bounds__(256) void k_colsum(const double *part, int k, int nb, double *h,
+                                                const int *skip) {
+  if (skip && *skip) return;
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= k) return;
   const double *p = part + size_t(j) * nb;
@@ -432,7 +442,8 @@ __global__ void k_gmres_start(double *x, double *r, const double *rhs, int n, do
 // column sweep from the last unknown, each lane updating its rows, which
 // subtracts the terms in the reference's order (j from it-1 down to i+1)
 __global__ __launch_bounds__(64) void k_backsub(const double *H, int ldh, const double *beta,
-                                                const int *flags, double *y) {
+                                                const int *flags, double *y, const int *skip) {
+  if (skip && *skip) return;
   extern __shared__ double rb[];
   const int it = flags[F_ITS];
   const int lane = threadIdx.x;
@@ -447,7 +458,8 @@ __global__ __launch_bounds__(64) void k_backsub(const double *H, int ldh, const 
 }
 
 __global__ void k_combine(const double *Qb, size_t ldq, const int *flags, const double *y,
-                          double *t, int n) {
+                          double *t, int n, const int *skip) {
+  if (skip && *skip) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int it = flags[F_ITS];
@@ -456,15 +468,18 @@ __global__ void k_combine(const double *Qb, size_t ldq, const int *flags, const 
   t[e] = s;
 }
 
-// out = a*x + b*y  (elementwise)
-__global__ void k_axpby(double *out, double a, const double *x, double b, const double *y, int n) {
+// out = a*x + b*y  (elementwise); skip: a no-op if non-null and nonzero
+__global__ void k_axpby(double *out, double a, const double *x, double b, const double *y, int n,
+                        const int *skip) {
+  if (skip && *skip) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) out[e] = a * x[e] + (y ? b * y[e] : 0.0);
 }
 
 // scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup)
 __global__ __launch_bounds__(1024) void k_norms(const double *x, const double *y, int n,
-                                                double *scal, int o0, int o1) {
+                                                double *scal, int o0, int o1, const int *skip) {
+  if (skip && *skip) return;
   __shared__ double sh[1024];
   for (int pass = 0; pass < 2; pass++) {
     const double *v = pass ? y : x;
@@ -487,7 +502,8 @@ __global__ void k_scale_by(double *x, int n, const double *scal, int o) {
   if (e < n) x[e] /= scal[o];
 }
 
-__global__ void k_set_ints(int *p, int n, int v) {
+__global__ void k_set_ints(int *p, int n, int v, const int *skip) {
+  if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -530,14 +546,16 @@ static DotShape dot_shape(int n) {
 // warm-started from s->x; CGS2 Arnoldi.  mixed: relaxed precision (FP64
 // operator until the residual drops below switch_rel, FP32 after).
 // n: vector length (padding rows stay zero); maxit: iteration cap.
-// Returns iterations via *its.
+// The stages: gmres_begin (initial residual and the start decisions on the
+// device), gmres_enqueue (Arnoldi steps it0..it1, every kernel a no-op once
+// the device flag says the solve has converged), gmres_chunks (chunks of
+// steps from it0, the host reading the flags after each), gmres_finish (the
+// least-squares solution and the update of x).
+
 template <class Op>
-int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixed, double tol,
-                 int *its) {
-  const size_t ldq = size_t(n);
+int gmres_begin(gmi_ctx *c, KrylovState *s, Op &op, int n, double tol) {
   const int nb = nblk(n, 256);
   int rc;
-  *its = 0;
   // r = rhs - A x ; norms ; the start decisions on the device (no host
   // round trip: a solve that needs no step runs one flag-skipped chunk)
   double *r = s->Qk;
@@ -545,70 +563,84 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
     // an exact preconditioner solves from x0 = 0 in one step: no warm start,
     // so no operator apply for the initial residual (r0 = rhs)
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 0.0, s->rhs, 0.0,
-                       nullptr, n);
+                       nullptr, n, nullptr);
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, 0.0, nullptr,
-                       n);
+                       n, nullptr);
   } else {
     if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, r, 1.0, s->rhs, -1.0, s->tmp,
-                       n);
+                       n, nullptr);
   }
-  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1);
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, r, n, s->scal, 0, 1,
+                     nullptr);
   hipLaunchKernelGGL(k_gmres_start, dim3(nb), dim3(256), 0, c->stream, s->x, r, s->rhs, n, s->scal,
                      s->beta, s->bpre, s->flags, tol);
-  if ((rc = launch_ok(c, "gmres start"))) return rc;
-  const double switch_rel = mixed ? tol / kFp32OperatorError : 0.0;
+  return launch_ok(c, "gmres start");
+}
+
+static GivensArgs givens_args(KrylovState *s, int maxit, bool mixed, double tol) {
   // scal[0] = ||rhs|| is read by k_givens
-  const int ldh = s->cols + 1;
-  // Arnoldi steps it0..it1; every kernel is a no-op once the device flag
-  // says the solve has converged, so a chunk needs no host decisions
+  return GivensArgs{s->h1, s->h2, s->H, s->cols + 1, s->cs, s->sn, s->beta, s->bpre,
+                    s->flags, s->scal, tol, maxit, mixed ? tol / kFp32OperatorError : 0.0};
+}
+
+// Arnoldi steps it0..it1 (each kernel a no-op once F_DONE is set)
+template <class Op>
+int gmres_enqueue(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixed, double tol,
+                  int it0, int it1) {
+  const size_t ldq = size_t(n);
   const DotShape ds = dot_shape(n);
   const size_t shm = sizeof(double) * ds.R;
   double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
-  const GivensArgs ga{s->h1, s->h2, s->H, ldh, s->cs, s->sn, s->beta, s->bpre,
-                      s->flags, s->scal, tol, maxit, switch_rel};
-  auto enqueue = [&](int it0, int it1) -> int {
-    int r;
-    if (op.fused()) {
-      // the operator runs each Arnoldi step on its own row partition (the
-      // Givens step of the previous iteration folded into the first kernel)
-      // and ends the chunk with a Givens step of its own, so that the host
-      // can read the flags
-      for (int it = it0; it <= it1; it++)
-        if ((r = op.fused_step(s, it, it == it0, mixed, ga))) return r;
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, op.fused_parts(),
-                         it1, ga);
-      return launch_ok(c, "givens");
-    }
-    for (int it = it0; it <= it1; it++) {
-      double *qprev = s->Qk + ldq * (it - 1);
-      double *w = s->Qk + ldq * it;
-      double *zk = s->Zk ? s->Zk + ldq * (it - 1) : s->z;
-      // the previous vector is normalised by the preconditioner's first
-      // kernel (the first one was scaled by beta0 already)
-      if ((r = op.precond(qprev, zk, s->flags, it >= 2 ? 3 : -1))) return r;
-      if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
-      // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
-      hipEvent_t gs_stop = nullptr;
-      if ((r = prof_gs_start(c, it, n, &gs_stop))) return r;
-      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
-                         ds.R, pp1, s->flags);
-      if (ds.wide)
-        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
-                           s->flags);
-      hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
-                         s->h1, w, n, ds.R, pp2, s->flags, int(ds.wide));
-      if (ds.wide)
-        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
-                           s->flags);
-      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
-                         s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
-      if (gs_stop && (r = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return r;
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, ga);
-      if ((r = launch_ok(c, "arnoldi"))) return r;
-    }
-    return GMI_OK;
-  };
+  const GivensArgs ga = givens_args(s, maxit, mixed, tol);
+  int r;
+  if (op.fused()) {
+    // the operator runs each Arnoldi step on its own row partition (the
+    // Givens step of the previous iteration folded into the first kernel)
+    // and ends the chunk with a Givens step of its own, so that the host
+    // can read the flags
+    for (int it = it0; it <= it1; it++)
+      if ((r = op.fused_step(s, it, it == it0, mixed, ga))) return r;
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, op.fused_parts(),
+                       it1, ga);
+    return launch_ok(c, "givens");
+  }
+  for (int it = it0; it <= it1; it++) {
+    double *qprev = s->Qk + ldq * (it - 1);
+    double *w = s->Qk + ldq * it;
+    double *zk = s->Zk ? s->Zk + ldq * (it - 1) : s->z;
+    // the previous vector is normalised by the preconditioner's first
+    // kernel (the first one was scaled by beta0 already)
+    if ((r = op.precond(qprev, zk, s->flags, it >= 2 ? 3 : -1))) return r;
+    if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
+    // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
+    hipEvent_t gs_stop = nullptr;
+    if ((r = prof_gs_start(c, it, n, &gs_stop))) return r;
+    hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
+                       ds.R, pp1, s->flags);
+    if (ds.wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
+                         s->flags);
+    hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
+                       s->h1, w, n, ds.R, pp2, s->flags, int(ds.wide));
+    if (ds.wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
+                         s->flags);
+    hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
+                       s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
+    if (gs_stop && (r = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return r;
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, ga);
+    if ((r = launch_ok(c, "arnoldi"))) return r;
+  }
+  return GMI_OK;
+}
+
+// Chunks of Arnoldi steps from it0 until the device flags say the solve has
+// converged; *its = the solve's iteration count.
+template <class Op>
+int gmres_chunks(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixed, double tol,
+                 int it0, int *its) {
+  int rc;
   if (s->graph_gen != c->list_generation) {
     s->drop_graphs();
     s->graph_gen = c->list_generation;
@@ -616,7 +648,7 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   // Gram-Schmidt records of this solve start here; steps launched past
   // convergence (flag-skipped) are dropped once the count is known
   const size_t gs_mark = c->gs_recs.size();
-  int it = 1;
+  int it = it0;
   int done = 0;
   while (!done) {
     const int last = std::min(maxit, it + s->chunk - 1);
@@ -627,13 +659,16 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
       auto g = s->graphs.find(key);
       if (g == s->graphs.end()) {
         CapturedGraph cg;
-        if ((rc = capture_graph(c, [&]() { return enqueue(it, last); }, &cg))) return rc;
+        if ((rc = capture_graph(
+                 c, [&]() { return gmres_enqueue(c, s, op, n, maxit, mixed, tol, it, last); },
+                 &cg)))
+          return rc;
         g = s->graphs.emplace(key, cg).first;
       }
       if ((rc = hip_check(c, hipGraphLaunch(g->second.exec, c->stream), "graph launch")))
         return rc;
     } else {
-      if ((rc = enqueue(it, last))) return rc;
+      if ((rc = gmres_enqueue(c, s, op, n, maxit, mixed, tol, it, last))) return rc;
     }
     it = last + 1;
     if ((rc = krylov_read_flags(c, s))) return rc;
@@ -644,22 +679,46 @@ int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixe
   c->gs_recs.erase(std::remove_if(c->gs_recs.begin() + gs_mark, c->gs_recs.end(),
                                   [nits](const gmi_ctx::GsRec &g) { return g.k > nits; }),
                    c->gs_recs.end());
+  *its = nits;
+  return GMI_OK;
+}
+
+// The least-squares solution and x += P^-1 Q y (or x += Z y); skip gates
+// every kernel (null: always run).
+template <class Op>
+int gmres_finish(gmi_ctx *c, KrylovState *s, Op &op, int n, const int *skip) {
+  const size_t ldq = size_t(n);
+  const int nb = nblk(n, 256);
+  const int ldh = s->cols + 1;
+  int rc;
   hipLaunchKernelGGL(k_backsub, dim3(1), dim3(64), sizeof(double) * (s->cols + 1), c->stream,
-                     s->H, ldh, s->beta, s->flags, s->y);
+                     s->H, ldh, s->beta, s->flags, s->y, skip);
   if (s->Zk) {
     // x += Z y
     hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Zk, ldq, s->flags, s->y,
-                       s->z, n);
+                       s->z, n, skip);
     if ((rc = launch_ok(c, "gmres solution"))) return rc;
   } else {
     // x += P^-1 (Q y)
     hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), 0, c->stream, s->Qk, ldq, s->flags, s->y,
-                       s->tmp, n);
+                       s->tmp, n, skip);
     if ((rc = launch_ok(c, "gmres solution"))) return rc;
-    if ((rc = op.precond(s->tmp, s->z, nullptr, -1))) return rc;
+    if ((rc = op.precond(s->tmp, s->z, skip, -1))) return rc;
   }
-  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n);
-  if ((rc = launch_ok(c, "gmres update"))) return rc;
+  hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->x, 1.0, s->x, 1.0, s->z, n,
+                     skip);
+  return launch_ok(c, "gmres update");
+}
+
+template <class Op>
+int krylov_gmres(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mixed, double tol,
+                 int *its) {
+  int rc;
+  *its = 0;
+  if ((rc = gmres_begin(c, s, op, n, tol))) return rc;
+  int nits = 0;
+  if ((rc = gmres_chunks(c, s, op, n, maxit, mixed, tol, 1, &nits))) return rc;
+  if ((rc = gmres_finish(c, s, op, n, nullptr))) return rc;
   *its = nits;
   return GMI_OK;
 }
@@ -684,8 +743,9 @@ int krylov_solve(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, int preci
     // true residual with the FP64 operator
     if ((rc = op.mult(s->x, s->tmp, 0, nullptr, false))) return rc;
     hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, c->stream, s->z, 1.0, s->rhs, -1.0,
-                       s->tmp, n);
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5);
+                       s->tmp, n, nullptr);
+    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, s->rhs, s->z, n, s->scal, 4, 5,
+                       nullptr);
     if ((rc = launch_ok(c, "refinement residual"))) return rc;
     if ((rc = hip_check(c, hipMemcpyAsync(s->h_scal + 4, s->scal + 4, 2 * sizeof(double),
                                           hipMemcpyDeviceToHost, c->stream), "scal")))

@@ -230,7 +230,8 @@ __device__ d3 strain_twist(const SimDev &a, int f, double t) {
 // The instability-driven build adds the follower force -efm t at the tip
 // and, when `perturb` (nt = 0), the two perturbation forces on segments 4 and
 // 12, after the link terms (filament.cpp:1177-1199).
-__global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, double t, int perturb) {
+__global__ __launch_bounds__(kFilLanes) void k_fil_forces(SimDev a, int perturb) {
+  const double t = a.step_t[0];
   __shared__ double sl[3 * kFilLanes], sM[3 * kFilLanes], sF[6 * kFilLanes];
   const int f = blockIdx.x, j = threadIdx.x;
   const int S = a.S, g0 = f * S, ld = a.lds;
@@ -1036,7 +1037,9 @@ __global__ void k_unit_rhs(SimDev a, double *rhs, int slot) {
 // -------------------------------------------------------------------------
 // residual (mobility_solver.cpp:1404-1504, 1641-1689)
 
-__global__ void k_errors(SimDev a, int nt, double *err, const double *vbody) {
+__global__ void k_errors(SimDev a, double *err, const double *vbody, const int *skip) {
+  if (skip && *skip) return;
+  const int nt = a.step_nt[0];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double c1 = -4.0 / 3.0, c2 = 1.0 / 3.0, c3 = -2.0 / 3.0;
   const bool euler = nt < a.neuler;
@@ -2000,7 +2003,8 @@ __device__ __forceinline__ int body_off(const SimDev &a, int n) {
   return 6 * a.S * (a.sw_fil0[n] + a.sw_nfil[n]) + 6 * n;
 }
 
-__global__ __launch_bounds__(64) void k_jim1(SimDev a, const double *in) {
+__global__ __launch_bounds__(64) void k_jim1(SimDev a, const double *in, const int *skip) {
+  if (skip && *skip) return;
   __shared__ double sh[64];
   const int f = blockIdx.x;
   const int n = 6 * a.S;
@@ -2016,7 +2020,9 @@ __global__ __launch_bounds__(64) void k_jim1(SimDev a, const double *in) {
 
 // One workgroup of 256 per swimmer: the filaments' B_i in_i are summed
 // across the workgroup, one lane applies S^-1.
-__global__ __launch_bounds__(256) void k_jim_body(SimDev a, const double *in, double *out) {
+__global__ __launch_bounds__(256) void k_jim_body(SimDev a, const double *in, double *out,
+                                                  const int *skip) {
+  if (skip && *skip) return;
   __shared__ double sh[4 * 6];
   const int n = blockIdx.x;
   const int o = body_off(a, n);
@@ -2037,7 +2043,9 @@ __global__ __launch_bounds__(256) void k_jim_body(SimDev a, const double *in, do
   }
 }
 
-__global__ __launch_bounds__(256) void k_jim2(SimDev a, const double *in, double *out) {
+__global__ __launch_bounds__(256) void k_jim2(SimDev a, const double *in, double *out,
+                                              const int *skip) {
+  if (skip && *skip) return;
   __shared__ double tmp[6 * kMaxSeg];
   const int f = blockIdx.x;
   const int S = a.S, n = 6 * S;
@@ -2085,7 +2093,8 @@ __global__ __launch_bounds__(256) void k_jim2(SimDev a, const double *in, double
 
 // upd = -gamma*upd - sum_i d[i] C_i   (find_update, and the new C column)
 __global__ void k_broyden_combine(double *out, const double *in, double gamma, const double *C,
-                                  size_t ldc, const double *d, int k, int n) {
+                                  size_t ldc, const double *d, int k, int n, const int *skip) {
+  if (skip && *skip) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   double s = -gamma * in[e];
@@ -2093,9 +2102,25 @@ __global__ void k_broyden_combine(double *out, const double *in, double gamma, c
   out[e] = s;
 }
 
-__global__ void k_scale2(double *x, double *y, int n, const double *scal, int o) {
+__global__ void k_scale2(double *x, double *y, int n, const double *scal, int o,
+                         const int *skip) {
+  if (skip && *skip) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) { x[e] /= scal[o]; y[e] /= scal[o]; }
+}
+
+__global__ void k_set_step(SimDev a, int nt, double t) {
+  a.step_nt[0] = nt;
+  a.step_t[0] = t;
+}
+
+// Gates of a captured Broyden iteration.  After the first GMRES chunk:
+// hold the rest of the iteration unless the solve has converged.  After the
+// residual: hold the Broyden update unless the residual is finite and too
+// large (the host then takes the iteration as converged or diverged).
+__global__ void k_hold_unsolved(int *flags) { flags[F_HOLD] = flags[F_DONE] ? 0 : 1; }
+__global__ void k_hold_unless_big(int *flags) {
+  flags[F_HOLD] = (flags[F_HOLD] || !flags[F_BIG] || flags[F_NAN]) ? 1 : 0;
 }
 
 // -------------------------------------------------------------------------
@@ -2354,7 +2379,9 @@ static int read_positions_and_forces(gmi_ctx *c, SimHost *s) {
   return launch_ok(c, "body repulsion");
 }
 
-static int compute_velocities(gmi_ctx *c, SimHost *s, int *its) {
+// The blob–blob solve's right-hand side from the current state: blob
+// positions, barrier, segment-segment and blob<-segment sums.
+static int velocities_before_solve(gmi_ctx *c, SimHost *s) {
   int rc;
   if ((rc = read_positions_and_forces(c, s))) return rc;
   const bool fp32 = s->precision == GMI_FP32_KAHAN;
@@ -2362,32 +2389,56 @@ static int compute_velocities(gmi_ctx *c, SimHost *s, int *its) {
     return rc;
   hipLaunchKernelGGL(k_assemble_rhs, dim3(nblk(s->d.npad, 256)), dim3(256), 0, c->stream, s->d,
                      s->rhs);
-  if ((rc = launch_ok(c, "rhs"))) return rc;
+  return launch_ok(c, "rhs");
+}
+
+// segment (v, w) += the blob forces' contribution (after the solve)
+static int velocities_after_solve(gmi_ctx *c, SimHost *s, const int *skip) {
+  return run_pairs(c, kSB, 1, s->precision == GMI_FP32_KAHAN, nullptr, s->x, nullptr, nullptr,
+                   nullptr, nullptr, skip);
+}
+
+static int compute_velocities(gmi_ctx *c, SimHost *s, int *its) {
+  int rc;
+  if ((rc = velocities_before_solve(c, s))) return rc;
   int it = 0;
   if ((rc = solve_linear_system(c, s, &it))) return rc;
   s->last_gmres = it;
   if (its) *its = it;
-  return run_pairs(c, kSB, 1, fp32, nullptr, s->x, nullptr, nullptr, nullptr, nullptr);
+  return velocities_after_solve(c, s, nullptr);
 }
 
-static int compute_errors(gmi_ctx *c, SimHost *s, int nt, double *dst, int *big, int *bad) {
+// the step's time and index for the kernels that read them on the device
+static int set_step(gmi_ctx *c, SimHost *s, int nt) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, c->stream, s->d, nt,
+                     (nt + 1) * s->p.dt);
+  return launch_ok(c, "step");
+}
+
+// Residual into dst and its flags (F_BIG, F_NAN) on the device.
+static int enqueue_errors(gmi_ctx *c, SimHost *s, double *dst, const int *skip) {
   SimDev &d = s->d;
-  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_BIG, 2, 0);
-  hipLaunchKernelGGL(k_errors, dim3(nblk(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d, nt, dst,
-                     s->x + 3 * d.ldb);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_BIG, 2, 0, skip);
+  hipLaunchKernelGGL(k_errors, dim3(nblk(d.Ns + d.Nsw, 256)), dim3(256), 0, c->stream, d, dst,
+                     s->x + 3 * d.ldb, skip);
+  return launch_ok(c, "errors");
+}
+
+static int compute_errors(gmi_ctx *c, SimHost *s, double *dst, int *big, int *bad) {
   int rc;
-  if ((rc = launch_ok(c, "errors"))) return rc;
+  if ((rc = enqueue_errors(c, s, dst, nullptr))) return rc;
   if ((rc = read_flags(c, s))) return rc;
   if (big) *big = s->h_flags[F_BIG];
   if (bad) *bad = s->h_flags[F_NAN];
   return GMI_OK;
 }
 
-static int jacobian_inv_mult(gmi_ctx *c, SimHost *s, const double *in, double *out) {
+static int jacobian_inv_mult(gmi_ctx *c, SimHost *s, const double *in, double *out,
+                             const int *skip) {
   SimDev &d = s->d;
-  hipLaunchKernelGGL(k_jim1, dim3(d.Nf), dim3(64), 0, c->stream, d, in);
-  hipLaunchKernelGGL(k_jim_body, dim3(d.Nsw), dim3(256), 0, c->stream, d, in, out);
-  hipLaunchKernelGGL(k_jim2, dim3(d.Nf), dim3(256), 0, c->stream, d, in, out);
+  hipLaunchKernelGGL(k_jim1, dim3(d.Nf), dim3(64), 0, c->stream, d, in, skip);
+  hipLaunchKernelGGL(k_jim_body, dim3(d.Nsw), dim3(256), 0, c->stream, d, in, out, skip);
+  hipLaunchKernelGGL(k_jim2, dim3(d.Nf), dim3(256), 0, c->stream, d, in, out, skip);
   return launch_ok(c, "jacobian_inv_mult");
 }
 
@@ -2466,6 +2517,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   if ((rc = dalloc(c, s, &swR, d.Nsw))) return rc; d.sw_R = swR;
   if ((rc = dalloc(c, s, &d.flags, F_N))) return rc;
   s->flags = d.flags;
+  if ((rc = dalloc(c, s, &d.step_t, 1)) || (rc = dalloc(c, s, &d.step_nt, 1))) return rc;
   s->cols = d.maxit;
   if ((rc = dalloc(c, s, &d.Xm1, 3 * ls))) return rc;
   if ((rc = dalloc(c, s, &d.Xm2, 3 * ls))) return rc;
@@ -2520,6 +2572,7 @@ int gmi_sim_create(gmi_ctx *c, const gmi_sim_params *p) {
   const size_t nbr = size_t(d.nbroy);
   if ((rc = dalloc(c, s, &s->err, nbr))) return rc;
   if ((rc = dalloc(c, s, &s->nerr, nbr))) return rc;
+  s->err_first = s->err;
   if ((rc = dalloc(c, s, &s->upd, nbr))) return rc;
   if ((rc = dalloc(c, s, &s->C, nbr * p->max_broyden_iter))) return rc;
   // D holds MAX_BROYDEN_ITER columns plus two GMRES-refinement scratch vectors
@@ -2690,6 +2743,15 @@ int gmi_sim_set_graphs(gmi_ctx *c, int use_graphs, int chunk) {
   return GMI_OK;
 }
 
+int gmi_sim_graph_stats(gmi_ctx *c, int *captured_iterations, int *graphs) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (!captured_iterations || !graphs) return fail(c, GMI_ERR_ARG, "null output");
+  *captured_iterations = s->captured_iters;
+  *graphs = int(s->graphs.size() + s->outer_graphs.size());
+  return GMI_OK;
+}
+
 int gmi_sim_set_body_preconditioner(gmi_ctx *c, int kind) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
@@ -2832,15 +2894,19 @@ int gmi_initial_guess(gmi_ctx *c, int nt) {
   return launch_ok(c, "initial guess");
 }
 
+// forces at the step set on the device (set_step)
+static int forces_and_torques(gmi_ctx *c, SimHost *s, int perturb) {
+  hipLaunchKernelGGL(k_fil_forces, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, perturb);
+  hipLaunchKernelGGL(k_body_forces, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d);
+  return launch_ok(c, "forces");
+}
+
 int gmi_forces_and_torques(gmi_ctx *c, int nt) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
-  const double t = (nt + 1) * s->p.dt;
-  const int perturb = (nt == 0 && s->d.pert) ? 1 : 0;
-  hipLaunchKernelGGL(k_fil_forces, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, t,
-                     perturb);
-  hipLaunchKernelGGL(k_body_forces, dim3(s->d.Nsw), dim3(256), 0, c->stream, s->d);
-  return launch_ok(c, "forces");
+  int rc;
+  if ((rc = set_step(c, s, nt))) return rc;
+  return forces_and_torques(c, s, (nt == 0 && s->d.pert) ? 1 : 0);
 }
 
 int gmi_compute_velocities(gmi_ctx *c, int nt, int *gmres_iters) {
@@ -2856,7 +2922,9 @@ int gmi_compute_errors(gmi_ctx *c, int nt, int which, int *too_large, int *not_f
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   if (which != 0 && which != 1) return fail(c, GMI_ERR_ARG, "which must be 0 or 1");
-  return compute_errors(c, s, nt, which ? s->nerr : s->err, too_large, not_finite);
+  int rc;
+  if ((rc = set_step(c, s, nt))) return rc;
+  return compute_errors(c, s, which ? s->nerr : s->err, too_large, not_finite);
 }
 
 int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
@@ -2864,7 +2932,8 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   SimDev &d = s->d;
   const double t = (nt + 1) * s->p.dt;
-  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0);
+  hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, c->stream, d.flags + F_SING, 1, 0,
+                     nullptr);
   hipLaunchKernelGGL(k_fil_jacobian, dim3(d.Nf), dim3(64), 0, c->stream, d, nt, t);
   if (6 * d.S <= kGjRegMax && s->gj_pivot_always) {
     hipLaunchKernelGGL(k_gj_invert_reg, dim3(d.Nf), dim3(256), 0, c->stream, d, nullptr);
@@ -2889,12 +2958,11 @@ int gmi_prepare_jacobian_inv(gmi_ctx *c, int nt) {
   return GMI_OK;
 }
 
-int gmi_broyden_find_update(gmi_ctx *c) {
-  SimHost *s = c ? sim_of(c) : nullptr;
-  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+// -H_k e: -gamma J^-1 e - sum_i (D_i . e) C_i (broyden_solver.cpp:26-67)
+static int find_update(gmi_ctx *c, SimHost *s) {
   SimDev &d = s->d;
   int rc;
-  if ((rc = jacobian_inv_mult(c, s, s->err, s->upd))) return rc;
+  if ((rc = jacobian_inv_mult(c, s, s->err, s->upd, nullptr))) return rc;
   const int n = d.nbroy;
   if (s->iter > 0) {
     const DotShape ds = dot_shape(n);
@@ -2905,53 +2973,169 @@ int gmi_broyden_find_update(gmi_ctx *c) {
                          s->bd, nullptr);
     else
       hipLaunchKernelGGL(k_colsum, dim3(nblk(s->iter, 256)), dim3(256), 0, c->stream, s->pp,
-                         s->iter, ds.nb, s->bd);
+                         s->iter, ds.nb, s->bd, nullptr);
   }
   hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, s->upd,
-                     s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n);
+                     s->upd, s->p.jacobian_confidence, s->C, size_t(n), s->bd, s->iter, n,
+                     nullptr);
   return launch_ok(c, "find_update");
 }
 
-int gmi_apply_update(gmi_ctx *c) {
+int gmi_broyden_find_update(gmi_ctx *c) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  return find_update(c, s);
+}
+
+static int apply_update(gmi_ctx *c, SimHost *s) {
   hipLaunchKernelGGL(k_body_update, dim3(nblk(s->d.Nsw, 64)), dim3(64), 0, c->stream, s->d, s->upd);
   hipLaunchKernelGGL(k_fil_update, dim3(s->d.Nf), dim3(kFilLanes), 0, c->stream, s->d, s->upd);
   return launch_ok(c, "update");
 }
 
+int gmi_apply_update(gmi_ctx *c) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  return apply_update(c, s);
+}
+
+// The new Broyden pair C_k, D_k from err (e) and nerr (e_new)
+// (broyden_solver.cpp:70-130): C_k = -H_k e_new / |e_new - e|,
+// D_k = (e_new - e) / |e_new - e|; skip gates every kernel.
+static int enqueue_end_of_iter(gmi_ctx *c, SimHost *s, const int *skip) {
+  SimDev &d = s->d;
+  const int n = d.nbroy;
+  int rc;
+  if ((rc = jacobian_inv_mult(c, s, s->nerr, s->upd, skip))) return rc;
+  const int it = s->iter;
+  double *Cit = s->C + size_t(n) * it, *Dit = s->D + size_t(n) * it;
+  if (it > 0) {
+    const DotShape ds = dot_shape(n);
+    // k_pdots and k_colsum_wide return when their flag word is set
+    hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
+                       size_t(n), it, s->nerr, n, ds.R, s->pp, skip);
+    if (ds.wide)
+      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, s->pp, ds.nb, s->bd,
+                         skip);
+    else
+      hipLaunchKernelGGL(k_colsum, dim3(nblk(it, 256)), dim3(256), 0, c->stream, s->pp, it,
+                         ds.nb, s->bd, skip);
+  }
+  hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
+                     s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n, skip);
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Dit, 1.0, s->nerr,
+                     -1.0, s->err, n, skip);
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, Dit, nullptr, n, s->scal, 8, 9,
+                     skip);
+  hipLaunchKernelGGL(k_scale2, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, Dit, n, s->scal,
+                     8, skip);
+  return launch_ok(c, "end_of_iter");
+}
+
+// host side of end_of_iter: the new pair is kept when the residual was too
+// large and there is room for it
+static bool stores_pair(const SimHost *s, int too_large) {
+  return too_large && s->iter < s->p.max_broyden_iter - 1;
+}
+
 int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
   SimHost *s = c ? sim_of(c) : nullptr;
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
-  SimDev &d = s->d;
-  const int n = d.nbroy;
-  if (too_large && s->iter < s->p.max_broyden_iter - 1) {
+  if (stores_pair(s, too_large)) {
     int rc;
-    if ((rc = jacobian_inv_mult(c, s, s->nerr, s->upd))) return rc;
-    const int it = s->iter;
-    double *Cit = s->C + size_t(n) * it, *Dit = s->D + size_t(n) * it;
-    if (it > 0) {
-      const DotShape ds = dot_shape(n);
-      hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), sizeof(double) * ds.R, c->stream, s->D,
-                         size_t(n), it, s->nerr, n, ds.R, s->pp, nullptr);
-      if (ds.wide)
-        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, s->pp, ds.nb, s->bd,
-                           nullptr);
-      else
-        hipLaunchKernelGGL(k_colsum, dim3(nblk(it, 256)), dim3(256), 0, c->stream, s->pp, it,
-                           ds.nb, s->bd);
-    }
-    hipLaunchKernelGGL(k_broyden_combine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, s->upd,
-                       s->p.jacobian_confidence, s->C, size_t(n), s->bd, it, n);
-    hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Dit, 1.0, s->nerr,
-                       -1.0, s->err, n);
-    hipLaunchKernelGGL(k_norms, dim3(1), dim3(1024), 0, c->stream, Dit, nullptr, n, s->scal, 8, 9);
-    hipLaunchKernelGGL(k_scale2, dim3(nblk(n, 256)), dim3(256), 0, c->stream, Cit, Dit, n, s->scal,
-                       8);
-    if ((rc = launch_ok(c, "end_of_iter"))) return rc;
+    if ((rc = enqueue_end_of_iter(c, s, nullptr))) return rc;
     std::swap(s->err, s->nerr);
   }
   s->iter++;
+  return GMI_OK;
+}
+
+// ---- captured Broyden iterations ------------------------------------------
+// One graph per (Broyden index, which buffer err is, precision, preconditioner)
+// holds a whole iteration of cilia_sim_main.cpp:266-314: find_update, the
+// state update, the forces, the velocities up to the first GMRES chunk, and,
+// gated on that chunk having converged (F_HOLD), the rest of the solve, the
+// segment velocities, the residual and the new Broyden pair (gated on the
+// residual being finite and too large).  The host reads the flags once per
+// iteration; a solve that needs more than the first chunk is finished with
+// eager launches of the same kernels.  The step's time and index are read on
+// the device, so the graphs serve every step.  Captured when GMRES runs in
+// FP64 (plain FP64, or under the exact body-frame preconditioner) without the
+// fitted guess (whose history bookkeeping is on the host), from the second
+// step on (the first step's forces carry the nt = 0 perturbations).
+static bool iteration_capturable(gmi_ctx *c, SimHost *s, int nt) {
+  return s->use_graphs && !c->profiling && s->hist_m == 0 && !(nt == 0 && s->d.pert) &&
+         (s->precision == GMI_FP64 || s->dense != nullptr);
+}
+
+static int gmres_maxit(const SimHost *s) {
+  return std::min(s->d.maxit, 3 * s->d.Nb + 6 * s->d.Nsw);
+}
+
+static int enqueue_iteration(gmi_ctx *c, SimHost *s) {
+  SimDev &d = s->d;
+  int rc;
+  if ((rc = find_update(c, s))) return rc;
+  if ((rc = apply_update(c, s))) return rc;
+  if ((rc = forces_and_torques(c, s, 0))) return rc;
+  if ((rc = velocities_before_solve(c, s))) return rc;
+  hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
+  SimOp op{c, s};
+  const int maxit = gmres_maxit(s);
+  const double tol = s->p.gmres_tol;
+  if ((rc = gmres_begin(c, s, op, d.npad, tol))) return rc;
+  if ((rc = gmres_enqueue(c, s, op, d.npad, maxit, false, tol, 1, std::min(maxit, s->chunk))))
+    return rc;
+  hipLaunchKernelGGL(k_hold_unsolved, dim3(1), dim3(1), 0, c->stream, d.flags);
+  const int *hold = d.flags + F_HOLD;
+  if ((rc = gmres_finish(c, s, op, d.npad, hold))) return rc;
+  if ((rc = velocities_after_solve(c, s, hold))) return rc;
+  if ((rc = enqueue_errors(c, s, s->nerr, hold))) return rc;
+  if (s->iter < s->p.max_broyden_iter - 1) {
+    hipLaunchKernelGGL(k_hold_unless_big, dim3(1), dim3(1), 0, c->stream, d.flags);
+    if ((rc = enqueue_end_of_iter(c, s, hold))) return rc;
+  }
+  return launch_ok(c, "captured iteration");
+}
+
+// One Broyden iteration from a graph: *big, *bad from the residual, *its the
+// GMRES iterations.  Leaves the host side of end_of_iter to the caller.
+static int captured_iteration(gmi_ctx *c, SimHost *s, int *big, int *bad, int *its) {
+  int rc;
+  if (s->graph_gen != c->list_generation) {
+    s->drop_graphs();
+    s->graph_gen = c->list_generation;
+  }
+  const long long key = (static_cast<long long>(s->iter) << 3) | (s->err == s->err_first ? 1 : 0) |
+                        (s->precision == GMI_FP32_KAHAN ? 2 : 0) | (s->dense ? 4 : 0);
+  auto g = s->outer_graphs.find(key);
+  if (g == s->outer_graphs.end()) {
+    CapturedGraph cg;
+    if ((rc = capture_graph(c, [&]() { return enqueue_iteration(c, s); }, &cg))) return rc;
+    g = s->outer_graphs.emplace(key, cg).first;
+  }
+  if ((rc = hip_check(c, hipGraphLaunch(g->second.exec, c->stream), "graph launch"))) return rc;
+  s->captured_iters++;
+  if ((rc = read_flags(c, s))) return rc;
+  if (s->h_flags[F_DONE]) {
+    *its = s->h_flags[F_ITS];
+    *big = s->h_flags[F_BIG];
+    *bad = s->h_flags[F_NAN];
+  } else {
+    // the solve needs more than the first chunk: the rest of it and the
+    // kernels the graph held back, eagerly
+    SimOp op{c, s};
+    const int maxit = gmres_maxit(s);
+    const double tol = s->p.gmres_tol;
+    if ((rc = gmres_chunks(c, s, op, s->d.npad, maxit, false, tol, std::min(maxit, s->chunk) + 1,
+                           its)))
+      return rc;
+    if ((rc = gmres_finish(c, s, op, s->d.npad, nullptr))) return rc;
+    if ((rc = velocities_after_solve(c, s, nullptr))) return rc;
+    if ((rc = compute_errors(c, s, s->nerr, big, bad))) return rc;
+    if (!*bad && stores_pair(s, *big) && (rc = enqueue_end_of_iter(c, s, nullptr))) return rc;
+  }
+  s->last_gmres = *its;
   return GMI_OK;
 }
 
@@ -2966,23 +3150,32 @@ static int broyden_solve(gmi_ctx *c, SimHost *s, int nt, int max_iter, bool chec
   if ((rc = gmi_forces_and_torques(c, nt))) return rc;
   if ((rc = compute_velocities(c, s, &its))) return rc;
   *gmres_total += its;
-  if ((rc = compute_errors(c, s, nt, s->err, &big, &bad))) return rc;
+  if ((rc = compute_errors(c, s, s->err, &big, &bad))) return rc;
   *iters = 0;
   if (bad && check_first) return GMI_ERR_NOT_FINITE;
   if (big && (rc = gmi_prepare_jacobian_inv(c, nt))) return rc;
   s->iter = 0;
+  const bool captured = iteration_capturable(c, s, nt);
+  const int perturb = (nt == 0 && s->d.pert) ? 1 : 0;
   while (big && s->iter < max_iter) {
-    if ((rc = gmi_broyden_find_update(c))) return rc;
-    if ((rc = gmi_apply_update(c))) return rc;
-    if ((rc = gmi_forces_and_torques(c, nt))) return rc;
-    if ((rc = compute_velocities(c, s, &its))) return rc;
+    if (captured) {
+      if ((rc = captured_iteration(c, s, &big, &bad, &its))) return rc;
+    } else {
+      if ((rc = find_update(c, s))) return rc;
+      if ((rc = apply_update(c, s))) return rc;
+      if ((rc = forces_and_torques(c, s, perturb))) return rc;
+      if ((rc = compute_velocities(c, s, &its))) return rc;
+      if ((rc = compute_errors(c, s, s->nerr, &big, &bad))) return rc;
+      if (!bad && stores_pair(s, big) && (rc = enqueue_end_of_iter(c, s, nullptr))) return rc;
+    }
     *gmres_total += its;
-    if ((rc = compute_errors(c, s, nt, s->nerr, &big, &bad))) return rc;
     if (bad) {
       *iters = s->iter + 1;
       return GMI_ERR_NOT_FINITE;
     }
-    if ((rc = gmi_broyden_end_of_iter(c, big))) return rc;
+    // host side of end_of_iter
+    if (stores_pair(s, big)) std::swap(s->err, s->nerr);
+    s->iter++;
   }
   *iters = s->iter;
   return big ? GMI_ERR_NOT_CONVERGED : GMI_OK;

@@ -39,6 +39,10 @@ struct SimDev {
   double *Jw, *Jinv, *ecb, *Bblk, *fil_BA, *fil_b2;
   int *flags;
   int *gj_redo;  // [Nf] 1: the pivot-free inverse failed its check (pivoted rerun)
+  // the step being solved, set once per step on the device so that captured
+  // Broyden iterations serve every step: step_t[0] = (nt + 1) dt, step_nt[0] = nt
+  double *step_t;
+  int *step_nt;
 };
 
 
@@ -61,7 +65,9 @@ struct SimHost : KrylovState {
   double *rot = nullptr;  // rotated N_ref and (K^T K)^-1 per swimmer [Nsw][72]
   double *err = nullptr, *nerr = nullptr, *upd = nullptr, *C = nullptr, *D = nullptr,
          *bd = nullptr;
+  double *err_first = nullptr;  // the buffer err pointed at first (err and nerr swap)
   int iter = 0;
+  int captured_iters = 0;  // Broyden iterations run from captured graphs
   int precision = GMI_FP64;
   int last_gmres = 0;
   double iters_avg = 0.0;  // moving average of Broyden iterations per step

@@ -115,8 +115,18 @@ int gmi_get_broyden(gmi_ctx *ctx, int which, double *out);
 int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
 /* GMRES runs its Arnoldi steps in chunks of `chunk` iterations between host
  * convergence checks; with use_graphs each chunk is captured once as a HIP
- * graph and replayed (no reference counterpart).  Default: graphs on, 6. */
+ * graph and replayed (no reference counterpart).  Default: graphs on, 6.
+ * With use_graphs, when GMRES runs in FP64 (GMI_FP64, or under
+ * GMI_PRECOND_BODY_INVERSE) and the fitted guess is off, each Broyden
+ * iteration after the first residual (cilia_sim_main.cpp:266-314) is one
+ * captured graph too -- update, forces, velocities, the first GMRES chunk
+ * and, gated on the device, the rest of the solve, the residual and the
+ * Broyden pair -- and the host reads the flags once per iteration. */
 int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
+/* Broyden iterations run from captured graphs since the context was created,
+ * and the captured graphs the context holds (Arnoldi chunks and Broyden
+ * iterations).  No reference counterpart. */
+int gmi_sim_graph_stats(gmi_ctx *ctx, int *captured_iterations, int *graphs);
 /* GMRES initial guess of the rigid-body solve.  m = 0 (default): the
  * previous solution, as the reference (mobility_solver.cpp:1060-1156).
  * m in [1, 16]: in the body frame, where the system matrix is fixed, the new

@@ -227,6 +227,41 @@ def test_graph_replay_matches_eager(gpu):
         assert np.array_equal(se["x"], sg["x"]) and np.array_equal(se["u"], sg["u"])
     me, mg = ge.mobility(), gg.mobility()
     assert np.array_equal(me["f_blobs"], mg["f_blobs"])
+    # FP64 GMRES without the fitted guess: every Broyden iteration after the
+    # first residual ran from a captured graph (its solve finished eagerly
+    # past the graph's first chunk)
+    assert ge.graph_stats()[0] == 0 and gg.graph_stats()[0] > 0
+
+
+@pytest.mark.parametrize("precision", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
+def test_captured_broyden_iterations_match_eager(gpu, precision):
+    """Under the exact body-frame preconditioner (FP64 Krylov products, one
+    Arnoldi step per check) each Broyden iteration is one captured graph.
+    Its kernels are the eager ones in the same order, so iteration counts,
+    trajectory and solver state equal the eager run's bit for bit, including
+    the solves that need a second Arnoldi step (finished eagerly) and, in
+    mixed precision, the FP32 pair sums of the velocities."""
+    cfg = geometry.sim_workload("step_962", seed=1)
+    sims = []
+    for graphs in (False, True):
+        g = capi.Sim(cfg)
+        g.set_sim_precision(precision)
+        g.set_graphs(graphs)
+        g.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+        g.make_body_reference_matrices()
+        sims.append(g)
+    ge, gg = sims
+    for nt in range(8):
+        re_, rg = ge.step(nt), gg.step(nt)
+        print(f"step {nt}: eager {re_} captured {rg} graph stats {gg.graph_stats()}", flush=True)
+        assert re_ == rg and re_[0] == 0
+        se, sg = ge.state(), gg.state()
+        for k in ("x", "u", "lam"):
+            assert np.array_equal(se[k], sg[k]), k
+    assert np.array_equal(ge.mobility()["f_blobs"], gg.mobility()["f_blobs"])
+    assert np.array_equal(ge.broyden(2), gg.broyden(2))
+    n_it, n_graphs = gg.graph_stats()
+    assert ge.graph_stats()[0] == 0 and n_it > 0 and n_graphs > 0
 
 
 def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
