@@ -38,7 +38,6 @@ int hip_check(gmi_ctx *c, hipError_t e, const char *what) {
 int pad64(int n) { return ((n + 63) / 64) * 64; }
 
 // Build the work list of one block mask.
-constexpr int kBarrierTargetBlocks = 1024, kBarrierMinChunk = 64;
 
 static int build_list(gmi_ctx *c, int mask, WorkList &wl, int target_blocks, int min_chunk) {
   wl.h_tiles.clear();
@@ -126,15 +125,6 @@ int rebuild_lists(gmi_ctx *c) {
     if ((rc = build_list(c, mask, c->lists[mask], c->target_blocks, c->min_chunk))) return rc;
     slots = std::max(slots, size_t(c->lists[mask].nslots));
   }
-  // the barrier visits every source of the target's group (the all-blocks
-  // list) with a shape of its own: a distance test per pair is far lighter
-  // than a mobility pair, so it wants short source splits on small systems
-  // (64-source chunks: 17.5 against 39.5 us at 962 particles with 256) but
-  // few partials to reduce on large ones (about 1,024 workgroups)
-  if ((rc = build_list(c, kSS | kSB | kBS | kBB, c->barrier_tiles, kBarrierTargetBlocks,
-                       kBarrierMinChunk)))
-    return rc;
-  slots = std::max(slots, size_t(c->barrier_tiles.nslots));
   if (slots > c->part_slots) {
     if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
@@ -330,6 +320,38 @@ int gmi_create(gmi_ctx **out, const gmi_params *p) {
   if ((rc = dalloc(&c->Fb, 3 * size_t(c->ldb)))) return rc;
   if ((rc = dalloc(&c->Vb, 3 * size_t(c->ldb)))) return rc;
   if ((rc = dalloc(&c->Frep, 3 * size_t(c->ldb)))) return rc;
+  {
+    // the barrier's cell grid: one to two buckets per particle, cells as
+    // wide as the largest contact distance 1.1 (a_i + a_j)
+    CellGrid &G = c->cells;
+    const int N = c->Ns + c->Nb;
+    G.H = kScanChunk;
+    while (G.H < N) G.H *= 2;
+    G.inv_h = 1.0 / (1.1 * 2.0 * std::max(c->as, c->ab));
+    std::vector<int> grp(std::max(N, 1), 0);
+    if (!c->interact)
+      for (int g = 0; g < c->nsw; ++g) {
+        for (int k = 0; k < c->nseg[g]; ++k) grp[c->seg_off[g] + k] = g;
+        for (int k = 0; k < c->nblob[g]; ++k) grp[c->Ns + c->blob_off[g] + k] = g;
+      }
+    int *gp = nullptr;
+    auto ialloc = [&](int **ptr, size_t n) {
+      return hip_check(c, hipMalloc(ptr, sizeof(int) * std::max<size_t>(n, 1)), "hipMalloc");
+    };
+    if ((rc = ialloc(&G.cnt, G.H)) || (rc = ialloc(&G.fill, G.H)) ||
+        (rc = ialloc(&G.start, G.H)) || (rc = ialloc(&G.boff, G.H / kScanChunk)) ||
+        (rc = ialloc(&G.ent, N)) || (rc = ialloc(&G.bucket, N)) || (rc = ialloc(&gp, N)) ||
+        (rc = hip_check(c, hipMalloc(&G.key, sizeof(int4) * std::max(N, 1)), "hipMalloc")) ||
+        (rc = hip_check(c, hipMalloc(&G.rk, sizeof(int4) * std::max(N, 1)), "hipMalloc")) ||
+        (rc = dalloc(&G.rx, N)) || (rc = dalloc(&G.ry, N)) || (rc = dalloc(&G.rz, N)))
+      return rc;
+    G.grp = gp;
+    if ((rc = hip_check(c, hipMemset(G.cnt, 0, sizeof(int) * G.H), "memset")) ||
+        (rc = hip_check(c, hipMemset(G.fill, 0, sizeof(int) * G.H), "memset")) ||
+        (rc = hip_check(c, hipMemcpy(gp, grp.data(), sizeof(int) * grp.size(),
+                                     hipMemcpyHostToDevice), "upload groups")))
+      return rc;
+  }
   // zero every vector once so that partial uploads never expose garbage
   double *bufs[] = {c->Xs, c->Fs, c->Vs, c->Xb, c->Fb, c->Vb, c->Frep};
   size_t lens[] = {3u * c->lds, 6u * c->lds, 6u * c->lds, 3u * c->ldb,
@@ -366,8 +388,13 @@ void gmi_destroy(gmi_ctx *c) {
     if (kv.second.d_tiles) (void)hipFree(kv.second.d_tiles);
     if (kv.second.d_work) (void)hipFree(kv.second.d_work);
   }
-  if (c->barrier_tiles.d_tiles) (void)hipFree(c->barrier_tiles.d_tiles);
-  if (c->barrier_tiles.d_work) (void)hipFree(c->barrier_tiles.d_work);
+  {
+    CellGrid &G = c->cells;
+    void *cb[] = {G.cnt, G.fill, G.start, G.boff, G.ent, G.bucket, G.key, G.rk, G.rx, G.ry, G.rz,
+                  const_cast<int *>(G.grp)};
+    for (void *b : cb)
+      if (b) (void)hipFree(b);
+  }
   for (auto &st : c->stage) {
     if (st.pinned) (void)hipHostFree(st.pinned);
     if (st.dev) (void)hipFree(st.dev);
@@ -426,9 +453,8 @@ int gmi_copy_segment_velocities_to_host(gmi_ctx *c, double *v) {
 int gmi_apply_interparticle_forces(gmi_ctx *c, double K) {
   if (!c) return GMI_ERR_ARG;
   if (!std::isfinite(K)) return fail(c, GMI_ERR_ARG, "barrier K not finite");
-  const WorkList &wl = c->barrier_tiles;
-  return hip_check(c, launch_barrier(c->Fs, c->lds, c->Frep, c->ldb, c->Xs, c->Xb, wl, c->part,
-                                     c->nseg_fil, c->as, c->ab, K, c->stream),
+  return hip_check(c, launch_barrier(c->Fs, c->lds, c->Frep, c->ldb, c->Xs, c->Xb, c->Ns, c->Nb,
+                                     c->cells, c->nseg_fil, c->as, c->ab, K, c->stream),
                    "barrier launch");
 }
 

@@ -48,7 +48,7 @@ struct gmi_ctx {
   int list_generation = 0;  // bumped whenever the work lists are rebuilt
 
   std::map<int, gmi::WorkList> lists;
-  gmi::WorkList barrier_tiles;
+  gmi::CellGrid cells{};  // the barrier's cell grid (device buffers)
   Stage stage[kNumStage];
   std::vector<Pending> pending;
   std::string err;

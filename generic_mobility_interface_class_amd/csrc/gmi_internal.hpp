@@ -68,11 +68,26 @@ hipError_t launch_pair_dyn(const PairArgs &a, int nwork, int ntiles_reduce, hipS
 // Segment-segment mobility above a no-slip wall at z = 0 (FP64; kSS lists).
 hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hipStream_t s,
                             hipEvent_t ev_mid = nullptr);
-// Barrier over the all-sources work list `wl` (split partials in `part`).
-hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb,
-                          const double *Xs, const double *Xb, const WorkList &wl,
-                          double *part, int nseg_per_fil, double a_s, double a_b, double K,
-                          hipStream_t s);
+// Hashed cell grid of the steric barrier: H buckets (a power of two and a
+// multiple of kScanChunk), cells of side 1/inv_h, per-particle group (the
+// swimmer when swimmers are independent, else 0).  cnt and fill must be zero
+// before a build; every build leaves them zero.
+constexpr int kScanChunk = 4096;
+struct CellGrid {
+  int *cnt, *fill, *start, *boff, *ent, *bucket;
+  int4 *key;        // per particle: cell x, y, z and group
+  // per bucket entry, in bucket order: position, cell and group
+  double *rx, *ry, *rz;
+  int4 *rk;
+  const int *grp;
+  int H;
+  double inv_h;
+};
+// Barrier forces of every particle (Ns segments, then Nb blobs): segments add
+// theirs to Fs, blobs write theirs to Frep.
+hipError_t launch_barrier(double *Fs, int lds, double *Frep_b, int ldb, const double *Xs,
+                          const double *Xb, int Ns, int Nb, const CellGrid &G,
+                          int nseg_per_fil, double a_s, double a_b, double K, hipStream_t s);
 
 // C = alpha op(A) B + beta C, column-major FP64 on the matrix cores
 // (gmi_gemm.hip); op(A) = A (m x k) or, with transA, A^T (A is k x m).

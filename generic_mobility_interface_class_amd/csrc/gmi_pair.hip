@@ -24,6 +24,7 @@
 // compensated sums across tiles; partial sums across source splits are
 // reduced in FP64.
 #include "gmi_internal.hpp"
+#include <climits>
 #include "gmi_wave.hpp"
 
 namespace gmi {
@@ -817,10 +818,17 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
   *v = a.accumulate ? *v + s : s;
 }
 
-// Steric barrier (cuda_functions.cu:961-1157).  One lane per target; the
-// target's group sources are staged in LDS tiles shared by the workgroup.
-// The contact test compares squared distances, so the square root and the
-// division run only for the few pairs inside 1.1 (a_i + a_j).
+// Steric barrier (cuda_functions.cu:961-1157) over a hashed cell grid.
+// Cells are cubes of side h >= 1.1 (a_i + a_j) for every radius pair, so a
+// contributing pair lies in the same or adjacent cells.  Each call bins every
+// particle (segments first, then blobs, in one index space) by its cell and
+// swimmer group into H buckets (a counting sort: integer atomics for the
+// counts, then each bucket sorted by particle index), and each target visits
+// the 27 cells around its own in a fixed order, skipping entries of other
+// cells or groups that share a bucket.  Sums therefore run in a fixed order
+// (cell by cell of the 27, index order within a cell).  The contact test compares
+// squared distances, so the square root and the division run only for the
+// pairs inside 1.1 (a_i + a_j).
 __device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, double asum,
                                              double K, double &fx, double &fy, double &fz) {
   const double d2 = dx * dx + dy * dy + dz * dz;
@@ -834,121 +842,194 @@ __device__ __forceinline__ void barrier_pair(double dx, double dy, double dz, do
   }
 }
 
-// Work item = (target tile, source split) of the all-sources work list;
-// split partials are summed by k_barrier_reduce.  A pair contributes only
-// inside 1.1 (a_i + a_j), so each wave skips the 64-source slices of a
-// staged tile whose bounding box lies farther than that from the bounding
-// box of its own 64 targets (wave-uniform tests; a skipped pair would have
-// added exactly nothing, so the sums are those of the full loop).
-__global__ __launch_bounds__(kBlock) void k_barrier(
-    double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
-    const double *__restrict__ Xs, const double *__restrict__ Xb,
-    const TileDesc *__restrict__ tiles, const WorkItem *__restrict__ work,
-    double *__restrict__ part, int nseg_fil, double as, double ab, double K) {
-  static_assert(kTile == kBlock && kBlock % 64 == 0, "one 64-source slice per wave");
-  constexpr int kSlices = kTile / 64;
-  __shared__ double sx[kTile], sy[kTile], sz[kTile];
-  __shared__ double sbox[kSlices][6];  // per slice: min x y z, max x y z
-  const WorkItem w = work[blockIdx.x];
-  const TileDesc T = tiles[w.tile];
-  const int tid = threadIdx.x, wave = tid >> 6;
-  const bool seg_t = (T.species == 0);
-  const int i = T.t0 + (tid < T.nt ? tid : 0);
-  const double *X = seg_t ? Xs : Xb;
-  const int ld = seg_t ? lds : ldb;
-  const double xi = X[i], yi = X[ld + i], zi = X[2 * ld + i];
-  // this wave's targets' box (lanes past the tile repeat its first target)
-  const double tlo[3] = {wave_min(xi), wave_min(yi), wave_min(zi)};
-  const double thi[3] = {wave_max(xi), wave_max(yi), wave_max(zi)};
-  // excluded sources: a segment itself and its neighbours in the same
-  // filament (:1040) -- the index range [lo, hi]; a blob itself (:1120)
-  int lo = i, hi = i;
-  if (seg_t) {
-    if (i % nseg_fil != 0) lo = i - 1;
-    if ((i + 1) % nseg_fil != 0) hi = i + 1;
-  }
-  const long L = long(T.nsseg) + T.nsblob;
-  const int beg = int(L * w.split / T.nsplit);
-  const int end = int(L * (w.split + 1) / T.nsplit);
-  double fx = 0, fy = 0, fz = 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    const bool src_seg = (pass == 0);
-    const double *S = src_seg ? Xs : Xb;
-    const int sld = src_seg ? lds : ldb;
-    const int r0 = src_seg ? beg : max(beg, T.nsseg) - T.nsseg;
-    const int r1 = src_seg ? min(end, T.nsseg) : end - T.nsseg;
-    const int j0 = (src_seg ? T.sseg0 : T.sblob0) + r0;
-    const int j1 = (src_seg ? T.sseg0 : T.sblob0) + r1;
-    const bool same = (src_seg == seg_t);
-    const double asum = src_seg ? (seg_t ? 2.0 * as : as + ab) : (seg_t ? as + ab : 2.0 * ab);
-    // boxes farther apart than the contact distance (with a relative margin
-    // far above rounding) hold no contributing pair
-    const double cut = 1.1 * asum * (1.0 + 1e-9), cut2 = cut * cut;
-    for (int base = j0; base < j1; base += kTile) {
-      const int n = min(kTile, j1 - base);
-      __syncthreads();
-      {
-        // lanes past the tile repeat its last source
-        const int j = base + min(tid, n - 1);
-        const double px = S[j], py = S[sld + j], pz = S[2 * sld + j];
-        if (tid < n) { sx[tid] = px; sy[tid] = py; sz[tid] = pz; }
-        const double b0 = wave_min(px), b1 = wave_min(py), b2 = wave_min(pz);
-        const double b3 = wave_max(px), b4 = wave_max(py), b5 = wave_max(pz);
-        if ((tid & 63) == 0) {
-          sbox[wave][0] = b0; sbox[wave][1] = b1; sbox[wave][2] = b2;
-          sbox[wave][3] = b3; sbox[wave][4] = b4; sbox[wave][5] = b5;
-        }
-      }
-      __syncthreads();
-      for (int sl = 0; sl < kSlices && 64 * sl < n; ++sl) {
-        double gap2 = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double g = fmax(0.0, fmax(sbox[sl][c] - thi[c], tlo[c] - sbox[sl][3 + c]));
-          gap2 += g * g;
-        }
-        if (gap2 > cut2) continue;
-        const int jj1 = min(n, 64 * sl + 64);
-        for (int jj = 64 * sl; jj < jj1; ++jj) {
-          const int j = base + jj;
-          if (same && j >= lo && j <= hi) continue;
-          barrier_pair(xi - sx[jj], yi - sy[jj], zi - sz[jj], asum, K, fx, fy, fz);
-        }
-      }
-    }
-  }
-  const bool live = tid < T.nt;
-  if (T.nsplit == 1) {
-    if (!live) return;
-    if (seg_t) {
-      Fs[i] += fx; Fs[lds + i] += fy; Fs[2 * lds + i] += fz;
-    } else {
-      Frep[i] = fx; Frep[ldb + i] = fy; Frep[2 * ldb + i] = fz;
-    }
-    return;
-  }
-  if (live) {
-    double *P = part + size_t(T.part0 + w.split) * 6 * kBlock + tid;
-    P[0] = fx; P[kBlock] = fy; P[2 * kBlock] = fz;
+__device__ __forceinline__ int cell_bucket(int cx, int cy, int cz, int g, int H) {
+  const unsigned h = (unsigned(cx) * 73856093u) ^ (unsigned(cy) * 19349663u) ^
+                     (unsigned(cz) * 83492791u) ^ (unsigned(g) * 2654435761u);
+  return int(h & unsigned(H - 1));
+}
+
+__device__ __forceinline__ void particle_pos(int p, const double *Xs, int lds, const double *Xb,
+                                             int ldb, int Ns, double &x, double &y, double &z) {
+  if (p < Ns) {
+    x = Xs[p]; y = Xs[lds + p]; z = Xs[2 * lds + p];
+  } else {
+    const int q = p - Ns;
+    x = Xb[q]; y = Xb[ldb + q]; z = Xb[2 * ldb + q];
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_barrier_reduce(
-    double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
-    const TileDesc *__restrict__ tiles, const double *__restrict__ part) {
-  const TileDesc T = tiles[blockIdx.x];
-  const int tid = threadIdx.x;
-  if (T.nsplit == 1 || tid >= T.nt) return;
-  const int i = T.t0 + tid;
-  double f[3] = {0.0, 0.0, 0.0};
-  const double *P = part + size_t(T.part0) * 6 * kBlock + tid;
-#pragma unroll 4
-  for (int q = 0; q < T.nsplit; ++q)
-    for (int c = 0; c < 3; ++c) f[c] += P[size_t(q) * 6 * kBlock + c * kBlock];
-  if (T.species == 0) {
+// cell and group of every particle; bucket counts
+__global__ __launch_bounds__(256) void k_cell_key(CellGrid G, const double *Xs, int lds,
+                                                  const double *Xb, int ldb, int Ns, int N) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  double x, y, z;
+  particle_pos(p, Xs, lds, Xb, ldb, Ns, x, y, z);
+  const int4 k = make_int4(int(floor(x * G.inv_h)), int(floor(y * G.inv_h)),
+                           int(floor(z * G.inv_h)), G.grp[p]);
+  G.key[p] = k;
+  const int b = cell_bucket(k.x, k.y, k.z, k.w, G.H);
+  G.bucket[p] = b;
+  atomicAdd(&G.cnt[b], 1);
+}
+
+// exclusive scan of kScanChunk bucket counts per workgroup; chunk totals to boff
+__global__ __launch_bounds__(1024) void k_cell_scan_local(CellGrid G) {
+  static_assert(kScanChunk == 4 * 1024, "four buckets per thread");
+  __shared__ int sh[1024];
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * kScanChunk + 4 * t;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = G.cnt[base + k];
+    s += v[k];
+  }
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int add = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  int e = sh[t] - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    G.start[base + k] = e;
+    e += v[k];
+  }
+  if (t == 1023) G.boff[blockIdx.x] = sh[1023];
+}
+
+// exclusive scan of the chunk totals (at most a few hundred; one thread)
+__global__ void k_cell_scan_top(CellGrid G) {
+  int acc = 0;
+  for (int b = 0; b < G.H / kScanChunk; ++b) {
+    const int v = G.boff[b];
+    G.boff[b] = acc;
+    acc += v;
+  }
+}
+
+__device__ __forceinline__ int bucket_begin(const CellGrid &G, int b) {
+  return G.start[b] + G.boff[b / kScanChunk];
+}
+__device__ __forceinline__ int bucket_end(const CellGrid &G, int b, int N) {
+  return b + 1 < G.H ? bucket_begin(G, b + 1) : N;
+}
+
+// every particle into its bucket; the counts are not needed past the scan
+// and are cleared for the next build
+__global__ __launch_bounds__(256) void k_cell_scatter(CellGrid G, int N) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  const int b = G.bucket[p];
+  G.ent[bucket_begin(G, b) + atomicAdd(&G.fill[b], 1)] = p;
+  G.cnt[b] = 0;
+}
+
+constexpr int kSortRegs = 8;
+// each bucket's entries into index order (buckets hold a few particles),
+// then the bucket's records (position, cell and group, index) in that order;
+// the fill counters are cleared for the next build
+__global__ __launch_bounds__(256) void k_cell_sort(CellGrid G, const double *Xs, int lds,
+                                                   const double *Xb, int ldb, int Ns, int N) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= G.H) return;
+  const int s = bucket_begin(G, b), e = bucket_end(G, b, N);
+  G.fill[b] = 0;
+  if (e - s <= kSortRegs) {
+    // short buckets (nearly all): sorted in registers
+    int v[kSortRegs];
+#pragma unroll
+    for (int k = 0; k < kSortRegs; ++k) v[k] = s + k < e ? G.ent[s + k] : INT_MAX;
+#pragma unroll
+    for (int a = 1; a < kSortRegs; ++a)
+#pragma unroll
+      for (int k = kSortRegs - 1; k >= a; --k)
+        if (v[k - 1] > v[k]) { const int w = v[k]; v[k] = v[k - 1]; v[k - 1] = w; }
+#pragma unroll
+    for (int k = 0; k < kSortRegs; ++k)
+      if (s + k < e) {
+        const int p = v[k];
+        G.ent[s + k] = p;
+        particle_pos(p, Xs, lds, Xb, ldb, Ns, G.rx[s + k], G.ry[s + k], G.rz[s + k]);
+        G.rk[s + k] = G.key[p];
+      }
+  } else {
+    for (int i = s + 1; i < e; ++i) {
+      const int v = G.ent[i];
+      int k = i - 1;
+      while (k >= s && G.ent[k] > v) {
+        G.ent[k + 1] = G.ent[k];
+        --k;
+      }
+      G.ent[k + 1] = v;
+    }
+    for (int i = s; i < e; ++i) {
+      const int p = G.ent[i];
+      particle_pos(p, Xs, lds, Xb, ldb, Ns, G.rx[i], G.ry[i], G.rz[i]);
+      G.rk[i] = G.key[p];
+    }
+  }
+}
+
+// 27 threads per target particle, one per cell of the 3 x 3 x 3 block around
+// its own (thread t of a workgroup takes cell t / kCellTargets for target
+// t % kCellTargets); the cells' forces are summed in cell order.  Segments
+// add the barrier force to their device force (translation rows), blobs
+// write their repulsion force.
+constexpr int kCellTargets = 32;
+__global__ __launch_bounds__(27 * kCellTargets) void k_barrier_cells(
+    CellGrid G, double *__restrict__ Fs, int lds, double *__restrict__ Frep, int ldb,
+    const double *__restrict__ Xs, const double *__restrict__ Xb, int Ns, int N, int nseg_fil,
+    double as, double ab, double K) {
+  __shared__ double sf[27][3][kCellTargets];
+  const int lane = threadIdx.x % kCellTargets, cell = threadIdx.x / kCellTargets;
+  const int i0 = blockIdx.x * kCellTargets + lane;
+  const bool live = i0 < N;
+  const int i = live ? i0 : N - 1;
+  const bool seg_i = i < Ns;
+  double xi, yi, zi;
+  particle_pos(i, Xs, lds, Xb, ldb, Ns, xi, yi, zi);
+  const int4 ki = G.key[i];
+  // excluded sources: a segment itself and its neighbours in the same
+  // filament (:1040); a blob itself (:1120)
+  int lo = i, hi = i;
+  if (seg_i) {
+    if (i % nseg_fil != 0) lo = i - 1;
+    if ((i + 1) % nseg_fil != 0) hi = i + 1;
+  }
+  const double ai = seg_i ? as : ab;
+  const int cx = ki.x + cell % 3 - 1, cy = ki.y + (cell / 3) % 3 - 1, cz = ki.z + cell / 9 - 1;
+  const int b = cell_bucket(cx, cy, cz, ki.w, G.H);
+  const int e1 = bucket_end(G, b, N);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int e = bucket_begin(G, b); e < e1; ++e) {
+    const int4 kj = G.rk[e];
+    if (kj.x != cx || kj.y != cy || kj.z != cz || kj.w != ki.w) continue;
+    const int j = G.ent[e];
+    const bool seg_j = j < Ns;
+    if (seg_i == seg_j && j >= lo && j <= hi) continue;
+    barrier_pair(xi - G.rx[e], yi - G.ry[e], zi - G.rz[e], ai + (seg_j ? as : ab), K, fx, fy, fz);
+  }
+  sf[cell][0][lane] = fx;
+  sf[cell][1][lane] = fy;
+  sf[cell][2][lane] = fz;
+  __syncthreads();
+  if (cell != 0 || !live) return;
+  double f[3];
+  for (int c = 0; c < 3; ++c) {
+    double t = 0.0;
+    for (int r = 0; r < 27; ++r) t += sf[r][c][lane];
+    f[c] = t;
+  }
+  if (seg_i) {
     Fs[i] += f[0]; Fs[lds + i] += f[1]; Fs[2 * lds + i] += f[2];
   } else {
-    Frep[i] = f[0]; Frep[ldb + i] = f[1]; Frep[2 * ldb + i] = f[2];
+    const int q = i - Ns;
+    Frep[q] = f[0]; Frep[ldb + q] = f[1]; Frep[2 * ldb + q] = f[2];
   }
 }
 
@@ -1013,17 +1094,21 @@ hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hip
   return e;
 }
 
-hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb,
-                          const double *Xs, const double *Xb, const WorkList &wl,
-                          double *part, int nseg_fil, double as, double ab, double K,
-                          hipStream_t s) {
-  if (wl.nwork <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_barrier, dim3(wl.nwork), dim3(kBlock), 0, s, Fs, lds, Frep, ldb, Xs, Xb,
-                     wl.d_tiles, wl.d_work, part, nseg_fil, as, ab, K);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || wl.nsplit_tiles == 0) return e;
-  hipLaunchKernelGGL(k_barrier_reduce, dim3(wl.ntiles), dim3(kBlock), 0, s, Fs, lds, Frep, ldb,
-                     wl.d_tiles, part);
+hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb, const double *Xs,
+                          const double *Xb, int Ns, int Nb, const CellGrid &G, int nseg_fil,
+                          double as, double ab, double K, hipStream_t s) {
+  const int N = Ns + Nb;
+  if (N <= 0) return hipSuccess;
+  const int nb = (N + 255) / 256;
+  hipLaunchKernelGGL(k_cell_key, dim3(nb), dim3(256), 0, s, G, Xs, lds, Xb, ldb, Ns, N);
+  hipLaunchKernelGGL(k_cell_scan_local, dim3(G.H / kScanChunk), dim3(1024), 0, s, G);
+  hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(1), 0, s, G);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, G, N);
+  hipLaunchKernelGGL(k_cell_sort, dim3((G.H + 255) / 256), dim3(256), 0, s, G, Xs, lds, Xb, ldb,
+                     Ns, N);
+  hipLaunchKernelGGL(k_barrier_cells, dim3((N + kCellTargets - 1) / kCellTargets),
+                     dim3(27 * kCellTargets), 0, s, G, Fs, lds, Frep, ldb, Xs, Xb, Ns, N, nseg_fil,
+                     as, ab, K);
   return hipGetLastError();
 }
 
