@@ -156,6 +156,38 @@ def test_barrier_parity(gpu):
     assert relerr(vs, orc.Mss(xs, f_ref, 1.0)) <= 1e-12
 
 
+@pytest.mark.parametrize("interact", [True, False])
+def test_barrier_two_swimmers_parity(gpu, interact):
+    """Two swimmers filling the same region: coupled, every particle pair
+    counts; independent, only pairs inside one swimmer (the cell grid keys
+    buckets by swimmer).  Against the oracle's barrier over the union, or
+    over each swimmer alone."""
+    rng = np.random.default_rng(11)
+    nseg, nfil, nb = 6, (7, 5), (60, 45)
+    K = 2.5
+    xs = [rng.uniform(-4, 4, (nseg * nf, 3)) for nf in nfil]
+    xb = [rng.uniform(-4, 4, (n, 3)) for n in nb]
+    fs = [rng.normal(size=(nseg * nf, 6)) for nf in nfil]
+    ctx = capi.Context([nseg * nf for nf in nfil], list(nb), nseg, a_blob=1.2, interact=interact)
+    ctx.upload(np.vstack(xs), np.vstack(fs), np.vstack(xb), np.zeros((sum(nb), 3)))
+    ctx.call("gmi_apply_interparticle_forces", K)
+    rep = np.zeros(3 * sum(nb))
+    ctx.call("gmi_copy_interparticle_blob_forces_to_host", capi.ptr(rep))
+    ctx.call("gmi_wait_for_device")
+    ctx.call("gmi_evaluate_segment_segment_mobility")
+    vs, _ = ctx.velocities()
+    if interact:
+        f_ref, rep_ref = orc.barrier(np.vstack(xs), np.vstack(xb), np.vstack(fs), nseg, 1.0, 1.2, K)
+        v_ref = orc.Mss(np.vstack(xs), f_ref, 1.0)
+    else:
+        parts = [orc.barrier(xs[k], xb[k], fs[k], nseg, 1.0, 1.2, K) for k in range(2)]
+        rep_ref = np.vstack([p[1] for p in parts])
+        v_ref = np.vstack([orc.Mss(xs[k], parts[k][0], 1.0) for k in range(2)])
+    assert np.abs(rep_ref).max() > 0
+    assert relerr(rep.reshape(-1, 3), rep_ref) <= 1e-13
+    assert relerr(vs, v_ref) <= 1e-12
+
+
 _REF_46722 = {}
 
 
