@@ -868,8 +868,9 @@ __global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *
 __global__ __launch_bounds__(256) void k_hist_dots(SimDev a, const BTile *tiles,
                                                    const int *sw_tile0, const double *rhs,
                                                    double *bnew, const double *Hb, size_t ldh,
-                                                   int count, double *dpart) {
+                                                   const int *hst, double *dpart) {
   __shared__ double sh[4 * (kHistMax + 1)];
+  const int count = hst[0];
   const BTile T = tiles[blockIdx.x];
   const int t = threadIdx.x;
   const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
@@ -911,13 +912,14 @@ __global__ __launch_bounds__(256) void k_hist_dots(SimDev a, const BTile *tiles,
 
 // Per swimmer (one thread): g = sum of the tile shares (tile order), then
 // (G + lambda I) c = g by Cholesky, lambda = 1e-14 trace(G) / count.
-// A failed factorisation leaves c = 0 (no guess).  g and bnew . bnew are
-// kept for the Gram update after the solve.
+// A failed factorisation (or an empty history) leaves c = 0 (no guess).  g
+// and bnew . bnew are kept for the Gram update after the solve.
 __global__ void k_hist_solve(SimDev a, const int *sw_tile0, const int *sw_ntile,
-                             const double *dpart, const double *G, double *Gw, int count,
+                             const double *dpart, const double *G, double *Gw, const int *hst,
                              double *coef, double *gsave) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.Nsw) return;
+  const int count = hst[0];
   constexpr int M = kHistMax;
   double g[M + 1];
   for (int i = 0; i <= M; i++) {
@@ -930,7 +932,7 @@ __global__ void k_hist_solve(SimDev a, const int *sw_tile0, const int *sw_ntile,
   double *L = Gw + size_t(n) * M * M;
   double tr = 0.0;
   for (int i = 0; i < count; i++) tr += Gn[M * i + i];
-  const double lam = 1e-14 * tr / count;
+  const double lam = count > 0 ? 1e-14 * tr / count : 0.0;
   bool ok = tr > 0.0;
   for (int j = 0; j < count && ok; j++) {
     double d = Gn[M * j + j] + lam;
@@ -961,11 +963,14 @@ __global__ void k_hist_solve(SimDev a, const int *sw_tile0, const int *sw_ntile,
   for (int i = 0; i < M; i++) coef[M * size_t(n) + i] = ok ? c[i] : 0.0;
 }
 
-// Per tile: x = R sum_i c_i Hx[i] (the guess, rotated back to the lab frame).
+// Per tile: x = R sum_i c_i Hx[i] (the guess, rotated back to the lab frame);
+// with an empty history x stays the previous solution.
 __global__ __launch_bounds__(256) void k_hist_combine(SimDev a, const BTile *tiles,
                                                       const int *sw_tile0, const double *Hx,
-                                                      size_t ldh, int count, const double *coef,
-                                                      double *x) {
+                                                      size_t ldh, const int *hst,
+                                                      const double *coef, double *x) {
+  const int count = hst[0];
+  if (count == 0) return;
   const BTile T = tiles[blockIdx.x];
   const int t = threadIdx.x;
   const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
@@ -986,12 +991,15 @@ __global__ __launch_bounds__(256) void k_hist_combine(SimDev a, const BTile *til
 }
 
 // After the solve: slot `head` <- (bnew, R^T x), and the swimmer's Gram
-// matrix row and column of that slot from the saved dots.
+// matrix row and column of that slot from the saved dots.  hst = (count,
+// head) before the push; k_hist_advance moves them on afterwards.
 __global__ __launch_bounds__(256) void k_hist_push(SimDev a, const BTile *tiles,
                                                    const int *sw_tile0, const double *bnew,
                                                    const double *x, double *Hb, double *Hx,
-                                                   size_t ldh, int head, int count_before,
-                                                   double *G, const double *gsave) {
+                                                   size_t ldh, const int *hst, double *G,
+                                                   const double *gsave, const int *skip) {
+  if (skip && *skip) return;
+  const int count_before = hst[0], head = hst[1];
   const BTile T = tiles[blockIdx.x];
   const int t = threadIdx.x;
   const m3 Qm = rotm(ga4(a.body + size_t(T.sw) * BREC + BQ));
@@ -1016,6 +1024,13 @@ __global__ __launch_bounds__(256) void k_hist_push(SimDev a, const BTile *tiles,
       Gn[M * head + head] = g[M];
     }
   }
+}
+
+// the history's (count, head) after a push into a history of length m
+__global__ void k_hist_advance(int *hst, int m, const int *skip) {
+  if (skip && *skip) return;
+  hst[0] = min(hst[0] + 1, m);
+  hst[1] = (hst[1] + 1) % m;
 }
 
 // copy U of every swimmer into column `col` of N_ref
@@ -2323,46 +2338,48 @@ struct SimOp {
   }
 };
 
+// The fitted GMRES initial guess of the coming solve (history on): the
+// right-hand side's fit onto the history, the combination of the history's
+// solutions as x (an empty history leaves the previous solution).
+static int hist_guess(gmi_ctx *c, SimHost *s) {
+  if (s->hist_m == 0) return GMI_OK;
+  const SimDev &d = s->d;
+  const size_t ldh = size_t(d.npad);
+  hipLaunchKernelGGL(k_hist_dots, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->rhs, s->hist_bnew, s->hist_b, ldh, s->hist_st, s->hist_part);
+  hipLaunchKernelGGL(k_hist_solve, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->sw_tile0,
+                     s->sw_ntile, s->hist_part, s->hist_G, s->hist_Gw, s->hist_st, s->hist_coef,
+                     s->hist_gsave);
+  hipLaunchKernelGGL(k_hist_combine, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->hist_x, ldh, s->hist_st, s->hist_coef, s->x);
+  return launch_ok(c, "fitted guess");
+}
+
+// The solved system into the history (history on); skip gates the kernels.
+static int hist_push(gmi_ctx *c, SimHost *s, const int *skip) {
+  if (s->hist_m == 0) return GMI_OK;
+  const SimDev &d = s->d;
+  hipLaunchKernelGGL(k_hist_push, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
+                     s->sw_tile0, s->hist_bnew, s->x, s->hist_b, s->hist_x, size_t(d.npad),
+                     s->hist_st, s->hist_G, s->hist_gsave, skip);
+  hipLaunchKernelGGL(k_hist_advance, dim3(1), dim3(1), 0, c->stream, s->hist_st, s->hist_m, skip);
+  return launch_ok(c, "history push");
+}
+
 // Rigid-body solve (mobility_solver.cpp:970-1354): the body matrices are
-// rotated once, then GMRES on the rigid-body operator.
+// rotated once, then GMRES on the rigid-body operator, its initial guess
+// fitted on the history when that is on.
 static int solve_linear_system(gmi_ctx *c, SimHost *s, int *its) {
   SimDev &d = s->d;
   hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
   int rc;
   if ((rc = launch_ok(c, "rotate body"))) return rc;
-  const size_t ldh = size_t(d.npad);
-  if (s->hist_m > 0) {
-    // the fitted initial guess replaces the previous solution once the
-    // history holds a right-hand side
-    hipLaunchKernelGGL(k_hist_dots, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                       s->sw_tile0, s->rhs, s->hist_bnew, s->hist_b, ldh, s->hist_count,
-                       s->hist_part);
-    if (s->hist_count > 0) {
-      hipLaunchKernelGGL(k_hist_solve, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
-                         s->sw_tile0, s->sw_ntile, s->hist_part, s->hist_G, s->hist_Gw,
-                         s->hist_count, s->hist_coef, s->hist_gsave);
-      hipLaunchKernelGGL(k_hist_combine, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                         s->sw_tile0, s->hist_x, ldh, s->hist_count, s->hist_coef, s->x);
-    } else {
-      hipLaunchKernelGGL(k_hist_solve, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d,
-                         s->sw_tile0, s->sw_ntile, s->hist_part, s->hist_G, s->hist_Gw, 0,
-                         s->hist_coef, s->hist_gsave);
-    }
-    if ((rc = launch_ok(c, "fitted guess"))) return rc;
-  }
+  if ((rc = hist_guess(c, s))) return rc;
   SimOp op{c, s};
   if ((rc = krylov_solve(c, s, op, d.npad, std::min(d.maxit, 3 * d.Nb + 6 * d.Nsw), s->precision,
                          s->p.gmres_tol, its)))
     return rc;
-  if (s->hist_m > 0) {
-    hipLaunchKernelGGL(k_hist_push, dim3(s->nbtiles), dim3(256), 0, c->stream, d, s->btiles,
-                       s->sw_tile0, s->hist_bnew, s->x, s->hist_b, s->hist_x, ldh, s->hist_head,
-                       s->hist_count, s->hist_G, s->hist_gsave);
-    if ((rc = launch_ok(c, "history push"))) return rc;
-    s->hist_count = std::min(s->hist_count + 1, s->hist_m);
-    s->hist_head = (s->hist_head + 1) % s->hist_m;
-  }
-  return GMI_OK;
+  return hist_push(c, s, nullptr);
 }
 
 // read_positions_and_forces (mobility_solver.cpp:132-192)
@@ -2789,14 +2806,17 @@ int gmi_sim_set_gmres_history(gmi_ctx *c, int m) {
         (rc = dalloc(c, s, &s->hist_Gw, Nsw * M * M)) ||
         (rc = dalloc(c, s, &s->hist_part, size_t(s->nbtiles) * (M + 1))) ||
         (rc = dalloc(c, s, &s->hist_coef, Nsw * M)) ||
-        (rc = dalloc(c, s, &s->hist_gsave, Nsw * (M + 1))))
+        (rc = dalloc(c, s, &s->hist_gsave, Nsw * (M + 1))) || (rc = dalloc(c, s, &s->hist_st, 2)))
       return rc;
   }
-  // a new length starts an empty history
+  // a new length starts an empty history; captured iterations hold the old
+  // length's kernels
+  s->drop_graphs();
   s->hist_m = m;
-  s->hist_count = 0;
-  s->hist_head = 0;
-  return GMI_OK;
+  if (s->hist_st && (rc = hip_check(c, hipMemsetAsync(s->hist_st, 0, 2 * sizeof(int), c->stream),
+                                    "history reset")))
+    return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
 int gmi_sim_set_precision(gmi_ctx *c, int precision) {
@@ -3064,7 +3084,7 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
 // fitted guess (whose history bookkeeping is on the host), from the second
 // step on (the first step's forces carry the nt = 0 perturbations).
 static bool iteration_capturable(gmi_ctx *c, SimHost *s, int nt) {
-  return s->use_graphs && !c->profiling && s->hist_m == 0 && !(nt == 0 && s->d.pert) &&
+  return s->use_graphs && !c->profiling && !(nt == 0 && s->d.pert) &&
          (s->precision == GMI_FP64 || s->dense != nullptr);
 }
 
@@ -3080,6 +3100,7 @@ static int enqueue_iteration(gmi_ctx *c, SimHost *s) {
   if ((rc = forces_and_torques(c, s, 0))) return rc;
   if ((rc = velocities_before_solve(c, s))) return rc;
   hipLaunchKernelGGL(k_rotate_body, dim3(nblk(d.Nsw, 64)), dim3(64), 0, c->stream, d, s->rot);
+  if ((rc = hist_guess(c, s))) return rc;
   SimOp op{c, s};
   const int maxit = gmres_maxit(s);
   const double tol = s->p.gmres_tol;
@@ -3089,6 +3110,7 @@ static int enqueue_iteration(gmi_ctx *c, SimHost *s) {
   hipLaunchKernelGGL(k_hold_unsolved, dim3(1), dim3(1), 0, c->stream, d.flags);
   const int *hold = d.flags + F_HOLD;
   if ((rc = gmres_finish(c, s, op, d.npad, hold))) return rc;
+  if ((rc = hist_push(c, s, hold))) return rc;
   if ((rc = velocities_after_solve(c, s, hold))) return rc;
   if ((rc = enqueue_errors(c, s, s->nerr, hold))) return rc;
   if (s->iter < s->p.max_broyden_iter - 1) {
@@ -3131,6 +3153,7 @@ static int captured_iteration(gmi_ctx *c, SimHost *s, int *big, int *bad, int *i
                            its)))
       return rc;
     if ((rc = gmres_finish(c, s, op, s->d.npad, nullptr))) return rc;
+    if ((rc = hist_push(c, s, nullptr))) return rc;
     if ((rc = velocities_after_solve(c, s, nullptr))) return rc;
     if ((rc = compute_errors(c, s, s->nerr, big, bad))) return rc;
     if (!*bad && stores_pair(s, *big) && (rc = enqueue_end_of_iter(c, s, nullptr))) return rc;

@@ -71,8 +71,11 @@ struct SimHost : KrylovState {
   int precision = GMI_FP64;
   int last_gmres = 0;
   double iters_avg = 0.0;  // moving average of Broyden iterations per step
-  // history-fitted GMRES initial guess (0: the previous solution)
-  int hist_m = 0, hist_count = 0, hist_head = 0;
+  // history-fitted GMRES initial guess (0: the previous solution); the
+  // history's (count, head) live on the device so that captured iterations
+  // can push into it
+  int hist_m = 0;
+  int *hist_st = nullptr;
   double *hist_b = nullptr, *hist_x = nullptr, *hist_bnew = nullptr, *hist_G = nullptr,
          *hist_Gw = nullptr, *hist_part = nullptr, *hist_coef = nullptr, *hist_gsave = nullptr;
   bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS

@@ -117,9 +117,8 @@ int gmi_sim_set_precision(gmi_ctx *ctx, int precision);
  * convergence checks; with use_graphs each chunk is captured once as a HIP
  * graph and replayed (no reference counterpart).  Default: graphs on, 6.
  * With use_graphs, when GMRES runs in FP64 (GMI_FP64, or under
- * GMI_PRECOND_BODY_INVERSE) and the fitted guess is off, each Broyden
- * iteration after the first residual (cilia_sim_main.cpp:266-314) is one
- * captured graph too -- update, forces, velocities, the first GMRES chunk
+ * GMI_PRECOND_BODY_INVERSE), each Broyden iteration after the first
+ * residual (cilia_sim_main.cpp:266-314) is one captured graph too -- update, forces, velocities, the first GMRES chunk
  * and, gated on the device, the rest of the solve, the residual and the
  * Broyden pair -- and the host reads the flags once per iteration. */
 int gmi_sim_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);

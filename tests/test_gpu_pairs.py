@@ -188,6 +188,35 @@ def test_barrier_two_swimmers_parity(gpu, interact):
     assert relerr(vs, v_ref) <= 1e-12
 
 
+@pytest.mark.parametrize("layout", ["crowded", "far"])
+def test_barrier_cell_grid_edge_cases(gpu, layout):
+    """The barrier's cell grid on inputs that reach its rare paths: many
+    particles in a few cells (buckets longer than the in-register sort), and
+    a cluster far from the origin on the negative side (large negative cell
+    coordinates in the hash and the packed keys)."""
+    rng = np.random.default_rng(5)
+    nseg, nfil, nb = 4, 40, 120
+    if layout == "crowded":
+        xs = rng.uniform(-1.2, 1.2, (nseg * nfil, 3)); xb = rng.uniform(-1.2, 1.2, (nb, 3))
+    else:
+        off = np.array([-9.5e3, 4.1e3, -7.7e3])
+        xs = off + rng.uniform(-6, 6, (nseg * nfil, 3)); xb = off + rng.uniform(-6, 6, (nb, 3))
+    fs = rng.normal(size=(nseg * nfil, 6))
+    K = 1.7
+    ctx = capi.Context([nseg * nfil], [nb], nseg, a_blob=0.8)
+    ctx.upload(xs, fs, xb, np.zeros((nb, 3)))
+    ctx.call("gmi_apply_interparticle_forces", K)
+    rep = np.zeros(3 * nb)
+    ctx.call("gmi_copy_interparticle_blob_forces_to_host", capi.ptr(rep))
+    ctx.call("gmi_wait_for_device")
+    ctx.call("gmi_evaluate_segment_segment_mobility")
+    vs, _ = ctx.velocities()
+    f_ref, rep_ref = orc.barrier(xs, xb, fs, nseg, 1.0, 0.8, K)
+    assert np.abs(rep_ref).max() > 0
+    assert relerr(rep.reshape(-1, 3), rep_ref) <= 1e-13
+    assert relerr(vs, orc.Mss(xs, f_ref, 1.0)) <= 1e-12
+
+
 _REF_46722 = {}
 
 

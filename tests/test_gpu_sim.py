@@ -233,21 +233,28 @@ def test_graph_replay_matches_eager(gpu):
     assert ge.graph_stats()[0] == 0 and gg.graph_stats()[0] > 0
 
 
-@pytest.mark.parametrize("precision", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
-def test_captured_broyden_iterations_match_eager(gpu, precision):
+@pytest.mark.parametrize("precision,fitted", [(capi.GMI_FP64, False), (capi.GMI_FP32_KAHAN, False),
+                                              (capi.GMI_FP64, True)])
+def test_captured_broyden_iterations_match_eager(gpu, precision, fitted):
     """Under the exact body-frame preconditioner (FP64 Krylov products, one
-    Arnoldi step per check) each Broyden iteration is one captured graph.
-    Its kernels are the eager ones in the same order, so iteration counts,
-    trajectory and solver state equal the eager run's bit for bit, including
-    the solves that need a second Arnoldi step (finished eagerly) and, in
-    mixed precision, the FP32 pair sums of the velocities."""
+    Arnoldi step per check) each Broyden iteration is one captured graph, and
+    so it is with the reference preconditioner in FP64 with the history-fitted
+    GMRES guess (its fit and push run inside the graph, the history's count
+    and head kept on the device).  The kernels are the eager ones in the same
+    order, so iteration counts, trajectory and solver state equal the eager
+    run's bit for bit, including the solves that need more than the graph's
+    first Arnoldi chunk (finished eagerly) and, in mixed precision, the FP32
+    pair sums of the velocities."""
     cfg = geometry.sim_workload("step_962", seed=1)
     sims = []
     for graphs in (False, True):
         g = capi.Sim(cfg)
         g.set_sim_precision(precision)
         g.set_graphs(graphs)
-        g.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
+        if fitted:
+            g.set_gmres_history(16)
+        else:
+            g.set_body_preconditioner(capi.GMI_PRECOND_BODY_INVERSE)
         g.make_body_reference_matrices()
         sims.append(g)
     ge, gg = sims
