@@ -476,24 +476,24 @@ __global__ void k_axpby(double *out, double a, const double *x, double b, const 
   if (e < n) out[e] = a * x[e] + (y ? b * y[e] : 0.0);
 }
 
-// scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup)
+// scal[o0] = ||x||, scal[o1] = ||y|| (one workgroup): four running sums per
+// thread over a stride of 4096, then the wave sums and a fixed tree
 __global__ __launch_bounds__(1024) void k_norms(const double *x, const double *y, int n,
                                                 double *scal, int o0, int o1, const int *skip) {
   if (skip && *skip) return;
-  __shared__ double sh[1024];
+  __shared__ double sh[16];
   for (int pass = 0; pass < 2; pass++) {
     const double *v = pass ? y : x;
     if (!v) break;
-    double s = 0;
-    for (int e = threadIdx.x; e < n; e += 1024) s += v[e] * v[e];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 512; st > 0; st >>= 1) {
-      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-      __syncthreads();
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int e = threadIdx.x;
+    for (; e + 3072 < n; e += 4096) {
+      const double a = v[e], b = v[e + 1024], c = v[e + 2048], d = v[e + 3072];
+      s0 += a * a; s1 += b * b; s2 += c * c; s3 += d * d;
     }
-    if (threadIdx.x == 0) scal[pass ? o1 : o0] = sqrt(sh[0]);
-    __syncthreads();
+    for (; e < n; e += 1024) s0 += v[e] * v[e];
+    const double s = block_sum<1024>((s0 + s1) + (s2 + s3), sh);
+    if (threadIdx.x == 0) scal[pass ? o1 : o0] = sqrt(s);
   }
 }
 
