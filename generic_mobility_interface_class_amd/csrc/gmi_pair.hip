@@ -154,15 +154,54 @@ template <typename R> struct Acc {
   __device__ void zero() { vx = vy = vz = wx = wy = wz = R(0); }
 };
 
-// One staged source: position (FP32 keeps a low part too), F', and for
-// segment sources T' and T2.
-template <typename R> struct Lds {
-  R x[kTile], y[kTile], z[kTile];
-  R fx[kTile], fy[kTile], fz[kTile];
-  R tx[kTile], ty[kTile], tz[kTile];
-  R ux[kTile], uy[kTile], uz[kTile];
-  float lx[kTile], ly[kTile], lz[kTile];   // used by FP32 only
+// One staged source, one record per source so that a pair reads it with
+// 16-byte LDS loads from one address (every lane reads the same source, a
+// broadcast): position (FP32 keeps a low part too), F', and for segment
+// sources T' and T2.  What a translation-only pair needs fills the first 48
+// bytes.
+template <typename R> struct Src;
+template <> struct alignas(16) Src<float> {
+  float x, y, z, lx, ly, lz, fx, fy, fz, tx, ty, tz, ux, uy, uz, pad;
 };
+template <> struct alignas(16) Src<double> {
+  double x, y, z, fx, fy, fz, tx, ty, tz, ux, uy, uz;
+};
+template <typename R> struct Lds {
+  Src<R> s[kTile];
+};
+
+// A staged source: the first 48 bytes, and all of it when SIX.
+template <bool SIX>
+__device__ __forceinline__ Src<float> fetch(const Src<float> *src) {
+  const float4 *q = reinterpret_cast<const float4 *>(src);
+  const float4 a = q[0], b = q[1], c = q[2];
+  Src<float> v;
+  v.x = a.x; v.y = a.y; v.z = a.z; v.lx = a.w;
+  v.ly = b.x; v.lz = b.y; v.fx = b.z; v.fy = b.w;
+  v.fz = c.x; v.tx = c.y; v.ty = c.z; v.tz = c.w;
+  if (SIX) {
+    const float4 d = q[3];
+    v.ux = d.x; v.uy = d.y; v.uz = d.z;
+  } else {
+    v.ux = v.uy = v.uz = 0.f;
+  }
+  v.pad = 0.f;
+  return v;
+}
+template <bool SIX>
+__device__ __forceinline__ Src<double> fetch(const Src<double> *src) {
+  const double2 *q = reinterpret_cast<const double2 *>(src);
+  const double2 a = q[0], b = q[1], c = q[2];
+  Src<double> v;
+  v.x = a.x; v.y = a.y; v.z = b.x; v.fx = b.y; v.fy = c.x; v.fz = c.y;
+  if (SIX) {
+    const double2 d = q[3], e = q[4], f = q[5];
+    v.tx = d.x; v.ty = d.y; v.tz = e.x; v.ux = e.y; v.uy = f.x; v.uz = f.y;
+  } else {
+    v.tx = v.ty = v.tz = v.ux = v.uy = v.uz = 0.0;
+  }
+  return v;
+}
 
 // ---- per-pair kernels ------------------------------------------------------
 // With d = x_i - x_j, ri = 1/|d| and the pre-scaled sources, the far forms
@@ -181,18 +220,18 @@ __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
 
 // v += At F' + Bt d
 template <typename R>
-__device__ __forceinline__ void acc_v(R At, R Bt, R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void acc_v(R At, R Bt, R dx, R dy, R dz, const Src<R> &s,
                                       Acc<R> &o) {
-  o.vx = fma(At, s.fx[j], o.vx); o.vx = fma(Bt, dx, o.vx);
-  o.vy = fma(At, s.fy[j], o.vy); o.vy = fma(Bt, dy, o.vy);
-  o.vz = fma(At, s.fz[j], o.vz); o.vz = fma(Bt, dz, o.vz);
+  o.vx = fma(At, s.fx, o.vx); o.vx = fma(Bt, dx, o.vx);
+  o.vy = fma(At, s.fy, o.vy); o.vy = fma(Bt, dy, o.vy);
+  o.vz = fma(At, s.fz, o.vz); o.vz = fma(Bt, dz, o.vz);
 }
 
 // v += Cg (T2 x d)
 template <typename R>
-__device__ __forceinline__ void acc_vrot(R Cg, R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void acc_vrot(R Cg, R dx, R dy, R dz, const Src<R> &s,
                                          Acc<R> &o) {
-  const R ux = s.ux[j], uy = s.uy[j], uz = s.uz[j];
+  const R ux = s.ux, uy = s.uy, uz = s.uz;
   o.vx = fma(Cg, fma(uy, dz, -(uz * dy)), o.vx);
   o.vy = fma(Cg, fma(uz, dx, -(ux * dz)), o.vy);
   o.vz = fma(Cg, fma(ux, dy, -(uy * dx)), o.vz);
@@ -200,10 +239,10 @@ __device__ __forceinline__ void acc_vrot(R Cg, R dx, R dy, R dz, const Lds<R> &s
 
 // ss: 6x6 equal radius.
 template <typename R>
-__device__ __forceinline__ void pair_ss(R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void pair_ss(R dx, R dy, R dz, const Src<R> &s,
                                         const PC<R> &k, Acc<R> &o) {
-  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
-  const R tx = s.tx[j], ty = s.ty[j], tz = s.tz[j];
+  const R fx = s.fx, fy = s.fy, fz = s.fz;
+  const R tx = s.tx, ty = s.ty, tz = s.tz;
   const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
   const R td = fma(tz, dz, fma(ty, dy, tx * dx));
@@ -220,8 +259,8 @@ __device__ __forceinline__ void pair_ss(R dx, R dy, R dz, const Lds<R> &s, int j
     const R E = nr ? E2 : R(0);
     o.wx = fma(E, tx, o.wx); o.wy = fma(E, ty, o.wy); o.wz = fma(E, tz, o.wz);
   }
-  acc_vrot(Cg, dx, dy, dz, s, j, o);
-  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  acc_vrot(Cg, dx, dy, dz, s, o);
+  acc_v(At, Bt, dx, dy, dz, s, o);
   o.wx = fma(Cg, fma(-fz, dy, fma(fy, dz, -tx)), o.wx); o.wx = fma(Br, dx, o.wx);
   o.wy = fma(Cg, fma(-fx, dz, fma(fz, dx, -ty)), o.wy); o.wy = fma(Br, dy, o.wy);
   o.wz = fma(Cg, fma(-fy, dx, fma(fx, dy, -tz)), o.wz); o.wz = fma(Br, dz, o.wz);
@@ -229,9 +268,9 @@ __device__ __forceinline__ void pair_ss(R dx, R dy, R dz, const Lds<R> &s, int j
 
 // sb: segment (v, w) <- blob f, unequal radii.
 template <typename R>
-__device__ __forceinline__ void pair_sb(R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void pair_sb(R dx, R dy, R dz, const Src<R> &s,
                                         const PC<R> &k, Acc<R> &o) {
-  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R fx = s.fx, fy = s.fy, fz = s.fz;
   const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
   const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
@@ -244,7 +283,7 @@ __device__ __forceinline__ void pair_sb(R dx, R dy, R dz, const Lds<R> &s, int j
     near_unequal(r2, ri, fd, k, At2, Bt2, Cg2);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Cg = nr ? Cg2 : Cg;
   }
-  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  acc_v(At, Bt, dx, dy, dz, s, o);
   o.wx = fma(Cg, fma(fy, dz, -(fz * dy)), o.wx);
   o.wy = fma(Cg, fma(fz, dx, -(fx * dz)), o.wy);
   o.wz = fma(Cg, fma(fx, dy, -(fy * dx)), o.wz);
@@ -252,9 +291,9 @@ __device__ __forceinline__ void pair_sb(R dx, R dy, R dz, const Lds<R> &s, int j
 
 // bs: blob v <- segment (f, tau), unequal radii.
 template <typename R>
-__device__ __forceinline__ void pair_bs(R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void pair_bs(R dx, R dy, R dz, const Src<R> &s,
                                         const PC<R> &k, Acc<R> &o) {
-  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R fx = s.fx, fy = s.fy, fz = s.fz;
   const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
   const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
@@ -267,17 +306,17 @@ __device__ __forceinline__ void pair_bs(R dx, R dy, R dz, const Lds<R> &s, int j
     near_unequal(r2, ri, fd, k, At2, Bt2, Cg2);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt; Cg = nr ? Cg2 : Cg;
   }
-  acc_vrot(Cg, dx, dy, dz, s, j, o);
-  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  acc_vrot(Cg, dx, dy, dz, s, o);
+  acc_v(At, Bt, dx, dy, dz, s, o);
 }
 
 // translation only, unequal radii: segment v <- blob f or blob v <- segment f
 // (the tt parts of sb / bs; the prescribed-cilia operator has no torques and
 // no angular velocities, cuda_functions.cu:646,808)
 template <typename R>
-__device__ __forceinline__ void pair_ut(R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void pair_ut(R dx, R dy, R dz, const Src<R> &s,
                                         const PC<R> &k, Acc<R> &o) {
-  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R fx = s.fx, fy = s.fy, fz = s.fz;
   const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
   const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
@@ -289,15 +328,15 @@ __device__ __forceinline__ void pair_ut(R dx, R dy, R dz, const Lds<R> &s, int j
     near_unequal(r2, ri, fd, k, At2, Bt2, Cg);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
-  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  acc_v(At, Bt, dx, dy, dz, s, o);
 }
 
 // bb: blob v <- blob f, equal radius (also segment v <- segment f when only
 // translations couple).
 template <typename R>
-__device__ __forceinline__ void pair_bb(R dx, R dy, R dz, const Lds<R> &s, int j,
+__device__ __forceinline__ void pair_bb(R dx, R dy, R dz, const Src<R> &s,
                                         const PC<R> &k, Acc<R> &o) {
-  const R fx = s.fx[j], fy = s.fy[j], fz = s.fz[j];
+  const R fx = s.fx, fy = s.fy, fz = s.fz;
   const R r2 = fma(dz, dz, fma(dy, dy, dx * dx));
   const R fd = fma(fz, dz, fma(fy, dy, fx * dx));
   const R ri = rsq(r2), ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
@@ -309,7 +348,7 @@ __device__ __forceinline__ void pair_bb(R dx, R dy, R dz, const Lds<R> &s, int j
     near_equal(r2, ri, fd, R(0), k, At2, Bt2, Br, Cg, E);
     At = nr ? At2 : At; Bt = nr ? Bt2 : Bt;
   }
-  acc_v(At, Bt, dx, dy, dz, s, j, o);
+  acc_v(At, Bt, dx, dy, dz, s, o);
 }
 
 // ---- precision traits ----------------------------------------------------
@@ -329,42 +368,37 @@ template <> struct Pos<float> {
   }
 };
 
-template <typename R>
-__device__ __forceinline__ void sep(const Pos<R> &p, const Lds<R> &s, int j,
-                                    R &dx, R &dy, R &dz);
-template <>
-__device__ __forceinline__ void sep<double>(const Pos<double> &p,
-                                            const Lds<double> &s, int j,
-                                            double &dx, double &dy, double &dz) {
-  dx = p.x - s.x[j]; dy = p.y - s.y[j]; dz = p.z - s.z[j];
+__device__ __forceinline__ void sep(const Pos<double> &p, const Src<double> &s,
+                                    double &dx, double &dy, double &dz) {
+  dx = p.x - s.x; dy = p.y - s.y; dz = p.z - s.z;
 }
-template <>
-__device__ __forceinline__ void sep<float>(const Pos<float> &p,
-                                           const Lds<float> &s, int j,
-                                           float &dx, float &dy, float &dz) {
-  dx = (p.x - s.x[j]) + (p.lx - s.lx[j]);
-  dy = (p.y - s.y[j]) + (p.ly - s.ly[j]);
-  dz = (p.z - s.z[j]) + (p.lz - s.lz[j]);
+__device__ __forceinline__ void sep(const Pos<float> &p, const Src<float> &s,
+                                    float &dx, float &dy, float &dz) {
+  dx = (p.x - s.x) + (p.lx - s.lx);
+  dy = (p.y - s.y) + (p.ly - s.ly);
+  dz = (p.z - s.z) + (p.lz - s.lz);
 }
 
-template <typename R>
-__device__ __forceinline__ void stage_pos(Lds<R> &s, int t, double px,
-                                          double py, double pz);
-template <>
-__device__ __forceinline__ void stage_pos<double>(Lds<double> &s, int t,
-                                                  double px, double py,
-                                                  double pz) {
-  s.x[t] = px; s.y[t] = py; s.z[t] = pz;
+// Source record from FP64 position, F' and T2 (T' = T2 / 2).
+__device__ __forceinline__ Src<double> make_src(double px, double py, double pz, double fx,
+                                                double fy, double fz, double ux, double uy,
+                                                double uz, double) {
+  Src<double> v;
+  v.x = px; v.y = py; v.z = pz; v.fx = fx; v.fy = fy; v.fz = fz;
+  v.tx = 0.5 * ux; v.ty = 0.5 * uy; v.tz = 0.5 * uz; v.ux = ux; v.uy = uy; v.uz = uz;
+  return v;
 }
-template <>
-__device__ __forceinline__ void stage_pos<float>(Lds<float> &s, int t,
-                                                 double px, double py,
-                                                 double pz) {
-  const float hx = float(px), hy = float(py), hz = float(pz);
-  s.x[t] = hx; s.y[t] = hy; s.z[t] = hz;
-  s.lx[t] = float(px - double(hx));
-  s.ly[t] = float(py - double(hy));
-  s.lz[t] = float(pz - double(hz));
+__device__ __forceinline__ Src<float> make_src(double px, double py, double pz, double fx,
+                                               double fy, double fz, double ux, double uy,
+                                               double uz, float) {
+  Src<float> v;
+  v.x = float(px); v.y = float(py); v.z = float(pz);
+  v.lx = float(px - double(v.x)); v.ly = float(py - double(v.y)); v.lz = float(pz - double(v.z));
+  v.fx = float(fx); v.fy = float(fy); v.fz = float(fz);
+  v.tx = float(0.5 * ux); v.ty = float(0.5 * uy); v.tz = float(0.5 * uz);
+  v.ux = float(ux); v.uy = float(uy); v.uz = float(uz);
+  v.pad = 0.f;
+  return v;
 }
 
 // Running sum of one target: FP64 plain, FP32 Kahan-compensated per tile.
@@ -402,6 +436,32 @@ template <> struct Sum<float> {
 
 enum Kind { SS, SB, BS, BB, UT };
 
+// Ends a pair's accumulation where it stands: an empty asm that reads and
+// writes the six sums, so the compiler cannot defer the sums of one pair past
+// the next pair (which keeps every pair's intermediates live at once).
+template <typename R> __device__ __forceinline__ void settle(Acc<R> &o) {
+  asm volatile("" : "+v"(o.vx), "+v"(o.vy), "+v"(o.vz), "+v"(o.wx), "+v"(o.wy), "+v"(o.wz));
+}
+
+// Target p against one staged source.
+template <typename R, int K>
+__device__ __forceinline__ void pair_at(const Pos<R> &p, const Src<R> *src,
+                                        const PC<R> &k, Acc<R> &acc) {
+  const Src<R> s = fetch<K == SS || K == BS>(src);
+  R dx, dy, dz;
+  sep(p, s, dx, dy, dz);
+  if (K == SS)
+    pair_ss<R>(dx, dy, dz, s, k, acc);
+  else if (K == SB)
+    pair_sb<R>(dx, dy, dz, s, k, acc);
+  else if (K == BS)
+    pair_bs<R>(dx, dy, dz, s, k, acc);
+  else if (K == UT)
+    pair_ut<R>(dx, dy, dz, s, k, acc);
+  else
+    pair_bb<R>(dx, dy, dz, s, k, acc);
+}
+
 // Walk sources [j0, j1) of one species in LDS tiles and accumulate into sum.
 // c8 = 1/(8 pi mu) scales the staged sources (F' = c8 f, T' = c8 tau / 2,
 // T2 = c8 tau).
@@ -418,38 +478,34 @@ __device__ __forceinline__ void sweep(Lds<R> &lds, const Pos<R> &p,
     __syncthreads();
     if (tid < n) {
       const int j = base + tid;
-      stage_pos<R>(lds, tid, X[j], X[ld + j], X[2 * ld + j]);
-      lds.fx[tid] = R(c8 * F[j]);
-      lds.fy[tid] = R(c8 * F[ld + j]);
-      lds.fz[tid] = R(c8 * F[2 * ld + j]);
+      double ux = 0.0, uy = 0.0, uz = 0.0;
       if (six) {
-        const double tx = c8 * F[3 * ld + j], ty = c8 * F[4 * ld + j], tz = c8 * F[5 * ld + j];
-        lds.ux[tid] = R(tx);
-        lds.uy[tid] = R(ty);
-        lds.uz[tid] = R(tz);
-        lds.tx[tid] = R(0.5 * tx);
-        lds.ty[tid] = R(0.5 * ty);
-        lds.tz[tid] = R(0.5 * tz);
+        ux = c8 * F[3 * ld + j]; uy = c8 * F[4 * ld + j]; uz = c8 * F[5 * ld + j];
       }
+      lds.s[tid] = make_src(X[j], X[ld + j], X[2 * ld + j], c8 * F[j], c8 * F[ld + j],
+                            c8 * F[2 * ld + j], ux, uy, uz, R(0));
     }
     __syncthreads();
     Acc<R> acc;
     acc.zero();
-#pragma unroll 2
-    for (int j = 0; j < n; ++j) {
-      R dx, dy, dz;
-      sep<R>(p, lds, j, dx, dy, dz);
-      if (K == SS)
-        pair_ss<R>(dx, dy, dz, lds, j, k, acc);
-      else if (K == SB)
-        pair_sb<R>(dx, dy, dz, lds, j, k, acc);
-      else if (K == BS)
-        pair_bs<R>(dx, dy, dz, lds, j, k, acc);
-      else if (K == UT)
-        pair_ut<R>(dx, dy, dz, lds, j, k, acc);
-      else
-        pair_bb<R>(dx, dy, dz, lds, j, k, acc);
+    // Four sources per trip: one LDS address serves the four, and the ballot
+    // inside each pair keeps the compiler from unrolling the loop itself.
+    // The source index is held in a vector register (the empty asm), so the
+    // four loads take it with immediate offsets instead of a copy each.
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+      int jv = j;
+      asm volatile("" : "+v"(jv));
+      const Src<R> *src = lds.s + jv;
+      pair_at<R, K>(p, src, k, acc);
+      settle(acc);
+      pair_at<R, K>(p, src + 1, k, acc);
+      settle(acc);
+      pair_at<R, K>(p, src + 2, k, acc);
+      settle(acc);
+      pair_at<R, K>(p, src + 3, k, acc);
     }
+    for (; j < n; ++j) pair_at<R, K>(p, lds.s + j, k, acc);
     sum.add(acc);
   }
 }
