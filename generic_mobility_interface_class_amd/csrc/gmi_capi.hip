@@ -340,7 +340,8 @@ int gmi_create(gmi_ctx **out, const gmi_params *p) {
     };
     if ((rc = ialloc(&G.cnt, G.H)) || (rc = ialloc(&G.fill, G.H)) ||
         (rc = ialloc(&G.start, G.H)) || (rc = ialloc(&G.boff, G.H / kScanChunk)) ||
-        (rc = ialloc(&G.ent, N)) || (rc = ialloc(&G.bucket, N)) || (rc = ialloc(&gp, N)) ||
+        (rc = ialloc(&G.ent, N)) || (rc = ialloc(&G.arr, N)) || (rc = ialloc(&G.bucket, N)) ||
+        (rc = ialloc(&gp, N)) ||
         (rc = hip_check(c, hipMalloc(&G.key, sizeof(int4) * std::max(N, 1)), "hipMalloc")) ||
         (rc = hip_check(c, hipMalloc(&G.rk, sizeof(int4) * std::max(N, 1)), "hipMalloc")) ||
         (rc = dalloc(&G.rx, N)) || (rc = dalloc(&G.ry, N)) || (rc = dalloc(&G.rz, N)))
@@ -390,8 +391,8 @@ void gmi_destroy(gmi_ctx *c) {
   }
   {
     CellGrid &G = c->cells;
-    void *cb[] = {G.cnt, G.fill, G.start, G.boff, G.ent, G.bucket, G.key, G.rk, G.rx, G.ry, G.rz,
-                  const_cast<int *>(G.grp)};
+    void *cb[] = {G.cnt, G.fill, G.start, G.boff, G.ent, G.arr, G.bucket, G.key, G.rk, G.rx, G.ry,
+                  G.rz, const_cast<int *>(G.grp)};
     for (void *b : cb)
       if (b) (void)hipFree(b);
   }

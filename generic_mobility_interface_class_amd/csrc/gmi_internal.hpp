@@ -74,7 +74,9 @@ hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hip
 // before a build; every build leaves them zero.
 constexpr int kScanChunk = 4096;
 struct CellGrid {
-  int *cnt, *fill, *start, *boff, *ent, *bucket;
+  int *cnt, *fill, *start, *boff, *bucket;
+  int *arr;         // per bucket entry: particle index in arrival order
+  int *ent;         // per bucket entry: particle index in index order
   int4 *key;        // per particle: cell x, y, z and group
   // per bucket entry, in bucket order: position, cell and group
   double *rx, *ry, *rz;

@@ -981,54 +981,29 @@ __global__ __launch_bounds__(256) void k_cell_scatter(CellGrid G, int N) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= N) return;
   const int b = G.bucket[p];
-  G.ent[bucket_begin(G, b) + atomicAdd(&G.fill[b], 1)] = p;
+  G.arr[bucket_begin(G, b) + atomicAdd(&G.fill[b], 1)] = p;
   G.cnt[b] = 0;
 }
 
-constexpr int kSortRegs = 8;
-// each bucket's entries into index order (buckets hold a few particles),
-// then the bucket's records (position, cell and group, index) in that order;
-// the fill counters are cleared for the next build
-__global__ __launch_bounds__(256) void k_cell_sort(CellGrid G, const double *Xs, int lds,
+// each particle's place in its bucket: after the scatter a bucket's
+// entries (arr) are in arrival order, and a particle's rank among them is the
+// number of entries with a smaller index, so the bucket's records (index,
+// position, cell and group) land in index order without a sort.  Every
+// particle of a bucket clears the bucket's fill counter for the next build
+// (all write the same zero, after the scatter's increments).
+__global__ __launch_bounds__(256) void k_cell_rank(CellGrid G, const double *Xs, int lds,
                                                    const double *Xb, int ldb, int Ns, int N) {
-  const int b = blockIdx.x * 256 + threadIdx.x;
-  if (b >= G.H) return;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  const int b = G.bucket[p];
   const int s = bucket_begin(G, b), e = bucket_end(G, b, N);
+  int r = 0;
+  for (int i = s; i < e; ++i) r += G.arr[i] < p;
+  const int o = s + r;
+  G.ent[o] = p;
+  particle_pos(p, Xs, lds, Xb, ldb, Ns, G.rx[o], G.ry[o], G.rz[o]);
+  G.rk[o] = G.key[p];
   G.fill[b] = 0;
-  if (e - s <= kSortRegs) {
-    // short buckets (nearly all): sorted in registers
-    int v[kSortRegs];
-#pragma unroll
-    for (int k = 0; k < kSortRegs; ++k) v[k] = s + k < e ? G.ent[s + k] : INT_MAX;
-#pragma unroll
-    for (int a = 1; a < kSortRegs; ++a)
-#pragma unroll
-      for (int k = kSortRegs - 1; k >= a; --k)
-        if (v[k - 1] > v[k]) { const int w = v[k]; v[k] = v[k - 1]; v[k - 1] = w; }
-#pragma unroll
-    for (int k = 0; k < kSortRegs; ++k)
-      if (s + k < e) {
-        const int p = v[k];
-        G.ent[s + k] = p;
-        particle_pos(p, Xs, lds, Xb, ldb, Ns, G.rx[s + k], G.ry[s + k], G.rz[s + k]);
-        G.rk[s + k] = G.key[p];
-      }
-  } else {
-    for (int i = s + 1; i < e; ++i) {
-      const int v = G.ent[i];
-      int k = i - 1;
-      while (k >= s && G.ent[k] > v) {
-        G.ent[k + 1] = G.ent[k];
-        --k;
-      }
-      G.ent[k + 1] = v;
-    }
-    for (int i = s; i < e; ++i) {
-      const int p = G.ent[i];
-      particle_pos(p, Xs, lds, Xb, ldb, Ns, G.rx[i], G.ry[i], G.rz[i]);
-      G.rk[i] = G.key[p];
-    }
-  }
 }
 
 // 27 threads per target particle, one per cell of the 3 x 3 x 3 block around
@@ -1160,8 +1135,7 @@ hipError_t launch_barrier(double *Fs, int lds, double *Frep, int ldb, const doub
   hipLaunchKernelGGL(k_cell_scan_local, dim3(G.H / kScanChunk), dim3(1024), 0, s, G);
   hipLaunchKernelGGL(k_cell_scan_top, dim3(1), dim3(1), 0, s, G);
   hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, s, G, N);
-  hipLaunchKernelGGL(k_cell_sort, dim3((G.H + 255) / 256), dim3(256), 0, s, G, Xs, lds, Xb, ldb,
-                     Ns, N);
+  hipLaunchKernelGGL(k_cell_rank, dim3(nb), dim3(256), 0, s, G, Xs, lds, Xb, ldb, Ns, N);
   hipLaunchKernelGGL(k_barrier_cells, dim3((N + kCellTargets - 1) / kCellTargets),
                      dim3(27 * kCellTargets), 0, s, G, Fs, lds, Frep, ldb, Xs, Xb, Ns, N, nseg_fil,
                      as, ab, K);
