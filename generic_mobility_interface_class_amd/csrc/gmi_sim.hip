@@ -3079,10 +3079,10 @@ int gmi_broyden_end_of_iter(gmi_ctx *c, int too_large) {
 // residual being finite and too large).  The host reads the flags once per
 // iteration; a solve that needs more than the first chunk is finished with
 // eager launches of the same kernels.  The step's time and index are read on
-// the device, so the graphs serve every step.  Captured when GMRES runs in
-// FP64 (plain FP64, or under the exact body-frame preconditioner) without the
-// fitted guess (whose history bookkeeping is on the host), from the second
-// step on (the first step's forces carry the nt = 0 perturbations).
+// the device, so the graphs serve every step; so are the fitted guess's
+// history count and head.  Captured when GMRES runs in FP64 (plain FP64, or
+// under the exact body-frame preconditioner), from the second step on (the
+// first step's forces carry the nt = 0 perturbations).
 static bool iteration_capturable(gmi_ctx *c, SimHost *s, int nt) {
   return s->use_graphs && !c->profiling && !(nt == 0 && s->d.pert) &&
          (s->precision == GMI_FP64 || s->dense != nullptr);
