@@ -846,32 +846,38 @@ __global__ __launch_bounds__(kBlock) void k_pair_wall(const PairArgs a) {
 }
 
 // Split partials of one (tile, component) per workgroup (blockIdx.y is the
-// component): each lane keeps four running sums over the splits, so it has
-// four loads in flight, and the sums are combined in a fixed order.
-__global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
+// component), four lanes per target: lane group g sums the splits
+// q = g (mod 4) below the last multiple of four, group 0 also the rest, and
+// the four sums are combined in a fixed order -- the sums a single lane with
+// four running sums would form, with four times the loads in flight.
+constexpr int kReduceGroups = 4;
+__global__ __launch_bounds__(kReduceGroups * kBlock) void k_reduce(const PairArgs a) {
   if (a.skip && *a.skip) return;
+  __shared__ double sh[kReduceGroups][kBlock];
   const TileDesc T = a.tiles[blockIdx.x];
-  const int tid = threadIdx.x, c = blockIdx.y;
-  if (T.nsplit == 1 || tid >= T.nt) return;
+  const int tid = threadIdx.x & (kBlock - 1), g = threadIdx.x / kBlock, c = blockIdx.y;
+  if (T.nsplit == 1) return;
   const bool seg_t = (T.species == 0);
   const int nc = (seg_t && !a.tt_only) ? 6 : 3;
   if (c >= nc) return;
+  const bool live = tid < T.nt;
+  double s = 0.0;
+  if (live) {
+    const double *P = a.part + size_t(T.part0) * 6 * kBlock + c * kBlock + tid;
+    const size_t stride = size_t(6) * kBlock;
+    const int main = T.nsplit & ~(kReduceGroups - 1);
+    for (int q = g; q < main; q += kReduceGroups) s += P[stride * q];
+    if (g == 0)
+      for (int q = main; q < T.nsplit; ++q) s += P[stride * q];
+  }
+  sh[g][tid] = s;
+  __syncthreads();
+  if (g != 0 || !live) return;
+  const double r = (sh[0][tid] + sh[1][tid]) + (sh[2][tid] + sh[3][tid]);
   double *V = seg_t ? a.Vs : a.Vb;
   const int ld = seg_t ? a.lds : a.ldb;
-  const double *P = a.part + size_t(T.part0) * 6 * kBlock + c * kBlock + tid;
-  const size_t stride = size_t(6) * kBlock;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int q = 0;
-  for (; q + 3 < T.nsplit; q += 4) {
-    s0 += P[stride * q];
-    s1 += P[stride * (q + 1)];
-    s2 += P[stride * (q + 2)];
-    s3 += P[stride * (q + 3)];
-  }
-  for (; q < T.nsplit; ++q) s0 += P[stride * q];
-  const double s = (s0 + s1) + (s2 + s3);
   double *v = V + c * ld + T.t0 + tid;
-  *v = a.accumulate ? *v + s : s;
+  *v = a.accumulate ? *v + r : r;
 }
 
 // Steric barrier (cuda_functions.cu:961-1157) over a hashed cell grid.
@@ -879,7 +885,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const PairArgs a) {
 // contributing pair lies in the same or adjacent cells.  Each call bins every
 // particle (segments first, then blobs, in one index space) by its cell and
 // swimmer group into H buckets (a counting sort: integer atomics for the
-// counts, then each bucket sorted by particle index), and each target visits
+// counts, then each bucket put in particle-index order), and each target visits
 // the 27 cells around its own in a fixed order, skipping entries of other
 // cells or groups that share a bucket.  Sums therefore run in a fixed order
 // (cell by cell of the 27, index order within a cell).  The contact test compares
@@ -1092,7 +1098,7 @@ hipError_t launch_pair(const PairArgs &a, int nwork, int ntiles_reduce, bool fp3
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
   if (ntiles_reduce > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kReduceGroups * kBlock), 0, s, a);
     e = hipGetLastError();
   }
   return e;
@@ -1104,7 +1110,7 @@ hipError_t launch_pair_dyn(const PairArgs &a, int nwork, int ntiles_reduce, hipS
   hipLaunchKernelGGL(k_pair_dyn, dim3(nwork), dim3(kBlock), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || ntiles_reduce <= 0) return e;
-  hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kReduceGroups * kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1119,7 +1125,7 @@ hipError_t launch_pair_wall(const PairArgs &a, int nwork, int ntiles_reduce, hip
   if (e != hipSuccess) return e;
   if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
   if (ntiles_reduce > 0) {
-    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_reduce, dim3(ntiles_reduce, 6), dim3(kReduceGroups * kBlock), 0, s, a);
     e = hipGetLastError();
   }
   return e;
