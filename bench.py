@@ -204,6 +204,7 @@ def bench_step(args):
         sim = capi.Sim(cfg, precision=capi.GMI_FP64)
         sim.set_sim_precision(capi.GMI_FP64)
         sim.set_graphs(not args.no_graphs)
+        sim.set_gram_schmidt(args.gs_passes)
         if fitted:
             sim.set_gmres_history(16)
         if inverse:
@@ -312,13 +313,14 @@ def _timed_steps(sim, nt0, k, name):
 
 
 def _gram_schmidt(sim, nt0, k, name):
-    """The Gram-Schmidt kernels of the fused Arnoldi step (k_kprod_dots,
-    k_gs_chunk<1>, k_gs_chunk<2>, with k_colsum_wide beyond 256 chunks) over k
-    steps from nt0, launched eagerly so that every Arnoldi step brackets them
-    with HIP events.  Algorithmic bytes per Arnoldi step with j previous
-    columns of length n: two CGS passes of (j + 2) n doubles (the columns and
-    w read, w written), against 8 TB/s; the K products and split sums that
-    k_kprod_dots also does are not counted."""
+    """The Gram-Schmidt kernels of the fused Arnoldi step (k_kprod_dots and
+    k_gs_chunk<2>, with k_gs_chunk<1> when it takes two passes and
+    k_colsum_wide beyond 256 chunks) over k steps from nt0, launched eagerly
+    so that every Arnoldi step brackets them with HIP events.  Algorithmic
+    bytes per Arnoldi step with j previous columns of length n: per
+    Gram-Schmidt pass (j + 2) n doubles (the columns and w read, w written),
+    against 8 TB/s; the K products and split sums that k_kprod_dots also does
+    are not counted."""
     sim.set_graphs(False)
     sim.call("gmi_profile_begin")
     for nt in range(nt0, nt0 + k):
@@ -328,8 +330,8 @@ def _gram_schmidt(sim, nt0, k, name):
     pair_ms, n = capi.C.c_double(0.0), capi.C.c_int(0)
     sim.call("gmi_profile_end", capi.C.byref(pair_ms), capi.C.byref(n))
     gbs = nbytes.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0
-    r = {"kernels": "k_kprod_dots + k_gs_chunk<1> + k_gs_chunk<2> (+ k_colsum_wide): "
-                    "K products and both CGS passes of the fused Arnoldi step",
+    r = {"kernels": "k_kprod_dots + k_gs_chunk<2> (+ k_colsum_wide): K products and the "
+                    "one Gram-Schmidt pass of the fused Arnoldi step (the reference's one pass)",
          "steps": f"{nt0}-{nt0 + k - 1}", "arnoldi_steps": steps.value, "ms": ms.value,
          "us_per_arnoldi_step": 1e3 * ms.value / max(1, steps.value),
          "algorithmic_bytes": nbytes.value, "achieved_gbs": gbs,
@@ -677,6 +679,9 @@ def main():
     ap.add_argument("--no-step", action="store_true")
     ap.add_argument("--step-modes", default=None,
                     help="comma list of step-leg modes (default: all nine)")
+    ap.add_argument("--gs-passes", type=int, default=1, choices=(1, 2),
+                    help="Gram-Schmidt passes of the reference-preconditioned Arnoldi step "
+                         "in the step leg (1: one, as the reference)")
     ap.add_argument("--step-shape", type=int, nargs=2, default=None,
                     metavar=("TARGET_WORKGROUPS", "MIN_CHUNK"),
                     help="pair launch shape of the step leg (default: STEP_LAUNCH_SHAPE)")

@@ -102,6 +102,7 @@ SIGNATURES = [
     ("gmi_sim_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("gmi_sim_graph_stats", C.c_int, [C.c_void_p, _ip, _ip]),
     ("gmi_sim_set_gmres_history", C.c_int, [C.c_void_p, C.c_int]),
+    ("gmi_sim_set_gram_schmidt", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_jacobian_pivoting", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_sim_set_body_preconditioner", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_warmstart_init", C.c_int, [C.c_void_p, C.c_int, C.c_ulonglong]),
@@ -367,6 +368,11 @@ class Sim(Context):
         """GMRES initial guess fitted on the last m right-hand sides (0: the
         reference's previous solution)."""
         self.call("gmi_sim_set_gmres_history", int(m))
+
+    def set_gram_schmidt(self, passes=1):
+        """Gram-Schmidt passes of the reference-preconditioned Arnoldi step
+        (1: one pass, as the reference; 2: with reorthogonalisation)."""
+        self.call("gmi_sim_set_gram_schmidt", int(passes))
 
     def set_jacobian_pivoting(self, always_pivot=True):
         self.call("gmi_sim_set_jacobian_pivoting", int(bool(always_pivot)))

@@ -155,7 +155,7 @@ int prof_start(gmi_ctx *c, hipEvent_t *ev_stop) {
   return GMI_OK;
 }
 
-int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop) {
+int prof_gs_start(gmi_ctx *c, int k, int n, int passes, hipEvent_t *ev_stop) {
   *ev_stop = nullptr;
   if (!c->profiling) return GMI_OK;
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
@@ -169,7 +169,7 @@ int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop) {
   }
   if ((rc = hip_check(c, hipEventRecord(c->gs_events[c->gs_used], c->stream), "event")))
     return rc;
-  c->gs_recs.push_back({c->gs_used, k, n});
+  c->gs_recs.push_back({c->gs_used, k, n, passes});
   *ev_stop = c->gs_events[c->gs_used + 1];
   c->gs_used += 2;
   return GMI_OK;
@@ -538,8 +538,8 @@ int gmi_profile_krylov(gmi_ctx *c, double *gs_ms, double *gs_bytes, int *steps) 
                         "elapsed")))
       return rc;
     tot += t;
-    // two Gram-Schmidt passes, each reading k columns and w and writing w
-    bytes += 2.0 * 8.0 * double(g.n) * double(g.k + 2);
+    // per Gram-Schmidt pass: k columns and w read, w written
+    bytes += double(g.passes) * 8.0 * double(g.n) * double(g.k + 2);
   }
   *gs_ms = tot;
   *gs_bytes = bytes;

@@ -59,10 +59,10 @@ struct gmi_ctx {
   size_t prof_used = 0;
   // Gram-Schmidt brackets of eager Arnoldi steps while profiling: events
   // gs_events[2i], gs_events[2i+1] around the step's Gram-Schmidt kernels,
-  // the step's column count k and vector length n
+  // the step's column count k, vector length n and Gram-Schmidt passes
   struct GsRec {
     size_t ev;
-    int k, n;
+    int k, n, passes;
   };
   std::vector<hipEvent_t> gs_events;
   size_t gs_used = 0;
@@ -91,7 +91,7 @@ int prof_start(gmi_ctx *c, hipEvent_t *ev_stop);
 // Gram-Schmidt profiling of one Arnoldi step with k previous columns of
 // length n: records the start event and returns the stop event in *ev_stop
 // (nullptr when not profiling or while the stream is captured).
-int prof_gs_start(gmi_ctx *c, int k, int n, hipEvent_t *ev_stop);
+int prof_gs_start(gmi_ctx *c, int k, int n, int passes, hipEvent_t *ev_stop);
 // skip: the launch is a no-op when non-null and *skip != 0
 int run_pairs(gmi_ctx *c, int mask, int accumulate, bool fp32, const double *Fs,
               const double *Fb, double *Vs, double *Vb, const double *Xs,

@@ -615,7 +615,7 @@ int gmres_enqueue(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mix
     if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
     // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
     hipEvent_t gs_stop = nullptr;
-    if ((r = prof_gs_start(c, it, n, &gs_stop))) return r;
+    if ((r = prof_gs_start(c, it, n, 2, &gs_stop))) return r;
     hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
                        ds.R, pp1, s->flags);
     if (ds.wide)

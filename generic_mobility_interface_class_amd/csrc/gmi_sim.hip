@@ -536,6 +536,8 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
 //   k_gs_chunk<1>      h1, w -= Q h1, the second pass's dots
 //   k_gs_chunk<2>      h2, w -= Q h2, |w|^2 shares and the -K^T w shares the
 //                      next preconditioner needs
+// With one Gram-Schmidt pass per step (gmi_sim_set_gram_schmidt) k_gs_chunk<2>
+// takes the first pass's dots and k_gs_chunk<1> is not launched.
 // Partials are per chunk and every consumer sums them in chunk order.  Rows
 // past a chunk's blobs (the padding of the SoA layout) are zero in every
 // Krylov vector and take no part.  Thread t < 198 owns slot t of its chunk:
@@ -785,7 +787,9 @@ __global__ __launch_bounds__(kGsThreads) void k_kprod_dots(
 // partials (workgroup 0 stores it in hout), w -= Q h on the chunk's rows
 // (thread t its slot t); PASS 1: the dots of the new w with the columns
 // (the next pass's partials); PASS 2: the chunk's share of |w|^2 (npart)
-// and of -K^T w (kpart).
+// and of -K^T w (kpart).  With one pass per step PASS 2 follows the first
+// dots directly and workgroup 0 zeroes hzero, the unused second pass's
+// coefficients that the Givens step adds.
 template <int PASS>
 __global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *chunks,
                                                          const int *sw_chunk0, const double *Qb,
@@ -793,8 +797,11 @@ __global__ __launch_bounds__(kGsThreads) void k_gs_chunk(SimDev a, const BTile *
                                                          double *hout, double *w,
                                                          double *part_out, int nparts,
                                                          double *npart, double *kpart,
-                                                         const int *flags, int h_ready) {
+                                                         const int *flags, int h_ready,
+                                                         double *hzero) {
   if (flags[F_DONE]) return;
+  if (hzero && blockIdx.x == 0)
+    for (int j = threadIdx.x; j < k; j += kGsThreads) hzero[j] = 0.0;
   __shared__ double hs[kMaxCols + 16];
   __shared__ double sv[kChunkRows];
   __shared__ double sq[4][256];
@@ -2312,7 +2319,7 @@ struct SimOp {
     if ((rc = launch_ok(c, "arnoldi precond"))) return rc;
     if ((rc = sys_pairs(c, s, s->z, mixed ? 2 : 0, s->flags))) return rc;
     hipEvent_t gs_stop = nullptr;
-    if ((rc = prof_gs_start(c, it, d.npad, &gs_stop))) return rc;
+    if ((rc = prof_gs_start(c, it, d.npad, s->gs_passes, &gs_stop))) return rc;
     double *pp1 = s->pp, *pp2 = s->pp + s->pp_half;
     const WorkList &wl = c->lists[kBB];
     hipLaunchKernelGGL(k_kprod_dots, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks, s->ch_tile,
@@ -2324,15 +2331,22 @@ struct SimOp {
     if (wide)
       hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, nc, s->h1,
                          s->flags);
-    hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
-                       s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, pp2, nc, nullptr, nullptr,
-                       s->flags, int(wide));
-    if (wide)
-      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, nc, s->h2,
-                         s->flags);
-    hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
-                       s->sw_chunk0, s->Qk, ldq, it, pp2, s->h2, w, nullptr, nc, s->part,
-                       s->ch_kpart, s->flags, int(wide));
+    if (s->gs_passes == 1) {
+      // one pass, as the reference's modified Gram-Schmidt takes one
+      hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
+                         s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, nullptr, nc, s->part,
+                         s->ch_kpart, s->flags, int(wide), s->h2);
+    } else {
+      hipLaunchKernelGGL(k_gs_chunk<1>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
+                         s->sw_chunk0, s->Qk, ldq, it, pp1, s->h1, w, pp2, nc, nullptr, nullptr,
+                         s->flags, int(wide), nullptr);
+      if (wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, nc, s->h2,
+                           s->flags);
+      hipLaunchKernelGGL(k_gs_chunk<2>, dim3(nc), dim3(kGsThreads), 0, c->stream, d, s->chunks,
+                         s->sw_chunk0, s->Qk, ldq, it, pp2, s->h2, w, nullptr, nc, s->part,
+                         s->ch_kpart, s->flags, int(wide), nullptr);
+    }
     if (gs_stop && (rc = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return rc;
     return launch_ok(c, "arnoldi step");
   }
@@ -2790,6 +2804,17 @@ int gmi_sim_set_jacobian_pivoting(gmi_ctx *c, int always_pivot) {
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
   if (always_pivot != 0 && always_pivot != 1) return fail(c, GMI_ERR_ARG, "always_pivot is 0 or 1");
   s->gj_pivot_always = always_pivot != 0;
+  return GMI_OK;
+}
+
+int gmi_sim_set_gram_schmidt(gmi_ctx *c, int passes) {
+  SimHost *s = c ? sim_of(c) : nullptr;
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no sim") : GMI_ERR_ARG;
+  if (passes != 1 && passes != 2) return fail(c, GMI_ERR_ARG, "passes is 1 or 2");
+  int rc;
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "sync"))) return rc;
+  s->drop_graphs();  // captured chunks hold the other pass count's kernels
+  s->gs_passes = passes;
   return GMI_OK;
 }
 

@@ -271,15 +271,17 @@ def test_captured_broyden_iterations_match_eager(gpu, precision, fitted):
     assert ge.graph_stats()[0] == 0 and n_it > 0 and n_graphs > 0
 
 
-def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
+@pytest.mark.parametrize("passes", [1, 2])
+def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu, passes):
     """gmi_profile_krylov: eagerly launched Arnoldi steps are bracketed one
     each; steps launched past convergence inside a chunk (flag-skipped) are
     not counted, so the count is the GMRES iteration total of the step, and
-    the bytes are 16 n (k + 2) summed over steps k = 1..its of each solve
-    (n the padded GMRES length).
+    the bytes are 8 p n (k + 2) summed over steps k = 1..its of each solve
+    (p Gram-Schmidt passes, n the padded GMRES length).
     Captured graphs record nothing."""
     cfg = geometry.sim_workload("step_962", seed=1)
     g = capi.Sim(cfg)
+    g.set_gram_schmidt(passes)
     g.make_body_reference_matrices()
     n = 3 * cfg.swimmers[0].nblob + 6
     for use_graphs in (False, True):
@@ -298,8 +300,8 @@ def test_gram_schmidt_profile_counts_the_arnoldi_steps(gpu):
             continue
         assert steps.value == gits
         assert ms.value > 0
-        # 16 n' (k + 2) per step with n' >= n (the padded length) and k >= 1
-        assert nbytes.value >= 16.0 * n * 3 * gits
+        # 8 p n' (k + 2) per step with n' >= n (the padded length) and k >= 1
+        assert nbytes.value >= 8.0 * passes * n * 3 * gits
 
 
 @pytest.mark.parametrize("nseg", [23, 30, 48, 64])
@@ -362,16 +364,19 @@ def _oracle_9122_steps(cfg, nsteps):
     return _ORACLE_9122["steps"]
 
 
-@pytest.mark.parametrize("precision", [capi.GMI_FP64, capi.GMI_FP32_KAHAN])
-def test_step_9122_matches_oracle(gpu, precision):
+@pytest.mark.parametrize("precision, passes", [(capi.GMI_FP64, 1), (capi.GMI_FP32_KAHAN, 1),
+                                               (capi.GMI_FP64, 2)])
+def test_step_9122_matches_oracle(gpu, precision, passes):
     """The 5,120-segment swimmer (256 x 20 segments + 4,002 blobs), three
-    steps on the GPU against the CPU oracle, FP64 and mixed precision:
-    Broyden counts within one, positions within DL*TOL, Lie elements within
-    TOL."""
+    steps on the GPU against the CPU oracle, FP64 and mixed precision, with
+    one Gram-Schmidt pass per Arnoldi step (as the oracle's modified
+    Gram-Schmidt) and with two: Broyden counts within one, positions within
+    DL*TOL, Lie elements within TOL."""
     cfg = geometry.sim_workload("step_9122", seed=1)
-    print(f"step_9122 precision {precision}", flush=True)
+    print(f"step_9122 precision {precision} passes {passes}", flush=True)
     ref = _oracle_9122_steps(cfg, 3)
     g = capi.Sim(cfg, precision=precision)
+    g.set_gram_schmidt(passes)
     if precision == capi.GMI_FP32_KAHAN:
         g.set_sim_precision(capi.GMI_FP32_KAHAN)
     g.make_body_reference_matrices()
