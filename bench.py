@@ -491,6 +491,7 @@ def bench_cilia(args):
         for prec_name, prec in (("fp64", capi.GMI_FP64), ("mixed", capi.GMI_FP32_KAHAN)):
             g = capi.Cilia(cfg, precision=prec)
             g.set_graphs(not args.no_graphs)
+            g.set_gram_schmidt(args.gs_passes)
             for nt in range(3):
                 assert g.step(nt)[0] == 0
             r[prec_name] = _cilia_window(g, 3, k, f"{name} {prec_name}")
@@ -524,6 +525,7 @@ def bench_carpet(args):
              "table_s": table_s}
         g = capi.Cilia(cfg)
         g.set_graphs(not args.no_graphs)
+        g.set_gram_schmidt(args.gs_passes)
         for nt in range(3):
             assert g.step(nt)[0] == 0
         r["fp64"] = _cilia_window(g, 3, k, f"{name} fp64")
@@ -680,8 +682,8 @@ def main():
     ap.add_argument("--step-modes", default=None,
                     help="comma list of step-leg modes (default: all nine)")
     ap.add_argument("--gs-passes", type=int, default=1, choices=(1, 2),
-                    help="Gram-Schmidt passes of the reference-preconditioned Arnoldi step "
-                         "in the step leg (1: one, as the reference)")
+                    help="Gram-Schmidt passes per Arnoldi step in the step, cilia and carpet "
+                         "legs (1: one, as the reference)")
     ap.add_argument("--step-shape", type=int, nargs=2, default=None,
                     metavar=("TARGET_WORKGROUPS", "MIN_CHUNK"),
                     help="pair launch shape of the step leg (default: STEP_LAUNCH_SHAPE)")

@@ -144,6 +144,7 @@ SIGNATURES = [
     ("gmi_cilia_apply_preconditioner", C.c_int, [C.c_void_p, _dp, _dp]),
     ("gmi_cilia_set_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("gmi_cilia_set_graphs", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("gmi_cilia_set_gram_schmidt", C.c_int, [C.c_void_p, C.c_int]),
 ]
 
 GMI_ERR_NOT_CONVERGED = -6
@@ -568,3 +569,8 @@ class Cilia(Context):
 
     def set_graphs(self, use_graphs=True, chunk=6):
         self.call("gmi_cilia_set_graphs", int(bool(use_graphs)), int(chunk))
+
+    def set_gram_schmidt(self, passes=1):
+        """Gram-Schmidt passes per Arnoldi step (1: one, as the reference; 2:
+        reorthogonalised)."""
+        self.call("gmi_cilia_set_gram_schmidt", int(passes))

@@ -1447,6 +1447,17 @@ int gmi_cilia_set_precision(gmi_ctx *c, int precision) {
   return GMI_OK;
 }
 
+int gmi_cilia_set_gram_schmidt(gmi_ctx *c, int passes) {
+  CiliaHost *s = cil_of(c);
+  if (!s) return c ? fail(c, GMI_ERR_STATE, "no prescribed-cilia stepper") : GMI_ERR_ARG;
+  if (passes != 1 && passes != 2) return fail(c, GMI_ERR_ARG, "passes is 1 or 2");
+  int rc;
+  if ((rc = cil_sync(c))) return rc;
+  s->drop_graphs();
+  s->gs_passes = passes;
+  return GMI_OK;
+}
+
 int gmi_cilia_set_graphs(gmi_ctx *c, int use_graphs, int chunk) {
   CiliaHost *s = cil_of(c);
   if (!s) return c ? fail(c, GMI_ERR_STATE, "no prescribed-cilia stepper") : GMI_ERR_ARG;

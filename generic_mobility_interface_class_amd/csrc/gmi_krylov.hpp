@@ -110,6 +110,7 @@ struct KrylovState {
   double *h_scal = nullptr;
   int cols = 0;           // Arnoldi steps the basis holds (H is (cols+1) x (cols+1))
   int chunk = 6;          // Arnoldi steps per host check
+  int gs_passes = 1;      // Gram-Schmidt passes per Arnoldi step (1, or 2: reorthogonalised)
   bool use_graphs = true;
   int graph_gen = -1;
   // captured Arnoldi chunks, keyed by (mixed, first iteration)
@@ -277,12 +278,17 @@ __global__ __launch_bounds__(256) void k_psub_dots(const double *Qb, size_t ldq,
 }
 
 // Gram-Schmidt pass 2: h2 = sum of the pass-2 partials; w -= sum_j h2[j] Q_j;
-// the workgroup's share of |w|^2
+// the workgroup's share of |w|^2.  With one pass per step it follows the
+// first dots directly (h2 is then the only pass's h) and workgroup 0 zeroes
+// hzero, the unused pass's coefficients that the Givens step adds.
 __global__ __launch_bounds__(256) void k_psub_norm(const double *Qb, size_t ldq, int k,
                                                    const double *part_in, double *h2, double *w,
                                                    int n, int R, double *norm_part,
-                                                   const int *flags, int h_ready) {
+                                                   const int *flags, int h_ready,
+                                                   double *hzero) {
   if (flags[F_DONE]) return;
+  if (hzero && blockIdx.x == 0)
+    for (int j = threadIdx.x; j < k; j += 256) hzero[j] = 0.0;
   __shared__ double hs[kMaxCols];
   __shared__ double sh[256];
   if (h_ready)
@@ -613,21 +619,27 @@ int gmres_enqueue(gmi_ctx *c, KrylovState *s, Op &op, int n, int maxit, bool mix
     // kernel (the first one was scaled by beta0 already)
     if ((r = op.precond(qprev, zk, s->flags, it >= 2 ? 3 : -1))) return r;
     if ((r = op.mult(zk, w, mixed ? 2 : 0, s->flags, true))) return r;
-    // CGS2: pass-1 dots, update + pass-2 dots, update + norm, Givens
+    // pass-1 dots, update + norm (one pass), or update + pass-2 dots,
+    // update + norm (two passes); then Givens
     hipEvent_t gs_stop = nullptr;
-    if ((r = prof_gs_start(c, it, n, 2, &gs_stop))) return r;
+    if ((r = prof_gs_start(c, it, n, s->gs_passes, &gs_stop))) return r;
     hipLaunchKernelGGL(k_pdots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, w, n,
                        ds.R, pp1, s->flags);
     if (ds.wide)
       hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp1, ds.nb, s->h1,
                          s->flags);
-    hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it, pp1,
-                       s->h1, w, n, ds.R, pp2, s->flags, int(ds.wide));
-    if (ds.wide)
-      hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
-                         s->flags);
-    hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
-                       s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide));
+    if (s->gs_passes == 1) {
+      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp1,
+                         s->h1, w, n, ds.R, s->part, s->flags, int(ds.wide), s->h2);
+    } else {
+      hipLaunchKernelGGL(k_psub_dots, dim3(ds.nb), dim3(256), shm, c->stream, s->Qk, ldq, it,
+                         pp1, s->h1, w, n, ds.R, pp2, s->flags, int(ds.wide));
+      if (ds.wide)
+        hipLaunchKernelGGL(k_colsum_wide, dim3(it), dim3(256), 0, c->stream, pp2, ds.nb, s->h2,
+                           s->flags);
+      hipLaunchKernelGGL(k_psub_norm, dim3(ds.nb), dim3(256), 0, c->stream, s->Qk, ldq, it, pp2,
+                         s->h2, w, n, ds.R, s->part, s->flags, int(ds.wide), nullptr);
+    }
     if (gs_stop && (r = hip_check(c, hipEventRecord(gs_stop, c->stream), "event"))) return r;
     hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, c->stream, s->part, ds.nb, it, ga);
     if ((r = launch_ok(c, "arnoldi"))) return r;

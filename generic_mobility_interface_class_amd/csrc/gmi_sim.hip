@@ -536,8 +536,8 @@ __global__ __launch_bounds__(256) void k_kprod(SimDev a, const BTile *tiles, con
 //   k_gs_chunk<1>      h1, w -= Q h1, the second pass's dots
 //   k_gs_chunk<2>      h2, w -= Q h2, |w|^2 shares and the -K^T w shares the
 //                      next preconditioner needs
-// With one Gram-Schmidt pass per step (gmi_sim_set_gram_schmidt) k_gs_chunk<2>
-// takes the first pass's dots and k_gs_chunk<1> is not launched.
+// With one Gram-Schmidt pass per step (the default; gmi_sim_set_gram_schmidt)
+// k_gs_chunk<2> takes the first pass's dots and k_gs_chunk<1> is not launched.
 // Partials are per chunk and every consumer sums them in chunk order.  Rows
 // past a chunk's blobs (the padding of the SoA layout) are zero in every
 // Krylov vector and take no part.  Thread t < 198 owns slot t of its chunk:

@@ -80,7 +80,6 @@ struct SimHost : KrylovState {
          *hist_Gw = nullptr, *hist_part = nullptr, *hist_coef = nullptr, *hist_gsave = nullptr;
   bool gj_lds = false;     // Gauss-Jordan inverse with the matrix in LDS
   bool gj_pivot_always = false;  // skip the checked pivot-free inverse
-  int gs_passes = 1;             // Gram-Schmidt passes of the fused Arnoldi step (1 or 2)
   Learn *learn = nullptr;
   DensePrec *dense = nullptr;  // null: the reference's self-mobility preconditioner
   // blob tiles for the preconditioner and K products

@@ -117,6 +117,10 @@ int gmi_cilia_apply_preconditioner(gmi_ctx *ctx, const double *in, double *out);
 int gmi_cilia_set_precision(gmi_ctx *ctx, int precision);
 /* HIP-graph capture of the Arnoldi chunks (default on, chunk 6). */
 int gmi_cilia_set_graphs(gmi_ctx *ctx, int use_graphs, int chunk);
+/* Gram-Schmidt passes per Arnoldi step: 1 (default, one pass as the
+ * reference's modified Gram-Schmidt, mobility_solver.cpp:1175-1183) or 2
+ * (reorthogonalised).  Captured graphs are dropped. */
+int gmi_cilia_set_gram_schmidt(gmi_ctx *ctx, int passes);
 
 #ifdef __cplusplus
 }

@@ -135,13 +135,12 @@ int gmi_sim_graph_stats(gmi_ctx *ctx, int *captured_iterations, int *graphs);
  * counterpart: a warm start of this build. */
 int gmi_sim_set_gmres_history(gmi_ctx *ctx, int m);
 
-/* Gram-Schmidt passes of each Arnoldi step under the reference's
- * preconditioner (the fused step): 1 (default) orthogonalises the new vector
- * once against the basis, as the reference's modified Gram-Schmidt does
- * (mobility_solver.cpp:1175-1183); 2 repeats the pass (classical
- * Gram-Schmidt with reorthogonalisation).  The other operators' GMRES always
- * takes two passes.  Captured graphs are dropped.  No reference counterpart
- * beyond the choice of algorithm. */
+/* Gram-Schmidt passes of each Arnoldi step of the rigid-body GMRES: 1
+ * (default) orthogonalises the new vector once against the basis, as the
+ * reference's modified Gram-Schmidt does (mobility_solver.cpp:1175-1183); 2
+ * repeats the pass (classical Gram-Schmidt with reorthogonalisation).
+ * Captured graphs are dropped.  No reference counterpart beyond the choice
+ * of algorithm. */
 int gmi_sim_set_gram_schmidt(gmi_ctx *ctx, int passes);
 
 /* How the filaments' approximate Jacobians are inverted (the reference's
