@@ -3,9 +3,10 @@
 //
 // Right-preconditioned GMRES (mobility_solver.cpp:970-1354) with the Krylov
 // basis, Hessenberg matrix and Givens rotations resident in HBM.  Arnoldi
-// uses classical Gram-Schmidt applied twice (CGS2): one step is two batched
-// dot-product sweeps over all previous columns instead of k dependent ones
-// (same basis to rounding).  The host only reads the convergence flag once
+// uses classical Gram-Schmidt, one pass per step by default (as the
+// reference's one modified Gram-Schmidt pass) or two (reorthogonalised):
+// a pass is one batched dot-product sweep over all previous columns instead
+// of k dependent ones.  The host only reads the convergence flag once
 // per chunk of Arnoldi steps; a chunk can be captured once as a HIP graph
 // and replayed by later solves.
 //
@@ -549,7 +550,8 @@ static DotShape dot_shape(int n) {
 }
 
 // GMRES with right preconditioning (mobility_solver.cpp:970-1354) on s->rhs,
-// warm-started from s->x; CGS2 Arnoldi.  mixed: relaxed precision (FP64
+// warm-started from s->x; classical Gram-Schmidt Arnoldi (s->gs_passes
+// passes per step).  mixed: relaxed precision (FP64
 // operator until the residual drops below switch_rel, FP32 after).
 // n: vector length (padding rows stay zero); maxit: iteration cap.
 // The stages: gmres_begin (initial residual and the start decisions on the

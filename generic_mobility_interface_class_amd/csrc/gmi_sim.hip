@@ -10,12 +10,12 @@
 //                  Jordan inverse, Schur blocks, J^-1 x
 //   per swimmer  : body state, Schur complement,       (thread / workgroup
 //                  preconditioner and K products        per swimmer)
-//   per vector   : GMRES CGS2 Arnoldi, Givens,          (grid-wide kernels)
+//   per vector   : GMRES Arnoldi, Givens,               (grid-wide kernels)
 //                  Broyden rank-one updates
 // Numerics follow the reference (file:line on every kernel); GMRES uses
-// classical Gram-Schmidt applied twice instead of modified Gram-Schmidt so
-// that one Arnoldi step is two batched dot-product sweeps rather than k
-// dependent ones (same Krylov basis to rounding).
+// classical Gram-Schmidt instead of modified Gram-Schmidt (one pass per
+// step by default, as the reference; two on request) so that a pass is one
+// batched dot-product sweep rather than k dependent ones.
 #include "gmi_internal.hpp"
 #include "gmi_ctx.hpp"
 #include "gmi_lie.hpp"
